@@ -1,0 +1,367 @@
+"""ctypes view of the C-ABI in include/dem3d.h and a thin host-side object.
+
+`System` mirrors the module interface the Fortran driver uses
+(src/MForces.f90:145 ComputeAccelerations, src/MDEM.f90:258-369
+MoveParticlesEuler/MoveParticles) over any library that exports the ABI under
+a given symbol prefix.  The product library is libdem3d_hip.so (prefix
+"dem_"); `load_hip()` refuses to run without it -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .spec import Reduced
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(HERE, "libdem3d_hip.so")
+
+DEM_OK = 0
+DEM_ERR_ESCAPE = -6
+DEM_FP64, DEM_MIXED = 0, 1
+DEM_HIST_REFERENCE, DEM_HIST_CANONICAL = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint64)
+_lp = C.POINTER(C.c_int64)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nc", C.c_int32), ("nw", C.c_int32), ("pbc", C.c_int32 * 3),
+        ("method_order", C.c_int32), ("n_euler", C.c_int32), ("max_ncell", C.c_int32),
+        ("ref_grid", C.c_int32 * 3), ("precision", C.c_int32), ("history_mode", C.c_int32),
+        ("max_neighbours", C.c_int32), ("max_contacts", C.c_int32),
+        ("lbox", C.c_double * 3), ("gravity", C.c_double * 3), ("time_step", C.c_double),
+        ("skin", C.c_double),
+        ("radius", _dp), ("mass", _dp), ("inertia", _dp), ("inv_mass", _dp),
+        ("inv_inertia", _dp), ("eff_mass", _dp), ("min_dij", _dp), ("min_dij_sq", _dp),
+        ("kn", _dp), ("yn", _dp), ("kt", _dp), ("yt", _dp), ("musq", _dp),
+        ("knw", _dp), ("ynw", _dp), ("ktw", _dp), ("ytw", _dp), ("muwsq", _dp),
+    ]
+
+
+class WallT(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32), ("order", C.c_int32), ("l1", C.c_double), ("l2", C.c_double),
+        ("l3", C.c_double), ("stiffness", C.c_double), ("shear_stiffness", C.c_double),
+        ("u", C.c_double * 3), ("w", C.c_double * 3), ("n", C.c_double * 3),
+        ("position", C.c_double * 3), ("velocity", C.c_double * 3), ("omega", C.c_double * 3),
+    ]
+
+
+class ParticleT(C.Structure):
+    _fields_ = [("index", C.c_int32), ("pad_", C.c_int32), ("position", C.c_double * 3),
+                ("velocity", C.c_double * 3), ("acceleration", C.c_double * 3),
+                ("omega", C.c_double * 3), ("alpha", C.c_double * 3)]
+
+
+assert C.sizeof(ParticleT) == 128
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("np", C.c_int64), ("pair_contacts", C.c_int64), ("wall_contacts", C.c_int64),
+        ("stored_pairs", C.c_int64), ("ncol", C.c_int64), ("evaluations", C.c_int64),
+        ("steps", C.c_int64), ("rebuilds", C.c_int64), ("neighbour_pairs", C.c_int64),
+        ("stale_dropped", C.c_int64), ("ref_ncell", C.c_int64 * 3), ("nb_ncell", C.c_int64 * 3),
+        ("kinetic_energy", C.c_double), ("max_overlap", C.c_double), ("ms_pass1", C.c_double),
+        ("ms_pass2", C.c_double), ("ms_rebuild", C.c_double), ("ms_step", C.c_double),
+    ]
+
+    def asdict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+def _ptr(a: Optional[np.ndarray], typ=_dp):
+    if a is None:
+        return C.cast(None, typ)
+    return a.ctypes.data_as(typ)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Library:
+    """Binds the ABI functions of one shared library under a symbol prefix."""
+
+    FUNCS = {
+        "create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+        "destroy": (None, [C.c_void_p]),
+        "last_error": (C.c_char_p, [C.c_void_p]),
+        "set_walls": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallT)]),
+        "upload_state": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip]),
+        "download_state": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+        "download_predicted": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
+        "upload_contacts": (C.c_int, [C.c_void_p, C.c_int64, _up, _dp, _dp, C.c_int64, _up, _dp, _dp]),
+        "download_contacts": (C.c_int, [C.c_void_p, C.c_int64, _lp, _up, _dp, _dp, C.c_int64, _lp,
+                                        _up, _dp, _dp]),
+        "download_cells": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
+        "compute_accelerations_aos": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ParticleT),
+                                                C.c_double, C.c_int32]),
+        "compute_accelerations": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+        "euler_startup": (C.c_int, [C.c_void_p]),
+        "step": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _lp, _ip]),
+        "stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    }
+    OPTIONAL = {
+        "synchronize": (C.c_int, [C.c_void_p]),
+        "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
+        "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    }
+
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built")
+        self.path = path
+        self.lib = C.CDLL(path)
+        self.prefix = prefix
+        for name, (res, args) in list(self.FUNCS.items()) + list(self.OPTIONAL.items()):
+            try:
+                f = getattr(self.lib, prefix + name)
+            except AttributeError:
+                if name in self.OPTIONAL:
+                    continue
+                raise
+            f.restype = res
+            f.argtypes = args
+            setattr(self, name, f)
+
+    def symbols(self):
+        return [self.prefix + n for n in self.FUNCS]
+
+
+_hip: Optional[Library] = None
+
+
+def load_hip() -> Library:
+    """The product library.  Raises if it is missing: there is no fallback."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError(
+                f"{HIP_LIB} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        _hip = Library(HIP_LIB, "dem_")
+    return _hip
+
+
+def make_params(red: Reduced, precision=DEM_FP64, history_mode=DEM_HIST_REFERENCE,
+                skin: float = 0.0, max_neighbours=0, max_contacts=0, max_ncell=None):
+    """dem_params_t from reduced constants; the numpy buffers are kept alive on
+    the returned object."""
+    p = Params()
+    keep = {}
+
+    def arr(name, a, shape):
+        a = np.asfortranarray(np.asarray(a, np.float64).reshape(shape)).ravel(order="F")
+        a = np.ascontiguousarray(a)
+        keep[name] = a
+        setattr(p, name, a.ctypes.data_as(_dp))
+
+    nc, nw = red.nc, red.nw
+    p.nc, p.nw = nc, nw
+    for d in range(3):
+        p.pbc[d] = int(red.pbc[d])
+        p.lbox[d] = float(red.lbox[d])
+        p.gravity[d] = float(red.gravity[d])
+    p.method_order = red.method_order
+    p.n_euler = red.n_euler
+    p.max_ncell = int(red.max_ncell if max_ncell is None else max_ncell)
+    if red.ref_grid is not None:
+        for d in range(3):
+            p.ref_grid[d] = int(red.ref_grid[d])
+    p.precision = precision
+    p.history_mode = history_mode
+    p.max_neighbours = max_neighbours
+    p.max_contacts = max_contacts
+    p.time_step = float(red.time_step)
+    p.skin = float(skin)
+    for name in ("radius", "mass", "inertia", "inv_mass", "inv_inertia", "eff_mass"):
+        arr(name, getattr(red, name), (nc,))
+    for name in ("min_dij", "min_dij_sq", "kn", "yn", "kt", "yt", "musq"):
+        arr(name, getattr(red, name), (nc, nc))
+    for name in ("knw", "ynw", "ktw", "ytw", "muwsq"):
+        a = np.asarray(getattr(red, name), np.float64).reshape(nc, nw)
+        if nw == 0:
+            a = np.zeros((nc, 1))
+        arr(name, a, a.shape)
+    p._keep = keep
+    return p
+
+
+def make_walls(red: Reduced):
+    arr = (WallT * max(red.nw, 1))()
+    for j, w in enumerate(red.walls):
+        t = arr[j]
+        t.shape = int(w["shape"])
+        t.order = 0
+        t.l1, t.l2, t.l3 = float(w["l1"]), float(w["l2"]), float(w["l3"])
+        t.stiffness = float(w["stiffness"])
+        t.shear_stiffness = float(w["shear_stiffness"])
+        for d in range(3):
+            t.u[d] = float(w["u"][d])
+            t.w[d] = float(w["w"][d])
+            t.n[d] = float(w["n"][d])
+            t.position[d] = float(w["position"][d])
+            t.velocity[d] = float(w["velocity"][d])
+            t.omega[d] = float(w["omega"][d])
+    return arr
+
+
+class System:
+    """One granular system resident in a library (GPU product or a checker)."""
+
+    def __init__(self, lib: Library, red: Reduced, **kw):
+        self.lib = lib
+        self.red = red
+        self.params = make_params(red, **kw)
+        h = C.c_void_p()
+        rc = lib.create(C.byref(self.params), C.byref(h))
+        if rc != DEM_OK:
+            raise DemError(rc, lib.last_error(None).decode())
+        self.h = h
+        self.np = 0
+        self.nprev = red.method_order - 1
+        if red.nw:
+            self.walls = make_walls(red)
+            self._check(lib.set_walls(self.h, red.nw, self.walls))
+
+    def _check(self, rc):
+        if rc != DEM_OK:
+            raise DemError(rc, self.lib.last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state
+    def upload(self, comp, x, v, w, a=None, alpha=None, ring=None, ipc=None, cell_rank=None):
+        comp = np.ascontiguousarray(comp, np.int32)
+        n = comp.shape[0]
+        self.np = n
+        conv = lambda q: None if q is None else _f64(np.asarray(q).reshape(3, n))
+        x, v, w, a, alpha = map(conv, (x, v, w, a, alpha))
+        ring = None if ring is None else _f64(ring)
+        ipc = None if ipc is None else np.ascontiguousarray(ipc, np.int32)
+        cr = None if cell_rank is None else np.ascontiguousarray(cell_rank, np.int32)
+        self._keep = (comp, x, v, w, a, alpha, ring, ipc, cr)
+        self._check(self.lib.upload_state(self.h, n, _ptr(comp, _ip), _ptr(x), _ptr(v), _ptr(w),
+                                          _ptr(a), _ptr(alpha), _ptr(ring), _ptr(ipc, _ip),
+                                          _ptr(cr, _ip)))
+
+    def download(self):
+        n = self.np
+        comp = np.zeros(n, np.int32)
+        x, v, w, a, al = (np.zeros((3, n)) for _ in range(5))
+        ring = np.zeros((max(self.nprev, 1), 9, n))
+        ipc = np.zeros(3, np.int32)
+        self._check(self.lib.download_state(self.h, n, _ptr(comp, _ip), _ptr(x), _ptr(v), _ptr(w),
+                                            _ptr(a), _ptr(al), _ptr(ring), _ptr(ipc, _ip)))
+        return dict(comp=comp, x=x, v=v, w=w, a=a, alpha=al, ring=ring[: self.nprev],
+                    ipc=ipc[: self.nprev])
+
+    def download_predicted(self):
+        n = self.np
+        x, v, w, a, al = (np.zeros((3, n)) for _ in range(5))
+        self._check(self.lib.download_predicted(self.h, n, _ptr(x), _ptr(v), _ptr(w), _ptr(a),
+                                                _ptr(al)))
+        return dict(x=x, v=v, w=w, a=a, alpha=al)
+
+    def upload_contacts(self, keys, ft, vt, wkeys=None, wft=None, wvt=None):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        ft, vt = _f64(np.asarray(ft).reshape(-1, 3)), _f64(np.asarray(vt).reshape(-1, 3))
+        if wkeys is None:
+            wkeys, wft, wvt = np.zeros(0, np.uint64), np.zeros((0, 3)), np.zeros((0, 3))
+        wkeys = np.ascontiguousarray(wkeys, np.uint64)
+        wft, wvt = _f64(np.asarray(wft).reshape(-1, 3)), _f64(np.asarray(wvt).reshape(-1, 3))
+        self._check(self.lib.upload_contacts(self.h, keys.shape[0], _ptr(keys, _up), _ptr(ft),
+                                             _ptr(vt), wkeys.shape[0], _ptr(wkeys, _up),
+                                             _ptr(wft), _ptr(wvt)))
+
+    def download_contacts(self):
+        n = C.c_int64()
+        m = C.c_int64()
+        self._check(self.lib.download_contacts(self.h, 0, C.byref(n), None, None, None, 0,
+                                               C.byref(m), None, None, None))
+        keys = np.zeros(n.value, np.uint64)
+        ft, vt = np.zeros((n.value, 3)), np.zeros((n.value, 3))
+        wkeys = np.zeros(m.value, np.uint64)
+        wft, wvt = np.zeros((m.value, 3)), np.zeros((m.value, 3))
+        self._check(self.lib.download_contacts(self.h, n.value, C.byref(n), _ptr(keys, _up),
+                                               _ptr(ft), _ptr(vt), m.value, C.byref(m),
+                                               _ptr(wkeys, _up), _ptr(wft), _ptr(wvt)))
+        return dict(keys=keys, ft=ft, vt=vt, wkeys=wkeys, wft=wft, wvt=wvt)
+
+    def download_cells(self):
+        c = np.zeros(self.np, np.int32)
+        self._check(self.lib.download_cells(self.h, self.np, _ptr(c, _ip)))
+        return c
+
+    # ---- hot path
+    def compute_accelerations_aos(self, comp, x, v, w, time_step=None, update=True):
+        n = self.np
+        arr = (ParticleT * n)()
+        buf = np.frombuffer(arr, dtype=np.uint8).reshape(n, 128)
+        rec = np.zeros((n, 16), np.float64)
+        rec[:, 1:4] = np.asarray(x).reshape(3, n).T
+        rec[:, 4:7] = np.asarray(v).reshape(3, n).T
+        rec[:, 10:13] = np.asarray(w).reshape(3, n).T
+        buf[:] = rec.view(np.uint8)
+        np.frombuffer(arr, dtype=np.int32).reshape(n, 32)[:, 0] = np.asarray(comp, np.int32)
+        dt = self.red.time_step if time_step is None else time_step
+        self._check(self.lib.compute_accelerations_aos(self.h, n, arr, float(dt), int(update)))
+        out = np.frombuffer(arr, dtype=np.float64).reshape(n, 16)
+        return out[:, 7:10].T.copy(), out[:, 13:16].T.copy()
+
+    def compute_accelerations(self, which=0, update=True):
+        self._check(self.lib.compute_accelerations(self.h, int(which), int(update)))
+
+    def euler_startup(self):
+        self._check(self.lib.euler_startup(self.h))
+
+    def startup(self):
+        """The driver's start-up loop (src/DEM.f90:87-92)."""
+        for _ in range(self.red.method_order - 1):
+            self.euler_startup()
+
+    def step(self, nsteps, first_step=None, allow_escape=False):
+        done = C.c_int64()
+        esc = C.c_int32()
+        first = self.red.method_order if first_step is None else first_step
+        rc = self.lib.step(self.h, int(nsteps), int(first), C.byref(done), C.byref(esc))
+        if rc == DEM_ERR_ESCAPE and allow_escape:
+            return done.value, esc.value
+        self._check(rc)
+        return done.value, esc.value
+
+    def synchronize(self):
+        if hasattr(self.lib, "synchronize"):
+            self._check(self.lib.synchronize(self.h))
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.stats(self.h, C.byref(s)))
+        return s.asdict()

@@ -1,0 +1,215 @@
+"""Seeded synthetic granular systems for parity tests and benchmarks.
+
+All systems use the reference's input mapping (src/DEM.f90:360-739): the pair
+spring kn enters as component Stiffness = 2 kn (series springs,
+src/DEM.f90:273,278-280), ShearStiffness = 2/7 Stiffness, walls have
+Stiffness 2 kn (so Knw = kn), tangential restitution equals normal
+restitution, MinStepsPerCollision = 50, MethodOrder = 4,
+NumberOfEulerIntervals = 10, no fluid.  The box is centred on the origin
+(src/MConfig.f90:3); the floor is a Rectangle wall at z = -Lz/2 whose
+half-lengths exceed the periodic box so every particle sees its face.
+
+Positions are returned in reduced units (divided by the length scale) as an
+array x[3, np]; components are 1-based.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from .spec import RECTANGLE, Component, Reduced, SystemSpec, Wall, reduce_spec
+
+
+@dataclass
+class Workload:
+    name: str
+    spec: SystemSpec
+    red: Reduced
+    comp: np.ndarray  # int32 [np], 1-based
+    x: np.ndarray     # [3, np] reduced
+    v: np.ndarray     # [3, np] reduced
+    w: np.ndarray     # [3, np] reduced
+    steps: int
+    note: str = ""
+
+    @property
+    def np(self) -> int:
+        return int(self.comp.shape[0])
+
+
+def _spec(lbox, diameters, density, kn, e, mu, pbc=(True, True, False), walls=None,
+          total_steps=1000, gravity=(0.0, 0.0, -9.81)):
+    nc = len(diameters)
+    comps = [Component(d, density, 2.0 * kn, 2.0 / 7.0 * 2.0 * kn) for d in diameters]
+    walls = walls or []
+    nw = len(walls)
+    spec = SystemSpec(
+        lbox=list(lbox), components=comps, walls=walls,
+        en=np.full((nc, nc), e), et=np.full((nc, nc), e), mu=np.full((nc, nc), mu),
+        enw=np.full((nc, nw), e), etw=np.full((nc, nw), e), muw=np.full((nc, nw), mu),
+        pbc=list(pbc), gravity=list(gravity), total_time=1.0,
+    )
+    # choose TotalTime so that NumberOfSteps = total_steps at dt = Tc/50
+    probe = reduce_spec(spec)
+    tc = probe.time_scale
+    spec.total_time = total_steps * tc / spec.min_steps_per_collision
+    return spec
+
+
+def _floor(lx, ly, lz, kn, top: Optional[float] = None, top_velocity=(0.0, 0.0, 0.0)):
+    walls = [Wall(RECTANGLE, 4.0 * lx, 4.0 * ly, 0.0, 2.0 * kn, 2.0 / 7.0 * 2.0 * kn,
+                  (0.0, 0.0, -0.5 * lz), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0))]
+    if top is not None:
+        walls.append(Wall(RECTANGLE, 4.0 * lx, 4.0 * ly, 0.0, 2.0 * kn, 2.0 / 7.0 * 2.0 * kn,
+                          (0.0, 0.0, top), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0),
+                          velocity=top_velocity))
+    return walls
+
+
+def config0(seed: int = 20240001, steps: int = 20000) -> Workload:
+    """2 000 monodisperse spheres (d = 1 mm, 2500 kg/m^3) on a 10x10x20 simple
+    cubic lattice at spacing 1.2 d with +-0.05 d jitter, box 12 x 12 x 48 mm,
+    periodic x,y, floor at z = -Lz/2, velocities U(+-0.05 m/s), kn = 1e3 N/m,
+    e = 0.9, mu = 0.5."""
+    d, kn = 1e-3, 1e3
+    lx, ly, lz = 12e-3, 12e-3, 48e-3
+    spec = _spec((lx, ly, lz), [d], 2500.0, kn, 0.9, 0.5, walls=_floor(lx, ly, lz, kn),
+                 total_steps=steps)
+    red = reduce_spec(spec)
+    rng = np.random.default_rng(seed)
+    a = 1.2 * d
+    ix, iy, iz = np.meshgrid(np.arange(10), np.arange(10), np.arange(20), indexing="ij")
+    pos = np.stack([-0.5 * lx + 0.5 * a + a * ix.ravel(),
+                    -0.5 * ly + 0.5 * a + a * iy.ravel(),
+                    -0.5 * lz + 0.5 * a + a * iz.ravel()])
+    pos = pos + rng.uniform(-0.05 * d, 0.05 * d, pos.shape)
+    vel = rng.uniform(-0.05, 0.05, pos.shape)
+    L, T = red.length_scale, red.time_scale
+    n = pos.shape[1]
+    return Workload("config0-mono-2k", spec, red, np.ones(n, np.int32), pos / L, vel * T / L,
+                    np.zeros_like(pos), steps)
+
+
+def config1(seed: int = 20240002, steps: int = 5000, n: int = 200_000, nbins: int = 16) -> Workload:
+    """200 000 polydisperse spheres, radii U[0.4, 0.6] mm discretised into 16
+    equal-width components, at volume fraction ~0.45 (jittered FCC lattice),
+    base 40 x 40 mm, periodic x,y, floor, settling under gravity."""
+    kn = 1e3
+    rng = np.random.default_rng(seed)
+    edges = np.linspace(0.4e-3, 0.6e-3, nbins + 1)
+    centres = 0.5 * (edges[1:] + edges[:-1])
+    r = rng.uniform(0.4e-3, 0.6e-3, n)
+    comp = np.clip(np.searchsorted(edges, r) - 1, 0, nbins - 1)
+    rr = centres[comp]
+    mean_vol = float(np.mean(4.0 / 3.0 * math.pi * rr ** 3))
+    lx = ly = 40e-3
+    # FCC lattice constant for phi = 0.45
+    dens = 0.45 / mean_vol
+    a = (4.0 / dens) ** (1.0 / 3.0)
+    nxy = int(round(lx / a))
+    a = lx / nxy
+    nz = int(math.ceil(n / (4 * nxy * nxy)))
+    lz = 2.0 * nz * a + 10e-3
+    pos = _fcc(nxy, nxy, nz, a)[:, :n]
+    pos[0] -= 0.5 * lx
+    pos[1] -= 0.5 * ly
+    pos[2] += -0.5 * lz + 0.6e-3
+    gap = a / math.sqrt(2.0) - 1.2e-3
+    pos = pos + rng.uniform(-0.45 * gap, 0.45 * gap, pos.shape)
+    spec = _spec((lx, ly, lz), list(2.0 * centres), 2500.0, kn, 0.9, 0.5,
+                 walls=_floor(lx, ly, lz, kn), total_steps=steps)
+    red = reduce_spec(spec)
+    L, T = red.length_scale, red.time_scale
+    vel = rng.uniform(-0.05, 0.05, pos.shape)
+    return Workload("config1-poly-200k", spec, red, (comp + 1).astype(np.int32), pos / L,
+                    vel * T / L, np.zeros_like(pos), steps,
+                    note=f"FCC a={a*1e3:.4f} mm, phi~0.45, Lz={lz*1e3:.2f} mm")
+
+
+def _fcc(nx, ny, nz, a):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cells = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1)
+    # order layer by layer in z so that truncation removes the top
+    order = np.lexsort((cells[:, 0], cells[:, 1], cells[:, 2]))
+    cells = cells[order]
+    pts = (cells[:, None, :] + base[None, :, :]).reshape(-1, 3)
+    pts = pts[np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))]
+    return (pts * a + 0.5 * a * 0.5).T.copy()
+
+
+def config2(seed: int = 20240003, steps: int = 2000, cells=(100, 100, 50),
+            shear_rate: float = 10.0) -> Workload:
+    """4 000 000 bidisperse spheres (d = 1.0 / 1.4 mm, 50/50 by number) in a
+    dense bed: rock-salt arrangement (large spheres on an FCC lattice, small
+    ones on its octahedral sites) with lattice constant 2(r_s + r_L), so every
+    sphere starts with 6 touching neighbours of the other size (Z = 6,
+    phi = 0.567).  Periodic x,y; floor at z = -Lz/2; a top Rectangle wall in
+    contact with the top layer moves in +x at shear_rate * H.  mu = 0.3,
+    e = 0.8, kn = 1e3 N/m.  cells = rock-salt unit cells per direction
+    (8 spheres each): the default gives 4 000 000 spheres."""
+    ds, dl, kn = 1.0e-3, 1.4e-3, 1e3
+    a = (ds + dl) * (1.0 - 1e-5)  # L-S distance a/2 = r_s + r_l, slightly compressed
+    nx, ny, nz = cells
+    lx, ly = nx * a, ny * a
+    h = nz * a
+    lz = h + 2.0e-3
+    rng = np.random.default_rng(seed)
+    ix, iy, iz = np.meshgrid(np.arange(2 * nx), np.arange(2 * ny), np.arange(2 * nz), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    large = ((ix + iy + iz) % 2) == 0
+    pos = np.stack([ix * 0.5 * a, iy * 0.5 * a, iz * 0.5 * a]).astype(np.float64)
+    pos[0] -= 0.5 * lx
+    pos[1] -= 0.5 * ly
+    z0 = -0.5 * lz + 0.5 * dl * (1.0 - 1e-5)
+    pos[2] += z0
+    pos = pos + rng.uniform(-1e-9, 1e-9, pos.shape)
+    ztop = float(pos[2].max()) + 0.5 * dl * (1.0 - 1e-5)
+    vtop = shear_rate * (ztop - (-0.5 * lz))
+    walls = _floor(lx, ly, lz, kn, top=ztop, top_velocity=(vtop, 0.0, 0.0))
+    # walls[1] normal must point into the bed: W x U flips N = U x W
+    walls[1].u, walls[1].w = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
+    spec = _spec((lx, ly, lz), [ds, dl], 2500.0, kn, 0.8, 0.3, walls=walls, total_steps=steps)
+    red = reduce_spec(spec)
+    comp = np.where(large, 2, 1).astype(np.int32)
+    L = red.length_scale
+    n = comp.shape[0]
+    return Workload("config2-bidisperse-sheared", spec, red, comp, pos / L,
+                    np.zeros((3, n)), np.zeros((3, n)), steps,
+                    note=f"rock-salt a={a*1e3:.5f} mm, {nx}x{ny}x{nz} cells, phi=0.567, "
+                         f"top wall v_x={vtop:.4f} m/s")
+
+
+def small_box(n_side=(6, 6, 6), seed=1, jitter=0.05, spacing=1.02, nc=2, ratio=1.3,
+              walls=True, pbc=(True, True, False), vel=0.5, steps=1000, gravity=-9.81,
+              mu=0.5, e=0.9) -> Workload:
+    """Small test system: jittered cubic lattice of spheres in a periodic box
+    (a few hundred particles), dense enough (spacing ~ max diameter) to have
+    many contacts and strong shear at the first steps."""
+    kn = 1e3
+    rng = np.random.default_rng(seed)
+    d0 = 1e-3
+    diams = [d0 * (1.0 + (ratio - 1.0) * k / max(nc - 1, 1)) for k in range(nc)]
+    dmax = max(diams)
+    a = spacing * dmax
+    nx, ny, nz = n_side
+    lx, ly = nx * a, ny * a
+    lz = nz * a + (0.0 if pbc[2] else 2.0 * dmax)
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = np.stack([-0.5 * lx + 0.5 * a + a * ix.ravel(),
+                    -0.5 * ly + 0.5 * a + a * iy.ravel(),
+                    -0.5 * lz + 0.5 * a + a * iz.ravel()]).astype(np.float64)
+    pos += rng.uniform(-jitter * d0, jitter * d0, pos.shape)
+    n = pos.shape[1]
+    comp = rng.integers(1, nc + 1, n).astype(np.int32)
+    wl = _floor(lx, ly, lz, kn) if walls else []
+    spec = _spec((lx, ly, lz), diams, 2500.0, kn, e, mu, pbc=pbc, walls=wl, total_steps=steps,
+                 gravity=(0.0, 0.0, gravity))
+    red = reduce_spec(spec)
+    L, T = red.length_scale, red.time_scale
+    v = rng.uniform(-vel, vel, pos.shape) * T / L
+    w = rng.uniform(-vel, vel, pos.shape) * T / L / 0.5
+    return Workload(f"small-{n}", spec, red, comp, pos / L, v, w, steps)

@@ -1,0 +1,220 @@
+/*
+ * dem3d.h -- C-ABI of the MI355X (gfx950) DEM per-timestep path.
+ *
+ * The library replaces, behind plain C types, the module interface that the
+ * Fortran driver of DEM3D uses for one time step:
+ *
+ *   MForces::ComputeAccelerations   src/MForces.f90:145-335
+ *   MForces::InitializeForce/CellList/CellDistribution  src/MForces.f90:54-117
+ *   MCells  (binning, stencil)      src/MCells.f90:155-227,276-327,498-513,607-643
+ *   MContacts (history store)       src/MContacts.f90:329-549
+ *   MDEM::MoveParticles             src/MDEM.f90:285-369
+ *   MDEM::MoveParticlesEuler        src/MDEM.f90:227-281
+ *   MDEM::ApplyPeriodicBoundaryConditions src/MDEM.f90:373-388
+ *   MDEM::UpdateParticleList (escape scan only) src/MDEM.f90:612-671
+ *   MDEM::KineticEnergy             src/MDEM.f90:199-213
+ *
+ * Conventions
+ *  - All quantities are in the reference's reduced units (src/DEM.f90:807-866).
+ *  - Particle ids at the ABI are the reference's 1-based indices; component
+ *    indices are 1-based (TParticle%Index).
+ *  - NC x NC and NC x NW tables are column-major (Fortran order):
+ *    element (i,j), 0-based, lives at i + j*nc.
+ *  - SoA particle arrays hold component d of particle i at [d*np + i]
+ *    (the layout of a Fortran array X(np,3)).
+ *  - The ABM ring is Previous(NPrev,NP) of (Velocity, Acceleration, Alpha):
+ *    ring[(s*9 + c)*np + i], s = storage slot 0..nprev-1, c = 0..2 velocity,
+ *    3..5 acceleration, 6..8 alpha (src/MDEM.f90:14-24).  ipc[] holds the
+ *    reference's 1-based slot order (ipc(1) = most recent).
+ *  - Pair keys are (imin << 32) | imax with 1-based ids, ascending; wall keys
+ *    are (i << 32) | wall.  ft/vt are 3 doubles per key, exactly the vectors the
+ *    reference stores in TPartner%Vector1/Vector2 (src/MContacts.f90:21-26).
+ *  - The library owns all device memory; caller arrays are copied and never
+ *    retained.  Functions return DEM_OK (0) or a negative DEM_ERR_* code; the
+ *    message is available from dem_last_error().
+ *  - One handle = one HIP stream; a handle is not thread-safe (like the
+ *    reference's module globals).  Distinct handles may run on distinct host
+ *    threads.
+ */
+#ifndef DEM3D_H
+#define DEM3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DEM_OK = 0,
+  DEM_ERR_ARG = -1,        /* invalid argument or size                       */
+  DEM_ERR_DEVICE = -2,     /* HIP runtime failure                             */
+  DEM_ERR_CAPACITY = -3,   /* a fixed capacity was exceeded (see message)     */
+  DEM_ERR_STATE = -4,      /* call out of order (e.g. step before upload)     */
+  DEM_ERR_UNSUPPORTED = -5,/* feature outside the implemented path            */
+  DEM_ERR_ESCAPE = -6      /* particles left the box: host must remove them   */
+};
+
+enum { DEM_FP64 = 0, DEM_MIXED = 1 };               /* force arithmetic          */
+enum { DEM_SHAPE_RECTANGLE = 1, DEM_SHAPE_CONE = 2 }; /* src/MConfig.f90:43-45   */
+
+/* Orientation of stored tangential history.
+ * REFERENCE: the stored vectors are the tangential force/velocity of whichever
+ *   particle played "i" in the reference's cell loop when they were written,
+ *   and are re-read for the current "i" without regard to orientation
+ *   (src/MForces.f90:220,262 with src/MContacts.f90:329-423).  The library
+ *   reproduces the reference's visitation roles, so forces match it.
+ * CANONICAL: the stored vectors always belong to the lower-id particle and are
+ *   negated for the other one (physically consistent; differs from the
+ *   reference whenever the reference's roles flip). */
+enum { DEM_HIST_REFERENCE = 0, DEM_HIST_CANONICAL = 1 };
+
+typedef struct dem_system* dem_handle_t;
+
+/* Constants of one granular system.  Mirrors the module variables of MForces
+ * (src/MForces.f90:11-48), MConfig (Lbox, Component) and MDEM (TimeStep,
+ * MethodOrder, NumberOfEulerIntervals) after InitializeForce and
+ * ScaleVariables have run. */
+typedef struct {
+  int32_t nc;               /* number of components (1..DEM_MAX_COMPONENTS)   */
+  int32_t nw;               /* number of walls (0..DEM_MAX_WALLS)             */
+  int32_t pbc[3];           /* PBC(dim) (nonzero = periodic)                  */
+  int32_t method_order;     /* MethodOrder, 1..4                              */
+  int32_t n_euler;          /* NumberOfEulerIntervals (>= 1)                  */
+  int32_t max_ncell;        /* MaxNCell of the reference grid (30000)         */
+  int32_t ref_grid[3];      /* reference grid Nx,Ny,Nz; all 0 = derive as
+                               InitCellStructure3D does (src/MCells.f90:155)  */
+  int32_t precision;        /* DEM_FP64 or DEM_MIXED                          */
+  int32_t history_mode;     /* DEM_HIST_REFERENCE or DEM_HIST_CANONICAL       */
+  int32_t max_neighbours;   /* neighbour-list capacity per particle, 0 = 64   */
+  int32_t max_contacts;     /* contact-table capacity per particle, 0 = 24    */
+  double lbox[3];           /* Lbox                                            */
+  double gravity[3];        /* Gravity                                         */
+  double time_step;         /* TimeStep                                        */
+  double skin;              /* neighbour-list skin; <= 0 selects 0.1*min diam  */
+  const double* radius;     /* [nc]  Component%Radius                          */
+  const double* mass;       /* [nc]  Component%Mass (energies)                 */
+  const double* inertia;    /* [nc]  Component%Inertia (energies)              */
+  const double* inv_mass;   /* [nc]  InvMass                                   */
+  const double* inv_inertia;/* [nc]  InvInertia                                */
+  const double* eff_mass;   /* [nc]  EffectiveMass                             */
+  const double* min_dij;    /* [nc*nc] MinDij                                  */
+  const double* min_dij_sq; /* [nc*nc] MinDijSq                                */
+  const double* kn;         /* [nc*nc] Kn  */
+  const double* yn;         /* [nc*nc] Yn  */
+  const double* kt;         /* [nc*nc] Kt  */
+  const double* yt;         /* [nc*nc] Yt  */
+  const double* musq;       /* [nc*nc] MuSq */
+  const double* knw;        /* [nc*nw] Knw */
+  const double* ynw;        /* [nc*nw] Ynw */
+  const double* ktw;        /* [nc*nw] Ktw */
+  const double* ytw;        /* [nc*nw] Ytw */
+  const double* muwsq;      /* [nc*nw] MuwSq */
+} dem_params_t;
+
+#define DEM_MAX_COMPONENTS 64
+#define DEM_MAX_WALLS 8
+
+/* TWall (src/MDimension3D.f90:45-59), lengths already halved as
+ * ReadSpecifications does (src/DEM.f90:477-479). */
+typedef struct {
+  int32_t shape;            /* DEM_SHAPE_*                                     */
+  int32_t order;            /* ignored: recomputed as ConfigureWalls does      */
+  double l1, l2, l3;
+  double stiffness, shear_stiffness;   /* informational (tables carry Knw..)  */
+  double u[3], w[3], n[3];
+  double position[3], velocity[3], omega[3];
+} dem_wall_t;
+
+/* TParticle (src/MDimension3D.f90:28-35) byte layout: 128 B records. */
+typedef struct {
+  int32_t index;            /* component, 1-based */
+  int32_t pad_;
+  double position[3], velocity[3], acceleration[3], omega[3], alpha[3];
+} dem_particle_t;
+
+/* Counters and observables of the last evaluation / step. */
+typedef struct {
+  int64_t np;               /* particles                                       */
+  int64_t pair_contacts;    /* P: overlapping particle pairs (i<j)             */
+  int64_t wall_contacts;    /* W: overlapping particle-wall pairs              */
+  int64_t stored_pairs;     /* entries in the pair history (incl. stale ones)  */
+  int64_t ncol;             /* Ncol (src/MForces.f90:47,227)                   */
+  int64_t evaluations;      /* force evaluations done                          */
+  int64_t steps;            /* MoveParticles steps done                        */
+  int64_t rebuilds;         /* neighbour-list rebuilds                         */
+  int64_t neighbour_pairs;  /* directed entries in the neighbour list          */
+  int64_t stale_dropped;    /* stale history entries lost at a rebuild         */
+  int64_t ref_ncell[3];     /* reference grid                                  */
+  int64_t nb_ncell[3];      /* neighbour-search grid                           */
+  double kinetic_energy;    /* KineticEnergy() (src/MDEM.f90:199-213)          */
+  double max_overlap;       /* max (MinDij - Dij)/MinDij over pair contacts    */
+  double ms_pass1;          /* mean device time of the last dem_step, per call */
+  double ms_pass2;
+  double ms_rebuild;        /* mean over the rebuilds that ran                 */
+  double ms_step;           /* mean device time per step                       */
+} dem_stats_t;
+
+/* ---- lifetime ------------------------------------------------------------ */
+int dem_create(const dem_params_t* params, dem_handle_t* out);
+void dem_destroy(dem_handle_t h);
+/* Message of the last error on h (h may be NULL for dem_create failures). */
+const char* dem_last_error(dem_handle_t h);
+
+/* ---- state in / out ------------------------------------------------------ */
+/* Walls; replaces Wall(:) (src/MConfig.f90:70) and runs ConfigureWalls. */
+int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* walls);
+
+/* Particle state.  a/alpha/ring/ipc/cell_rank may be NULL (zeros; fresh ring;
+ * fresh cell lists as DistributeParticles3D builds them).  cell_rank gives the
+ * order of the reference's cell lists: within one cell a particle with a larger
+ * rank precedes one with a smaller rank (ties: larger id first). */
+int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp,
+                     const double* x, const double* v, const double* w,
+                     const double* a, const double* alpha,
+                     const double* ring, const int32_t* ipc,
+                     const int32_t* cell_rank);
+int dem_download_state(dem_handle_t h, int32_t np, int32_t* comp,
+                       double* x, double* v, double* w,
+                       double* a, double* alpha, double* ring, int32_t* ipc);
+/* Predicted(:) of the last MoveParticles (src/MDEM.f90:20). */
+int dem_download_predicted(dem_handle_t h, int32_t np, double* x, double* v,
+                           double* w, double* a, double* alpha);
+
+/* Contact history (MContacts lists).  Keys ascending, 1-based. */
+int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys,
+                        const double* ft, const double* vt, int64_t nwall,
+                        const uint64_t* wkeys, const double* wft,
+                        const double* wvt);
+/* Any output pointer may be NULL to query counts only. */
+int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs,
+                          uint64_t* keys, double* ft, double* vt, int64_t wcap,
+                          int64_t* nwall, uint64_t* wkeys, double* wft,
+                          double* wvt);
+/* CellIndex(NP) (1-based) of the reference grid after the last evaluation. */
+int dem_download_cells(dem_handle_t h, int32_t np, int32_t* cell_index);
+
+/* ---- the hot path -------------------------------------------------------- */
+/* Drop-in for ComputeAccelerations(Particle, NP, TimeStep, Update)
+ * (src/MForces.f90:145): reads Index/Position/Velocity/Omega of p[0..np),
+ * writes Acceleration and Alpha.  Contact history and cell order stay resident. */
+int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p,
+                                  double time_step, int32_t update);
+/* Same on the resident state: which = 0 Particle, 1 Predicted. */
+int dem_compute_accelerations(dem_handle_t h, int32_t which, int32_t update);
+/* One MoveParticlesEuler (src/MDEM.f90:258-281) on the resident state. */
+int dem_euler_startup(dem_handle_t h);
+/* nsteps calls of MoveParticles(step) (src/MDEM.f90:285-369) followed each by
+ * the escape scan of UpdateParticleList.  Stops after the first step in which
+ * a particle left the box (returns DEM_ERR_ESCAPE, *escaped > 0).
+ * steps_done / escaped may be NULL. */
+int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step,
+             int64_t* steps_done, int32_t* escaped);
+/* Waits for the handle's stream. */
+int dem_synchronize(dem_handle_t h);
+int dem_stats(dem_handle_t h, dem_stats_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEM3D_H */

@@ -1,0 +1,62 @@
+#!/usr/bin/env bash
+# Builds the checkers of the DEM hot path.  TEST INFRASTRUCTURE ONLY.
+#
+#  oracle/liboracle.so            C restatement (dem3d_oracle.c), gcc, no FMA
+#                                 contraction.  Always built.
+#  oracle/_ref/libdem3d_ref.so    the reference Fortran modules, compiled from
+#                                 their own files under $REF/src with amdflang
+#                                 -O2 -ffp-contract=off, plus ref_harness.f90.
+#  oracle/_ref/libdem3d_ref_big.so  same, with the two capacity constants of
+#                                 MForces raised (MaxNeighbours 50 -> 65536,
+#                                 MaxNCell 30000 -> 2147483647) so that systems
+#                                 beyond ~20 k particles do not overflow the
+#                                 unchecked Neighbour(50) buffer.  The sources
+#                                 are streamed through sed into the compiler;
+#                                 no copy of them is written anywhere.
+#
+# The reference part is skipped when $REF/src is absent (e.g. on a machine that
+# only runs the prebuilt libraries).
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+REF="${DEM3D_REFERENCE:-/root/reference}"
+CC="${CC:-gcc}"
+
+"$CC" -O2 -ffp-contract=off -fno-fast-math -fPIC -shared -std=c99 \
+  -Wall -Wextra -Wno-unused-parameter \
+  -o "$HERE/liboracle.so" "$HERE/dem3d_oracle.c" -lm
+
+if [ ! -d "$REF/src" ]; then
+  echo "build_ref.sh: $REF/src not found; reference checker not rebuilt" >&2
+  exit 0
+fi
+FC="${FC:-amdflang}"
+FFLAGS="-O2 -ffp-contract=off -fPIC -cpp"
+MODS="MTypes MTime BinFiles MDimension3D MUtil MCells MConfig MContacts mFluid MForces MDEM"
+
+build_variant() {
+  local name="$1" patch="$2"
+  local obj="$HERE/_ref/obj_$name"
+  rm -rf "$obj"
+  mkdir -p "$obj"
+  local objs=()
+  for m in $MODS; do
+    if [ "$m" = "MForces" ] && [ -n "$patch" ]; then
+      sed -e 's/MaxNeighbours = 50/MaxNeighbours = 65536/' \
+          -e 's/MaxNCell = 30000_ib/MaxNCell = 2147483647_ib/' "$REF/src/$m.f90" |
+        (cd "$obj" && "$FC" $FFLAGS -x f95-cpp-input -c - -o "$obj/$m.o")
+      grep -q 'MaxNeighbours = 50' "$REF/src/$m.f90"
+      grep -q 'MaxNCell = 30000_ib' "$REF/src/$m.f90"
+    else
+      (cd "$obj" && "$FC" $FFLAGS -c "$REF/src/$m.f90" -o "$obj/$m.o")
+    fi
+    objs+=("$obj/$m.o")
+  done
+  (cd "$obj" && "$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o "$obj/ref_harness.o")
+  "$FC" -shared -fPIC -o "$HERE/_ref/libdem3d_ref$name.so" "${objs[@]}" "$obj/ref_harness.o"
+  rm -rf "$obj"
+}
+
+mkdir -p "$HERE/_ref"
+build_variant "" ""
+build_variant "_big" "yes"
+echo "build_ref.sh: built $HERE/_ref/libdem3d_ref.so and libdem3d_ref_big.so"

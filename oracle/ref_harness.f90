@@ -1,0 +1,276 @@
+!
+! ref_harness.f90 -- C-callable harness around the reference DEM3D modules.
+!
+! TEST INFRASTRUCTURE ONLY.  oracle/build_ref.sh compiles this file together
+! with the reference's own modules (taken unmodified from src/ of the reference
+! checkout, except for the two capacity constants of the "big" variant) into
+! oracle/_ref/libdem3d_ref*.so.  Tests and the golden-fixture script load it to
+! pin the C restatement (dem3d_oracle.c) to the reference, and bench.py times
+! it as the CPU baseline.  Nothing in the product links it.
+!
+! The harness sets the reference's module variables directly (the state that
+! src/DEM.f90:67-83 would build from a specification file), then calls the
+! reference routines:
+!   InitializeAuxiliaries          src/MDEM.f90:110-162
+!   ConfigureWalls                 src/MConfig.f90:636-688
+!   InitializeCellDistribution     src/MForces.f90:110-117
+!   InitializeIntegrationVariables src/MDEM.f90:93-106
+!   InitializeContacts             src/MContacts.f90:43-49
+!   ComputeAccelerations           src/MForces.f90:145-335
+!   MoveParticles / MoveParticlesEuler src/MDEM.f90:258-369
+!   KineticEnergy                  src/MDEM.f90:199-213
+! and moves private state (cell lists, ABM ring, contact lists) through the
+! reference's own binary save/read routines (src/MForces.f90:121-141,
+! src/MDEM.f90:52-89, src/MContacts.f90:75-169).
+!
+module ref_harness
+
+use iso_c_binding
+use MTypes
+use MDimension
+use MConfig
+use MCells
+use MContacts
+use MFluid
+use MForces
+use MDEM
+use BinFiles
+
+implicit none
+
+contains
+
+  subroutine c2f_string( cstr, n, fstr )
+    character(kind=c_char), intent(in) :: cstr(*)
+    integer(c_int32_t), intent(in) :: n
+    character(len=*), intent(out) :: fstr
+    integer :: k
+    fstr = ' '
+    do k = 1, n
+      fstr(k:k) = cstr(k)
+    end do
+  end subroutine c2f_string
+
+  ! Constants of the system in reduced units (what ReadSpecifications,
+  ! CalculateProperties and ScaleVariables leave in the modules).
+  subroutine ref_init( nc_, nw_, pbc_, lbox_, gravity_, dt_, order_, neuler_, &
+                       radius, mass, inertia, density,                      &
+                       kn_, yn_, kt_, yt_, mu_,                              &
+                       knw_, ynw_, ktw_, ytw_, muw_ ) bind(C, name="ref_init")
+    integer(c_int32_t), value :: nc_, nw_, order_, neuler_
+    integer(c_int32_t), intent(in) :: pbc_(3)
+    real(c_double), intent(in) :: lbox_(3), gravity_(3)
+    real(c_double), value :: dt_
+    real(c_double), intent(in) :: radius(nc_), mass(nc_), inertia(nc_), density(nc_)
+    real(c_double), intent(in) :: kn_(nc_,nc_), yn_(nc_,nc_), kt_(nc_,nc_), &
+                                  yt_(nc_,nc_), mu_(nc_,nc_)
+    real(c_double), intent(in) :: knw_(nc_,max(nw_,1)), ynw_(nc_,max(nw_,1)), &
+                                  ktw_(nc_,max(nw_,1)), ytw_(nc_,max(nw_,1)), &
+                                  muw_(nc_,max(nw_,1))
+    integer :: i
+
+    NC = nc_
+    NW = nw_
+    allocate( Component(NC), Wall(NW) )
+    do i = 1, NC
+      Component(i)%Number = 0
+      Component(i)%Radius = radius(i)
+      Component(i)%Density = density(i)
+      Component(i)%Mass = mass(i)
+      Component(i)%Inertia = inertia(i)
+      Component(i)%Stiffness = 0.0_rb
+      Component(i)%ShearStiffness = 0.0_rb
+    end do
+    Lbox = lbox_
+    InvLbox = 1.0_rb/Lbox
+    PBC = pbc_ /= 0
+    Gravity = gravity_
+    FluidDensity = 0.0_rb
+    FluidFlow = .false.
+    allocate( Kn(NC,NC), Yn(NC,NC), Kt(NC,NC), Yt(NC,NC), Mu(NC,NC) )
+    Kn = kn_ ; Yn = yn_ ; Kt = kt_ ; Yt = yt_ ; Mu = mu_
+    allocate( Knw(NC,NW), Ynw(NC,NW), Ktw(NC,NW), Ytw(NC,NW), Muw(NC,NW) )
+    if (NW > 0) then
+      Knw = knw_(:,1:NW) ; Ynw = ynw_(:,1:NW) ; Ktw = ktw_(:,1:NW)
+      Ytw = ytw_(:,1:NW) ; Muw = muw_(:,1:NW)
+    end if
+    TimeStep = dt_
+    MethodOrder = order_
+    NumberOfEulerIntervals = neuler_
+    WallsInMovement = .false.
+    call InitializeAuxiliaries
+  end subroutine ref_init
+
+  ! One wall (TWall, lengths already halved); ConfigureWalls after the last.
+  subroutine ref_set_wall( j, shape, l1, l2, l3, u, w, n, pos, vel, omg ) &
+      bind(C, name="ref_set_wall")
+    integer(c_int32_t), value :: j, shape
+    real(c_double), value :: l1, l2, l3
+    real(c_double), intent(in) :: u(3), w(3), n(3), pos(3), vel(3), omg(3)
+    Wall(j)%Shape = shape
+    Wall(j)%L1 = l1 ; Wall(j)%L2 = l2 ; Wall(j)%L3 = l3
+    Wall(j)%Stiffness = 1.0_rb ; Wall(j)%ShearStiffness = 1.0_rb
+    Wall(j)%U = u ; Wall(j)%W = w ; Wall(j)%N = n
+    Wall(j)%Position = pos ; Wall(j)%Velocity = vel ; Wall(j)%Omega = omg
+  end subroutine ref_set_wall
+
+  subroutine ref_configure_walls() bind(C, name="ref_configure_walls")
+    call ConfigureWalls
+  end subroutine ref_configure_walls
+
+  ! Tables derived by InitializeForce (src/MForces.f90:54-87).
+  subroutine ref_get_tables( mindij_, mindijsq_, invmass_, invinertia_, effmass_, &
+                             musq_, muwsq_ ) bind(C, name="ref_get_tables")
+    real(c_double), intent(out) :: mindij_(NC,NC), mindijsq_(NC,NC), invmass_(NC), &
+                                   invinertia_(NC), effmass_(NC), musq_(NC,NC),    &
+                                   muwsq_(NC,max(NW,1))
+    mindij_ = MinDij ; mindijsq_ = MinDijSq
+    invmass_ = InvMass ; invinertia_ = InvInertia ; effmass_ = EffectiveMass
+    musq_ = MuSq
+    if (NW > 0) muwsq_(:,1:NW) = MuwSq
+  end subroutine ref_get_tables
+
+  subroutine ref_get_grid( ncell ) bind(C, name="ref_get_grid")
+    integer(c_int32_t), intent(out) :: ncell(3)
+    ncell = (/ MCellx, MCelly, Mcellz /)
+  end subroutine ref_get_grid
+
+  ! Particle(1:np) from SoA arrays X(np,3).
+  subroutine ref_set_particles( np_, comp, x, v, w, a, alpha ) &
+      bind(C, name="ref_set_particles")
+    integer(c_int32_t), value :: np_
+    integer(c_int32_t), intent(in) :: comp(np_)
+    real(c_double), intent(in) :: x(np_,3), v(np_,3), w(np_,3), a(np_,3), alpha(np_,3)
+    integer :: i
+    if (.not.allocated(Particle)) then
+      NP = np_
+      allocate( Particle(NP) )
+    end if
+    do i = 1, NP
+      Particle(i)%Index = comp(i)
+      Particle(i)%Position = x(i,:)
+      Particle(i)%Velocity = v(i,:)
+      Particle(i)%Omega = w(i,:)
+      Particle(i)%Acceleration = a(i,:)
+      Particle(i)%Alpha = alpha(i,:)
+    end do
+  end subroutine ref_set_particles
+
+  subroutine ref_get_particles( np_, comp, x, v, w, a, alpha ) &
+      bind(C, name="ref_get_particles")
+    integer(c_int32_t), value :: np_
+    integer(c_int32_t), intent(out) :: comp(np_)
+    real(c_double), intent(out) :: x(np_,3), v(np_,3), w(np_,3), a(np_,3), alpha(np_,3)
+    integer :: i
+    do i = 1, np_
+      comp(i) = Particle(i)%Index
+      x(i,:) = Particle(i)%Position
+      v(i,:) = Particle(i)%Velocity
+      w(i,:) = Particle(i)%Omega
+      a(i,:) = Particle(i)%Acceleration
+      alpha(i,:) = Particle(i)%Alpha
+    end do
+  end subroutine ref_get_particles
+
+  ! New simulation (InitializeState, src/DEM.f90:910-922).
+  subroutine ref_fresh_state() bind(C, name="ref_fresh_state")
+    MaxNP = NP
+    call InitializeCellDistribution
+    call InitializeIntegrationVariables
+    call InitializeContacts( NP )
+  end subroutine ref_fresh_state
+
+  ! Resume (ReadState, src/DEM.f90:1087-1093): cell lists, ABM ring and
+  ! contact lists from one file in the reference's binary layout.
+  subroutine ref_load_state( path, n ) bind(C, name="ref_load_state")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n
+    character(len=1024) :: f
+    integer :: u
+    call c2f_string( path, n, f )
+    MaxNP = NP
+    u = BinOpen( trim(f), 'read' )
+    call ReadCellDistribution( u )
+    call ReadIntegrationVariables( u )
+    call ReadContactList( u )
+    call BinClose( u )
+  end subroutine ref_load_state
+
+  ! SaveState's private parts (src/DEM.f90:987-993), same order as load.
+  subroutine ref_save_state( path, n ) bind(C, name="ref_save_state")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n
+    character(len=1024) :: f
+    integer :: u
+    call c2f_string( path, n, f )
+    u = BinOpen( trim(f), 'write' )
+    call SaveCellDistribution( u )
+    call SaveIntegrationVariables( u )
+    call SaveContactList( u )
+    call BinClose( u )
+  end subroutine ref_save_state
+
+  subroutine ref_get_cell_index( np_, ci ) bind(C, name="ref_get_cell_index")
+    integer(c_int32_t), value :: np_
+    integer(c_int32_t), intent(out) :: ci(np_)
+    ci = CellIndex(1:np_)
+  end subroutine ref_get_cell_index
+
+  ! ComputeAccelerations on a caller-provided particle set (the drop-in
+  ! signature of src/MForces.f90:145); accelerations are returned in a/alpha.
+  subroutine ref_compute_accelerations_soa( np_, comp, x, v, w, a, alpha, dt_, upd ) &
+      bind(C, name="ref_compute_accelerations_soa")
+    integer(c_int32_t), value :: np_, upd
+    integer(c_int32_t), intent(in) :: comp(np_)
+    real(c_double), intent(in) :: x(np_,3), v(np_,3), w(np_,3)
+    real(c_double), intent(out) :: a(np_,3), alpha(np_,3)
+    real(c_double), value :: dt_
+    type(TParticle), allocatable :: P(:)
+    integer :: i
+    allocate( P(np_) )
+    do i = 1, np_
+      P(i)%Index = comp(i)
+      P(i)%Position = x(i,:)
+      P(i)%Velocity = v(i,:)
+      P(i)%Omega = w(i,:)
+      P(i)%Acceleration = 0.0_rb
+      P(i)%Alpha = 0.0_rb
+    end do
+    call ComputeAccelerations( P, np_, dt_, upd /= 0 )
+    do i = 1, np_
+      a(i,:) = P(i)%Acceleration
+      alpha(i,:) = P(i)%Alpha
+    end do
+    deallocate( P )
+  end subroutine ref_compute_accelerations_soa
+
+  subroutine ref_compute_accelerations( upd ) bind(C, name="ref_compute_accelerations")
+    integer(c_int32_t), value :: upd
+    call ComputeAccelerations( Particle, NP, TimeStep, upd /= 0 )
+  end subroutine ref_compute_accelerations
+
+  subroutine ref_move_particles_euler() bind(C, name="ref_move_particles_euler")
+    call MoveParticlesEuler
+  end subroutine ref_move_particles_euler
+
+  ! Steps first..first+n-1 of the time loop (src/DEM.f90:94-99) without I/O
+  ! and without UpdateParticleList.
+  subroutine ref_move_particles( first, n ) bind(C, name="ref_move_particles")
+    integer(c_int32_t), value :: first, n
+    integer(ib) :: s
+    do s = first, first + n - 1
+      call MoveParticles( s )
+    end do
+  end subroutine ref_move_particles
+
+  function ref_ncol() result( n ) bind(C, name="ref_ncol")
+    integer(c_int32_t) :: n
+    n = Ncol
+  end function ref_ncol
+
+  function ref_kinetic_energy() result( e ) bind(C, name="ref_kinetic_energy")
+    real(c_double) :: e
+    e = KineticEnergy()
+  end function ref_kinetic_energy
+
+end module ref_harness

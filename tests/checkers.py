@@ -1,0 +1,274 @@
+"""Test-side access to the checkers: the C restatement (oracle/liboracle.so)
+and the reference Fortran modules (oracle/_ref/libdem3d_ref*.so).
+
+Only tests, smoke() and bench.py's cpu_baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import multiprocessing as mp
+import os
+import struct
+import tempfile
+from typing import Optional
+
+import numpy as np
+
+from dem3d_amd.native import Library, System
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref.so")
+REF_LIB_BIG = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_big.so")
+
+_oracle = None
+
+
+def oracle_lib() -> Library:
+    global _oracle
+    if _oracle is None:
+        _oracle = Library(ORACLE_LIB, "orc_")
+    return _oracle
+
+
+def oracle_system(red, **kw) -> System:
+    return System(oracle_lib(), red, **kw)
+
+
+def ref_available(big=False) -> bool:
+    return os.path.exists(REF_LIB_BIG if big else REF_LIB)
+
+
+# ---------------------------------------------------------------------------
+# The reference's binary state layout (BinFiles byte stream, src/BinFiles.f90):
+# SaveCellStructure (src/MCells.f90:958-983), SaveIntegrationVariables
+# (src/MDEM.f90:52-68), SaveContactList (src/MContacts.f90:75-118).
+
+def parse_state_file(path, np_, nprev):
+    data = open(path, "rb").read()
+    off = 0
+
+    def i32():
+        nonlocal off
+        v = struct.unpack_from("<i", data, off)[0]
+        off += 4
+        return v
+
+    def f64(n):
+        nonlocal off
+        v = np.frombuffer(data, "<f8", n, off).copy()
+        off += 8 * n
+        return v
+
+    nt = i32()
+    cells = []
+    for _ in range(nt):
+        k = i32()
+        cells.append([i32() for _ in range(k)])
+    ipc = [i32() for _ in range(nprev)]
+    ring = np.zeros((nprev, 9, np_))
+    for j in range(nprev):
+        for i in range(np_):
+            ring[j, :, i] = f64(9)
+    npl = i32()
+    pairs, walls = [], []
+    for i in range(npl):
+        for m in range(2):
+            k = i32()
+            for _ in range(k):
+                idx = i32()
+                v1 = f64(3)
+                v2 = f64(3)
+                (pairs if m == 0 else walls).append((i + 1, idx, v1, v2))
+    return dict(cells=cells, ipc=np.array(ipc, np.int32), ring=ring, pairs=pairs, walls=walls)
+
+
+def write_state_file(path, cells, ipc, ring, pairs, walls, np_):
+    """Inverse of parse_state_file; pairs/walls are (i, j, v1, v2) 1-based."""
+    out = bytearray()
+    out += struct.pack("<i", len(cells))
+    for c in cells:
+        out += struct.pack("<i", len(c))
+        for q in c:
+            out += struct.pack("<i", q)
+    for q in ipc:
+        out += struct.pack("<i", int(q))
+    for j in range(len(ipc)):
+        for i in range(np_):
+            out += np.asarray(ring[j, :, i], "<f8").tobytes()
+    out += struct.pack("<i", np_)
+    by_i = {}
+    for (i, j, v1, v2) in pairs:
+        by_i.setdefault((i, 0), []).append((j, v1, v2))
+    for (i, j, v1, v2) in walls:
+        by_i.setdefault((i, 1), []).append((j, v1, v2))
+    for i in range(1, np_ + 1):
+        for m in range(2):
+            lst = sorted(by_i.get((i, m), []), key=lambda t: t[0])
+            out += struct.pack("<i", len(lst))
+            for (j, v1, v2) in lst:
+                out += struct.pack("<i", int(j))
+                out += np.asarray(v1, "<f8").tobytes() + np.asarray(v2, "<f8").tobytes()
+    pad = (-len(out)) % 1024
+    out += b"\0" * pad
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+def contacts_to_keys(pairs):
+    keys = np.array([(i << 32) | j for (i, j, _, _) in pairs], np.uint64)
+    ft = np.array([v1 for (_, _, v1, _) in pairs]).reshape(-1, 3)
+    vt = np.array([v2 for (_, _, _, v2) in pairs]).reshape(-1, 3)
+    order = np.argsort(keys, kind="stable")
+    return keys[order], ft[order], vt[order]
+
+
+def cells_to_rank(cells, np_):
+    rank = np.zeros(np_, np.int32)
+    for c in cells:
+        n = len(c)
+        for k, q in enumerate(c):
+            rank[q - 1] = n - 1 - k
+    return rank
+
+
+# ---------------------------------------------------------------------------
+# Running the reference modules.  They keep global module state, so every job
+# runs in a fresh process.
+
+def _ref_job(job, q):
+    try:
+        q.put(("ok", _ref_job_body(job)))
+    except BaseException as e:  # pragma: no cover - reported to the parent
+        import traceback
+        q.put(("err", traceback.format_exc()))
+
+
+def _d(a):
+    return np.asfortranarray(np.asarray(a, np.float64))
+
+
+def _ref_job_body(job):
+    red = job["red"]
+    lib = C.CDLL(REF_LIB_BIG if job.get("big") else REF_LIB)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    P = lambda a: a.ctypes.data_as(dp)
+    nc, nw = red.nc, red.nw
+    keep = []
+
+    def tab(a, shape):
+        a = _d(np.asarray(a, np.float64).reshape(shape))
+        keep.append(a)
+        return P(a)
+
+    pbc = np.array(red.pbc, np.int32)
+    lbox = _d(red.lbox)
+    grav = _d(red.gravity)
+    wshape = (nc, max(nw, 1))
+    zw = np.zeros(wshape)
+    lib.ref_init(C.c_int32(nc), C.c_int32(nw), pbc.ctypes.data_as(ip), P(lbox), P(grav),
+                 C.c_double(red.time_step), C.c_int32(red.method_order), C.c_int32(red.n_euler),
+                 tab(red.radius, nc), tab(red.mass, nc), tab(red.inertia, nc), tab(red.density, nc),
+                 tab(red.kn, (nc, nc)), tab(red.yn, (nc, nc)), tab(red.kt, (nc, nc)),
+                 tab(red.yt, (nc, nc)), tab(red.mu, (nc, nc)),
+                 tab(red.knw if nw else zw, wshape), tab(red.ynw if nw else zw, wshape),
+                 tab(red.ktw if nw else zw, wshape), tab(red.ytw if nw else zw, wshape),
+                 tab(red.muw if nw else zw, wshape))
+    for j, w in enumerate(red.walls):
+        vec = lambda k: tab(w[k], 3)
+        lib.ref_set_wall(C.c_int32(j + 1), C.c_int32(int(w["shape"])), C.c_double(w["l1"]),
+                         C.c_double(w["l2"]), C.c_double(w["l3"]), vec("u"), vec("w"), vec("n"),
+                         vec("position"), vec("velocity"), vec("omega"))
+    lib.ref_configure_walls()
+    out = {}
+    md, mdsq, musq = (np.zeros((nc, nc), order="F") for _ in range(3))
+    invm, invi, effm = (np.zeros(nc) for _ in range(3))
+    muwsq = np.zeros(wshape, order="F")
+    lib.ref_get_tables(P(md), P(mdsq), P(invm), P(invi), P(effm), P(musq), P(muwsq))
+    out["tables"] = dict(min_dij=md, min_dij_sq=mdsq, inv_mass=invm, inv_inertia=invi,
+                         eff_mass=effm, musq=musq, muwsq=muwsq[:, :nw])
+    grid = np.zeros(3, np.int32)
+    lib.ref_get_grid(grid.ctypes.data_as(ip))
+    out["grid"] = grid
+    n = int(job["comp"].shape[0])
+    comp = np.ascontiguousarray(job["comp"], np.int32)
+    cols = lambda a: _d(np.asarray(a).reshape(3, n).T)
+    x, v, w = cols(job["x"]), cols(job["v"]), cols(job["w"])
+    a = cols(job["a"]) if job.get("a") is not None else np.zeros((n, 3), order="F")
+    al = cols(job["alpha"]) if job.get("alpha") is not None else np.zeros((n, 3), order="F")
+    lib.ref_set_particles(C.c_int32(n), comp.ctypes.data_as(ip), P(x), P(v), P(w), P(a), P(al))
+    tmp = tempfile.mkdtemp(prefix="dem3d_ref_")
+    nprev = red.method_order - 1
+    if job.get("state") is not None:
+        st = job["state"]
+        path = os.path.join(tmp, "in.stt").encode()
+        write_state_file(path, st["cells"], st["ipc"], st["ring"], st["pairs"], st["walls"], n)
+        lib.ref_load_state(C.c_char_p(path), C.c_int32(len(path)))
+    else:
+        lib.ref_fresh_state()
+    lib.ref_kinetic_energy.restype = C.c_double
+    lib.ref_ncol.restype = C.c_int32
+
+    def snapshot(tag):
+        c2 = np.zeros(n, np.int32)
+        xs, vs, ws, as_, als = (np.zeros((n, 3), order="F") for _ in range(5))
+        lib.ref_get_particles(C.c_int32(n), c2.ctypes.data_as(ip), P(xs), P(vs), P(ws), P(as_),
+                              P(als))
+        ci = np.zeros(n, np.int32)
+        lib.ref_get_cell_index(C.c_int32(n), ci.ctypes.data_as(ip))
+        path = os.path.join(tmp, f"{tag}.stt").encode()
+        lib.ref_save_state(C.c_char_p(path), C.c_int32(len(path)))
+        st = parse_state_file(path, n, nprev)
+        os.remove(path)
+        return dict(x=xs.T.copy(), v=vs.T.copy(), w=ws.T.copy(), a=as_.T.copy(),
+                    alpha=als.T.copy(), cell_index=ci, state=st, ncol=int(lib.ref_ncol()),
+                    ke=float(lib.ref_kinetic_energy()))
+
+    out["snapshots"] = []
+    for act in job["actions"]:
+        kind = act[0]
+        if kind == "compute":
+            lib.ref_compute_accelerations(C.c_int32(int(act[1])))
+        elif kind == "compute_soa":
+            _, upd, px, pv, pw, dt = act
+            xs, vs, ws = cols(px), cols(pv), cols(pw)
+            as_, als = np.zeros((n, 3), order="F"), np.zeros((n, 3), order="F")
+            lib.ref_compute_accelerations_soa(C.c_int32(n), comp.ctypes.data_as(ip), P(xs),
+                                              P(vs), P(ws), P(as_), P(als), C.c_double(dt),
+                                              C.c_int32(int(upd)))
+            out.setdefault("soa", []).append(dict(a=as_.T.copy(), alpha=als.T.copy()))
+        elif kind == "euler":
+            lib.ref_move_particles_euler()
+        elif kind == "step":
+            lib.ref_move_particles(C.c_int32(int(act[1])), C.c_int32(int(act[2])))
+        elif kind == "snapshot":
+            out["snapshots"].append(snapshot(act[1]))
+        elif kind == "time":
+            import time as _t
+            t0 = _t.perf_counter()
+            lib.ref_move_particles(C.c_int32(int(act[1])), C.c_int32(int(act[2])))
+            out["seconds"] = _t.perf_counter() - t0
+        else:
+            raise ValueError(kind)
+    os.rmdir(tmp)
+    return out
+
+
+def run_reference(job: dict, timeout: Optional[float] = 600, method: str = "fork"):
+    """Run one job of the reference modules in a fresh process (fork by
+    default: callers that have initialised the GPU pass method="spawn" from
+    a __main__-guarded script)."""
+    ctx = mp.get_context(method)
+    q = ctx.Queue()
+    p = ctx.Process(target=_ref_job, args=(job, q))
+    p.start()
+    try:
+        status, payload = q.get(timeout=timeout)
+    finally:
+        p.join(timeout=30)
+        if p.is_alive():
+            p.kill()
+    if status != "ok":
+        raise RuntimeError(payload)
+    return payload
