@@ -1,0 +1,1252 @@
+// dem3d.hip -- host runtime of the gfx950 DEM per-timestep path behind the
+// C-ABI of include/dem3d.h.
+//
+// One handle owns one HIP stream and all device memory of one granular
+// system.  The state stays resident; host round trips happen only in the
+// upload/download calls and once at the end of dem_step (to read the control
+// block: errors, escapes).
+#include "../../include/dem3d.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local char g_err[512];
+
+struct Timer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  void ensure(size_t n) {
+    while (ev.size() < n) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      ev.push_back(e);
+    }
+  }
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+struct dem_system {
+  char err[512];
+  hipStream_t stream = nullptr;
+  SysConst sc{};
+  int order = 4, nprev = 3, n_euler = 1, precision = 0, hist_mode = 0;
+  double dt = 0.0, skin = 0.0;
+  double ab0 = 0, am0 = 0, ab[3] = {0, 0, 0}, am[3] = {0, 0, 0};
+  std::vector<WallGeom> walls_h;
+  std::vector<WallConst> wallc_h;
+  std::vector<double> radius_h;
+  // device constant tables
+  PairConst* d_pair = nullptr;
+  CompConst* d_comp = nullptr;
+  WallConst* d_wallc = nullptr;
+  WallGeom* d_walls = nullptr;
+  // state
+  int np = 0, cap = 0;
+  bool has_state = false;
+  double4* pos[2] = {nullptr, nullptr};
+  double* vw[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* acc = nullptr;   // Particle%Acceleration/Alpha when acc_in_ring == false
+  double* acc_p = nullptr; // Predicted%Acceleration/Alpha (optional)
+  double* ring = nullptr;
+  double* ring2 = nullptr;
+  int ipc[3] = {1, 2, 3};
+  bool acc_in_ring = false;
+  int* id = nullptr;
+  int* id2 = nullptr;
+  int* rc[2] = {nullptr, nullptr};
+  int* st[2] = {nullptr, nullptr};
+  double4* xref = nullptr;
+  int* nbr = nullptr;
+  int* nnb = nullptr;
+  int* ccnt[2] = {nullptr, nullptr};
+  int* cpart[2] = {nullptr, nullptr};
+  double* chist[2] = {nullptr, nullptr};
+  int tcur = 0;
+  unsigned* wflag[2] = {nullptr, nullptr};
+  double* whist[2] = {nullptr, nullptr};
+  // binning scratch
+  int* key = nullptr;
+  int* cell_count = nullptr;
+  int* cell_start = nullptr;
+  int* cursor = nullptr;
+  int* cell_idx = nullptr;
+  int* tile_sum = nullptr;
+  int* old_to_new = nullptr;
+  int gncell = 0, ntiles = 0;
+  // staging
+  dem_particle_t* d_aos = nullptr;
+  double* d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  double* d_epart = nullptr;
+  // control
+  Ctl* ctl = nullptr;
+  int* rb = nullptr;  // rb[2]
+  long long eval = 0;
+  long long stamp_base = 0;
+  long long last_update_eval = -1;
+  bool rc_ready = false;
+  long long steps = 0;
+  int resort_interval = 500;
+  long long steps_since_resort = 0;
+  bool keep_pred_acc = false;
+  bool timing = false;
+  Timer timer;
+  double ms_pass1 = 0, ms_pass2 = 0, ms_rebuild = 0, ms_step = 0;
+};
+
+static int fail(dem_handle_t h, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(h ? h->err : g_err, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((h), DEM_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+template <typename T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+static inline int nblocks(long long n, int bs = DEM_BLOCK) { return (int)((n + bs - 1) / bs); }
+static inline int gs_blocks(long long n) {
+  long long b = (n + DEM_BLOCK - 1) / DEM_BLOCK;
+  return (int)std::max(1LL, std::min(b, 4096LL));
+}
+
+static void free_state(dem_handle_t h) {
+  for (int k = 0; k < 2; ++k) {
+    dfree(h->pos[k]); dfree(h->vw[k]); dfree(h->rc[k]); dfree(h->st[k]);
+    dfree(h->ccnt[k]); dfree(h->cpart[k]); dfree(h->chist[k]);
+    dfree(h->wflag[k]); dfree(h->whist[k]);
+  }
+  dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->id); dfree(h->id2);
+  dfree(h->xref); dfree(h->nbr); dfree(h->nnb); dfree(h->key); dfree(h->cell_count);
+  dfree(h->cell_start); dfree(h->cursor); dfree(h->cell_idx); dfree(h->tile_sum);
+  dfree(h->old_to_new); dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart);
+  h->np = h->cap = 0;
+  h->has_state = false;
+  h->tmp_bytes = 0;
+}
+
+extern "C" {
+
+const char* dem_last_error(dem_handle_t h) { return h ? h->err : g_err; }
+
+int dem_create(const dem_params_t* p, dem_handle_t* out) {
+  if (!p || !out) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->nc < 1 || p->nc > DEM_MAX_COMPONENTS || p->nw < 0 || p->nw > DEM_MAX_WALLS)
+    return fail(nullptr, DEM_ERR_ARG, "nc must be 1..%d and nw 0..%d", DEM_MAX_COMPONENTS,
+                DEM_MAX_WALLS);
+  if (p->method_order < 1 || p->method_order > 4 || p->n_euler < 1)
+    return fail(nullptr, DEM_ERR_ARG, "method_order must be 1..4 and n_euler >= 1");
+  if (p->precision != DEM_FP64)
+    return fail(nullptr, DEM_ERR_UNSUPPORTED, "precision mode %d is not built", p->precision);
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+    return fail(nullptr, DEM_ERR_DEVICE, "no HIP device available");
+  dem_handle_t h = new dem_system();
+  h->err[0] = 0;
+  const int nc = p->nc, nw = p->nw;
+  SysConst& s = h->sc;
+  s.nc = nc;
+  s.nw = nw;
+  s.hist_mode = h->hist_mode = p->history_mode;
+  h->order = p->method_order;
+  h->nprev = s.nprev = p->method_order - 1;
+  h->n_euler = p->n_euler;
+  h->dt = p->time_step;
+  s.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 64;
+  s.max_c = p->max_contacts > 0 ? p->max_contacts : 24;
+  if (s.max_c > 64) {
+    delete h;
+    return fail(nullptr, DEM_ERR_ARG, "max_contacts must be <= 64");
+  }
+  double rmax = 0.0, rmin = 1e300;
+  for (int c = 0; c < nc; ++c) {
+    rmax = std::max(rmax, p->radius[c]);
+    rmin = std::min(rmin, p->radius[c]);
+    h->radius_h.push_back(p->radius[c]);
+  }
+  h->skin = p->skin > 0.0 ? p->skin : 0.2 * rmin;  // 0.1 of the smallest diameter
+  s.skin_half_sq = 0.25 * h->skin * h->skin;
+  for (int d = 0; d < 3; ++d) {
+    s.pbc[d] = p->pbc[d] != 0;
+    s.lbox[d] = p->lbox[d];
+    s.invl[d] = 1.0 / p->lbox[d];
+    s.phalf[d] = 0.5 * p->lbox[d];
+    s.grav[d] = p->gravity[d];
+  }
+  // reference grid (InitCellStructure3D, src/MCells.f90:155-227)
+  int rn[3];
+  if (p->ref_grid[0] > 0 && p->ref_grid[1] > 0 && p->ref_grid[2] > 0) {
+    for (int d = 0; d < 3; ++d) rn[d] = p->ref_grid[d];
+  } else {
+    const double cutoff = 2.0 * rmax;
+    const int max_ncell = p->max_ncell > 0 ? p->max_ncell : 30000;
+    int n4[3];
+    for (int d = 0; d < 3; ++d) n4[d] = (int)(p->lbox[d] / cutoff);
+    if ((long long)n4[0] * n4[1] * n4[2] > max_ncell) {
+      const double lcell =
+          std::pow(p->lbox[0] * p->lbox[1] * p->lbox[2] / (double)max_ncell, 1.0 / 3.0);
+      for (int d = 0; d < 3; ++d) rn[d] = (int)(p->lbox[d] / lcell);
+    } else {
+      for (int d = 0; d < 3; ++d) rn[d] = n4[d];
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (rn[d] < 1 || (s.pbc[d] && rn[d] < 3)) {
+      delete h;
+      return fail(nullptr, DEM_ERR_UNSUPPORTED,
+                  "reference grid has %d cells along axis %d; periodic axes need >= 3 "
+                  "(with fewer the reference visits pairs twice)", rn[d], d + 1);
+    }
+    s.rn[d] = rn[d];
+    s.rn_by_l[d] = (double)rn[d] / p->lbox[d];
+    s.rn_by2[d] = (double)rn[d] * 0.5;
+  }
+  // neighbour-search grid: cells of at least 2*rmax + skin
+  const double rcmax = 2.0 * rmax + h->skin;
+  long long gn = 1;
+  for (int d = 0; d < 3; ++d) {
+    s.gn[d] = std::max(1, (int)(p->lbox[d] / rcmax));
+    s.g_by_l[d] = (double)s.gn[d] / p->lbox[d];
+    s.g_by2[d] = (double)s.gn[d] * 0.5;
+    gn *= s.gn[d];
+  }
+  if (gn > (1LL << 30)) {
+    delete h;
+    return fail(nullptr, DEM_ERR_ARG, "neighbour grid too large (%lld cells)", gn);
+  }
+  s.gncell = (int)gn;
+  h->gncell = (int)gn;
+  // ABM coefficients (InitializeAuxiliaries, src/MDEM.f90:127-154)
+  const double dt = h->dt;
+  double aux;
+  switch (h->order) {
+    case 4:
+      aux = dt / 24.0;
+      h->ab0 = 55.0 * aux; h->am0 = 9.0 * aux;
+      h->ab[0] = -59.0 * aux; h->am[0] = 19.0 * aux;
+      h->ab[1] = 37.0 * aux; h->am[1] = -5.0 * aux;
+      h->ab[2] = -9.0 * aux; h->am[2] = 1.0 * aux;
+      break;
+    case 3:
+      aux = dt / 12.0;
+      h->ab0 = 23.0 * aux; h->am0 = 5.0 * aux;
+      h->ab[0] = -16.0 * aux; h->am[0] = 8.0 * aux;
+      h->ab[1] = 5.0 * aux; h->am[1] = -1.0 * aux;
+      break;
+    case 2:
+      h->ab0 = 1.5 * dt; h->am0 = 0.5 * dt;
+      h->ab[0] = -0.5 * dt; h->am[0] = 0.5 * dt;
+      break;
+    default:
+      h->ab0 = dt; h->am0 = dt;
+  }
+  // constant tables
+  std::vector<PairConst> pair(nc * nc);
+  for (int t = 0; t < nc * nc; ++t) {
+    PairConst& q = pair[t];
+    q.kn = p->kn[t]; q.yn = p->yn[t]; q.kt = p->kt[t]; q.yt = p->yt[t]; q.musq = p->musq[t];
+    q.mindij = p->min_dij[t]; q.mindijsq = p->min_dij_sq[t];
+    const double rc = p->min_dij[t] + h->skin;
+    q.rc2 = rc * rc;
+  }
+  std::vector<CompConst> comp(nc);
+  for (int c = 0; c < nc; ++c) {
+    CompConst& q = comp[c];
+    q.radius = p->radius[c];
+    q.inv_mass = p->inv_mass[c];
+    q.inv_inertia = p->inv_inertia[c];
+    q.eff_mass = p->eff_mass[c];
+    q.mass = p->mass ? p->mass[c] : 0.0;
+    q.inertia = p->inertia ? p->inertia[c] : 0.0;
+    q.pad0 = q.pad1 = 0.0;
+  }
+  h->wallc_h.assign(std::max(1, nc * nw), WallConst{});
+  for (int t = 0; t < nc * nw; ++t) {
+    WallConst& q = h->wallc_h[t];
+    q.knw = p->knw[t]; q.ynw = p->ynw[t]; q.ktw = p->ktw[t]; q.ytw = p->ytw[t];
+    q.muwsq = p->muwsq[t];
+  }
+  h->walls_h.assign(std::max(1, nw), WallGeom{});
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = dalloc(&h->d_pair, pair.size());
+  if (e == hipSuccess) e = dalloc(&h->d_comp, comp.size());
+  if (e == hipSuccess) e = dalloc(&h->d_wallc, h->wallc_h.size());
+  if (e == hipSuccess) e = dalloc(&h->d_walls, h->walls_h.size());
+  if (e == hipSuccess) e = dalloc(&h->ctl, 1);
+  if (e == hipSuccess) e = dalloc(&h->rb, 2);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_pair, pair.data(), pair.size() * sizeof(PairConst), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_comp, comp.data(), comp.size() * sizeof(CompConst), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_wallc, h->wallc_h.data(), h->wallc_h.size() * sizeof(WallConst),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(h->ctl, 0, sizeof(Ctl));
+  if (e == hipSuccess) e = hipMemset(h->rb, 0, 2 * sizeof(int));
+  if (e != hipSuccess) {
+    int rc = fail(nullptr, DEM_ERR_DEVICE, "device setup failed: %s", hipGetErrorString(e));
+    dem_destroy(h);
+    return rc;
+  }
+  s.pair = h->d_pair;
+  s.comp = h->d_comp;
+  s.wallc = h->d_wallc;
+  s.walls = h->d_walls;
+  *out = h;
+  return DEM_OK;
+}
+
+void dem_destroy(dem_handle_t h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_state(h);
+  dfree(h->d_pair); dfree(h->d_comp); dfree(h->d_wallc); dfree(h->d_walls);
+  dfree(h->ctl); dfree(h->rb);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int dem_set_option(dem_handle_t h, const char* name, double value) {
+  if (!h || !name) return DEM_ERR_ARG;
+  if (!strcmp(name, "timing")) h->timing = value != 0.0;
+  else if (!strcmp(name, "resort_interval")) h->resort_interval = (int)value;
+  else if (!strcmp(name, "keep_predicted_acc")) h->keep_pred_acc = value != 0.0;
+  else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
+  return DEM_OK;
+}
+
+// ConfigureWalls (src/MConfig.f90:636-688).
+int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* w) {
+  if (!h) return DEM_ERR_ARG;
+  if (nw != h->sc.nw) return fail(h, DEM_ERR_ARG, "nw=%d but the system has %d walls", nw, h->sc.nw);
+  const int nc = h->sc.nc;
+  for (int j = 0; j < nw; ++j) {
+    if (w[j].shape != DEM_SHAPE_RECTANGLE && w[j].shape != DEM_SHAPE_CONE)
+      return fail(h, DEM_ERR_ARG, "wall %d has invalid shape %d", j + 1, w[j].shape);
+    WallGeom& g = h->walls_h[j];
+    g.shape = w[j].shape;
+    g.l1 = w[j].l1; g.l2 = w[j].l2; g.l3 = w[j].l3;
+    for (int d = 0; d < 3; ++d) {
+      g.u[d] = w[j].u[d]; g.w[d] = w[j].w[d]; g.n[d] = w[j].n[d];
+      g.pos[d] = w[j].position[d]; g.vel[d] = w[j].velocity[d]; g.omg[d] = w[j].omega[d];
+    }
+    g.rap = g.sin_t = g.cos_t = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      WallConst& q = h->wallc_h[c + j * nc];
+      q.max_alpha_u = q.min_aw_sq = q.max_aw_sq = 0.0;
+    }
+    if (g.shape == DEM_SHAPE_CONE) {
+      const double l_2 = g.l1, l = 2.0 * l_2, ra = g.l2, rp = g.l3;
+      const double aux = std::sqrt(l * l + (rp - ra) * (rp - ra));
+      g.rap = aux;
+      g.sin_t = l / aux;
+      g.cos_t = (rp - ra) / aux;
+      for (int c = 0; c < nc; ++c) {
+        const double ri = h->radius_h[c];
+        double mn = std::min(ra, rp) - ri;
+        mn = mn > 0.0 ? mn : 0.0;
+        const double mx = std::max(ra, rp) + ri;
+        WallConst& q = h->wallc_h[c + j * nc];
+        q.max_alpha_u = l_2 + ri;
+        q.min_aw_sq = mn * mn;
+        q.max_aw_sq = mx * mx;
+      }
+    }
+  }
+  HIPCHK(h, hipMemcpy(h->d_walls, h->walls_h.data(), h->walls_h.size() * sizeof(WallGeom),
+                      hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_wallc, h->wallc_h.data(), h->wallc_h.size() * sizeof(WallConst),
+                      hipMemcpyHostToDevice));
+  return DEM_OK;
+}
+
+static int ensure_tmp(dem_handle_t h, size_t bytes) {
+  if (bytes <= h->tmp_bytes) return DEM_OK;
+  dfree(h->d_tmp);
+  HIPCHK(h, hipMalloc((void**)&h->d_tmp, bytes));
+  h->tmp_bytes = bytes;
+  return DEM_OK;
+}
+
+static int alloc_state(dem_handle_t h, int np) {
+  free_state(h);
+  const size_t N = (size_t)std::max(np, 1);
+  const SysConst& s = h->sc;
+  const size_t nring = (size_t)std::max(h->nprev, 1) * 9;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = dalloc(&h->pos[k], N);
+    if (e == hipSuccess) e = dalloc(&h->vw[k], 6 * N);
+    if (e == hipSuccess) e = dalloc(&h->rc[k], N);
+    if (e == hipSuccess) e = dalloc(&h->st[k], N);
+    if (e == hipSuccess) e = dalloc(&h->ccnt[k], N);
+    if (e == hipSuccess) e = dalloc(&h->cpart[k], (size_t)s.max_c * N);
+    if (e == hipSuccess) e = dalloc(&h->chist[k], (size_t)s.max_c * 6 * N);
+    if (e == hipSuccess) e = dalloc(&h->wflag[k], N);
+    if (e == hipSuccess) e = dalloc(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
+  }
+  if (e == hipSuccess) e = dalloc(&h->acc, 6 * N);
+  if (e == hipSuccess) e = dalloc(&h->acc_p, 6 * N);
+  if (e == hipSuccess) e = dalloc(&h->ring, nring * N);
+  if (e == hipSuccess) e = dalloc(&h->ring2, nring * N);
+  if (e == hipSuccess) e = dalloc(&h->id, N);
+  if (e == hipSuccess) e = dalloc(&h->id2, N);
+  if (e == hipSuccess) e = dalloc(&h->xref, N);
+  if (e == hipSuccess) e = dalloc(&h->nbr, (size_t)s.max_nb * N);
+  if (e == hipSuccess) e = dalloc(&h->nnb, N);
+  if (e == hipSuccess) e = dalloc(&h->key, N);
+  if (e == hipSuccess) e = dalloc(&h->cell_count, (size_t)h->gncell + 1);
+  if (e == hipSuccess) e = dalloc(&h->cell_start, (size_t)h->gncell + 1);
+  if (e == hipSuccess) e = dalloc(&h->cursor, (size_t)h->gncell + 1);
+  if (e == hipSuccess) e = dalloc(&h->cell_idx, N);
+  h->ntiles = (h->gncell + SCAN_TILE - 1) / SCAN_TILE;
+  if (e == hipSuccess) e = dalloc(&h->tile_sum, (size_t)h->ntiles + 1);
+  if (e == hipSuccess) e = dalloc(&h->old_to_new, N);
+  if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
+  if (e != hipSuccess) {
+    free_state(h);
+    return fail(h, DEM_ERR_DEVICE, "allocation for %d particles failed: %s", np,
+                hipGetErrorString(e));
+  }
+  // zero the arrays whose unused parts are read
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(h, hipMemsetAsync(h->ccnt[k], 0, N * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), h->stream));
+  }
+  HIPCHK(h, hipMemsetAsync(h->acc_p, 0, 6 * N * sizeof(double), h->stream));
+  h->np = np;
+  h->cap = np;
+  h->sc.np = np;
+  return DEM_OK;
+}
+
+// ---- neighbour list ---------------------------------------------------------
+
+// Enqueues the counting sort + list build, gated on *gate unless force.
+static void enqueue_binning(dem_handle_t h, const double4* pos, const int* gate, int force) {
+  const SysConst& s = h->sc;
+  hipStream_t st = h->stream;
+  const int ng = gs_blocks(h->gncell + 1);
+  const int npb = gs_blocks(s.np);
+  hipLaunchKernelGGL(k_fill_i32, dim3(ng), dim3(DEM_BLOCK), 0, st, h->cell_count, h->gncell + 1,
+                     0, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_bin_count, dim3(npb), dim3(DEM_BLOCK), 0, st, s, pos, h->key,
+                     h->cell_count, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles), dim3(DEM_BLOCK), 0, st, h->cell_count,
+                     h->gncell, h->tile_sum, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles, gate,
+                     force, h->ctl);
+  hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles), dim3(DEM_BLOCK), 0, st, h->cell_count,
+                     h->gncell, h->tile_sum, h->cell_start, h->cursor, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(npb), dim3(DEM_BLOCK), 0, st, s.np, h->key, h->cursor,
+                     h->cell_idx, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_cell_sort, dim3(ng), dim3(DEM_BLOCK), 0, st, h->gncell, h->cell_start,
+                     h->cell_idx, gate, force, h->ctl);
+}
+
+static void enqueue_rebuild(dem_handle_t h, const int* gate, int force) {
+  const SysConst& s = h->sc;
+  if (s.np == 0) return;
+  enqueue_binning(h, h->pos[h->cur], gate, force);
+  hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s,
+                     h->pos[h->cur], h->key, h->cell_start, h->cell_idx, h->nbr, h->nnb, h->xref,
+                     gate, force, h->ctl);
+}
+
+// Re-sort the particle arrays into neighbour-grid cell order.  The cell state
+// that the next evaluation reads (rc/st of parity eval&1 when rc_ready, else
+// of the previous parity) moves with the particles.
+static int resort(dem_handle_t h) {
+  SysConst& s = h->sc;
+  if (s.np == 0) return DEM_OK;
+  enqueue_binning(h, h->pos[h->cur], h->rb, 1);
+  hipLaunchKernelGGL(k_old_to_new, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s.np,
+                     h->cell_idx, h->old_to_new);
+  const int pc = h->rc_ready ? (int)(h->eval & 1) : (int)((h->eval + 1) & 1);
+  PermArgs a;
+  a.src = h->cell_idx;
+  a.old_to_new = h->old_to_new;
+  a.pos = h->pos[h->cur]; a.pos2 = h->pos[h->cur ^ 1];
+  a.vw = h->vw[h->cur]; a.vw2 = h->vw[h->cur ^ 1];
+  a.acc = h->acc; a.acc2 = h->acc_p;
+  a.ring = h->ring; a.ring2 = h->ring2; a.nring = 9 * h->nprev;
+  a.id = h->id; a.id2 = h->id2;
+  a.rc = h->rc[pc]; a.rc2 = h->rc[pc ^ 1];
+  a.st = h->st[pc]; a.st2 = h->st[pc ^ 1];
+  a.ccnt = h->ccnt[h->tcur]; a.ccnt2 = h->ccnt[h->tcur ^ 1];
+  a.cpart = h->cpart[h->tcur]; a.cpart2 = h->cpart[h->tcur ^ 1];
+  a.chist = h->chist[h->tcur]; a.chist2 = h->chist[h->tcur ^ 1];
+  a.wflag = h->wflag[0]; a.wflag2 = h->wflag[1];
+  a.whist = h->whist[0]; a.whist2 = h->whist[1];
+  a.nw = s.nw;
+  hipLaunchKernelGGL(k_permute, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s, a);
+  HIPCHK(h, hipGetLastError());
+  h->cur ^= 1;
+  std::swap(h->acc, h->acc_p);
+  std::swap(h->ring, h->ring2);
+  std::swap(h->id, h->id2);
+  std::swap(h->rc[0], h->rc[1]);
+  std::swap(h->st[0], h->st[1]);
+  h->tcur ^= 1;
+  std::swap(h->wflag[0], h->wflag[1]);
+  std::swap(h->whist[0], h->whist[1]);
+  h->steps_since_resort = 0;
+  enqueue_rebuild(h, h->rb, 1);
+  HIPCHK(h, hipGetLastError());
+  return DEM_OK;
+}
+
+static int read_ctl(dem_handle_t h, Ctl* c) {
+  HIPCHK(h, hipMemcpyAsync(c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+static int check_errors(dem_handle_t h) {
+  Ctl c;
+  int rc = read_ctl(h, &c);
+  if (rc) return rc;
+  if (c.error) {
+    std::string m;
+    if (c.error & DEM_KERR_NBR_OVERFLOW)
+      m += "neighbour list overflow (max_neighbours too small); ";
+    if (c.error & DEM_KERR_CONTACT_OVERFLOW)
+      m += "contact table overflow (max_contacts too small); ";
+    if (c.error & DEM_KERR_SKIN) m += "a particle moved more than skin/2 in one step; ";
+    if (c.error & DEM_KERR_ROLE) m += "overlapping pair in non-adjacent reference cells; ";
+    if (c.error & DEM_KERR_CONE_AXIS) m += "particle on the axis of a cone wall; ";
+    if (c.error & DEM_KERR_NAN) m += "non-finite position; ";
+    int err = c.error;
+    HIPCHK(h, hipMemsetAsync(&h->ctl->error, 0, sizeof(int), h->stream));
+    return fail(h, (err & (DEM_KERR_NBR_OVERFLOW | DEM_KERR_CONTACT_OVERFLOW)) ? DEM_ERR_CAPACITY
+                                                                              : DEM_ERR_STATE,
+                "%s", m.c_str());
+  }
+  return DEM_OK;
+}
+
+// ---- upload / download ----------------------------------------------------------
+
+int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const double* x,
+                     const double* v, const double* w, const double* a, const double* alpha,
+                     const double* ring, const int32_t* ipc, const int32_t* cell_rank) {
+  if (!h || np < 0 || (np > 0 && (!comp || !x || !v || !w)))
+    return fail(h, DEM_ERR_ARG, "bad state arguments");
+  if (!h->has_state || np != h->np) {
+    int rc = alloc_state(h, np);
+    if (rc) return rc;
+  }
+  const size_t N = (size_t)np;
+  SysConst& s = h->sc;
+  s.np = np;
+  h->cur = 0;
+  h->tcur = 0;
+  h->acc_in_ring = false;
+  for (int k = 0; k < 3; ++k) h->ipc[k] = k + 1;
+  if (ipc)
+    for (int k = 0; k < h->nprev; ++k) {
+      if (ipc[k] < 1 || ipc[k] > h->nprev) return fail(h, DEM_ERR_ARG, "bad ipc");
+      h->ipc[k] = ipc[k];
+    }
+  std::vector<double4> hp(N);
+  std::vector<double> hv(6 * N), ha(6 * N, 0.0);
+  std::vector<int> hid(N), hrank(N, 0);
+  int max_rank = 0;
+  for (size_t i = 0; i < N; ++i) {
+    if (comp[i] < 1 || comp[i] > s.nc)
+      return fail(h, DEM_ERR_ARG, "particle %zu has component %d", i + 1, comp[i]);
+    hp[i] = make_double4(x[i], x[N + i], x[2 * N + i], (double)(comp[i] - 1));
+    for (int d = 0; d < 3; ++d) {
+      hv[6 * i + d] = v[d * N + i];
+      hv[6 * i + 3 + d] = w[d * N + i];
+      if (a) ha[6 * i + d] = a[d * N + i];
+      if (alpha) ha[6 * i + 3 + d] = alpha[d * N + i];
+    }
+    hid[i] = (int)i;
+    if (cell_rank) {
+      hrank[i] = cell_rank[i];
+      max_rank = std::max(max_rank, cell_rank[i]);
+    }
+  }
+  if (np == 0) {
+    h->has_state = true;
+    return DEM_OK;
+  }
+  hipStream_t st = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->pos[0], hp.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync(h->vw[0], hv.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync(h->acc, ha.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync(h->id, hid.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+  if (h->nprev > 0) {
+    if (ring)
+      HIPCHK(h, hipMemcpyAsync(h->ring, ring, (size_t)h->nprev * 9 * N * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+    else
+      HIPCHK(h, hipMemsetAsync(h->ring, 0, (size_t)h->nprev * 9 * N * sizeof(double), st));
+  }
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(h, hipMemsetAsync(h->ccnt[k], 0, N * sizeof(int), st));
+    HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), st));
+  }
+  HIPCHK(h, hipMemsetAsync(h->ctl, 0, sizeof(Ctl), st));
+  HIPCHK(h, hipMemsetAsync(h->rb, 0, 2 * sizeof(int), st));
+  // cell state of "the last evaluation" = the uploaded lists
+  // (DistributeParticles3D, src/MCells.f90:542-564, or the given order)
+  const int q = (int)((h->eval + 1) & 1);
+  HIPCHK(h, hipMemcpyAsync(h->st[q], hrank.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_refcell, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, s, h->pos[0], h->rc[q],
+                     h->st[q], h->rc[q], h->st[q], 0, 1);
+  h->stamp_base = (long long)max_rank + 1 - h->eval;
+  h->rc_ready = false;
+  h->has_state = true;
+  h->last_update_eval = -1;
+  int rc = resort(h);
+  if (rc) return rc;
+  return check_errors(h);
+}
+
+// out[c*np + ref] for c in [0, ncomp): element c of record i at
+// in[i*stride_rec + c*stride_comp].
+static int gather_ref(dem_handle_t h, const double* in, int ncomp, int stride_rec,
+                      int stride_comp, double* host_out) {
+  const size_t N = (size_t)h->np;
+  int rc = ensure_tmp(h, ncomp * N * sizeof(double));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_to_ref_d, dim3(nblocks(h->np)), dim3(DEM_BLOCK), 0, h->stream, h->np, ncomp,
+                     in, stride_rec, stride_comp, h->id, h->d_tmp);
+  HIPCHK(h, hipMemcpyAsync(host_out, h->d_tmp, ncomp * N * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+static int acc_source(dem_handle_t h, const double** p, int* rec, int* cs) {
+  if (h->acc_in_ring && h->nprev > 0) {
+    *p = h->ring + (size_t)(h->ipc[0] - 1) * 9 * h->np + 3 * (size_t)h->np;
+    *rec = 1;
+    *cs = h->np;
+  } else {
+    *p = h->acc;
+    *rec = 6;
+    *cs = 1;
+  }
+  return DEM_OK;
+}
+
+int dem_download_state(dem_handle_t h, int32_t np, int32_t* comp, double* x, double* v, double* w,
+                       double* a, double* alpha, double* ring, int32_t* ipc) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "no state");
+  if (np != h->np) return fail(h, DEM_ERR_ARG, "np=%d but the state holds %d", np, h->np);
+  const size_t N = (size_t)np;
+  if (np > 0) {
+    std::vector<double> tmp(4 * N);
+    int rc = gather_ref(h, (const double*)h->pos[h->cur], 4, 4, 1, tmp.data());
+    if (rc) return rc;
+    if (x) memcpy(x, tmp.data(), 3 * N * sizeof(double));
+    if (comp)
+      for (size_t i = 0; i < N; ++i) comp[i] = (int)tmp[3 * N + i] + 1;
+    std::vector<double> t6(6 * N);
+    if (v || w) {
+      rc = gather_ref(h, h->vw[h->cur], 6, 6, 1, t6.data());
+      if (rc) return rc;
+      if (v) memcpy(v, t6.data(), 3 * N * sizeof(double));
+      if (w) memcpy(w, t6.data() + 3 * N, 3 * N * sizeof(double));
+    }
+    if (a || alpha) {
+      const double* src;
+      int rec, cs;
+      acc_source(h, &src, &rec, &cs);
+      rc = gather_ref(h, src, 6, rec, cs, t6.data());
+      if (rc) return rc;
+      if (a) memcpy(a, t6.data(), 3 * N * sizeof(double));
+      if (alpha) memcpy(alpha, t6.data() + 3 * N, 3 * N * sizeof(double));
+    }
+    if (ring && h->nprev > 0) {
+      for (int sl = 0; sl < h->nprev; ++sl) {
+        rc = gather_ref(h, h->ring + (size_t)sl * 9 * N, 9, 1, np, ring + (size_t)sl * 9 * N);
+        if (rc) return rc;
+      }
+    }
+  }
+  if (ipc)
+    for (int k = 0; k < h->nprev; ++k) ipc[k] = h->ipc[k];
+  return DEM_OK;
+}
+
+int dem_download_predicted(dem_handle_t h, int32_t np, double* x, double* v, double* w, double* a,
+                           double* alpha) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "no state");
+  if (np != h->np) return fail(h, DEM_ERR_ARG, "np mismatch");
+  const size_t N = (size_t)np;
+  if (np == 0) return DEM_OK;
+  std::vector<double> tmp(6 * N);
+  int rc = gather_ref(h, (const double*)h->pos[h->cur ^ 1], 3, 4, 1, tmp.data());
+  if (rc) return rc;
+  if (x) memcpy(x, tmp.data(), 3 * N * sizeof(double));
+  rc = gather_ref(h, h->vw[h->cur ^ 1], 6, 6, 1, tmp.data());
+  if (rc) return rc;
+  if (v) memcpy(v, tmp.data(), 3 * N * sizeof(double));
+  if (w) memcpy(w, tmp.data() + 3 * N, 3 * N * sizeof(double));
+  if (a || alpha) {
+    rc = gather_ref(h, h->acc_p, 6, 6, 1, tmp.data());
+    if (rc) return rc;
+    if (a) memcpy(a, tmp.data(), 3 * N * sizeof(double));
+    if (alpha) memcpy(alpha, tmp.data() + 3 * N, 3 * N * sizeof(double));
+  }
+  return DEM_OK;
+}
+
+int dem_download_cells(dem_handle_t h, int32_t np, int32_t* cell_index) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "no state");
+  if (np != h->np) return fail(h, DEM_ERR_ARG, "np mismatch");
+  if (np == 0) return DEM_OK;
+  // CellIndex after the last evaluation
+  const int q = (int)((h->eval + 1) & 1);
+  int rc = ensure_tmp(h, (size_t)np * sizeof(int));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_to_ref_i, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, h->stream, np, h->rc[q],
+                     h->id, (int*)h->d_tmp, 1);
+  HIPCHK(h, hipMemcpyAsync(cell_index, h->d_tmp, (size_t)np * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+static int download_ids(dem_handle_t h, std::vector<int>& idh) {
+  idh.resize(h->np);
+  if (h->np == 0) return DEM_OK;
+  HIPCHK(h, hipMemcpyAsync(idh.data(), h->id, (size_t)h->np * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, const double* ft,
+                        const double* vt, int64_t nwall, const uint64_t* wkeys, const double* wft,
+                        const double* wvt) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  const size_t N = (size_t)h->np;
+  const SysConst& s = h->sc;
+  std::vector<int> idh;
+  int rc = download_ids(h, idh);
+  if (rc) return rc;
+  std::vector<int> inv(N);
+  for (size_t i = 0; i < N; ++i) inv[idh[i]] = (int)i;
+  std::vector<int> cnt(N, 0), part((size_t)s.max_c * N, 0);
+  std::vector<double> hist((size_t)s.max_c * 6 * N, 0.0);
+  auto put = [&](int p, int q, const double* f, const double* g) -> bool {
+    int c = cnt[p];
+    if (c >= s.max_c) return false;
+    part[(size_t)c * N + p] = q;
+    for (int d = 0; d < 3; ++d) {
+      hist[((size_t)c * 6 + d) * N + p] = f[d];
+      hist[((size_t)c * 6 + 3 + d) * N + p] = g[d];
+    }
+    cnt[p] = c + 1;
+    return true;
+  };
+  for (int64_t k = 0; k < npairs; ++k) {
+    const long long a = (long long)(keys[k] >> 32) - 1, b = (long long)(keys[k] & 0xffffffffu) - 1;
+    if (a < 0 || b <= a || b >= (long long)N)
+      return fail(h, DEM_ERR_ARG, "bad pair key at %lld", (long long)k);
+    const int p = inv[a], q = inv[b];
+    if (!put(p, q, ft + 3 * k, vt + 3 * k) || !put(q, p, ft + 3 * k, vt + 3 * k))
+      return fail(h, DEM_ERR_CAPACITY, "more than max_contacts=%d contacts on one particle",
+                  s.max_c);
+  }
+  std::vector<unsigned> wf(N, 0u);
+  std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N, 0.0);
+  for (int64_t k = 0; k < nwall; ++k) {
+    const long long a = (long long)(wkeys[k] >> 32) - 1, jw = (long long)(wkeys[k] & 0xffffffffu) - 1;
+    if (a < 0 || a >= (long long)N || jw < 0 || jw >= s.nw)
+      return fail(h, DEM_ERR_ARG, "bad wall key at %lld", (long long)k);
+    const int p = inv[a];
+    wf[p] |= 1u << jw;
+    for (int d = 0; d < 3; ++d) {
+      wh[((size_t)jw * 6 + d) * N + p] = wft[3 * k + d];
+      wh[((size_t)jw * 6 + 3 + d) * N + p] = wvt[3 * k + d];
+    }
+  }
+  hipStream_t st = h->stream;
+  const int t = h->tcur;
+  if (N > 0) {
+    HIPCHK(h, hipMemcpyAsync(h->ccnt[t], cnt.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->cpart[t], part.data(), part.size() * sizeof(int),
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->chist[t], hist.data(), hist.size() * sizeof(double),
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->wflag[0], wf.data(), N * sizeof(unsigned), hipMemcpyHostToDevice,
+                             st));
+    if (s.nw > 0)
+      HIPCHK(h, hipMemcpyAsync(h->whist[0], wh.data(), (size_t)s.nw * 6 * N * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(h, hipStreamSynchronize(st));
+  return DEM_OK;
+}
+
+int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t* keys, double* ft,
+                          double* vt, int64_t wcap, int64_t* nwall, uint64_t* wkeys, double* wft,
+                          double* wvt) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "no state");
+  const size_t N = (size_t)h->np;
+  const SysConst& s = h->sc;
+  std::vector<int> idh;
+  int rc = download_ids(h, idh);
+  if (rc) return rc;
+  std::vector<int> cnt(N), part((size_t)s.max_c * N);
+  std::vector<double> hist((size_t)s.max_c * 6 * N);
+  std::vector<unsigned> wf(N);
+  std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N);
+  hipStream_t st = h->stream;
+  const int t = h->tcur;
+  if (N > 0) {
+    HIPCHK(h, hipMemcpyAsync(cnt.data(), h->ccnt[t], N * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->cpart[t], part.size() * sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(hist.data(), h->chist[t], hist.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(wf.data(), h->wflag[0], N * sizeof(unsigned), hipMemcpyDeviceToHost,
+                             st));
+    if (s.nw > 0)
+      HIPCHK(h, hipMemcpyAsync(wh.data(), h->whist[0], (size_t)s.nw * 6 * N * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+  }
+  struct E {
+    uint64_t key;
+    double f[3], g[3];
+  };
+  std::vector<E> pe, we;
+  for (size_t i = 0; i < N; ++i) {
+    const int a = idh[i];
+    for (int c = 0; c < cnt[i]; ++c) {
+      const int q = part[(size_t)c * N + i];
+      const int b = idh[q];
+      if (a >= b) continue;  // each pair is stored on both particles
+      E e;
+      e.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(b + 1);
+      for (int d = 0; d < 3; ++d) {
+        e.f[d] = hist[((size_t)c * 6 + d) * N + i];
+        e.g[d] = hist[((size_t)c * 6 + 3 + d) * N + i];
+      }
+      pe.push_back(e);
+    }
+    for (int jw = 0; jw < s.nw; ++jw) {
+      if (!((wf[i] >> jw) & 1u)) continue;
+      E e;
+      e.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(jw + 1);
+      for (int d = 0; d < 3; ++d) {
+        e.f[d] = wh[((size_t)jw * 6 + d) * N + i];
+        e.g[d] = wh[((size_t)jw * 6 + 3 + d) * N + i];
+      }
+      we.push_back(e);
+    }
+  }
+  auto by_key = [](const E& x, const E& y) { return x.key < y.key; };
+  std::sort(pe.begin(), pe.end(), by_key);
+  std::sort(we.begin(), we.end(), by_key);
+  if (npairs) *npairs = (int64_t)pe.size();
+  if (nwall) *nwall = (int64_t)we.size();
+  if ((keys && (int64_t)pe.size() > cap) || (wkeys && (int64_t)we.size() > wcap))
+    return fail(h, DEM_ERR_CAPACITY, "contact arrays too small");
+  for (size_t k = 0; k < pe.size() && keys; ++k) {
+    keys[k] = pe[k].key;
+    if (ft) memcpy(ft + 3 * k, pe[k].f, 3 * sizeof(double));
+    if (vt) memcpy(vt + 3 * k, pe[k].g, 3 * sizeof(double));
+  }
+  for (size_t k = 0; k < we.size() && wkeys; ++k) {
+    wkeys[k] = we[k].key;
+    if (wft) memcpy(wft + 3 * k, we[k].f, 3 * sizeof(double));
+    if (wvt) memcpy(wvt + 3 * k, we[k].g, 3 * sizeof(double));
+  }
+  return DEM_OK;
+}
+
+// ---- evaluation ------------------------------------------------------------------
+
+struct EvalCfg {
+  const double4* pos_in;
+  const double* vw_in;
+};
+
+static PassArgs base_args(dem_handle_t h, bool update) {
+  PassArgs a{};
+  const int p = (int)(h->eval & 1);
+  a.id = h->id;
+  a.rc_in = h->rc[p];
+  a.st_in = h->st[p];
+  a.rc_out = h->rc[p ^ 1];
+  a.st_out = h->st[p ^ 1];
+  a.stamp_next = (int)(h->stamp_base + h->eval + 1);
+  a.par = p;
+  a.xref = h->xref;
+  a.nbr = h->nbr;
+  a.nnb = h->nnb;
+  a.ccnt_in = h->ccnt[h->tcur];
+  a.cpart_in = h->cpart[h->tcur];
+  a.chist_in = h->chist[h->tcur];
+  a.ccnt_out = h->ccnt[h->tcur ^ 1];
+  a.cpart_out = h->cpart[h->tcur ^ 1];
+  a.chist_out = h->chist[h->tcur ^ 1];
+  a.wflag = h->wflag[0];
+  a.whist = h->whist[0];
+  a.rb_clear = h->rb + p;
+  a.rb_set = h->rb + (p ^ 1);
+  a.ctl = h->ctl;
+  a.half_dt = 0.5 * h->dt;
+  (void)update;
+  return a;
+}
+
+static void after_eval(dem_handle_t h, bool update) {
+  if (update) {
+    h->tcur ^= 1;
+    h->last_update_eval = h->eval;
+  }
+  h->eval += 1;
+}
+
+// rc/st for the positions about to be evaluated (eval parity), from the cell
+// state of the previous evaluation.
+static void enqueue_refcell(dem_handle_t h) {
+  const int p = (int)(h->eval & 1);
+  hipLaunchKernelGGL(k_refcell, dim3(nblocks(h->np)), dim3(DEM_BLOCK), 0, h->stream, h->sc,
+                     h->pos[h->cur], h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
+                     (int)(h->stamp_base + h->eval), 0);
+  h->rc_ready = true;
+}
+
+int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p, double time_step,
+                                  int32_t update) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (np != h->np) return fail(h, DEM_ERR_STATE, "state holds %d particles, call passes %d", h->np, np);
+  if (np == 0) return DEM_OK;
+  const size_t N = (size_t)np;
+  hipStream_t st = h->stream;
+  if (!h->d_aos) HIPCHK(h, hipMalloc((void**)&h->d_aos, N * sizeof(dem_particle_t)));
+  HIPCHK(h, hipMemcpyAsync(h->d_aos, p, N * sizeof(dem_particle_t), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_aos_in, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->d_aos, h->id,
+                     h->pos[h->cur], h->vw[h->cur]);
+  enqueue_refcell(h);
+  const int par = (int)(h->eval & 1);
+  hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc,
+                     h->pos[h->cur], h->xref, h->rb + par);
+  enqueue_rebuild(h, h->rb + par, 0);
+  PassArgs a = base_args(h, update != 0);
+  a.pos_in = h->pos[h->cur];
+  a.vw_in = h->vw[h->cur];
+  a.acc_out = h->acc;
+  a.half_dt = 0.5 * time_step;
+  if (update)
+    hipLaunchKernelGGL((k_pass<MODE_ACC, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+  else
+    hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+  HIPCHK(h, hipGetLastError());
+  after_eval(h, update != 0);
+  h->rc_ready = false;
+  h->acc_in_ring = false;
+  hipLaunchKernelGGL(k_aos_out, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->d_aos, h->id,
+                     h->acc);
+  HIPCHK(h, hipMemcpyAsync(p, h->d_aos, N * sizeof(dem_particle_t), hipMemcpyDeviceToHost, st));
+  return check_errors(h);
+}
+
+int dem_compute_accelerations(dem_handle_t h, int32_t which, int32_t update) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (h->np == 0) return DEM_OK;
+  const int np = h->np;
+  hipStream_t st = h->stream;
+  const int c = which ? (h->cur ^ 1) : h->cur;
+  {
+    const int p = (int)(h->eval & 1);
+    hipLaunchKernelGGL(k_refcell, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[c],
+                       h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
+                       (int)(h->stamp_base + h->eval), 0);
+    hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[c],
+                       h->xref, h->rb + p);
+    if (c == h->cur) enqueue_rebuild(h, h->rb + p, 0);
+    else {
+      // rebuild around the evaluated (predicted) positions
+      std::swap(h->pos[0], h->pos[1]);
+      std::swap(h->vw[0], h->vw[1]);
+      enqueue_rebuild(h, h->rb + p, 0);
+      std::swap(h->pos[0], h->pos[1]);
+      std::swap(h->vw[0], h->vw[1]);
+    }
+  }
+  PassArgs a = base_args(h, update != 0);
+  a.pos_in = h->pos[c];
+  a.vw_in = h->vw[c];
+  a.acc_out = which ? h->acc_p : h->acc;
+  if (update)
+    hipLaunchKernelGGL((k_pass<MODE_ACC, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+  else
+    hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+  HIPCHK(h, hipGetLastError());
+  after_eval(h, update != 0);
+  h->rc_ready = false;
+  if (!which) h->acc_in_ring = false;
+  return check_errors(h);
+}
+
+__global__ void k_store_ring(int np, const double* vw, const double* acc, double* slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const size_t N = (size_t)np;
+  for (int d = 0; d < 3; ++d) {
+    slot[d * N + i] = vw[6 * (size_t)i + d];
+    slot[(3 + d) * N + i] = acc[6 * (size_t)i + d];
+    slot[(6 + d) * N + i] = acc[6 * (size_t)i + 3 + d];
+  }
+}
+
+// ApplyPeriodicBoundaryConditions (src/MDEM.f90:373-388) + the cell state of
+// the wrapped positions for the next evaluation.
+__global__ void k_wrap(SysConst s, double4* pos, const int* rc_prev, const int* st_prev,
+                       int* rc_out, int* st_out, int stamp_next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.np) return;
+  double4 P = pos[i];
+  if (s.pbc[0]) P.x = P.x - s.lbox[0] * fnint(P.x * s.invl[0]);
+  if (s.pbc[1]) P.y = P.y - s.lbox[1] * fnint(P.y * s.invl[1]);
+  if (s.pbc[2]) P.z = P.z - s.lbox[2] * fnint(P.z * s.invl[2]);
+  pos[i] = P;
+  const int rc = ref_cell(s, P.x, P.y, P.z);
+  rc_out[i] = rc;
+  st_out[i] = (rc != rc_prev[i]) ? stamp_next : st_prev[i];
+}
+
+int dem_euler_startup(dem_handle_t h) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (h->nprev == 0) return fail(h, DEM_ERR_STATE, "Euler start-up needs MethodOrder > 1");
+  if (h->np == 0) return DEM_OK;
+  const int np = h->np;
+  hipStream_t st = h->stream;
+  if (h->acc_in_ring)
+    return fail(h, DEM_ERR_STATE, "Euler start-up after MoveParticles is not supported");
+  // ipc = cshift(ipc, -1); Previous(ipc(1)) = (v, a, alpha)  (src/MDEM.f90:266-271)
+  {
+    const int last = h->ipc[h->nprev - 1];
+    for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+    h->ipc[0] = last;
+  }
+  hipLaunchKernelGGL(k_store_ring, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->vw[h->cur],
+                     h->acc, h->ring + (size_t)(h->ipc[0] - 1) * 9 * np);
+  if (!h->rc_ready) enqueue_refcell(h);
+  const double hs = h->dt / (double)h->n_euler;
+  for (int sub = 0; sub < h->n_euler; ++sub) {
+    const int p = (int)(h->eval & 1);
+    PassArgs a = base_args(h, true);
+    a.pos_in = h->pos[h->cur];
+    a.vw_in = h->vw[h->cur];
+    a.pos_out = h->pos[h->cur ^ 1];
+    a.vw_out = h->vw[h->cur ^ 1];
+    a.acc_out = h->acc;
+    a.h = hs;
+    a.hsq2 = 0.5 * (hs * hs);
+    a.half_dt = 0.5 * hs;
+    hipLaunchKernelGGL((k_pass<MODE_EULER, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st,
+                       h->sc, a);
+    after_eval(h, true);
+    h->cur ^= 1;
+    (void)p;
+    enqueue_rebuild(h, h->rb + (h->eval & 1), 0);
+  }
+  // PBC after the sub-steps (src/MDEM.f90:279)
+  const int p = (int)(h->eval & 1);
+  hipLaunchKernelGGL(k_wrap, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[h->cur],
+                     h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
+                     (int)(h->stamp_base + h->eval));
+  HIPCHK(h, hipGetLastError());
+  h->rc_ready = true;
+  h->acc_in_ring = false;
+  return check_errors(h);
+}
+
+int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_done,
+             int32_t* escaped) {
+  (void)first_step;
+  if (steps_done) *steps_done = 0;
+  if (escaped) *escaped = 0;
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (nsteps <= 0 || h->np == 0) return DEM_OK;
+  const int np = h->np;
+  const size_t N = (size_t)np;
+  hipStream_t st = h->stream;
+  {
+    int big = 0x7fffffff;
+    HIPCHK(h, hipMemcpyAsync(&h->ctl->stop_step, &big, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemsetAsync(&h->ctl->escaped, 0, sizeof(int), st));
+  }
+  if (!h->rc_ready) enqueue_refcell(h);
+  const int nb = nblocks(np);
+  if (h->timing) h->timer.ensure((size_t)6 * nsteps + 2);
+  size_t ev = 0;
+  double t1 = 0, t2 = 0, tr = 0;
+  bool first = true;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    if (h->resort_interval > 0 && h->steps_since_resort >= h->resort_interval) {
+      int rc = resort(h);
+      if (rc) return rc;
+    }
+    // pass 1: forces at x_n (Update), AB predictor, ring store
+    PassArgs a = base_args(h, true);
+    a.pos_in = h->pos[h->cur];
+    a.vw_in = h->vw[h->cur];
+    a.pos_out = h->pos[h->cur ^ 1];
+    a.vw_out = h->vw[h->cur ^ 1];
+    a.acc_out = h->acc;
+    a.coef0 = h->ab0;
+    for (int k = 0; k < h->nprev; ++k) {
+      a.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
+      a.coef[k] = h->ab[k];
+    }
+    a.slot_wr = h->nprev > 0 ? h->ring + (size_t)(h->ipc[h->nprev - 1] - 1) * 9 * N : nullptr;
+    a.check_pred = first ? 1 : 0;
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    hipLaunchKernelGGL((k_pass<MODE_PRED, true>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    after_eval(h, true);
+    if (h->nprev > 0) {
+      const int last = h->ipc[h->nprev - 1];
+      for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+      h->ipc[0] = last;
+    }
+    // pass 2: forces at the predicted state, AM corrector, look-ahead
+    PassArgs b = base_args(h, false);
+    b.pos_in = h->pos[h->cur ^ 1];
+    b.vw_in = h->vw[h->cur ^ 1];
+    b.pos = h->pos[h->cur];
+    b.vw = h->vw[h->cur];
+    b.acc_out = h->keep_pred_acc ? h->acc_p : nullptr;
+    b.coef0 = h->am0;
+    for (int k = 0; k < h->nprev; ++k) {
+      b.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
+      b.coef[k] = h->am[k];
+      b.ab[k] = h->ab[k];
+    }
+    b.ab0 = h->ab0;
+    b.step_idx = (int)s;
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    hipLaunchKernelGGL((k_pass<MODE_CORR, false>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, b);
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    after_eval(h, false);
+    h->acc_in_ring = h->nprev > 0;
+    // neighbour list, when pass 2 asked for it
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    enqueue_rebuild(h, h->rb + (h->eval & 1), 0);
+    if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    h->rc_ready = true;
+    h->steps_since_resort += 1;
+    first = false;
+  }
+  HIPCHK(h, hipGetLastError());
+  Ctl c;
+  int rc = read_ctl(h, &c);
+  if (rc) return rc;
+  if (h->timing) {
+    float ms;
+    for (size_t k = 0; k + 5 < ev; k += 6) {
+      if (hipEventElapsedTime(&ms, h->timer.ev[k], h->timer.ev[k + 1]) == hipSuccess) t1 += ms;
+      if (hipEventElapsedTime(&ms, h->timer.ev[k + 2], h->timer.ev[k + 3]) == hipSuccess) t2 += ms;
+      if (hipEventElapsedTime(&ms, h->timer.ev[k + 4], h->timer.ev[k + 5]) == hipSuccess) tr += ms;
+    }
+    const double n = (double)nsteps;
+    h->ms_pass1 = t1 / n;
+    h->ms_pass2 = t2 / n;
+    h->ms_rebuild = tr / n;
+    if (ev >= 2 && hipEventElapsedTime(&ms, h->timer.ev[0], h->timer.ev[ev - 1]) == hipSuccess)
+      h->ms_step = ms / n;
+  }
+  int64_t done = nsteps;
+  if (c.escaped > 0) done = (int64_t)c.stop_step + 1;
+  h->steps += done;
+  if (steps_done) *steps_done = done;
+  if (escaped) *escaped = c.escaped;
+  rc = check_errors(h);
+  if (rc) return rc;
+  if (c.escaped > 0)
+    return fail(h, DEM_ERR_ESCAPE, "%d particles left the box in step %lld of the call",
+                c.escaped, (long long)done);
+  return DEM_OK;
+}
+
+int dem_synchronize(dem_handle_t h) {
+  if (!h) return DEM_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+int dem_stats(dem_handle_t h, dem_stats_t* out) {
+  if (!h || !out) return DEM_ERR_ARG;
+  memset(out, 0, sizeof *out);
+  out->np = h->np;
+  for (int d = 0; d < 3; ++d) {
+    out->ref_ncell[d] = h->sc.rn[d];
+    out->nb_ncell[d] = h->sc.gn[d];
+  }
+  if (!h->has_state) return DEM_OK;
+  Ctl c;
+  int rc = read_ctl(h, &c);
+  if (rc) return rc;
+  const int q = (int)((h->eval + 1) & 1);
+  out->pair_contacts = (int64_t)c.pairs[q];
+  out->wall_contacts = (int64_t)c.wallc[q];
+  if (h->last_update_eval >= 0) out->stored_pairs = (int64_t)(c.stored[h->last_update_eval & 1] / 2);
+  out->ncol = (int64_t)c.ncol;
+  out->evaluations = h->eval;
+  out->steps = h->steps;
+  out->rebuilds = (int64_t)c.rebuilds;
+  out->neighbour_pairs = (int64_t)c.nbpairs;
+  if (h->np > 0) {
+    const int nb = nblocks(h->np);
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(DEM_BLOCK), 0, h->stream, h->sc, h->pos[h->cur],
+                       h->vw[h->cur], h->d_epart);
+    std::vector<double> part(nb);
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->d_epart, nb * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double e = 0.0;
+    for (double x : part) e += x;
+    out->kinetic_energy = 0.5 * e;
+  }
+  out->ms_pass1 = h->ms_pass1;
+  out->ms_pass2 = h->ms_pass2;
+  out->ms_rebuild = h->ms_rebuild;
+  out->ms_step = h->ms_step;
+  return DEM_OK;
+}
+
+}  // extern "C"

@@ -214,6 +214,16 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step,
 int dem_synchronize(dem_handle_t h);
 int dem_stats(dem_handle_t h, dem_stats_t* out);
 
+/* Runtime options (no counterpart in the reference):
+ *   "timing"            1 = time pass 1, pass 2 and rebuild kernels with HIP
+ *                       events inside dem_step (reported by dem_stats)
+ *   "resort_interval"   steps between re-sorts of the particle arrays into
+ *                       cell order (0 = only at upload); default 500
+ *   "keep_predicted_acc" 1 = keep Predicted%Acceleration/Alpha for
+ *                       dem_download_predicted (one extra 48 B write per
+ *                       particle and step); default 0 */
+int dem_set_option(dem_handle_t h, const char* name, double value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -438,18 +438,20 @@ static int overlap(const orc_handle_t h, const particle_t* ip, int jw, double ri
   double ra_aw = w->l2 - alpha_w;
   double lambda = lby2_au * cn->sin_t - ra_aw * cn->cos_t;
   double wv[3];
-  if (alpha_w == 0.0) return -1;
-  for (int k = 0; k < 3; k++) wv[k] = rijw[k] / alpha_w;
   if ((lambda >= 0.0) && (lambda <= cn->rap)) {
     double sd = -(lby2_au * cn->cos_t + ra_aw * cn->sin_t);
     *dij = fabs(sd);
     if (!(*dij < ri)) return 0;
+    if (alpha_w == 0.0) return -1;
+    for (int k = 0; k < 3; k++) wv[k] = rijw[k] / alpha_w;
     for (int k = 0; k < 3; k++) nij[k] = cn->sin_t * wv[k] + cn->cos_t * w->u[k];
     for (int k = 0; k < 3; k++) qji[k] = sd * nij[k] - rij[k];
     for (int k = 0; k < 3; k++) nij[k] = (qji[k] + rij[k]) / *dij;
     return 1;
   }
   lambda = fmax(0.0, fmin(lambda, cn->rap));
+  if (alpha_w == 0.0) return -1;
+  for (int k = 0; k < 3; k++) wv[k] = rijw[k] / alpha_w;
   double cu = cn->sin_t * lambda - lby2_au;
   double cw = cn->cos_t * lambda + ra_aw;
   for (int k = 0; k < 3; k++) nij[k] = cu * w->u[k] - cw * wv[k];

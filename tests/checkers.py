@@ -9,6 +9,7 @@ import ctypes as C
 import multiprocessing as mp
 import os
 import struct
+import shutil
 import tempfile
 from typing import Optional
 
@@ -198,7 +199,9 @@ def _ref_job_body(job):
     a = cols(job["a"]) if job.get("a") is not None else np.zeros((n, 3), order="F")
     al = cols(job["alpha"]) if job.get("alpha") is not None else np.zeros((n, 3), order="F")
     lib.ref_set_particles(C.c_int32(n), comp.ctypes.data_as(ip), P(x), P(v), P(w), P(a), P(al))
-    tmp = tempfile.mkdtemp(prefix="dem3d_ref_")
+    scratch = os.path.join(ROOT, ".scratch")
+    os.makedirs(scratch, exist_ok=True)
+    tmp = tempfile.mkdtemp(prefix="ref_", dir=scratch)
     nprev = red.method_order - 1
     if job.get("state") is not None:
         st = job["state"]
@@ -251,7 +254,7 @@ def _ref_job_body(job):
             out["seconds"] = _t.perf_counter() - t0
         else:
             raise ValueError(kind)
-    os.rmdir(tmp)
+    shutil.rmtree(tmp, ignore_errors=True)
     return out
 
 
