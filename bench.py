@@ -1,0 +1,174 @@
+"""Benchmark of the DEM per-timestep path on one MI355X.
+
+Workload: the 4 000 000-sphere bidisperse sheared bed (rock-salt packing of
+1.0/1.4 mm spheres, phi = 0.567, periodic x,y, floor, top wall moving at
+10 s^-1 * H; mu = 0.3, e = 0.8), FP64.  One step = MoveParticles
+(src/MDEM.f90:285-369: two force evaluations, AB predictor, AM corrector) +
+the escape scan, with the state resident in HBM.
+
+Prints one JSON line:
+  value        particle-steps/s over exactly --steps timed steps
+  roofline     pass-1 force kernel (the dominant one): algorithmic bytes per
+               launch / mean launch time from HIP events on the library's
+               stream, against 8 TB/s
+  cpu_baseline the reference Fortran modules (oracle/_ref, capacity
+               constants raised) timed on one host core on a bounded sample
+               of the same bed
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of the fused two-pass design (FP64): per particle 436 B in
+# each pass; per contact 112 B in pass 1 (history read + write) and 56 B in
+# pass 2 (history read).  Neighbour-list and rebuild traffic excluded.
+PASS1_B_PER_PARTICLE, PASS1_B_PER_CONTACT = 436.0, 112.0
+PASS2_B_PER_PARTICLE, PASS2_B_PER_CONTACT = 436.0, 56.0
+
+
+def cpu_baseline(cells, steps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from dem3d_amd import workloads
+    from checkers import ref_available, run_reference
+    if not ref_available(big=True):
+        return None
+    wl = workloads.config2(cells=cells)
+    res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, big=True,
+                             actions=[("time", 1, steps)]), timeout=900)
+    sec = res["seconds"]
+    return {
+        "value": wl.np * steps / sec,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"reference Fortran MoveParticles (MaxNeighbours/MaxNCell raised), "
+                   f"{steps} steps of the same sheared bed cut to {cells[0]}x{cells[1]}x{cells[2]} "
+                   f"rock-salt cells ({wl.np} spheres), zero ABM ring, {sec:.2f} s, 1 core, "
+                   f"{cpu_model()}"),
+    }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per pass-1 launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), committed in profiles/; None when absent."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("pass1_bytes_per_launch")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cells", default="100,100,50", help="rock-salt cells (8 spheres each)")
+    ap.add_argument("--cpu-cells", default="100,100,12")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skin", type=float, default=0.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    cells = tuple(int(x) for x in args.cells.split(","))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # before the GPU is initialised: the reference runs in a forked process
+        cpu = cpu_baseline(tuple(int(x) for x in args.cpu_cells.split(",")), args.cpu_steps)
+        print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    from dem3d_amd import workloads
+    from dem3d_amd.native import System, load_hip
+    wl = workloads.config2(cells=cells)
+    print(f"workload: {wl.name} np={wl.np} {wl.note}", file=sys.stderr, flush=True)
+    g = System(load_hip(), wl.red, skin=args.skin)
+    g.upload(wl.comp, wl.x, wl.v, wl.w)
+    g.startup()
+    if args.warmup:
+        g.step(args.warmup)
+    g.lib.set_option(g.h, b"timing", 1.0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    st = g.stats()
+    sec = t1 - t0
+    ms = 1e3 * sec / args.steps
+    value = wl.np * args.steps / sec
+    P, W = st["pair_contacts"], st["wall_contacts"]
+    b1 = PASS1_B_PER_PARTICLE * wl.np + PASS1_B_PER_CONTACT * (P + W)
+    b2 = PASS2_B_PER_PARTICLE * wl.np + PASS2_B_PER_CONTACT * (P + W)
+    dom_ms, dom_b, dom = (st["ms_pass1"], b1, "pass1")
+    if st["ms_pass2"] > st["ms_pass1"]:
+        dom_ms, dom_b, dom = (st["ms_pass2"], b2, "pass2")
+    achieved = dom_b / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = traffic_from_profiles(wl.name)
+    out = {
+        "metric": "particle-steps/sec at 4M spheres (FP64)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "spheres": wl.np,
+            "packing": wl.note,
+            "precision": "fp64",
+            "history_mode": "reference",
+            "pair_contacts": P,
+            "wall_contacts": W,
+            "rebuilds": st["rebuilds"],
+            "ms_pass1": st["ms_pass1"],
+            "ms_pass2": st["ms_pass2"],
+            "ms_rebuild_per_step": st["ms_rebuild"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": dom_b,
+            "step_frac": (b1 + b2) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -120,6 +120,7 @@ class Library:
         "stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     }
     OPTIONAL = {
+        "set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),

@@ -181,7 +181,7 @@ def test_escape_is_reported(hip):
     x[2, 0] = 0.5 * red.lbox[2] - 0.01      # just below the open top
     v = wl.v.copy()
     v[:, 0] = 0.0
-    v[2, 0] = 5.0                            # flying up
+    v[2, 0] = 1.0                            # flying up, 0.02 d per step (< skin/2)
     g = System(hip, red)
     g.upload(wl.comp, x, v, wl.w)
     g.startup()
