@@ -87,7 +87,22 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--history", choices=("reference", "canonical"), default="reference",
+                    help="orientation of stored tangential history (see include/dem3d.h)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy kernel (reads 1 GiB, writes "
+                         "1 GiB) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        import torch
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
@@ -100,10 +115,11 @@ def main(argv=None):
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     from dem3d_amd import workloads
-    from dem3d_amd.native import System, load_hip
+    from dem3d_amd.native import DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, System, load_hip
     wl = workloads.config2(cells=cells)
     print(f"workload: {wl.name} np={wl.np} {wl.note}", file=sys.stderr, flush=True)
-    g = System(load_hip(), wl.red, skin=args.skin)
+    hmode = DEM_HIST_REFERENCE if args.history == "reference" else DEM_HIST_CANONICAL
+    g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
     if args.warmup:
@@ -144,7 +160,7 @@ def main(argv=None):
             "spheres": wl.np,
             "packing": wl.note,
             "precision": "fp64",
-            "history_mode": "reference",
+            "history_mode": args.history,
             "pair_contacts": P,
             "wall_contacts": W,
             "rebuilds": st["rebuilds"],

@@ -49,6 +49,7 @@ struct Ctl {
   unsigned long long wallc[2];      // wall contacts of the eval, by parity
   unsigned long long stored[2];     // stored pair entries after an update eval
   unsigned long long nbpairs;       // directed neighbour entries of last build
+  unsigned long long stale_dropped; // stored pairs lost from the list at a rebuild
 };
 
 enum {

@@ -22,7 +22,6 @@ thread_local char g_err[512];
 
 struct Timer {
   std::vector<hipEvent_t> ev;
-  size_t used = 0;
   void ensure(size_t n) {
     while (ev.size() < n) {
       hipEvent_t e;
@@ -41,7 +40,7 @@ struct dem_system {
   char err[512];
   hipStream_t stream = nullptr;
   SysConst sc{};
-  int order = 4, nprev = 3, n_euler = 1, precision = 0, hist_mode = 0;
+  int order = 4, nprev = 3, n_euler = 1, hist_mode = 0;
   double dt = 0.0, skin = 0.0;
   double ab0 = 0, am0 = 0, ab[3] = {0, 0, 0}, am[3] = {0, 0, 0};
   std::vector<WallGeom> walls_h;
@@ -52,29 +51,26 @@ struct dem_system {
   CompConst* d_comp = nullptr;
   WallConst* d_wallc = nullptr;
   WallGeom* d_walls = nullptr;
-  // state
-  int np = 0, cap = 0;
+  // state: buffer cur is Particle, cur^1 Predicted (or Euler scratch)
+  int np = 0;
   bool has_state = false;
   double4* pos[2] = {nullptr, nullptr};
-  double* vw[2] = {nullptr, nullptr};
+  Kin* kin[2] = {nullptr, nullptr};
   int cur = 0;
-  double* acc = nullptr;   // Particle%Acceleration/Alpha when acc_in_ring == false
-  double* acc_p = nullptr; // Predicted%Acceleration/Alpha (optional)
+  int last_eval_buf = 0;     // buffer whose kin holds the cell state of the last evaluation
+  bool rc_ready = false;     // kin[cur] holds the cell state for the next evaluation
+  double2* acc = nullptr;    // Particle%Acceleration/Alpha when !acc_in_ring
+  double2* acc_p = nullptr;  // Predicted%Acceleration/Alpha (optional)
   double* ring = nullptr;
   double* ring2 = nullptr;
   int ipc[3] = {1, 2, 3};
   bool acc_in_ring = false;
   int* id = nullptr;
   int* id2 = nullptr;
-  int* rc[2] = {nullptr, nullptr};
-  int* st[2] = {nullptr, nullptr};
   double4* xref = nullptr;
-  int* nbr = nullptr;
-  int* nnb = nullptr;
-  int* ccnt[2] = {nullptr, nullptr};
-  int* cpart[2] = {nullptr, nullptr};
-  double* chist[2] = {nullptr, nullptr};
-  int tcur = 0;
+  EdgeSets es{};             // host copy of the two edge sets
+  EdgeSets* d_es = nullptr;  // device copy (for kernels that select by *epar)
+  size_t ecap = 0;           // edges per set
   unsigned* wflag[2] = {nullptr, nullptr};
   double* whist[2] = {nullptr, nullptr};
   // binning scratch
@@ -84,8 +80,10 @@ struct dem_system {
   int* cursor = nullptr;
   int* cell_idx = nullptr;
   int* tile_sum = nullptr;
+  int* perm_src = nullptr;   // new -> old of the last resort
   int* old_to_new = nullptr;
-  int gncell = 0, ntiles = 0;
+  int* nbr_cnt = nullptr;
+  int gncell = 0, ntiles = 0, ntiles_np = 0;
   // staging
   dem_particle_t* d_aos = nullptr;
   double* d_tmp = nullptr;
@@ -97,11 +95,11 @@ struct dem_system {
   long long eval = 0;
   long long stamp_base = 0;
   long long last_update_eval = -1;
-  bool rc_ready = false;
   long long steps = 0;
   int resort_interval = 500;
   long long steps_since_resort = 0;
   bool keep_pred_acc = false;
+  int64_t uploaded_unlisted = 0;  // uploaded pairs with no edge in the list
   bool timing = false;
   Timer timer;
   double ms_pass1 = 0, ms_pass2 = 0, ms_rebuild = 0, ms_step = 0;
@@ -142,15 +140,18 @@ static inline int gs_blocks(long long n) {
 
 static void free_state(dem_handle_t h) {
   for (int k = 0; k < 2; ++k) {
-    dfree(h->pos[k]); dfree(h->vw[k]); dfree(h->rc[k]); dfree(h->st[k]);
-    dfree(h->ccnt[k]); dfree(h->cpart[k]); dfree(h->chist[k]);
+    dfree(h->pos[k]); dfree(h->kin[k]);
+    dfree(h->es.off[k]); dfree(h->es.nbr[k]); dfree(h->es.flag[k]); dfree(h->es.hist[k]);
     dfree(h->wflag[k]); dfree(h->whist[k]);
   }
+  dfree(h->es.epar);
+  dfree(h->d_es);
   dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->id); dfree(h->id2);
-  dfree(h->xref); dfree(h->nbr); dfree(h->nnb); dfree(h->key); dfree(h->cell_count);
+  dfree(h->xref); dfree(h->key); dfree(h->cell_count);
   dfree(h->cell_start); dfree(h->cursor); dfree(h->cell_idx); dfree(h->tile_sum);
-  dfree(h->old_to_new); dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart);
-  h->np = h->cap = 0;
+  dfree(h->perm_src); dfree(h->old_to_new); dfree(h->nbr_cnt);
+  dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart);
+  h->np = 0;
   h->has_state = false;
   h->tmp_bytes = 0;
 }
@@ -183,12 +184,8 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   h->nprev = s.nprev = p->method_order - 1;
   h->n_euler = p->n_euler;
   h->dt = p->time_step;
-  s.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 64;
-  s.max_c = p->max_contacts > 0 ? p->max_contacts : 24;
-  if (s.max_c > 64) {
-    delete h;
-    return fail(nullptr, DEM_ERR_ARG, "max_contacts must be <= 64");
-  }
+  s.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 48;
+  s.max_c = 0;
   double rmax = 0.0, rmin = 1e300;
   for (int c = 0; c < nc; ++c) {
     rmax = std::max(rmax, p->radius[c]);
@@ -391,6 +388,8 @@ int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* w) {
   return DEM_OK;
 }
 
+}  // extern "C"
+
 static int ensure_tmp(dem_handle_t h, size_t bytes) {
   if (bytes <= h->tmp_bytes) return DEM_OK;
   dfree(h->d_tmp);
@@ -404,56 +403,66 @@ static int alloc_state(dem_handle_t h, int np) {
   const size_t N = (size_t)std::max(np, 1);
   const SysConst& s = h->sc;
   const size_t nring = (size_t)std::max(h->nprev, 1) * 9;
+  h->ecap = N * (size_t)s.max_nb;
   hipError_t e = hipSuccess;
   for (int k = 0; k < 2 && e == hipSuccess; ++k) {
     e = dalloc(&h->pos[k], N);
-    if (e == hipSuccess) e = dalloc(&h->vw[k], 6 * N);
-    if (e == hipSuccess) e = dalloc(&h->rc[k], N);
-    if (e == hipSuccess) e = dalloc(&h->st[k], N);
-    if (e == hipSuccess) e = dalloc(&h->ccnt[k], N);
-    if (e == hipSuccess) e = dalloc(&h->cpart[k], (size_t)s.max_c * N);
-    if (e == hipSuccess) e = dalloc(&h->chist[k], (size_t)s.max_c * 6 * N);
+    if (e == hipSuccess) e = dalloc(&h->kin[k], N);
+    if (e == hipSuccess) e = dalloc(&h->es.off[k], N + 1);
+    if (e == hipSuccess) e = dalloc(&h->es.nbr[k], h->ecap);
+    if (e == hipSuccess) e = dalloc(&h->es.flag[k], h->ecap);
+    if (e == hipSuccess) e = dalloc(&h->es.hist[k], 3 * h->ecap);
     if (e == hipSuccess) e = dalloc(&h->wflag[k], N);
     if (e == hipSuccess) e = dalloc(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
   }
-  if (e == hipSuccess) e = dalloc(&h->acc, 6 * N);
-  if (e == hipSuccess) e = dalloc(&h->acc_p, 6 * N);
+  if (e == hipSuccess) e = dalloc(&h->es.epar, 1);
+  if (e == hipSuccess) e = dalloc(&h->d_es, 1);
+  if (e == hipSuccess) e = dalloc(&h->acc, 3 * N);
+  if (e == hipSuccess) e = dalloc(&h->acc_p, 3 * N);
   if (e == hipSuccess) e = dalloc(&h->ring, nring * N);
   if (e == hipSuccess) e = dalloc(&h->ring2, nring * N);
   if (e == hipSuccess) e = dalloc(&h->id, N);
   if (e == hipSuccess) e = dalloc(&h->id2, N);
   if (e == hipSuccess) e = dalloc(&h->xref, N);
-  if (e == hipSuccess) e = dalloc(&h->nbr, (size_t)s.max_nb * N);
-  if (e == hipSuccess) e = dalloc(&h->nnb, N);
   if (e == hipSuccess) e = dalloc(&h->key, N);
   if (e == hipSuccess) e = dalloc(&h->cell_count, (size_t)h->gncell + 1);
   if (e == hipSuccess) e = dalloc(&h->cell_start, (size_t)h->gncell + 1);
   if (e == hipSuccess) e = dalloc(&h->cursor, (size_t)h->gncell + 1);
   if (e == hipSuccess) e = dalloc(&h->cell_idx, N);
   h->ntiles = (h->gncell + SCAN_TILE - 1) / SCAN_TILE;
-  if (e == hipSuccess) e = dalloc(&h->tile_sum, (size_t)h->ntiles + 1);
+  h->ntiles_np = ((int)N + SCAN_TILE - 1) / SCAN_TILE;
+  if (e == hipSuccess) e = dalloc(&h->tile_sum, (size_t)std::max(h->ntiles, h->ntiles_np) + 1);
+  if (e == hipSuccess) e = dalloc(&h->perm_src, N);
   if (e == hipSuccess) e = dalloc(&h->old_to_new, N);
+  if (e == hipSuccess) e = dalloc(&h->nbr_cnt, N);
   if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
   if (e != hipSuccess) {
     free_state(h);
     return fail(h, DEM_ERR_DEVICE, "allocation for %d particles failed: %s", np,
                 hipGetErrorString(e));
   }
-  // zero the arrays whose unused parts are read
+  HIPCHK(h, hipMemcpyAsync(h->d_es, &h->es, sizeof(EdgeSets), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->acc_p, 0, 3 * N * sizeof(double2), h->stream));
+  h->np = np;
+  h->sc.np = np;
+  return DEM_OK;
+}
+
+// Empty edge sets, no wall history, current set 0.
+static int clear_contacts(dem_handle_t h) {
+  const size_t N = (size_t)std::max(h->np, 1);
   for (int k = 0; k < 2; ++k) {
-    HIPCHK(h, hipMemsetAsync(h->ccnt[k], 0, N * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->es.off[k], 0, (N + 1) * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), h->stream));
   }
-  HIPCHK(h, hipMemsetAsync(h->acc_p, 0, 6 * N * sizeof(double), h->stream));
-  h->np = np;
-  h->cap = np;
-  h->sc.np = np;
+  HIPCHK(h, hipMemsetAsync(h->es.epar, 0, sizeof(int), h->stream));
   return DEM_OK;
 }
 
 // ---- neighbour list ---------------------------------------------------------
 
-// Enqueues the counting sort + list build, gated on *gate unless force.
+// Enqueues the counting sort of particles into the neighbour grid, gated on
+// *gate unless force.
 static void enqueue_binning(dem_handle_t h, const double4* pos, const int* gate, int force) {
   const SysConst& s = h->sc;
   hipStream_t st = h->stream;
@@ -468,61 +477,73 @@ static void enqueue_binning(dem_handle_t h, const double4* pos, const int* gate,
   hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles, gate,
                      force, h->ctl);
   hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles), dim3(DEM_BLOCK), 0, st, h->cell_count,
-                     h->gncell, h->tile_sum, h->cell_start, h->cursor, gate, force, h->ctl);
+                     h->gncell, h->tile_sum, h->cell_start, h->cursor, (const EdgeSets*)nullptr,
+                     gate, force, h->ctl);
   hipLaunchKernelGGL(k_bin_scatter, dim3(npb), dim3(DEM_BLOCK), 0, st, s.np, h->key, h->cursor,
                      h->cell_idx, gate, force, h->ctl);
   hipLaunchKernelGGL(k_cell_sort, dim3(ng), dim3(DEM_BLOCK), 0, st, h->gncell, h->cell_start,
                      h->cell_idx, gate, force, h->ctl);
 }
 
-static void enqueue_rebuild(dem_handle_t h, const int* gate, int force) {
+// Rebuild of the edge set around pos[b]: counting sort into the neighbour
+// grid, count per particle, scan into the other set's offsets, fill with the
+// history carried over, flip.  src/o2n translate indices when the particles
+// were just re-sorted (null otherwise).
+static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
+                            const int* src = nullptr, const int* o2n = nullptr) {
   const SysConst& s = h->sc;
   if (s.np == 0) return;
-  enqueue_binning(h, h->pos[h->cur], gate, force);
-  hipLaunchKernelGGL(k_nbr_build, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s,
-                     h->pos[h->cur], h->key, h->cell_start, h->cell_idx, h->nbr, h->nnb, h->xref,
+  hipStream_t st = h->stream;
+  const double4* pos = h->pos[b];
+  enqueue_binning(h, pos, gate, force);
+  const int nb = nblocks(s.np);
+  hipLaunchKernelGGL(k_nbr_count, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key,
+                     h->cell_start, h->cell_idx, h->nbr_cnt, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->nbr_cnt, s.np,
+                     h->tile_sum, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles_np,
                      gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->nbr_cnt, s.np,
+                     h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key, h->cell_start,
+                     h->cell_idx, h->es, src, o2n, (const Kin*)h->kin[b], h->xref, gate, force,
+                     h->ctl);
+  hipLaunchKernelGGL(k_nbr_flip, dim3(1), dim3(64), 0, st, h->es, gate, force, h->ctl);
 }
 
-// Re-sort the particle arrays into neighbour-grid cell order.  The cell state
-// that the next evaluation reads (rc/st of parity eval&1 when rc_ready, else
-// of the previous parity) moves with the particles.
+// Re-sort the particle arrays of Particle (cur) into neighbour-grid cell
+// order.  The cell state in kin[cur] moves with the particles; the edge set
+// is rebuilt through the permutation so the history follows.
 static int resort(dem_handle_t h) {
   SysConst& s = h->sc;
   if (s.np == 0) return DEM_OK;
+  hipStream_t st = h->stream;
   enqueue_binning(h, h->pos[h->cur], h->rb, 1);
-  hipLaunchKernelGGL(k_old_to_new, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s.np,
-                     h->cell_idx, h->old_to_new);
-  const int pc = h->rc_ready ? (int)(h->eval & 1) : (int)((h->eval + 1) & 1);
+  HIPCHK(h, hipMemcpyAsync(h->perm_src, h->cell_idx, (size_t)s.np * sizeof(int),
+                           hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_old_to_new, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, st, s.np,
+                     h->perm_src, h->old_to_new);
   PermArgs a;
-  a.src = h->cell_idx;
-  a.old_to_new = h->old_to_new;
+  a.src = h->perm_src;
   a.pos = h->pos[h->cur]; a.pos2 = h->pos[h->cur ^ 1];
-  a.vw = h->vw[h->cur]; a.vw2 = h->vw[h->cur ^ 1];
+  a.kin = h->kin[h->cur]; a.kin2 = h->kin[h->cur ^ 1];
   a.acc = h->acc; a.acc2 = h->acc_p;
   a.ring = h->ring; a.ring2 = h->ring2; a.nring = 9 * h->nprev;
   a.id = h->id; a.id2 = h->id2;
-  a.rc = h->rc[pc]; a.rc2 = h->rc[pc ^ 1];
-  a.st = h->st[pc]; a.st2 = h->st[pc ^ 1];
-  a.ccnt = h->ccnt[h->tcur]; a.ccnt2 = h->ccnt[h->tcur ^ 1];
-  a.cpart = h->cpart[h->tcur]; a.cpart2 = h->cpart[h->tcur ^ 1];
-  a.chist = h->chist[h->tcur]; a.chist2 = h->chist[h->tcur ^ 1];
   a.wflag = h->wflag[0]; a.wflag2 = h->wflag[1];
   a.whist = h->whist[0]; a.whist2 = h->whist[1];
   a.nw = s.nw;
-  hipLaunchKernelGGL(k_permute, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, h->stream, s, a);
+  hipLaunchKernelGGL(k_permute, dim3(nblocks(s.np)), dim3(DEM_BLOCK), 0, st, s, a);
   HIPCHK(h, hipGetLastError());
   h->cur ^= 1;
+  h->last_eval_buf = h->cur;
   std::swap(h->acc, h->acc_p);
   std::swap(h->ring, h->ring2);
   std::swap(h->id, h->id2);
-  std::swap(h->rc[0], h->rc[1]);
-  std::swap(h->st[0], h->st[1]);
-  h->tcur ^= 1;
   std::swap(h->wflag[0], h->wflag[1]);
   std::swap(h->whist[0], h->whist[1]);
   h->steps_since_resort = 0;
-  enqueue_rebuild(h, h->rb, 1);
+  enqueue_rebuild(h, h->cur, h->rb, 1, h->perm_src, h->old_to_new);
   HIPCHK(h, hipGetLastError());
   return DEM_OK;
 }
@@ -541,20 +562,26 @@ static int check_errors(dem_handle_t h) {
     std::string m;
     if (c.error & DEM_KERR_NBR_OVERFLOW)
       m += "neighbour list overflow (max_neighbours too small); ";
-    if (c.error & DEM_KERR_CONTACT_OVERFLOW)
-      m += "contact table overflow (max_contacts too small); ";
     if (c.error & DEM_KERR_SKIN) m += "a particle moved more than skin/2 in one step; ";
     if (c.error & DEM_KERR_ROLE) m += "overlapping pair in non-adjacent reference cells; ";
     if (c.error & DEM_KERR_CONE_AXIS) m += "particle on the axis of a cone wall; ";
     if (c.error & DEM_KERR_NAN) m += "non-finite position; ";
     int err = c.error;
     HIPCHK(h, hipMemsetAsync(&h->ctl->error, 0, sizeof(int), h->stream));
-    return fail(h, (err & (DEM_KERR_NBR_OVERFLOW | DEM_KERR_CONTACT_OVERFLOW)) ? DEM_ERR_CAPACITY
-                                                                              : DEM_ERR_STATE,
-                "%s", m.c_str());
+    return fail(h, (err & DEM_KERR_NBR_OVERFLOW) ? DEM_ERR_CAPACITY : DEM_ERR_STATE, "%s",
+                m.c_str());
   }
   return DEM_OK;
 }
+
+// Cell state of buffer b from its positions, against the last evaluation.
+static void enqueue_refcell(dem_handle_t h, int b, int wrap = 0) {
+  hipLaunchKernelGGL(k_refcell, dim3(nblocks(h->np)), dim3(DEM_BLOCK), 0, h->stream, h->sc,
+                     h->pos[b], (const Kin*)h->kin[h->last_eval_buf], h->kin[b],
+                     (int)(h->stamp_base + h->eval), wrap);
+}
+
+extern "C" {
 
 // ---- upload / download ----------------------------------------------------------
 
@@ -571,7 +598,7 @@ int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const doub
   SysConst& s = h->sc;
   s.np = np;
   h->cur = 0;
-  h->tcur = 0;
+  h->last_eval_buf = 0;
   h->acc_in_ring = false;
   for (int k = 0; k < 3; ++k) h->ipc[k] = k + 1;
   if (ipc)
@@ -580,33 +607,38 @@ int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const doub
       h->ipc[k] = ipc[k];
     }
   std::vector<double4> hp(N);
-  std::vector<double> hv(6 * N), ha(6 * N, 0.0);
-  std::vector<int> hid(N), hrank(N, 0);
+  std::vector<Kin> hk(N);
+  std::vector<double> ha(6 * N, 0.0);
+  std::vector<int> hid(N);
   int max_rank = 0;
   for (size_t i = 0; i < N; ++i) {
     if (comp[i] < 1 || comp[i] > s.nc)
       return fail(h, DEM_ERR_ARG, "particle %zu has component %d", i + 1, comp[i]);
     hp[i] = make_double4(x[i], x[N + i], x[2 * N + i], (double)(comp[i] - 1));
+    Kin& k = hk[i];
+    k.v0 = make_double2(v[i], v[N + i]);
+    k.v1 = make_double2(v[2 * N + i], w[i]);
+    k.v2 = make_double2(w[N + i], w[2 * N + i]);
+    k.id = (int)i;
+    k.rc = 0;
+    k.st = cell_rank ? cell_rank[i] : 0;
+    k.pad = 0;
     for (int d = 0; d < 3; ++d) {
-      hv[6 * i + d] = v[d * N + i];
-      hv[6 * i + 3 + d] = w[d * N + i];
       if (a) ha[6 * i + d] = a[d * N + i];
       if (alpha) ha[6 * i + 3 + d] = alpha[d * N + i];
     }
     hid[i] = (int)i;
-    if (cell_rank) {
-      hrank[i] = cell_rank[i];
-      max_rank = std::max(max_rank, cell_rank[i]);
-    }
+    if (cell_rank) max_rank = std::max(max_rank, cell_rank[i]);
   }
-  if (np == 0) {
-    h->has_state = true;
-    return DEM_OK;
-  }
+  h->has_state = true;
+  h->last_update_eval = -1;
+  h->rc_ready = false;
+  if (np == 0) return DEM_OK;
   hipStream_t st = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->pos[0], hp.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
-  HIPCHK(h, hipMemcpyAsync(h->vw[0], hv.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, st));
-  HIPCHK(h, hipMemcpyAsync(h->acc, ha.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync(h->kin[0], hk.data(), N * sizeof(Kin), hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync((double*)h->acc, ha.data(), 6 * N * sizeof(double),
+                           hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->id, hid.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
   if (h->nprev > 0) {
     if (ring)
@@ -615,26 +647,24 @@ int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const doub
     else
       HIPCHK(h, hipMemsetAsync(h->ring, 0, (size_t)h->nprev * 9 * N * sizeof(double), st));
   }
-  for (int k = 0; k < 2; ++k) {
-    HIPCHK(h, hipMemsetAsync(h->ccnt[k], 0, N * sizeof(int), st));
-    HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), st));
+  {
+    int rc = clear_contacts(h);
+    if (rc) return rc;
   }
   HIPCHK(h, hipMemsetAsync(h->ctl, 0, sizeof(Ctl), st));
   HIPCHK(h, hipMemsetAsync(h->rb, 0, 2 * sizeof(int), st));
-  // cell state of "the last evaluation" = the uploaded lists
-  // (DistributeParticles3D, src/MCells.f90:542-564, or the given order)
-  const int q = (int)((h->eval + 1) & 1);
-  HIPCHK(h, hipMemcpyAsync(h->st[q], hrank.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_refcell, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, s, h->pos[0], h->rc[q],
-                     h->st[q], h->rc[q], h->st[q], 0, 1);
+  // cell state of the uploaded lists, as DistributeParticles3D
+  // (src/MCells.f90:542-564) or the given order leaves them: CellIndex from
+  // the positions, entry stamps = the ranks (a later transfer stamps larger)
   h->stamp_base = (long long)max_rank + 1 - h->eval;
-  h->rc_ready = false;
-  h->has_state = true;
-  h->last_update_eval = -1;
+  hipLaunchKernelGGL(k_refcell, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, s, h->pos[0],
+                     (const Kin*)h->kin[0], h->kin[0], 0, -1);
   int rc = resort(h);
   if (rc) return rc;
   return check_errors(h);
 }
+
+}  // extern "C"
 
 // out[c*np + ref] for c in [0, ncomp): element c of record i at
 // in[i*stride_rec + c*stride_comp].
@@ -651,18 +681,63 @@ static int gather_ref(dem_handle_t h, const double* in, int ncomp, int stride_re
   return DEM_OK;
 }
 
-static int acc_source(dem_handle_t h, const double** p, int* rec, int* cs) {
+static void acc_source(dem_handle_t h, const double** p, int* rec, int* cs) {
   if (h->acc_in_ring && h->nprev > 0) {
     *p = h->ring + (size_t)(h->ipc[0] - 1) * 9 * h->np + 3 * (size_t)h->np;
     *rec = 1;
     *cs = h->np;
   } else {
-    *p = h->acc;
+    *p = (const double*)h->acc;
     *rec = 6;
     *cs = 1;
   }
+}
+
+static int download_ids(dem_handle_t h, std::vector<int>& idh) {
+  idh.resize(h->np);
+  if (h->np == 0) return DEM_OK;
+  HIPCHK(h, hipMemcpyAsync(idh.data(), h->id, (size_t)h->np * sizeof(int), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return DEM_OK;
 }
+
+// The current edge set on the host: offsets, neighbours, flags, history.
+struct HostEdges {
+  int par = 0;
+  std::vector<int> off, nbr;
+  std::vector<unsigned char> flag;
+  std::vector<double> hist;  // 6 per edge
+};
+
+static int download_edges(dem_handle_t h, HostEdges& he, bool with_hist) {
+  const size_t N = (size_t)h->np;
+  hipStream_t st = h->stream;
+  HIPCHK(h, hipMemcpyAsync(&he.par, h->es.epar, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  he.off.resize(N + 1);
+  HIPCHK(h, hipMemcpyAsync(he.off.data(), h->es.off[he.par], (N + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  const size_t E = (size_t)he.off[N];
+  he.nbr.resize(E);
+  he.flag.resize(E);
+  if (E) {
+    HIPCHK(h, hipMemcpyAsync(he.nbr.data(), h->es.nbr[he.par], E * sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(he.flag.data(), h->es.flag[he.par], E, hipMemcpyDeviceToHost, st));
+  }
+  if (with_hist) {
+    he.hist.resize(6 * E);
+    if (E)
+      HIPCHK(h, hipMemcpyAsync(he.hist.data(), (const double*)h->es.hist[he.par],
+                               6 * E * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(h, hipStreamSynchronize(st));
+  return DEM_OK;
+}
+
+extern "C" {
 
 int dem_download_state(dem_handle_t h, int32_t np, int32_t* comp, double* x, double* v, double* w,
                        double* a, double* alpha, double* ring, int32_t* ipc) {
@@ -678,7 +753,7 @@ int dem_download_state(dem_handle_t h, int32_t np, int32_t* comp, double* x, dou
       for (size_t i = 0; i < N; ++i) comp[i] = (int)tmp[3 * N + i] + 1;
     std::vector<double> t6(6 * N);
     if (v || w) {
-      rc = gather_ref(h, h->vw[h->cur], 6, 6, 1, t6.data());
+      rc = gather_ref(h, (const double*)h->kin[h->cur], 6, 8, 1, t6.data());
       if (rc) return rc;
       if (v) memcpy(v, t6.data(), 3 * N * sizeof(double));
       if (w) memcpy(w, t6.data() + 3 * N, 3 * N * sizeof(double));
@@ -714,12 +789,12 @@ int dem_download_predicted(dem_handle_t h, int32_t np, double* x, double* v, dou
   int rc = gather_ref(h, (const double*)h->pos[h->cur ^ 1], 3, 4, 1, tmp.data());
   if (rc) return rc;
   if (x) memcpy(x, tmp.data(), 3 * N * sizeof(double));
-  rc = gather_ref(h, h->vw[h->cur ^ 1], 6, 6, 1, tmp.data());
+  rc = gather_ref(h, (const double*)h->kin[h->cur ^ 1], 6, 8, 1, tmp.data());
   if (rc) return rc;
   if (v) memcpy(v, tmp.data(), 3 * N * sizeof(double));
   if (w) memcpy(w, tmp.data() + 3 * N, 3 * N * sizeof(double));
   if (a || alpha) {
-    rc = gather_ref(h, h->acc_p, 6, 6, 1, tmp.data());
+    rc = gather_ref(h, (const double*)h->acc_p, 6, 6, 1, tmp.data());
     if (rc) return rc;
     if (a) memcpy(a, tmp.data(), 3 * N * sizeof(double));
     if (alpha) memcpy(alpha, tmp.data() + 3 * N, 3 * N * sizeof(double));
@@ -732,21 +807,11 @@ int dem_download_cells(dem_handle_t h, int32_t np, int32_t* cell_index) {
   if (np != h->np) return fail(h, DEM_ERR_ARG, "np mismatch");
   if (np == 0) return DEM_OK;
   // CellIndex after the last evaluation
-  const int q = (int)((h->eval + 1) & 1);
   int rc = ensure_tmp(h, (size_t)np * sizeof(int));
   if (rc) return rc;
-  hipLaunchKernelGGL(k_to_ref_i, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, h->stream, np, h->rc[q],
-                     h->id, (int*)h->d_tmp, 1);
+  hipLaunchKernelGGL(k_cells_to_ref, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, h->stream, np,
+                     (const Kin*)h->kin[h->last_eval_buf], h->id, (int*)h->d_tmp);
   HIPCHK(h, hipMemcpyAsync(cell_index, h->d_tmp, (size_t)np * sizeof(int), hipMemcpyDeviceToHost,
-                           h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return DEM_OK;
-}
-
-static int download_ids(dem_handle_t h, std::vector<int>& idh) {
-  idh.resize(h->np);
-  if (h->np == 0) return DEM_OK;
-  HIPCHK(h, hipMemcpyAsync(idh.data(), h->id, (size_t)h->np * sizeof(int), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return DEM_OK;
@@ -761,32 +826,42 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
   std::vector<int> idh;
   int rc = download_ids(h, idh);
   if (rc) return rc;
-  std::vector<int> inv(N);
-  for (size_t i = 0; i < N; ++i) inv[idh[i]] = (int)i;
-  std::vector<int> cnt(N, 0), part((size_t)s.max_c * N, 0);
-  std::vector<double> hist((size_t)s.max_c * 6 * N, 0.0);
-  auto put = [&](int p, int q, const double* f, const double* g) -> bool {
-    int c = cnt[p];
-    if (c >= s.max_c) return false;
-    part[(size_t)c * N + p] = q;
-    for (int d = 0; d < 3; ++d) {
-      hist[((size_t)c * 6 + d) * N + p] = f[d];
-      hist[((size_t)c * 6 + 3 + d) * N + p] = g[d];
-    }
-    cnt[p] = c + 1;
-    return true;
-  };
+  HostEdges he;
+  rc = download_edges(h, he, false);
+  if (rc) return rc;
+  const size_t E = he.nbr.size();
+  std::vector<unsigned char> flag(E, 0);
+  std::vector<double> hist(6 * E, 0.0);
+  std::vector<std::pair<uint64_t, int64_t>> kv((size_t)std::max<int64_t>(npairs, 0));
   for (int64_t k = 0; k < npairs; ++k) {
     const long long a = (long long)(keys[k] >> 32) - 1, b = (long long)(keys[k] & 0xffffffffu) - 1;
     if (a < 0 || b <= a || b >= (long long)N)
       return fail(h, DEM_ERR_ARG, "bad pair key at %lld", (long long)k);
-    const int p = inv[a], q = inv[b];
-    if (!put(p, q, ft + 3 * k, vt + 3 * k) || !put(q, p, ft + 3 * k, vt + 3 * k))
-      return fail(h, DEM_ERR_CAPACITY, "more than max_contacts=%d contacts on one particle",
-                  s.max_c);
+    kv[k] = {keys[k], k};
   }
+  std::sort(kv.begin(), kv.end());
+  size_t used = 0;
+  for (size_t i = 0; i < N; ++i) {
+    for (int e = he.off[i]; e < he.off[i + 1]; ++e) {
+      const int a = idh[i], b = idh[he.nbr[e]];
+      const uint64_t key = ((uint64_t)(std::min(a, b) + 1) << 32) | (uint64_t)(std::max(a, b) + 1);
+      auto it = std::lower_bound(kv.begin(), kv.end(), std::make_pair(key, (int64_t)-1));
+      if (it == kv.end() || it->first != key) continue;
+      const int64_t k = it->second;
+      flag[e] = 1;
+      for (int d = 0; d < 3; ++d) {
+        hist[6 * (size_t)e + d] = ft[3 * k + d];
+        hist[6 * (size_t)e + 3 + d] = vt[3 * k + d];
+      }
+      ++used;
+    }
+  }
+  // A stored pair farther apart than r_i + r_j + skin has no edge; it is
+  // dropped (counted in uploaded_unlisted).
   std::vector<unsigned> wf(N, 0u);
   std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N, 0.0);
+  std::vector<int> inv(N);
+  for (size_t i = 0; i < N; ++i) inv[idh[i]] = (int)i;
   for (int64_t k = 0; k < nwall; ++k) {
     const long long a = (long long)(wkeys[k] >> 32) - 1, jw = (long long)(wkeys[k] & 0xffffffffu) - 1;
     if (a < 0 || a >= (long long)N || jw < 0 || jw >= s.nw)
@@ -799,13 +874,12 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
     }
   }
   hipStream_t st = h->stream;
-  const int t = h->tcur;
-  if (N > 0) {
-    HIPCHK(h, hipMemcpyAsync(h->ccnt[t], cnt.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(h->cpart[t], part.data(), part.size() * sizeof(int),
+  if (E) {
+    HIPCHK(h, hipMemcpyAsync(h->es.flag[he.par], flag.data(), E, hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync((double*)h->es.hist[he.par], hist.data(), 6 * E * sizeof(double),
                              hipMemcpyHostToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(h->chist[t], hist.data(), hist.size() * sizeof(double),
-                             hipMemcpyHostToDevice, st));
+  }
+  if (N) {
     HIPCHK(h, hipMemcpyAsync(h->wflag[0], wf.data(), N * sizeof(unsigned), hipMemcpyHostToDevice,
                              st));
     if (s.nw > 0)
@@ -813,6 +887,7 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
                                hipMemcpyHostToDevice, st));
   }
   HIPCHK(h, hipStreamSynchronize(st));
+  h->uploaded_unlisted = (int64_t)npairs - (int64_t)(used / 2);
   return DEM_OK;
 }
 
@@ -825,18 +900,13 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
   std::vector<int> idh;
   int rc = download_ids(h, idh);
   if (rc) return rc;
-  std::vector<int> cnt(N), part((size_t)s.max_c * N);
-  std::vector<double> hist((size_t)s.max_c * 6 * N);
+  HostEdges he;
+  rc = download_edges(h, he, true);
+  if (rc) return rc;
   std::vector<unsigned> wf(N);
   std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N);
   hipStream_t st = h->stream;
-  const int t = h->tcur;
   if (N > 0) {
-    HIPCHK(h, hipMemcpyAsync(cnt.data(), h->ccnt[t], N * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(part.data(), h->cpart[t], part.size() * sizeof(int),
-                             hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(hist.data(), h->chist[t], hist.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipMemcpyAsync(wf.data(), h->wflag[0], N * sizeof(unsigned), hipMemcpyDeviceToHost,
                              st));
     if (s.nw > 0)
@@ -844,37 +914,37 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
                                hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
   }
-  struct E {
+  struct Ent {
     uint64_t key;
     double f[3], g[3];
   };
-  std::vector<E> pe, we;
+  std::vector<Ent> pe, we;
   for (size_t i = 0; i < N; ++i) {
     const int a = idh[i];
-    for (int c = 0; c < cnt[i]; ++c) {
-      const int q = part[(size_t)c * N + i];
-      const int b = idh[q];
-      if (a >= b) continue;  // each pair is stored on both particles
-      E e;
-      e.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(b + 1);
+    for (int e = he.off[i]; e < he.off[i + 1]; ++e) {
+      if (!(he.flag[e] & 1)) continue;
+      const int b = idh[he.nbr[e]];
+      if (a >= b) continue;  // each pair is stored on both of its edges
+      Ent x;
+      x.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(b + 1);
       for (int d = 0; d < 3; ++d) {
-        e.f[d] = hist[((size_t)c * 6 + d) * N + i];
-        e.g[d] = hist[((size_t)c * 6 + 3 + d) * N + i];
+        x.f[d] = he.hist[6 * (size_t)e + d];
+        x.g[d] = he.hist[6 * (size_t)e + 3 + d];
       }
-      pe.push_back(e);
+      pe.push_back(x);
     }
     for (int jw = 0; jw < s.nw; ++jw) {
       if (!((wf[i] >> jw) & 1u)) continue;
-      E e;
-      e.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(jw + 1);
+      Ent x;
+      x.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(jw + 1);
       for (int d = 0; d < 3; ++d) {
-        e.f[d] = wh[((size_t)jw * 6 + d) * N + i];
-        e.g[d] = wh[((size_t)jw * 6 + 3 + d) * N + i];
+        x.f[d] = wh[((size_t)jw * 6 + d) * N + i];
+        x.g[d] = wh[((size_t)jw * 6 + 3 + d) * N + i];
       }
-      we.push_back(e);
+      we.push_back(x);
     }
   }
-  auto by_key = [](const E& x, const E& y) { return x.key < y.key; };
+  auto by_key = [](const Ent& x, const Ent& y) { return x.key < y.key; };
   std::sort(pe.begin(), pe.end(), by_key);
   std::sort(we.begin(), we.end(), by_key);
   if (npairs) *npairs = (int64_t)pe.size();
@@ -894,59 +964,33 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
   return DEM_OK;
 }
 
+}  // extern "C"
+
 // ---- evaluation ------------------------------------------------------------------
 
-struct EvalCfg {
-  const double4* pos_in;
-  const double* vw_in;
-};
-
-static PassArgs base_args(dem_handle_t h, bool update) {
+static PassArgs base_args(dem_handle_t h) {
   PassArgs a{};
   const int p = (int)(h->eval & 1);
-  a.id = h->id;
-  a.rc_in = h->rc[p];
-  a.st_in = h->st[p];
-  a.rc_out = h->rc[p ^ 1];
-  a.st_out = h->st[p ^ 1];
   a.stamp_next = (int)(h->stamp_base + h->eval + 1);
   a.par = p;
   a.xref = h->xref;
-  a.nbr = h->nbr;
-  a.nnb = h->nnb;
-  a.ccnt_in = h->ccnt[h->tcur];
-  a.cpart_in = h->cpart[h->tcur];
-  a.chist_in = h->chist[h->tcur];
-  a.ccnt_out = h->ccnt[h->tcur ^ 1];
-  a.cpart_out = h->cpart[h->tcur ^ 1];
-  a.chist_out = h->chist[h->tcur ^ 1];
+  a.E = h->es;
   a.wflag = h->wflag[0];
   a.whist = h->whist[0];
   a.rb_clear = h->rb + p;
   a.rb_set = h->rb + (p ^ 1);
   a.ctl = h->ctl;
   a.half_dt = 0.5 * h->dt;
-  (void)update;
   return a;
 }
 
-static void after_eval(dem_handle_t h, bool update) {
-  if (update) {
-    h->tcur ^= 1;
-    h->last_update_eval = h->eval;
-  }
+static void after_eval(dem_handle_t h, bool update, int b) {
+  if (update) h->last_update_eval = h->eval;
+  h->last_eval_buf = b;
   h->eval += 1;
 }
 
-// rc/st for the positions about to be evaluated (eval parity), from the cell
-// state of the previous evaluation.
-static void enqueue_refcell(dem_handle_t h) {
-  const int p = (int)(h->eval & 1);
-  hipLaunchKernelGGL(k_refcell, dim3(nblocks(h->np)), dim3(DEM_BLOCK), 0, h->stream, h->sc,
-                     h->pos[h->cur], h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
-                     (int)(h->stamp_base + h->eval), 0);
-  h->rc_ready = true;
-}
+extern "C" {
 
 int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p, double time_step,
                                   int32_t update) {
@@ -957,16 +1001,17 @@ int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p,
   hipStream_t st = h->stream;
   if (!h->d_aos) HIPCHK(h, hipMalloc((void**)&h->d_aos, N * sizeof(dem_particle_t)));
   HIPCHK(h, hipMemcpyAsync(h->d_aos, p, N * sizeof(dem_particle_t), hipMemcpyHostToDevice, st));
+  const int b = h->cur;
   hipLaunchKernelGGL(k_aos_in, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->d_aos, h->id,
-                     h->pos[h->cur], h->vw[h->cur]);
-  enqueue_refcell(h);
+                     h->pos[b], h->kin[b]);
+  enqueue_refcell(h, b);
   const int par = (int)(h->eval & 1);
-  hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc,
-                     h->pos[h->cur], h->xref, h->rb + par);
-  enqueue_rebuild(h, h->rb + par, 0);
-  PassArgs a = base_args(h, update != 0);
-  a.pos_in = h->pos[h->cur];
-  a.vw_in = h->vw[h->cur];
+  hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[b],
+                     h->xref, h->rb + par);
+  enqueue_rebuild(h, b, h->rb + par, 0);
+  PassArgs a = base_args(h);
+  a.pos_in = h->pos[b];
+  a.kin_in = h->kin[b];
   a.acc_out = h->acc;
   a.half_dt = 0.5 * time_step;
   if (update)
@@ -974,11 +1019,11 @@ int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p,
   else
     hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
   HIPCHK(h, hipGetLastError());
-  after_eval(h, update != 0);
+  after_eval(h, update != 0, b);
   h->rc_ready = false;
   h->acc_in_ring = false;
   hipLaunchKernelGGL(k_aos_out, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->d_aos, h->id,
-                     h->acc);
+                     (const double2*)h->acc);
   HIPCHK(h, hipMemcpyAsync(p, h->d_aos, N * sizeof(dem_particle_t), hipMemcpyDeviceToHost, st));
   return check_errors(h);
 }
@@ -988,107 +1033,65 @@ int dem_compute_accelerations(dem_handle_t h, int32_t which, int32_t update) {
   if (h->np == 0) return DEM_OK;
   const int np = h->np;
   hipStream_t st = h->stream;
-  const int c = which ? (h->cur ^ 1) : h->cur;
-  {
-    const int p = (int)(h->eval & 1);
-    hipLaunchKernelGGL(k_refcell, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[c],
-                       h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
-                       (int)(h->stamp_base + h->eval), 0);
-    hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[c],
-                       h->xref, h->rb + p);
-    if (c == h->cur) enqueue_rebuild(h, h->rb + p, 0);
-    else {
-      // rebuild around the evaluated (predicted) positions
-      std::swap(h->pos[0], h->pos[1]);
-      std::swap(h->vw[0], h->vw[1]);
-      enqueue_rebuild(h, h->rb + p, 0);
-      std::swap(h->pos[0], h->pos[1]);
-      std::swap(h->vw[0], h->vw[1]);
-    }
-  }
-  PassArgs a = base_args(h, update != 0);
-  a.pos_in = h->pos[c];
-  a.vw_in = h->vw[c];
+  const int b = which ? (h->cur ^ 1) : h->cur;
+  enqueue_refcell(h, b);
+  const int par = (int)(h->eval & 1);
+  hipLaunchKernelGGL(k_check_disp, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[b],
+                     h->xref, h->rb + par);
+  enqueue_rebuild(h, b, h->rb + par, 0);
+  PassArgs a = base_args(h);
+  a.pos_in = h->pos[b];
+  a.kin_in = h->kin[b];
   a.acc_out = which ? h->acc_p : h->acc;
   if (update)
     hipLaunchKernelGGL((k_pass<MODE_ACC, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
   else
     hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
   HIPCHK(h, hipGetLastError());
-  after_eval(h, update != 0);
+  after_eval(h, update != 0, b);
   h->rc_ready = false;
   if (!which) h->acc_in_ring = false;
   return check_errors(h);
 }
 
-__global__ void k_store_ring(int np, const double* vw, const double* acc, double* slot) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const size_t N = (size_t)np;
-  for (int d = 0; d < 3; ++d) {
-    slot[d * N + i] = vw[6 * (size_t)i + d];
-    slot[(3 + d) * N + i] = acc[6 * (size_t)i + d];
-    slot[(6 + d) * N + i] = acc[6 * (size_t)i + 3 + d];
-  }
-}
-
-// ApplyPeriodicBoundaryConditions (src/MDEM.f90:373-388) + the cell state of
-// the wrapped positions for the next evaluation.
-__global__ void k_wrap(SysConst s, double4* pos, const int* rc_prev, const int* st_prev,
-                       int* rc_out, int* st_out, int stamp_next) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.np) return;
-  double4 P = pos[i];
-  if (s.pbc[0]) P.x = P.x - s.lbox[0] * fnint(P.x * s.invl[0]);
-  if (s.pbc[1]) P.y = P.y - s.lbox[1] * fnint(P.y * s.invl[1]);
-  if (s.pbc[2]) P.z = P.z - s.lbox[2] * fnint(P.z * s.invl[2]);
-  pos[i] = P;
-  const int rc = ref_cell(s, P.x, P.y, P.z);
-  rc_out[i] = rc;
-  st_out[i] = (rc != rc_prev[i]) ? stamp_next : st_prev[i];
-}
-
 int dem_euler_startup(dem_handle_t h) {
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
   if (h->nprev == 0) return fail(h, DEM_ERR_STATE, "Euler start-up needs MethodOrder > 1");
+  if (h->acc_in_ring)
+    return fail(h, DEM_ERR_STATE, "Euler start-up after MoveParticles is not supported");
   if (h->np == 0) return DEM_OK;
   const int np = h->np;
   hipStream_t st = h->stream;
-  if (h->acc_in_ring)
-    return fail(h, DEM_ERR_STATE, "Euler start-up after MoveParticles is not supported");
   // ipc = cshift(ipc, -1); Previous(ipc(1)) = (v, a, alpha)  (src/MDEM.f90:266-271)
   {
     const int last = h->ipc[h->nprev - 1];
     for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
     h->ipc[0] = last;
   }
-  hipLaunchKernelGGL(k_store_ring, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np, h->vw[h->cur],
-                     h->acc, h->ring + (size_t)(h->ipc[0] - 1) * 9 * np);
-  if (!h->rc_ready) enqueue_refcell(h);
+  hipLaunchKernelGGL(k_store_ring, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, np,
+                     (const Kin*)h->kin[h->cur], (const double2*)h->acc,
+                     h->ring + (size_t)(h->ipc[0] - 1) * 9 * np);
+  if (!h->rc_ready) enqueue_refcell(h, h->cur);
   const double hs = h->dt / (double)h->n_euler;
   for (int sub = 0; sub < h->n_euler; ++sub) {
-    const int p = (int)(h->eval & 1);
-    PassArgs a = base_args(h, true);
+    PassArgs a = base_args(h);
     a.pos_in = h->pos[h->cur];
-    a.vw_in = h->vw[h->cur];
+    a.kin_in = h->kin[h->cur];
     a.pos_out = h->pos[h->cur ^ 1];
-    a.vw_out = h->vw[h->cur ^ 1];
+    a.kin_out = h->kin[h->cur ^ 1];
     a.acc_out = h->acc;
     a.h = hs;
     a.hsq2 = 0.5 * (hs * hs);
     a.half_dt = 0.5 * hs;
     hipLaunchKernelGGL((k_pass<MODE_EULER, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st,
                        h->sc, a);
-    after_eval(h, true);
+    after_eval(h, true, h->cur);
     h->cur ^= 1;
-    (void)p;
-    enqueue_rebuild(h, h->rb + (h->eval & 1), 0);
+    enqueue_rebuild(h, h->cur, h->rb + (h->eval & 1), 0);
   }
-  // PBC after the sub-steps (src/MDEM.f90:279)
-  const int p = (int)(h->eval & 1);
-  hipLaunchKernelGGL(k_wrap, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, h->pos[h->cur],
-                     h->rc[p ^ 1], h->st[p ^ 1], h->rc[p], h->st[p],
-                     (int)(h->stamp_base + h->eval));
+  // PBC after the sub-steps (src/MDEM.f90:279), then the cell state of the
+  // wrapped positions against the last sub-step's
+  enqueue_refcell(h, h->cur, 1);
   HIPCHK(h, hipGetLastError());
   h->rc_ready = true;
   h->acc_in_ring = false;
@@ -1110,7 +1113,10 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     HIPCHK(h, hipMemcpyAsync(&h->ctl->stop_step, &big, sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(h, hipMemsetAsync(&h->ctl->escaped, 0, sizeof(int), st));
   }
-  if (!h->rc_ready) enqueue_refcell(h);
+  if (!h->rc_ready) {
+    enqueue_refcell(h, h->cur);
+    h->rc_ready = true;
+  }
   const int nb = nblocks(np);
   if (h->timing) h->timer.ensure((size_t)6 * nsteps + 2);
   size_t ev = 0;
@@ -1122,11 +1128,11 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
       if (rc) return rc;
     }
     // pass 1: forces at x_n (Update), AB predictor, ring store
-    PassArgs a = base_args(h, true);
+    PassArgs a = base_args(h);
     a.pos_in = h->pos[h->cur];
-    a.vw_in = h->vw[h->cur];
+    a.kin_in = h->kin[h->cur];
     a.pos_out = h->pos[h->cur ^ 1];
-    a.vw_out = h->vw[h->cur ^ 1];
+    a.kin_out = h->kin[h->cur ^ 1];
     a.acc_out = h->acc;
     a.coef0 = h->ab0;
     for (int k = 0; k < h->nprev; ++k) {
@@ -1138,18 +1144,18 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     hipLaunchKernelGGL((k_pass<MODE_PRED, true>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, a);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    after_eval(h, true);
+    after_eval(h, true, h->cur);
     if (h->nprev > 0) {
       const int last = h->ipc[h->nprev - 1];
       for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
       h->ipc[0] = last;
     }
     // pass 2: forces at the predicted state, AM corrector, look-ahead
-    PassArgs b = base_args(h, false);
+    PassArgs b = base_args(h);
     b.pos_in = h->pos[h->cur ^ 1];
-    b.vw_in = h->vw[h->cur ^ 1];
+    b.kin_in = h->kin[h->cur ^ 1];
     b.pos = h->pos[h->cur];
-    b.vw = h->vw[h->cur];
+    b.kin = h->kin[h->cur];
     b.acc_out = h->keep_pred_acc ? h->acc_p : nullptr;
     b.coef0 = h->am0;
     for (int k = 0; k < h->nprev; ++k) {
@@ -1162,13 +1168,12 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     hipLaunchKernelGGL((k_pass<MODE_CORR, false>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, b);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    after_eval(h, false);
+    after_eval(h, false, h->cur ^ 1);
     h->acc_in_ring = h->nprev > 0;
     // neighbour list, when pass 2 asked for it
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    enqueue_rebuild(h, h->rb + (h->eval & 1), 0);
+    enqueue_rebuild(h, h->cur, h->rb + (h->eval & 1), 0);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    h->rc_ready = true;
     h->steps_since_resort += 1;
     first = false;
   }
@@ -1230,10 +1235,11 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->steps = h->steps;
   out->rebuilds = (int64_t)c.rebuilds;
   out->neighbour_pairs = (int64_t)c.nbpairs;
+  out->stale_dropped = (int64_t)c.stale_dropped + h->uploaded_unlisted;
   if (h->np > 0) {
     const int nb = nblocks(h->np);
     hipLaunchKernelGGL(k_energy, dim3(nb), dim3(DEM_BLOCK), 0, h->stream, h->sc, h->pos[h->cur],
-                       h->vw[h->cur], h->d_epart);
+                       (const Kin*)h->kin[h->cur], h->d_epart);
     std::vector<double> part(nb);
     HIPCHK(h, hipMemcpyAsync(part.data(), h->d_epart, nb * sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));

@@ -1,23 +1,68 @@
 // kernels.hpp -- gfx950 kernels of the DEM per-timestep path.
 //
-// Layout in HBM (internal particle order = cell order of the last resort):
-//   pos   double4[N]   x, y, z, component (0-based, as a double)
-//   vw    double[6N]   48-byte records vx vy vz wx wy wz
-//   ring  double[nprev][9][N]  ABM ring Previous(:,i) (src/MDEM.f90:14-24)
-//   nbr   int[max_nb][N]       directed neighbour list, slot-major (ELL)
-//   contact table (two copies, swapped by every Update evaluation):
-//         cnt int[N], part int[max_c][N], hist double[max_c][6][N]
-//   wall history: wflag uint[N] (bit w = contact with wall w), whist [nw][6][N]
-// A wavefront owns 64 consecutive particles; every particle accumulates its
-// own force and torque in registers (the reference's scattered Force(j,:)
-// updates, src/MForces.f90:256-259, become a per-particle sum over its
-// directed list: atomic-free and run-to-run deterministic).
+// Layout in HBM (internal particle order = neighbour-grid cell order of the
+// last resort).  Two state buffers b = 0, 1 hold Particle and Predicted
+// (src/MDEM.f90:20); which one is Particle is host state:
+//   pos[b]  double4[N]  x, y, z, component (0-based, as a double)
+//   kin[b]  Kin[N]      64-byte line: velocity, angular velocity, reference id,
+//                       and the reference-grid cell state (CellIndex and the
+//                       cell-list entry stamp) of the positions in pos[b]
+//   ring    double[nprev][9][N]  ABM ring Previous(:,i) (src/MDEM.f90:14-24)
+//   edges (two sets, selected by the device word *epar; a rebuild writes the
+//         other set and flips it):
+//         off  int[N+1]    CSR offsets of each particle's directed list
+//         nbr  int[E]      neighbour j of edge e
+//         flag u8[E]       bit 0: the pair has stored history
+//         hist double2[3E] tangential force FT | velocity VT of the pair as the
+//                          reference stores them (TPartner%Vector1/Vector2)
+//   wall history: wflag uint[N] (bit w: contact with wall w), whist [nw][6][N]
+//
+// A force pass runs one 256-lane workgroup per 256 consecutive particles.
+// The block stages its particles in LDS, then walks their edges 256 at a
+// time, one lane per edge (independent gathers, full wavefronts), writes each
+// contact's normal force, tangential force and torque to LDS, and every
+// particle lane sums its own edges in edge order: an atomic-free segmented
+// reduction, bitwise reproducible.  The same lanes then apply the integrator
+// to their particle.
 #pragma once
 #pragma clang fp contract(off)
 
 #include "physics.hpp"
 
 enum { MODE_ACC = 0, MODE_EULER = 1, MODE_PRED = 2, MODE_CORR = 3 };
+
+struct __align__(64) Kin {
+  double2 v0, v1, v2;  // vx vy | vz wx | wy wz
+  int id;              // reference id, 0-based
+  int rc;              // reference-grid cell (0-based) of the positions
+  int st;              // entry stamp into that cell's list
+  int pad;
+};
+
+__device__ __forceinline__ Kin load_kin(const Kin* k) {
+  const int4* q = reinterpret_cast<const int4*>(k);
+  int4 a = q[0], b = q[1], c = q[2], d = q[3];
+  Kin r;
+  r.v0 = make_double2(__hiloint2double(a.y, a.x), __hiloint2double(a.w, a.z));
+  r.v1 = make_double2(__hiloint2double(b.y, b.x), __hiloint2double(b.w, b.z));
+  r.v2 = make_double2(__hiloint2double(c.y, c.x), __hiloint2double(c.w, c.z));
+  r.id = d.x;
+  r.rc = d.y;
+  r.st = d.z;
+  r.pad = d.w;
+  return r;
+}
+
+__device__ __forceinline__ void store_kin(Kin* k, d3 v, d3 w, int id, int rc, int st) {
+  double2* q = reinterpret_cast<double2*>(k);
+  q[0] = make_double2(v.x, v.y);
+  q[1] = make_double2(v.z, w.x);
+  q[2] = make_double2(w.y, w.z);
+  reinterpret_cast<int4*>(k)[3] = make_int4(id, rc, st, 0);
+}
+
+__device__ __forceinline__ d3 kin_v(const Kin& k) { return mk3(k.v0.x, k.v0.y, k.v1.x); }
+__device__ __forceinline__ d3 kin_w(const Kin& k) { return mk3(k.v1.y, k.v2.x, k.v2.y); }
 
 // Blocks are dispatched round-robin over the 8 XCDs; map consecutive block
 // ranges to one XCD so a spatial slab of particles shares one L2.
@@ -40,41 +85,37 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+struct EdgeSets {
+  int* off[2];
+  int* nbr[2];
+  unsigned char* flag[2];
+  double2* hist[2];
+  int* epar;  // device word: index of the current set
+};
+
 struct PassArgs {
-  // evaluated positions / velocities (Particle or Predicted)
+  // evaluated state (Particle or Predicted): positions, kin with cell state
   const double4* pos_in;
-  const double* vw_in;
-  // Particle state (read by PRED/CORR/EULER; written by CORR; EULER writes *_out)
+  const Kin* kin_in;
+  // Particle state (CORR reads and rewrites it in place)
   double4* pos;
-  double* vw;
-  double4* pos_out;     // PRED: predicted; EULER: next Particle buffer
-  double* vw_out;
-  double* acc_out;      // ACC/EULER: a, alpha of the evaluated set; CORR: predicted accel (optional)
-  double* ring;
+  Kin* kin;
+  // PRED: Predicted; EULER: the next Particle buffer
+  double4* pos_out;
+  Kin* kin_out;
+  double2* acc_out;     // ACC/EULER/PRED(order 1): a, alpha; CORR: predicted (optional)
   const double* slot_rd[3];  // ring slots in ipc order (k = 1..nprev)
   double* slot_wr;           // PRED: slot that becomes ipc(1)
   double coef0, coef[3];     // AB0/AB(k) (PRED), AM0/AM(k) (CORR)
   double ab0, ab[3];         // CORR: AB coefficients for the look-ahead predictor
   double h, hsq2;            // EULER sub-step and 0.5*h**2
   double half_dt;            // HalfTimeStep
-  const int* id;
-  const int* rc_in;
-  const int* st_in;
-  int* rc_out;
-  int* st_out;
-  int stamp_next;
+  int stamp_next;            // entry stamp of the next evaluation
   int check_pred;            // PRED: verify the predicted positions against xref
   int step_idx;              // CORR: step index within the current dem_step call
   int par;                   // evaluation parity (counters)
   const double4* xref;
-  const int* nbr;
-  const int* nnb;
-  const int* ccnt_in;
-  const int* cpart_in;
-  const double* chist_in;
-  int* ccnt_out;
-  int* cpart_out;
-  double* chist_out;
+  EdgeSets E;
   unsigned* wflag;
   double* whist;
   int* rb_clear;             // rebuild flag consumed before this evaluation
@@ -82,133 +123,179 @@ struct PassArgs {
   Ctl* ctl;
 };
 
+__device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
+  o[0] = make_double2(a.x, a.y);
+  o[1] = make_double2(a.z, b.x);
+  o[2] = make_double2(b.y, b.z);
+}
+
 template <int MODE, bool UPDATE>
 __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
+  __shared__ double4 sh_pos[DEM_BLOCK];
+  __shared__ double2 sh_vw[DEM_BLOCK * 3];
+  __shared__ int sh_rc[DEM_BLOCK];
+  __shared__ int sh_st[DEM_BLOCK];
+  __shared__ int sh_id[DEM_BLOCK];
+  __shared__ int sh_off[DEM_BLOCK + 1];
+  __shared__ double sh_c[9 * DEM_BLOCK];   // per edge: FN, FT, torque contribution
+  __shared__ unsigned char sh_ok[DEM_BLOCK];
+
   Ctl* ctl = a.ctl;
   if (*(volatile int*)&ctl->escaped) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t == 0) {
     *a.rb_clear = 0;
     ctl->pairs[a.par ^ 1] = 0;
     ctl->wallc[a.par ^ 1] = 0;
     if (UPDATE) ctl->stored[a.par ^ 1] = 0;
   }
   const size_t N = (size_t)s.np;
-  const int i = xcd_block() * DEM_BLOCK + threadIdx.x;
+  const int ep = *(volatile const int*)a.E.epar;
+  const int* __restrict__ off = a.E.off[ep];
+  const int* __restrict__ nbr = a.E.nbr[ep];
+  unsigned char* __restrict__ flag = a.E.flag[ep];
+  double2* __restrict__ hist = a.E.hist[ep];
+
+  const int p0 = xcd_block() * DEM_BLOCK;
+  const int nb = min(DEM_BLOCK, s.np - p0);
+  const int i = p0 + t;
+  const bool active = t < nb;
   unsigned long long n_col = 0, n_pair = 0, n_wall = 0, n_stored = 0;
-  int err = 0, need_rb = 0, n_esc = 0, cnt_hw = 0;
-  if (i < s.np) {
-    const double4 P = a.pos_in[i];
-    const int ci = (int)P.w;
-    const d3 xi = mk3(P.x, P.y, P.z);
-    const double2* vwp = reinterpret_cast<const double2*>(a.vw_in + 6 * (size_t)i);
-    const double2 u0 = vwp[0], u1 = vwp[1], u2 = vwp[2];
-    const d3 vi = mk3(u0.x, u0.y, u1.x);
-    const d3 wi = mk3(u1.y, u2.x, u2.y);
-    const CompConst cc = s.comp[ci];
-    const double ri = cc.radius;
-    // iForce = EffectiveMass*Gravity (src/MForces.f90:190)
-    d3 F = scl3(cc.eff_mass, mk3(s.grav[0], s.grav[1], s.grav[2]));
-    d3 T = mk3(0.0, 0.0, 0.0);
-    const int cell_p = a.rc_in[i];
-    const int st_p = a.st_in[i];
-    const int id_p = a.id[i];
-    const int cnt_old = a.ccnt_in[i];
-    unsigned long long visited = 0;
-    int cnt_new = 0;
-    const int nn = a.nnb[i];
-    for (int k = 0; k < nn; ++k) {
-      const int j = a.nbr[(size_t)k * N + i];
+  int err = 0, need_rb = 0, n_esc = 0;
+
+  // ---- stage the block's particles
+  double4 P = make_double4(0, 0, 0, 0);
+  d3 vi = mk3(0, 0, 0), wi = vi;
+  int cell_p = 0, st_p = 0, id_p = 0;
+  if (active) {
+    P = a.pos_in[i];
+    const Kin K = load_kin(a.kin_in + i);
+    vi = kin_v(K);
+    wi = kin_w(K);
+    cell_p = K.rc;
+    st_p = K.st;
+    id_p = K.id;
+    sh_pos[t] = P;
+    sh_vw[3 * t] = K.v0;
+    sh_vw[3 * t + 1] = K.v1;
+    sh_vw[3 * t + 2] = K.v2;
+    sh_rc[t] = cell_p;
+    sh_st[t] = st_p;
+    sh_id[t] = id_p;
+    sh_off[t] = off[i];
+  }
+  if (t == 0) sh_off[nb] = off[p0 + nb];
+  const int ci = (int)P.w;
+  const CompConst cc = s.comp[active ? ci : 0];
+  const double ri = cc.radius;
+  // iForce = EffectiveMass*Gravity (src/MForces.f90:190)
+  d3 F = scl3(cc.eff_mass, mk3(s.grav[0], s.grav[1], s.grav[2]));
+  d3 T = mk3(0.0, 0.0, 0.0);
+  __syncthreads();
+
+  // ---- particle-pair contacts, one lane per directed edge
+  const int e_begin = sh_off[0], e_end = sh_off[nb];
+  for (int c0 = e_begin; c0 < e_end; c0 += DEM_BLOCK) {
+    const int e = c0 + t;
+    unsigned char ok = 0;
+    if (e < e_end) {
+      int lo = 0, hi = nb - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh_off[mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      const int li = lo;
+      // Everything the edge may need is requested before the overlap test:
+      // the edge record and its history (coalesced), then both records of
+      // the partner together, so a contact costs two memory round trips.
+      const int j = nbr[e];
+      const unsigned char fl = flag[e];
+      const double2* hp = hist + 3 * (size_t)e;
+      const double2 h0 = hp[0], h1 = hp[1], h2 = hp[2];
       const double4 Q = a.pos_in[j];
+      const Kin KJ = load_kin(a.kin_in + j);
+      const double4 Pi = sh_pos[li];
+      const int cie = (int)Pi.w;
       const int cj = (int)Q.w;
-      const d3 rij = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
+      const d3 rij = min_image3(s, mk3(Q.x - Pi.x, Q.y - Pi.y, Q.z - Pi.z));
       const double dsq = dot3(rij, rij);
-      const PairConst& pc = s.pair[ci + cj * s.nc];
-      const double delta = pc.mindijsq - dsq;
-      if (!(delta > 0.0)) continue;  // overlap test (src/MForces.f90:223-224)
-      // contact history of the pair (VerifyPairContact, src/MContacts.f90:329)
-      int c = -1;
-      for (int m = 0; m < cnt_old; ++m)
-        if (a.cpart_in[(size_t)m * N + i] == j) { c = m; break; }
-      const int id_q = a.id[j];
-      int role;
-      if (s.hist_mode == 0) {
-        role = ref_role(s, cell_p, a.rc_in[j], st_p, a.st_in[j], id_p, id_q);
-        if (role < 0) { err |= DEM_KERR_ROLE; role = id_p < id_q; }
-      } else {
-        role = id_p < id_q;
-      }
-      const double sg = role ? 1.0 : -1.0;
-      ContactIn in;
-      in.contacting = c >= 0;
-      if (in.contacting) {
-        visited |= 1ull << c;
-        const double* hp = a.chist_in + (size_t)c * 6 * N + i;
-        in.ft0 = scl3(sg, mk3(hp[0], hp[N], hp[2 * N]));
-        in.vt0 = scl3(sg, mk3(hp[3 * N], hp[4 * N], hp[5 * N]));
-      } else {
-        in.ft0 = mk3(0.0, 0.0, 0.0);
-        in.vt0 = in.ft0;
-        if (role) ++n_col;
-      }
-      if (role) ++n_pair;
-      const CompConst cq = s.comp[cj];
-      const double rj = cq.radius;
-      const double2* vwq = reinterpret_cast<const double2*>(a.vw_in + 6 * (size_t)j);
-      const double2 q0 = vwq[0], q1 = vwq[1], q2 = vwq[2];
-      const d3 vj = mk3(q0.x, q0.y, q1.x);
-      const d3 wj = mk3(q1.y, q2.x, q2.y);
-      const double dij = sqrt(dsq);
-      in.nij = div3(rij, dij);
-      in.disp_n = scl3(dij - pc.mindij, in.nij);
-      in.vij = sub3(vj, vi);
-      in.vijc = sub3(in.vij, cross3(add3(scl3(ri, wi), scl3(rj, wj)), in.nij));
-      in.delta_dsq = delta;
-      in.half_dt = a.half_dt;
-      const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
-      F = add3(add3(F, o.fn), o.ft);
-      // torque of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
-      if (role) T = add3(T, cross3(scl3(ri, in.nij), o.ft));
-      else T = add3(T, scl3(ri, cross3(in.nij, o.ft)));
-      if (UPDATE) {
-        if (cnt_new < s.max_c) {
-          a.cpart_out[(size_t)cnt_new * N + i] = j;
-          double* hq = a.chist_out + (size_t)cnt_new * 6 * N + i;
-          const d3 sft = scl3(sg, o.ft), svt = scl3(sg, o.vt);
-          hq[0] = sft.x; hq[N] = sft.y; hq[2 * N] = sft.z;
-          hq[3 * N] = svt.x; hq[4 * N] = svt.y; hq[5 * N] = svt.z;
+      const PairConst& pc = s.pair[cie + cj * s.nc];
+      const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
+      if (delta > 0.0) {
+        const int idi = sh_id[li];
+        int role;
+        if (s.hist_mode == 0) {
+          role = ref_role(s, sh_rc[li], KJ.rc, sh_st[li], KJ.st, idi, KJ.id);
+          if (role < 0) { err |= DEM_KERR_ROLE; role = idi < KJ.id; }
         } else {
-          err |= DEM_KERR_CONTACT_OVERFLOW;
+          role = idi < KJ.id;
         }
-        ++cnt_new;
+        const double sg = role ? 1.0 : -1.0;
+        ContactIn in;
+        in.contacting = fl & 1;
+        if (in.contacting) {
+          in.ft0 = scl3(sg, mk3(h0.x, h0.y, h1.x));
+          in.vt0 = scl3(sg, mk3(h1.y, h2.x, h2.y));
+        } else {
+          in.ft0 = mk3(0.0, 0.0, 0.0);
+          in.vt0 = in.ft0;
+          if (role) ++n_col;
+        }
+        if (role) ++n_pair;
+        const double rip = s.comp[cie].radius;
+        const double rj = s.comp[cj].radius;
+        const double2 s0 = sh_vw[3 * li], s1 = sh_vw[3 * li + 1], s2 = sh_vw[3 * li + 2];
+        const d3 vie = mk3(s0.x, s0.y, s1.x), wie = mk3(s1.y, s2.x, s2.y);
+        const double dij = sqrt(dsq);
+        in.nij = div3(rij, dij);
+        in.disp_n = scl3(dij - pc.mindij, in.nij);
+        in.vij = sub3(kin_v(KJ), vie);
+        in.vijc = sub3(in.vij, cross3(add3(scl3(rip, wie), scl3(rj, kin_w(KJ))), in.nij));
+        in.delta_dsq = delta;
+        in.half_dt = a.half_dt;
+        const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
+        // torque of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
+        const d3 tq = role ? cross3(scl3(rip, in.nij), o.ft) : scl3(rip, cross3(in.nij, o.ft));
+        double* c = sh_c + 9 * t;
+        c[0] = o.fn.x; c[1] = o.fn.y; c[2] = o.fn.z;
+        c[3] = o.ft.x; c[4] = o.ft.y; c[5] = o.ft.z;
+        c[6] = tq.x; c[7] = tq.y; c[8] = tq.z;
+        ok = 1;
+        if (UPDATE) {
+          store_d3x2(hist + 3 * (size_t)e, scl3(sg, o.ft), scl3(sg, o.vt));
+          if (!(fl & 1)) flag[e] = 1;
+          ++n_stored;
+        }
+      } else if (UPDATE && (fl & 1)) {
+        // The reference removes a stored pair only when its cell loop visits
+        // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
+        const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, sh_rc[li], KJ.rc);
+        if (keep) ++n_stored;
+        else flag[e] = 0;
       }
     }
-    if (UPDATE) {
-      // Entries not refreshed above belong to non-overlapping pairs.  The
-      // reference removes them only when its cell loop visits the pair
-      // (RemovePairContact, src/MForces.f90:268); otherwise they persist.
-      for (int m = 0; m < cnt_old; ++m) {
-        if (visited & (1ull << m)) continue;
-        const int q = a.cpart_in[(size_t)m * N + i];
-        bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, a.rc_in[q]);
-        if (!keep) continue;
-        if (cnt_new < s.max_c) {
-          a.cpart_out[(size_t)cnt_new * N + i] = q;
-          const double* hp = a.chist_in + (size_t)m * 6 * N + i;
-          double* hq = a.chist_out + (size_t)cnt_new * 6 * N + i;
-          for (int d = 0; d < 6; ++d) hq[d * N] = hp[d * N];
-        } else {
-          err |= DEM_KERR_CONTACT_OVERFLOW;
-        }
-        ++cnt_new;
+    sh_ok[t] = ok;
+    __syncthreads();
+    if (active) {
+      const int b = max(sh_off[t], c0), f = min(sh_off[t + 1], c0 + DEM_BLOCK);
+      for (int q = b; q < f; ++q) {
+        const int k = q - c0;
+        if (!sh_ok[k]) continue;
+        const double* c = sh_c + 9 * k;
+        // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381)
+        F = add3(add3(F, mk3(c[0], c[1], c[2])), mk3(c[3], c[4], c[5]));
+        T = add3(T, mk3(c[6], c[7], c[8]));
       }
-      if (cnt_new > s.max_c) cnt_new = s.max_c;
-      a.ccnt_out[i] = cnt_new;
-      n_stored = cnt_new;
-      cnt_hw = cnt_new;
     }
+    __syncthreads();
+  }
+
+  if (active) {
+    const d3 xi = mk3(P.x, P.y, P.z);
     // walls (src/MForces.f90:275-314)
     if (s.nw > 0) {
-      unsigned wf = a.wflag[i];
+      const unsigned wf = a.wflag[i];
       unsigned wf_new = wf;
       for (int jw = 0; jw < s.nw; ++jw) {
         const WallGeom& w = s.walls[jw];
@@ -224,21 +311,19 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
           in.vt0 = in.ft0;
         }
         const WallHit wh = wall_overlap(s, w, xi, ri, wc.max_alpha_u, wc.min_aw_sq, wc.max_aw_sq);
-        const double dij = wh.dij;
-        const d3 nij = wh.nij, qji = wh.qji;
         if (wh.ov < 0) err |= DEM_KERR_CONE_AXIS;
         if (wh.ov > 0) {
           ++n_wall;
-          in.nij = nij;
-          in.delta_dsq = ri * ri - dij * dij;
-          in.disp_n = scl3(dij - ri, nij);
-          in.vij = sub3(add3(vget(w.vel), cross3(vget(w.omg), qji)), vi);
-          in.vijc = sub3(in.vij, cross3(scl3(ri, wi), nij));
+          in.nij = wh.nij;
+          in.delta_dsq = ri * ri - wh.dij * wh.dij;
+          in.disp_n = scl3(wh.dij - ri, wh.nij);
+          in.vij = sub3(add3(vget(w.vel), cross3(vget(w.omg), wh.qji)), vi);
+          in.vijc = sub3(in.vij, cross3(scl3(ri, wi), wh.nij));
           in.half_dt = a.half_dt;
-          const d3 rc = sub3(add3(vget(w.pos), qji), xi);
+          const d3 rc = sub3(add3(vget(w.pos), wh.qji), xi);
           const ContactOut o = common_calc(in, rc, wc.knw, wc.ynw, wc.ktw, wc.ytw, wc.muwsq);
           F = add3(add3(F, o.fn), o.ft);
-          T = add3(T, cross3(scl3(ri, nij), o.ft));
+          T = add3(T, cross3(scl3(ri, wh.nij), o.ft));
           if (UPDATE) {
             hp[0] = o.ft.x; hp[N] = o.ft.y; hp[2 * N] = o.ft.z;
             hp[3 * N] = o.vt.x; hp[4 * N] = o.vt.y; hp[5 * N] = o.vt.z;
@@ -248,34 +333,22 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
           wf_new &= ~(1u << jw);
         }
       }
-      if (UPDATE) a.wflag[i] = wf_new;
+      if (UPDATE && wf_new != wf) a.wflag[i] = wf_new;
     }
     // accelerations (src/MForces.f90:330-333)
     const d3 acc = scl3(cc.inv_mass, F);
     const d3 alp = scl3(cc.inv_inertia, T);
-    // Particle%Acceleration = Force*InvMass is Force(i,:)*InvMass: same product.
     if (MODE == MODE_ACC) {
-      double2* o = reinterpret_cast<double2*>(a.acc_out + 6 * (size_t)i);
-      o[0] = make_double2(acc.x, acc.y);
-      o[1] = make_double2(acc.z, alp.x);
-      o[2] = make_double2(alp.y, alp.z);
+      store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
     } else if (MODE == MODE_EULER) {
       // Euler (src/MDEM.f90:240-252)
-      d3 x = add3(add3(xi, scl3(a.h, vi)), scl3(a.hsq2, acc));
+      const d3 x = add3(add3(xi, scl3(a.h, vi)), scl3(a.hsq2, acc));
       const d3 v = add3(vi, scl3(a.h, acc));
       const d3 wv = add3(wi, scl3(a.h, alp));
       a.pos_out[i] = make_double4(x.x, x.y, x.z, P.w);
-      double2* o = reinterpret_cast<double2*>(a.vw_out + 6 * (size_t)i);
-      o[0] = make_double2(v.x, v.y);
-      o[1] = make_double2(v.z, wv.x);
-      o[2] = make_double2(wv.y, wv.z);
-      double2* q = reinterpret_cast<double2*>(a.acc_out + 6 * (size_t)i);
-      q[0] = make_double2(acc.x, acc.y);
-      q[1] = make_double2(acc.z, alp.x);
-      q[2] = make_double2(alp.y, alp.z);
       const int rc = ref_cell(s, x.x, x.y, x.z);
-      a.rc_out[i] = rc;
-      a.st_out[i] = (rc != cell_p) ? a.stamp_next : st_p;
+      store_kin(a.kin_out + i, v, wv, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
+      store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       const double4 R = a.xref[i];
       const d3 dd = min_image3(s, mk3(x.x - R.x, x.y - R.y, x.z - R.z));
       if (dot3(dd, dd) > s.skin_half_sq) need_rb = 1;
@@ -299,24 +372,16 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
       if (s.pbc[1]) xp.y = xp.y - s.lbox[1] * fnint(xp.y * s.invl[1]);
       if (s.pbc[2]) xp.z = xp.z - s.lbox[2] * fnint(xp.z * s.invl[2]);
       a.pos_out[i] = make_double4(xp.x, xp.y, xp.z, P.w);
-      double2* o = reinterpret_cast<double2*>(a.vw_out + 6 * (size_t)i);
-      o[0] = make_double2(vp.x, vp.y);
-      o[1] = make_double2(vp.z, wp.x);
-      o[2] = make_double2(wp.y, wp.z);
+      const int rc = ref_cell(s, xp.x, xp.y, xp.z);
+      store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
       if (s.nprev > 0) {
         double* r = a.slot_wr + i;
         r[0] = vi.x; r[N] = vi.y; r[2 * N] = vi.z;
         r[3 * N] = acc.x; r[4 * N] = acc.y; r[5 * N] = acc.z;
         r[6 * N] = alp.x; r[7 * N] = alp.y; r[8 * N] = alp.z;
       } else {
-        double2* q = reinterpret_cast<double2*>(a.acc_out + 6 * (size_t)i);
-        q[0] = make_double2(acc.x, acc.y);
-        q[1] = make_double2(acc.z, alp.x);
-        q[2] = make_double2(alp.y, alp.z);
+        store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       }
-      const int rc = ref_cell(s, xp.x, xp.y, xp.z);
-      a.rc_out[i] = rc;
-      a.st_out[i] = (rc != cell_p) ? a.stamp_next : st_p;
       if (a.check_pred) {
         const double4 R = a.xref[i];
         const d3 dd = min_image3(s, mk3(xp.x - R.x, xp.y - R.y, xp.z - R.z));
@@ -326,17 +391,10 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
     } else {  // MODE_CORR
       // AM corrector (src/MDEM.f90:341-367); vi, wi hold the predicted
       // velocities, acc/alp the predicted accelerations.
-      if (a.acc_out) {
-        double2* q = reinterpret_cast<double2*>(a.acc_out + 6 * (size_t)i);
-        q[0] = make_double2(acc.x, acc.y);
-        q[1] = make_double2(acc.z, alp.x);
-        q[2] = make_double2(alp.y, alp.z);
-      }
+      if (a.acc_out) store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       const double4 X = a.pos[i];
-      const double2* vp0 = reinterpret_cast<const double2*>(a.vw + 6 * (size_t)i);
-      const double2 p0 = vp0[0], p1 = vp0[1], p2 = vp0[2];
-      const d3 v0 = mk3(p0.x, p0.y, p1.x);
-      const d3 w0 = mk3(p1.y, p2.x, p2.y);
+      const Kin K0 = load_kin(a.kin + i);
+      const d3 v0 = kin_v(K0), w0 = kin_w(K0);
       d3 sv = mk3(0, 0, 0), sa = sv, sl = sv, sb = sv;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -356,13 +414,8 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
       if (s.pbc[1]) x.y = x.y - s.lbox[1] * fnint(x.y * s.invl[1]);
       if (s.pbc[2]) x.z = x.z - s.lbox[2] * fnint(x.z * s.invl[2]);
       a.pos[i] = make_double4(x.x, x.y, x.z, X.w);
-      double2* o = reinterpret_cast<double2*>(a.vw + 6 * (size_t)i);
-      o[0] = make_double2(v.x, v.y);
-      o[1] = make_double2(v.z, wv.x);
-      o[2] = make_double2(wv.y, wv.z);
       const int rc = ref_cell(s, x.x, x.y, x.z);
-      a.rc_out[i] = rc;
-      a.st_out[i] = (rc != cell_p) ? a.stamp_next : st_p;
+      store_kin(a.kin + i, v, wv, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
       // escape scan of UpdateParticleList (src/MDEM.f90:624-625)
       if ((x.x > s.phalf[0]) || (x.x < -s.phalf[0]) || (x.y > s.phalf[1]) ||
           (x.y < -s.phalf[1]) || (x.z > s.phalf[2]) || (x.z < -s.phalf[2]))
@@ -389,15 +442,13 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
   err = wave_or(err);
   need_rb = wave_or(need_rb);
   n_esc = (int)wave_sum_u64((unsigned long long)n_esc);
-  cnt_hw = wave_max(cnt_hw);
-  if ((threadIdx.x & 63) == 0) {
+  if ((t & 63) == 0) {
     if (n_col) atomicAdd(&ctl->ncol, n_col);
     if (n_pair) atomicAdd(&ctl->pairs[a.par], n_pair);
     if (n_wall) atomicAdd(&ctl->wallc[a.par], n_wall);
     if (UPDATE && n_stored) atomicAdd(&ctl->stored[a.par], n_stored);
     if (err) atomicOr(&ctl->error, err);
     if (need_rb) atomicOr(a.rb_set, 1);
-    if (UPDATE && cnt_hw) atomicMax(&ctl->max_cnt, cnt_hw);
     if (MODE == MODE_CORR && n_esc) {
       atomicAdd(&ctl->escaped, n_esc);
       atomicMin(&ctl->stop_step, a.step_idx);
@@ -406,16 +457,28 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Reference-grid cell assignment of uploaded positions (CurrentCell3D and the
-// transfer bookkeeping of RedistributeParticles3D, src/MCells.f90:607-643).
-__global__ void k_refcell(SysConst s, const double4* pos, const int* rc_prev, const int* st_prev,
-                          int* rc_out, int* st_out, int stamp_next, int fresh) {
+// Cell state of positions set from outside (upload, the ComputeAccelerations
+// drop-in, the PBC wrap after the Euler start-up): CurrentCell3D plus the
+// transfer bookkeeping of RedistributeParticles3D (src/MCells.f90:607-643)
+// against the cell state of the previous evaluation (prev; may alias kin).
+// wrap != 0 applies ApplyPeriodicBoundaryConditions (src/MDEM.f90:373-388)
+// to the positions first; wrap < 0 only assigns the cells and keeps the
+// stamps (a fresh upload, whose stamps are the given list ranks).
+__global__ void k_refcell(SysConst s, double4* pos, const Kin* prev, Kin* kin, int stamp_next,
+                          int wrap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= s.np) return;
-  const double4 P = pos[i];
+  double4 P = pos[i];
+  if (wrap > 0) {
+    if (s.pbc[0]) P.x = P.x - s.lbox[0] * fnint(P.x * s.invl[0]);
+    if (s.pbc[1]) P.y = P.y - s.lbox[1] * fnint(P.y * s.invl[1]);
+    if (s.pbc[2]) P.z = P.z - s.lbox[2] * fnint(P.z * s.invl[2]);
+    pos[i] = P;
+  }
+  const int rc_prev = prev[i].rc, st_prev = prev[i].st;
   const int rc = ref_cell(s, P.x, P.y, P.z);
-  rc_out[i] = rc;
-  st_out[i] = fresh ? st_prev[i] : ((rc != rc_prev[i]) ? stamp_next : st_prev[i]);
+  kin[i].rc = rc;
+  kin[i].st = (wrap < 0 || rc == rc_prev) ? st_prev : stamp_next;
 }
 
 // Displacement check of uploaded positions against the neighbour-list origin.
@@ -515,11 +578,16 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int nti
   }
 }
 
+// Exclusive scan of in[0..n) into start[0..n] (start[n] = total); cursor (may
+// be null) receives a copy of start[0..n).  With out_sel != null the output
+// is the offsets array of the edge set being built (1 - *epar).
 __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
                                                           const int* tile_off, int* start,
-                                                          int* cursor, const int* gate,
-                                                          int force, const Ctl* ctl) {
+                                                          int* cursor, const EdgeSets* out_sel,
+                                                          const int* gate, int force,
+                                                          const Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
+  if (out_sel) start = out_sel->off[1 - *(volatile const int*)out_sel->epar];
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
@@ -532,7 +600,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) {
       start[base + k] = ex;
-      cursor[base + k] = ex;
+      if (cursor) cursor[base + k] = ex;
     }
     ex += v[k];
   }
@@ -567,96 +635,163 @@ __global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const in
   }
 }
 
-// Build the directed neighbour list: all j != i with |r_ij| < r_i + r_j + skin
-// (minimum image), scanning the 27 cells around i's cell in a fixed order.
-__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_build(SysConst s, const double4* pos,
-                                                         const int* key, const int* start,
-                                                         const int* cell_idx, int* nbr, int* nnb,
-                                                         double4* xref, const int* gate,
-                                                         int force, Ctl* ctl) {
-  if (gate_closed(gate, force, ctl)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->rebuilds, 1ull);
-  const size_t N = (size_t)s.np;
-  const int i = xcd_block() * DEM_BLOCK + threadIdx.x;
-  int cnt = 0, err = 0;
-  if (i < s.np) {
-    const double4 P = pos[i];
-    const int ci = (int)P.w;
-    const int k = key[i];
-    const int gx = k % s.gn[0], gy = (k / s.gn[0]) % s.gn[1], gz = k / (s.gn[0] * s.gn[1]);
-    for (int dz = -1; dz <= 1; ++dz) {
-      int cz = gz + dz;
-      if (s.pbc[2]) {
-        if (s.gn[2] == 1 && dz != 0) continue;
-        if (s.gn[2] == 2 && dz == 1) continue;
-        cz = (cz + s.gn[2]) % s.gn[2];
-      } else if (cz < 0 || cz >= s.gn[2]) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int cy = gy + dy;
-        if (s.pbc[1]) {
-          if (s.gn[1] == 1 && dy != 0) continue;
-          if (s.gn[1] == 2 && dy == 1) continue;
-          cy = (cy + s.gn[1]) % s.gn[1];
-        } else if (cy < 0 || cy >= s.gn[1]) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          int cx = gx + dx;
-          if (s.pbc[0]) {
-            if (s.gn[0] == 1 && dx != 0) continue;
-            if (s.gn[0] == 2 && dx == 1) continue;
-            cx = (cx + s.gn[0]) % s.gn[0];
-          } else if (cx < 0 || cx >= s.gn[0]) continue;
-          const int c = cx + (cy + cz * s.gn[1]) * s.gn[0];
-          const int e = start[c + 1];
-          for (int p = start[c]; p < e; ++p) {
-            const int j = cell_idx[p];
-            if (j == i) continue;
-            const double4 Q = pos[j];
-            const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
-            const double dsq = dot3(r, r);
-            if (dsq < s.pair[ci + (int)Q.w * s.nc].rc2) {
-              if (cnt < s.max_nb) nbr[(size_t)cnt * N + i] = j;
-              else err = DEM_KERR_NBR_OVERFLOW;
-              ++cnt;
-            }
-          }
+// Visits every j != i with |r_ij|^2 < (r_i + r_j + skin)^2 (minimum image),
+// scanning the 27 cells around i's cell in a fixed order.
+template <typename F>
+__device__ __forceinline__ void for_each_neighbour(const SysConst& s, const double4* pos,
+                                                   int i, const double4& P, int k,
+                                                   const int* start, const int* cell_idx,
+                                                   F&& fn) {
+  const int ci = (int)P.w;
+  const int gx = k % s.gn[0], gy = (k / s.gn[0]) % s.gn[1], gz = k / (s.gn[0] * s.gn[1]);
+  for (int dz = -1; dz <= 1; ++dz) {
+    int cz = gz + dz;
+    if (s.pbc[2]) {
+      if (s.gn[2] == 1 && dz != 0) continue;
+      if (s.gn[2] == 2 && dz == 1) continue;
+      cz = (cz + s.gn[2]) % s.gn[2];
+    } else if (cz < 0 || cz >= s.gn[2]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int cy = gy + dy;
+      if (s.pbc[1]) {
+        if (s.gn[1] == 1 && dy != 0) continue;
+        if (s.gn[1] == 2 && dy == 1) continue;
+        cy = (cy + s.gn[1]) % s.gn[1];
+      } else if (cy < 0 || cy >= s.gn[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int cx = gx + dx;
+        if (s.pbc[0]) {
+          if (s.gn[0] == 1 && dx != 0) continue;
+          if (s.gn[0] == 2 && dx == 1) continue;
+          cx = (cx + s.gn[0]) % s.gn[0];
+        } else if (cx < 0 || cx >= s.gn[0]) continue;
+        const int c = cx + (cy + cz * s.gn[1]) * s.gn[0];
+        const int e = start[c + 1];
+        for (int p = start[c]; p < e; ++p) {
+          const int j = cell_idx[p];
+          if (j == i) continue;
+          const double4 Q = pos[j];
+          const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
+          if (dot3(r, r) < s.pair[ci + (int)Q.w * s.nc].rc2) fn(j);
         }
       }
     }
-    nnb[i] = cnt < s.max_nb ? cnt : s.max_nb;
-    xref[i] = P;
   }
-  unsigned long long tot = wave_sum_u64((unsigned long long)(cnt < s.max_nb ? cnt : s.max_nb));
+}
+
+// Pass 1 of the list build: neighbours per particle (capped at max_nb).
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const double4* pos,
+                                                         const int* key, const int* start,
+                                                         const int* cell_idx, int* cnt,
+                                                         const int* gate, int force, Ctl* ctl) {
+  if (gate_closed(gate, force, ctl)) return;
+  const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
+  int n = 0, err = 0;
+  if (i < s.np) {
+    const double4 P = pos[i];
+    for_each_neighbour(s, pos, i, P, key[i], start, cell_idx, [&](int) { ++n; });
+    if (n > s.max_nb) {
+      err = DEM_KERR_NBR_OVERFLOW;
+      n = s.max_nb;
+    }
+    cnt[i] = n;
+  }
   err = wave_or(err);
-  const int hw = wave_max(cnt);
+  const int hw = wave_max(n);
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&ctl->nbpairs, tot);
     if (err) atomicOr(&ctl->error, err);
     atomicMax(&ctl->max_nnb, hw);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nbpairs = 0;
+}
+
+// Pass 2 of the list build: fill the new edge set and carry the history of
+// every pair that is still listed.  src maps new particle indices to the ones
+// the current edge set uses (null = identity; a resort passes its
+// permutation, and o2n its inverse).  Stored pairs that drop out of the list
+// are counted when the reference would have kept them (cells not adjacent).
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
+                                                        const int* key, const int* start,
+                                                        const int* cell_idx, EdgeSets E,
+                                                        const int* src, const int* o2n,
+                                                        const Kin* kin_cur, double4* xref,
+                                                        const int* gate, int force, Ctl* ctl) {
+  if (gate_closed(gate, force, ctl)) return;
+  const int p = *(volatile const int*)E.epar, q = 1 - p;
+  const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
+  unsigned long long nedges = 0, dropped = 0;
+  if (i < s.np) {
+    const double4 P = pos[i];
+    const int o = src ? src[i] : i;
+    const int ob = E.off[p][o], oe = E.off[p][o + 1];
+    const int* onbr = E.nbr[p];
+    const unsigned char* oflag = E.flag[p];
+    const double2* ohist = E.hist[p];
+    const int nb0 = E.off[q][i], nb1 = E.off[q][i + 1];
+    int e = nb0;
+    unsigned long long found = 0;
+    for_each_neighbour(s, pos, i, P, key[i], start, cell_idx, [&](int j) {
+      if (e >= nb1) return;
+      E.nbr[q][e] = j;
+      const int jo = src ? src[j] : j;
+      unsigned char fl = 0;
+      for (int m = ob; m < oe; ++m) {
+        if (onbr[m] == jo) {
+          fl = oflag[m];
+          if (fl & 1) {
+            const double2* h = ohist + 3 * (size_t)m;
+            double2* g = E.hist[q] + 3 * (size_t)e;
+            g[0] = h[0]; g[1] = h[1]; g[2] = h[2];
+          }
+          if (m - ob < 64) found |= 1ull << (m - ob);
+          break;
+        }
+      }
+      E.flag[q][e] = fl;
+      ++e;
+    });
+    nedges = (unsigned long long)(nb1 - nb0);
+    for (int m = ob; m < oe; ++m) {
+      if (!(oflag[m] & 1)) continue;
+      if (m - ob < 64 && (found >> (m - ob)) & 1ull) continue;
+      const int jn = o2n ? o2n[onbr[m]] : onbr[m];
+      if (s.hist_mode == 0 && !ref_adjacent(s, kin_cur[i].rc, kin_cur[jn].rc)) ++dropped;
+    }
+    xref[i] = P;
+  }
+  nedges = wave_sum_u64(nedges);
+  dropped = wave_sum_u64(dropped);
+  if ((threadIdx.x & 63) == 0) {
+    if (dropped) atomicAdd(&ctl->stale_dropped, dropped);
+    if (nedges) atomicAdd(&ctl->nbpairs, nedges);
+  }
+}
+
+// Make the freshly built edge set current.
+__global__ void k_nbr_flip(EdgeSets E, const int* gate, int force, Ctl* ctl) {
+  if (gate_closed(gate, force, ctl)) return;
+  if (threadIdx.x == 0) {
+    *E.epar = 1 - *E.epar;
+    ctl->rebuilds += 1;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Resort: move every per-particle array from old order to new order
-// (new i <- old cell_idx[i]); contact partners are renumbered.
+// (new i <- old src[i]).  Edges are rebuilt afterwards through src.
 
-__global__ void k_old_to_new(int np, const int* cell_idx, int* old_to_new) {
+__global__ void k_old_to_new(int np, const int* src, int* old_to_new) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) old_to_new[cell_idx[i]] = i;
+  if (i < np) old_to_new[src[i]] = i;
 }
 
 struct PermArgs {
-  const int* src;       // new -> old
-  const int* old_to_new;
+  const int* src;
   const double4* pos;  double4* pos2;
-  const double* vw;    double* vw2;
-  const double* acc;   double* acc2;
+  const Kin* kin;      Kin* kin2;
+  const double2* acc;  double2* acc2;
   const double* ring;  double* ring2;  int nring;  // 9*nprev components
   const int* id;       int* id2;
-  const int* rc;       int* rc2;
-  const int* st;       int* st2;
-  const int* ccnt;     int* ccnt2;
-  const int* cpart;    int* cpart2;
-  const double* chist; double* chist2;
   const unsigned* wflag; unsigned* wflag2;
   const double* whist;   double* whist2;
   int nw;
@@ -668,18 +803,10 @@ __global__ void k_permute(SysConst s, PermArgs a) {
   if (i >= s.np) return;
   const int o = a.src[i];
   a.pos2[i] = a.pos[o];
-  for (int d = 0; d < 6; ++d) a.vw2[6 * (size_t)i + d] = a.vw[6 * (size_t)o + d];
-  for (int d = 0; d < 6; ++d) a.acc2[6 * (size_t)i + d] = a.acc[6 * (size_t)o + d];
+  a.kin2[i] = a.kin[o];
+  for (int d = 0; d < 3; ++d) a.acc2[3 * (size_t)i + d] = a.acc[3 * (size_t)o + d];
   for (int c = 0; c < a.nring; ++c) a.ring2[c * N + i] = a.ring[c * N + o];
   a.id2[i] = a.id[o];
-  a.rc2[i] = a.rc[o];
-  a.st2[i] = a.st[o];
-  const int n = a.ccnt[o];
-  a.ccnt2[i] = n;
-  for (int m = 0; m < n; ++m) {
-    a.cpart2[m * N + i] = a.old_to_new[a.cpart[m * N + o]];
-    for (int d = 0; d < 6; ++d) a.chist2[(m * 6 + d) * N + i] = a.chist[(m * 6 + d) * N + o];
-  }
   if (a.nw > 0) {
     a.wflag2[i] = a.wflag[o];
     for (int c = 0; c < 6 * a.nw; ++c) a.whist2[c * N + i] = a.whist[c * N + o];
@@ -689,22 +816,23 @@ __global__ void k_permute(SysConst s, PermArgs a) {
 // ---------------------------------------------------------------------------
 // Host <-> device reordering.
 
-// Particle records (TParticle layout) in reference order -> internal arrays.
-__global__ void k_aos_in(int np, const dem_particle_t* p, const int* id, double4* pos, double* vw) {
+// Particle records (TParticle layout) in reference order -> internal arrays
+// (velocities only; id and cell state stay).
+__global__ void k_aos_in(int np, const dem_particle_t* p, const int* id, double4* pos, Kin* kin) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const dem_particle_t& r = p[id[i]];
   pos[i] = make_double4(r.position[0], r.position[1], r.position[2], (double)(r.index - 1));
-  double* o = vw + 6 * (size_t)i;
-  o[0] = r.velocity[0]; o[1] = r.velocity[1]; o[2] = r.velocity[2];
-  o[3] = r.omega[0]; o[4] = r.omega[1]; o[5] = r.omega[2];
+  kin[i].v0 = make_double2(r.velocity[0], r.velocity[1]);
+  kin[i].v1 = make_double2(r.velocity[2], r.omega[0]);
+  kin[i].v2 = make_double2(r.omega[1], r.omega[2]);
 }
 
-__global__ void k_aos_out(int np, dem_particle_t* p, const int* id, const double* acc) {
+__global__ void k_aos_out(int np, dem_particle_t* p, const int* id, const double2* acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   dem_particle_t& r = p[id[i]];
-  const double* a = acc + 6 * (size_t)i;
+  const double* a = reinterpret_cast<const double*>(acc + 3 * (size_t)i);
   r.acceleration[0] = a[0]; r.acceleration[1] = a[1]; r.acceleration[2] = a[2];
   r.alpha[0] = a[3]; r.alpha[1] = a[4]; r.alpha[2] = a[5];
 }
@@ -720,25 +848,24 @@ __global__ void k_to_ref_d(int np, int ncomp, const double* in, int stride_rec, 
     out[(size_t)c * np + r] = in[(size_t)i * stride_rec + (size_t)c * stride_comp];
 }
 
-__global__ void k_to_ref_i(int np, const int* in, const int* id, int* out, int add) {
+__global__ void k_cells_to_ref(int np, const Kin* kin, const int* id, int* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  out[id[i]] = in[i] + add;
+  out[id[i]] = kin[i].rc + 1;
 }
 
 // Deterministic partial sums for the kinetic energy (src/MDEM.f90:199-213):
 // each block sums its particles in index order, the host sums the blocks.
 __global__ __launch_bounds__(DEM_BLOCK) void k_energy(SysConst s, const double4* pos,
-                                                      const double* vw, double* part) {
+                                                      const Kin* kin, double* part) {
   __shared__ double buf[DEM_BLOCK];
   const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
   double e = 0.0;
   if (i < s.np) {
     const CompConst c = s.comp[(int)pos[i].w];
-    const double* v = vw + 6 * (size_t)i;
-    const double v2 = dot3(mk3(v[0], v[1], v[2]), mk3(v[0], v[1], v[2]));
-    const double w2 = dot3(mk3(v[3], v[4], v[5]), mk3(v[3], v[4], v[5]));
-    e = c.mass * v2 + c.inertia * w2;
+    const Kin K = kin[i];
+    const d3 v = kin_v(K), w = kin_w(K);
+    e = c.mass * dot3(v, v) + c.inertia * dot3(w, w);
   }
   buf[threadIdx.x] = e;
   __syncthreads();
@@ -746,5 +873,21 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_energy(SysConst s, const double4*
     double t = 0.0;
     for (int k = 0; k < DEM_BLOCK; ++k) t += buf[k];
     part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_store_ring(int np, const Kin* kin, const double2* acc2, double* slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const size_t N = (size_t)np;
+  const Kin K = kin[i];
+  const d3 v = kin_v(K);
+  const double* acc = reinterpret_cast<const double*>(acc2);
+  slot[i] = v.x;
+  slot[N + i] = v.y;
+  slot[2 * N + i] = v.z;
+  for (int d = 0; d < 3; ++d) {
+    slot[(3 + d) * N + i] = acc[6 * (size_t)i + d];
+    slot[(6 + d) * N + i] = acc[6 * (size_t)i + 3 + d];
   }
 }

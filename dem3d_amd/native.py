@@ -17,7 +17,7 @@ import numpy as np
 from .spec import Reduced
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-HIP_LIB = os.path.join(HERE, "libdem3d_hip.so")
+HIP_LIB = os.environ.get("DEM3D_HIP_LIB") or os.path.join(HERE, "libdem3d_hip.so")
 
 DEM_OK = 0
 DEM_ERR_ESCAPE = -6
