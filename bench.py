@@ -104,8 +104,9 @@ def main(argv=None):
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the DEM path runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(a multi-GPU run would decompose the box into slabs, one per GPU)",
+              file=sys.stderr)
         return 2
     cells = tuple(int(x) for x in args.cells.split(","))
     cpu = None

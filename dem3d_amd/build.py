@@ -21,7 +21,8 @@ HIPCC_FLAGS = [
     "-shared",
     # every multiply and add rounds on its own, as in the reference's arithmetic
     "-ffp-contract=off",
-    # no packed FP32 instructions (build flags)
+    # no packed FP32 instructions: some of their forms have returned wrong
+    # results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",

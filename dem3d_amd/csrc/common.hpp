@@ -42,7 +42,7 @@ struct Ctl {
   int stop_step;        // index (within the dem_step call) of that step
   int max_nnb;          // high-water mark of neighbour-list lengths
   int max_cnt;          // high-water mark of contact-table rows
-  int pad;
+  int esc_pending;      // escapes found by the running corrector pass
   unsigned long long ncol;          // cumulative (src/MForces.f90:227)
   unsigned long long rebuilds;
   unsigned long long pairs[2];      // pair contacts of the eval, by parity

@@ -70,7 +70,8 @@ struct dem_system {
   double4* xref = nullptr;
   EdgeSets es{};             // host copy of the two edge sets
   EdgeSets* d_es = nullptr;  // device copy (for kernels that select by *epar)
-  size_t ecap = 0;           // edges per set
+  size_t ecap = 0;           // edge slots per set (64 * slices * max_nb)
+  int nslices = 0;           // 64-particle slices
   unsigned* wflag[2] = {nullptr, nullptr};
   double* whist[2] = {nullptr, nullptr};
   // binning scratch
@@ -82,13 +83,14 @@ struct dem_system {
   int* tile_sum = nullptr;
   int* perm_src = nullptr;   // new -> old of the last resort
   int* old_to_new = nullptr;
-  int* nbr_cnt = nullptr;
+  int* slice_w = nullptr;    // 64 * widest list of each slice (rebuild scratch)
   int gncell = 0, ntiles = 0, ntiles_np = 0;
   // staging
   dem_particle_t* d_aos = nullptr;
   double* d_tmp = nullptr;
   size_t tmp_bytes = 0;
   double* d_epart = nullptr;
+  unsigned* part = nullptr;  // per-block pass counters
   // control
   Ctl* ctl = nullptr;
   int* rb = nullptr;  // rb[2]
@@ -141,7 +143,8 @@ static inline int gs_blocks(long long n) {
 static void free_state(dem_handle_t h) {
   for (int k = 0; k < 2; ++k) {
     dfree(h->pos[k]); dfree(h->kin[k]);
-    dfree(h->es.off[k]); dfree(h->es.nbr[k]); dfree(h->es.flag[k]); dfree(h->es.hist[k]);
+    dfree(h->es.cnt[k]); dfree(h->es.soff[k]);
+    dfree(h->es.nbr[k]); dfree(h->es.flag[k]); dfree(h->es.hist[k]);
     dfree(h->wflag[k]); dfree(h->whist[k]);
   }
   dfree(h->es.epar);
@@ -149,8 +152,8 @@ static void free_state(dem_handle_t h) {
   dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->id); dfree(h->id2);
   dfree(h->xref); dfree(h->key); dfree(h->cell_count);
   dfree(h->cell_start); dfree(h->cursor); dfree(h->cell_idx); dfree(h->tile_sum);
-  dfree(h->perm_src); dfree(h->old_to_new); dfree(h->nbr_cnt);
-  dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart);
+  dfree(h->perm_src); dfree(h->old_to_new); dfree(h->slice_w);
+  dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart); dfree(h->part);
   h->np = 0;
   h->has_state = false;
   h->tmp_bytes = 0;
@@ -168,6 +171,8 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
                 DEM_MAX_WALLS);
   if (p->method_order < 1 || p->method_order > 4 || p->n_euler < 1)
     return fail(nullptr, DEM_ERR_ARG, "method_order must be 1..4 and n_euler >= 1");
+  if (p->max_neighbours < 0 || p->max_neighbours > 64)
+    return fail(nullptr, DEM_ERR_ARG, "max_neighbours must be 0 (default 48) or 1..64");
   if (p->precision != DEM_FP64)
     return fail(nullptr, DEM_ERR_UNSUPPORTED, "precision mode %d is not built", p->precision);
   int dev_count = 0;
@@ -403,12 +408,15 @@ static int alloc_state(dem_handle_t h, int np) {
   const size_t N = (size_t)std::max(np, 1);
   const SysConst& s = h->sc;
   const size_t nring = (size_t)std::max(h->nprev, 1) * 9;
-  h->ecap = N * (size_t)s.max_nb;
+  h->nslices = (int)((N + 63) / 64);
+  h->ecap = (size_t)h->nslices * 64 * (size_t)s.max_nb;
+  h->es.ecap = (long long)h->ecap;
   hipError_t e = hipSuccess;
   for (int k = 0; k < 2 && e == hipSuccess; ++k) {
     e = dalloc(&h->pos[k], N);
     if (e == hipSuccess) e = dalloc(&h->kin[k], N);
-    if (e == hipSuccess) e = dalloc(&h->es.off[k], N + 1);
+    if (e == hipSuccess) e = dalloc(&h->es.cnt[k], N);
+    if (e == hipSuccess) e = dalloc(&h->es.soff[k], (size_t)h->nslices + 1);
     if (e == hipSuccess) e = dalloc(&h->es.nbr[k], h->ecap);
     if (e == hipSuccess) e = dalloc(&h->es.flag[k], h->ecap);
     if (e == hipSuccess) e = dalloc(&h->es.hist[k], 3 * h->ecap);
@@ -430,12 +438,13 @@ static int alloc_state(dem_handle_t h, int np) {
   if (e == hipSuccess) e = dalloc(&h->cursor, (size_t)h->gncell + 1);
   if (e == hipSuccess) e = dalloc(&h->cell_idx, N);
   h->ntiles = (h->gncell + SCAN_TILE - 1) / SCAN_TILE;
-  h->ntiles_np = ((int)N + SCAN_TILE - 1) / SCAN_TILE;
+  h->ntiles_np = (h->nslices + SCAN_TILE - 1) / SCAN_TILE;
   if (e == hipSuccess) e = dalloc(&h->tile_sum, (size_t)std::max(h->ntiles, h->ntiles_np) + 1);
   if (e == hipSuccess) e = dalloc(&h->perm_src, N);
   if (e == hipSuccess) e = dalloc(&h->old_to_new, N);
-  if (e == hipSuccess) e = dalloc(&h->nbr_cnt, N);
+  if (e == hipSuccess) e = dalloc(&h->slice_w, (size_t)h->nslices);
   if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
+  if (e == hipSuccess) e = dalloc(&h->part, 4 * (size_t)nblocks(N));
   if (e != hipSuccess) {
     free_state(h);
     return fail(h, DEM_ERR_DEVICE, "allocation for %d particles failed: %s", np,
@@ -452,7 +461,9 @@ static int alloc_state(dem_handle_t h, int np) {
 static int clear_contacts(dem_handle_t h) {
   const size_t N = (size_t)std::max(h->np, 1);
   for (int k = 0; k < 2; ++k) {
-    HIPCHK(h, hipMemsetAsync(h->es.off[k], 0, (N + 1) * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->es.cnt[k], 0, N * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->es.soff[k], 0, ((size_t)h->nslices + 1) * sizeof(int),
+                             h->stream));
     HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), h->stream));
   }
   HIPCHK(h, hipMemsetAsync(h->es.epar, 0, sizeof(int), h->stream));
@@ -498,13 +509,14 @@ static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
   enqueue_binning(h, pos, gate, force);
   const int nb = nblocks(s.np);
   hipLaunchKernelGGL(k_nbr_count, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key,
-                     h->cell_start, h->cell_idx, h->nbr_cnt, gate, force, h->ctl);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->nbr_cnt, s.np,
-                     h->tile_sum, gate, force, h->ctl);
+                     h->cell_start, h->cell_idx, h->es, h->slice_w, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
+                     h->nslices, h->tile_sum, gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles_np,
                      gate, force, h->ctl);
-  hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->nbr_cnt, s.np,
-                     h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
+                     h->nslices, h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, gate, force,
+                     h->ctl);
   hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key, h->cell_start,
                      h->cell_idx, h->es, src, o2n, (const Kin*)h->kin[b], h->xref, gate, force,
                      h->ctl);
@@ -702,24 +714,32 @@ static int download_ids(dem_handle_t h, std::vector<int>& idh) {
   return DEM_OK;
 }
 
-// The current edge set on the host: offsets, neighbours, flags, history.
+// The current edge set on the host, in the device's sliced-ELL layout.
 struct HostEdges {
   int par = 0;
-  std::vector<int> off, nbr;
+  size_t E = 0;  // used edge slots (soff of the last slice end)
+  std::vector<int> cnt, soff, nbr;
   std::vector<unsigned char> flag;
-  std::vector<double> hist;  // 6 per edge
+  std::vector<double2> hist;  // three planes of E entries
+  size_t edge(size_t i, int k) const { return (size_t)soff[i >> 6] + 64 * (size_t)k + (i & 63); }
 };
 
 static int download_edges(dem_handle_t h, HostEdges& he, bool with_hist) {
   const size_t N = (size_t)h->np;
+  const size_t ns = (size_t)h->nslices;
   hipStream_t st = h->stream;
   HIPCHK(h, hipMemcpyAsync(&he.par, h->es.epar, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  he.off.resize(N + 1);
-  HIPCHK(h, hipMemcpyAsync(he.off.data(), h->es.off[he.par], (N + 1) * sizeof(int),
+  he.cnt.resize(N);
+  he.soff.resize(ns + 1);
+  if (N)
+    HIPCHK(h, hipMemcpyAsync(he.cnt.data(), h->es.cnt[he.par], N * sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipMemcpyAsync(he.soff.data(), h->es.soff[he.par], (ns + 1) * sizeof(int),
                            hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  const size_t E = (size_t)he.off[N];
+  he.E = N ? (size_t)he.soff[ns] : 0;
+  const size_t E = he.E;
   he.nbr.resize(E);
   he.flag.resize(E);
   if (E) {
@@ -728,10 +748,10 @@ static int download_edges(dem_handle_t h, HostEdges& he, bool with_hist) {
     HIPCHK(h, hipMemcpyAsync(he.flag.data(), h->es.flag[he.par], E, hipMemcpyDeviceToHost, st));
   }
   if (with_hist) {
-    he.hist.resize(6 * E);
-    if (E)
-      HIPCHK(h, hipMemcpyAsync(he.hist.data(), (const double*)h->es.hist[he.par],
-                               6 * E * sizeof(double), hipMemcpyDeviceToHost, st));
+    he.hist.resize(3 * E);
+    for (int c = 0; c < 3 && E; ++c)
+      HIPCHK(h, hipMemcpyAsync(he.hist.data() + c * E, h->es.hist[he.par] + c * h->ecap,
+                               E * sizeof(double2), hipMemcpyDeviceToHost, st));
   }
   HIPCHK(h, hipStreamSynchronize(st));
   return DEM_OK;
@@ -829,9 +849,9 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
   HostEdges he;
   rc = download_edges(h, he, false);
   if (rc) return rc;
-  const size_t E = he.nbr.size();
+  const size_t E = he.E;
   std::vector<unsigned char> flag(E, 0);
-  std::vector<double> hist(6 * E, 0.0);
+  std::vector<double2> hist(3 * E, make_double2(0.0, 0.0));
   std::vector<std::pair<uint64_t, int64_t>> kv((size_t)std::max<int64_t>(npairs, 0));
   for (int64_t k = 0; k < npairs; ++k) {
     const long long a = (long long)(keys[k] >> 32) - 1, b = (long long)(keys[k] & 0xffffffffu) - 1;
@@ -842,17 +862,17 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
   std::sort(kv.begin(), kv.end());
   size_t used = 0;
   for (size_t i = 0; i < N; ++i) {
-    for (int e = he.off[i]; e < he.off[i + 1]; ++e) {
+    for (int m = 0; m < he.cnt[i]; ++m) {
+      const size_t e = he.edge(i, m);
       const int a = idh[i], b = idh[he.nbr[e]];
       const uint64_t key = ((uint64_t)(std::min(a, b) + 1) << 32) | (uint64_t)(std::max(a, b) + 1);
       auto it = std::lower_bound(kv.begin(), kv.end(), std::make_pair(key, (int64_t)-1));
       if (it == kv.end() || it->first != key) continue;
       const int64_t k = it->second;
       flag[e] = 1;
-      for (int d = 0; d < 3; ++d) {
-        hist[6 * (size_t)e + d] = ft[3 * k + d];
-        hist[6 * (size_t)e + 3 + d] = vt[3 * k + d];
-      }
+      hist[e] = make_double2(ft[3 * k], ft[3 * k + 1]);
+      hist[E + e] = make_double2(ft[3 * k + 2], vt[3 * k]);
+      hist[2 * E + e] = make_double2(vt[3 * k + 1], vt[3 * k + 2]);
       ++used;
     }
   }
@@ -876,8 +896,9 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
   hipStream_t st = h->stream;
   if (E) {
     HIPCHK(h, hipMemcpyAsync(h->es.flag[he.par], flag.data(), E, hipMemcpyHostToDevice, st));
-    HIPCHK(h, hipMemcpyAsync((double*)h->es.hist[he.par], hist.data(), 6 * E * sizeof(double),
-                             hipMemcpyHostToDevice, st));
+    for (int c = 0; c < 3; ++c)
+      HIPCHK(h, hipMemcpyAsync(h->es.hist[he.par] + c * h->ecap, hist.data() + c * E,
+                               E * sizeof(double2), hipMemcpyHostToDevice, st));
   }
   if (N) {
     HIPCHK(h, hipMemcpyAsync(h->wflag[0], wf.data(), N * sizeof(unsigned), hipMemcpyHostToDevice,
@@ -921,16 +942,16 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
   std::vector<Ent> pe, we;
   for (size_t i = 0; i < N; ++i) {
     const int a = idh[i];
-    for (int e = he.off[i]; e < he.off[i + 1]; ++e) {
+    for (int m = 0; m < he.cnt[i]; ++m) {
+      const size_t e = he.edge(i, m);
       if (!(he.flag[e] & 1)) continue;
       const int b = idh[he.nbr[e]];
       if (a >= b) continue;  // each pair is stored on both of its edges
       Ent x;
       x.key = ((uint64_t)(a + 1) << 32) | (uint64_t)(b + 1);
-      for (int d = 0; d < 3; ++d) {
-        x.f[d] = he.hist[6 * (size_t)e + d];
-        x.g[d] = he.hist[6 * (size_t)e + 3 + d];
-      }
+      const double2 p0 = he.hist[e], p1 = he.hist[he.E + e], p2 = he.hist[2 * he.E + e];
+      x.f[0] = p0.x; x.f[1] = p0.y; x.f[2] = p1.x;
+      x.g[0] = p1.y; x.g[1] = p2.x; x.g[2] = p2.y;
       pe.push_back(x);
     }
     for (int jw = 0; jw < s.nw; ++jw) {
@@ -979,9 +1000,19 @@ static PassArgs base_args(dem_handle_t h) {
   a.whist = h->whist[0];
   a.rb_clear = h->rb + p;
   a.rb_set = h->rb + (p ^ 1);
+  a.part = h->part;
   a.ctl = h->ctl;
   a.half_dt = 0.5 * h->dt;
   return a;
+}
+
+// One force evaluation: the pass kernel, then the sum of its counters.
+template <int MODE, bool UPDATE>
+static void launch_pass(dem_handle_t h, const PassArgs& a) {
+  const int nb = nblocks(h->np);
+  hipLaunchKernelGGL((k_pass<MODE, UPDATE>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream, h->sc, a);
+  hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl,
+                     (const unsigned*)a.part, nb, a.par, UPDATE ? 1 : 0);
 }
 
 static void after_eval(dem_handle_t h, bool update, int b) {
@@ -1015,9 +1046,9 @@ int dem_compute_accelerations_aos(dem_handle_t h, int32_t np, dem_particle_t* p,
   a.acc_out = h->acc;
   a.half_dt = 0.5 * time_step;
   if (update)
-    hipLaunchKernelGGL((k_pass<MODE_ACC, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    launch_pass<MODE_ACC, true>(h, a);
   else
-    hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    launch_pass<MODE_ACC, false>(h, a);
   HIPCHK(h, hipGetLastError());
   after_eval(h, update != 0, b);
   h->rc_ready = false;
@@ -1044,9 +1075,9 @@ int dem_compute_accelerations(dem_handle_t h, int32_t which, int32_t update) {
   a.kin_in = h->kin[b];
   a.acc_out = which ? h->acc_p : h->acc;
   if (update)
-    hipLaunchKernelGGL((k_pass<MODE_ACC, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    launch_pass<MODE_ACC, true>(h, a);
   else
-    hipLaunchKernelGGL((k_pass<MODE_ACC, false>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    launch_pass<MODE_ACC, false>(h, a);
   HIPCHK(h, hipGetLastError());
   after_eval(h, update != 0, b);
   h->rc_ready = false;
@@ -1083,8 +1114,7 @@ int dem_euler_startup(dem_handle_t h) {
     a.h = hs;
     a.hsq2 = 0.5 * (hs * hs);
     a.half_dt = 0.5 * hs;
-    hipLaunchKernelGGL((k_pass<MODE_EULER, true>), dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st,
-                       h->sc, a);
+    launch_pass<MODE_EULER, true>(h, a);
     after_eval(h, true, h->cur);
     h->cur ^= 1;
     enqueue_rebuild(h, h->cur, h->rb + (h->eval & 1), 0);
@@ -1112,19 +1142,25 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     int big = 0x7fffffff;
     HIPCHK(h, hipMemcpyAsync(&h->ctl->stop_step, &big, sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(h, hipMemsetAsync(&h->ctl->escaped, 0, sizeof(int), st));
+    HIPCHK(h, hipMemsetAsync(&h->ctl->esc_pending, 0, sizeof(int), st));
   }
   if (!h->rc_ready) {
     enqueue_refcell(h, h->cur);
     h->rc_ready = true;
   }
-  const int nb = nblocks(np);
   if (h->timing) h->timer.ensure((size_t)6 * nsteps + 2);
   size_t ev = 0;
   double t1 = 0, t2 = 0, tr = 0;
   bool first = true;
+  int64_t enqueued = 0;
   for (int64_t s = 0; s < nsteps; ++s) {
     if (h->resort_interval > 0 && h->steps_since_resort >= h->resort_interval) {
-      int rc = resort(h);
+      // a resort moves every particle array; never after an escape
+      Ctl c0;
+      int rc = read_ctl(h, &c0);
+      if (rc) return rc;
+      if (c0.escaped > 0) break;
+      rc = resort(h);
       if (rc) return rc;
     }
     // pass 1: forces at x_n (Update), AB predictor, ring store
@@ -1142,7 +1178,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     a.slot_wr = h->nprev > 0 ? h->ring + (size_t)(h->ipc[h->nprev - 1] - 1) * 9 * N : nullptr;
     a.check_pred = first ? 1 : 0;
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    hipLaunchKernelGGL((k_pass<MODE_PRED, true>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, a);
+    launch_pass<MODE_PRED, true>(h, a);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     after_eval(h, true, h->cur);
     if (h->nprev > 0) {
@@ -1166,7 +1202,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     b.ab0 = h->ab0;
     b.step_idx = (int)s;
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    hipLaunchKernelGGL((k_pass<MODE_CORR, false>), dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, b);
+    launch_pass<MODE_CORR, false>(h, b);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     after_eval(h, false, h->cur ^ 1);
     h->acc_in_ring = h->nprev > 0;
@@ -1176,6 +1212,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     h->steps_since_resort += 1;
     first = false;
+    enqueued = s + 1;
   }
   HIPCHK(h, hipGetLastError());
   Ctl c;
@@ -1195,8 +1232,22 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     if (ev >= 2 && hipEventElapsedTime(&ms, h->timer.ev[0], h->timer.ev[ev - 1]) == hipSuccess)
       h->ms_step = ms / n;
   }
-  int64_t done = nsteps;
-  if (c.escaped > 0) done = (int64_t)c.stop_step + 1;
+  int64_t done = enqueued;
+  if (c.escaped > 0) {
+    // The kernels of the steps after the escape returned at once: undo their
+    // host-side bookkeeping (ring rotation, evaluation count).
+    done = (int64_t)c.stop_step + 1;
+    for (int64_t k = done; k < enqueued; ++k) {
+      if (h->nprev > 0) {
+        const int first_slot = h->ipc[0];
+        for (int q = 0; q + 1 < h->nprev; ++q) h->ipc[q] = h->ipc[q + 1];
+        h->ipc[h->nprev - 1] = first_slot;
+      }
+      h->eval -= 2;
+      h->steps_since_resort -= 1;
+    }
+    h->last_update_eval = h->eval - 2;
+  }
   h->steps += done;
   if (steps_done) *steps_done = done;
   if (escaped) *escaped = c.escaped;

@@ -9,25 +9,36 @@
 //                       cell-list entry stamp) of the positions in pos[b]
 //   ring    double[nprev][9][N]  ABM ring Previous(:,i) (src/MDEM.f90:14-24)
 //   edges (two sets, selected by the device word *epar; a rebuild writes the
-//         other set and flips it):
-//         off  int[N+1]    CSR offsets of each particle's directed list
-//         nbr  int[E]      neighbour j of edge e
-//         flag u8[E]       bit 0: the pair has stored history
-//         hist double2[3E] tangential force FT | velocity VT of the pair as the
-//                          reference stores them (TPartner%Vector1/Vector2)
+//         other set and flips it), sliced ELL with 64-particle slices (one
+//         slice per wavefront): the k-th neighbour of particle i is edge
+//         e = soff[i/64] + 64*k + i%64, so a wavefront reading the k-th
+//         edge of its 64 particles touches one contiguous run.
+//         cnt  int[N]          neighbours of each particle
+//         soff int[N/64+1]     first edge of each slice (64 * slice width)
+//         nbr  int[E]          neighbour j of edge e
+//         flag u8[E]           bit 0: the pair has stored history
+//         hist double2[3][E]   planes (FTx,FTy) (FTz,VTx) (VTy,VTz): tangential
+//                              force FT and velocity VT of the pair as the
+//                              reference stores them (TPartner%Vector1/Vector2)
 //   wall history: wflag uint[N] (bit w: contact with wall w), whist [nw][6][N]
 //
-// A force pass runs one 256-lane workgroup per 256 consecutive particles.
-// The block stages its particles in LDS, then walks their edges 256 at a
-// time, one lane per edge (independent gathers, full wavefronts), writes each
-// contact's normal force, tangential force and torque to LDS, and every
-// particle lane sums its own edges in edge order: an atomic-free segmented
-// reduction, bitwise reproducible.  The same lanes then apply the integrator
-// to their particle.
+// A force pass runs one lane per particle.  Each lane walks its own edges in
+// list order (all lanes of a wavefront read their k-th edge together),
+// gathers the partner, evaluates the contact and accumulates force and
+// torque in registers: no atomics and no inter-lane traffic, so the sums are
+// bitwise reproducible.  The same lane then applies the integrator to its
+// particle.
 #pragma once
 #pragma clang fp contract(off)
 
 #include "physics.hpp"
+
+#ifndef DEM_PASS_WAVES
+#define DEM_PASS_WAVES 2
+#endif
+// Occupancy target of the force passes (wavefronts per SIMD): the register
+// budget is 512 / DEM_PASS_WAVES VGPRs.
+#define DEM_PASS_ATTR __attribute__((amdgpu_waves_per_eu(DEM_PASS_WAVES, 8)))
 
 enum { MODE_ACC = 0, MODE_EULER = 1, MODE_PRED = 2, MODE_CORR = 3 };
 
@@ -86,12 +97,18 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 struct EdgeSets {
-  int* off[2];
+  int* cnt[2];
+  int* soff[2];
   int* nbr[2];
   unsigned char* flag[2];
-  double2* hist[2];
+  double2* hist[2];  // three planes of ecap entries each
+  long long ecap;
   int* epar;  // device word: index of the current set
 };
+
+__device__ __forceinline__ long long sell_edge(const int* soff, int i, int k) {
+  return (long long)soff[i >> 6] + 64LL * k + (i & 63);
+}
 
 struct PassArgs {
   // evaluated state (Particle or Predicted): positions, kin with cell state
@@ -120,6 +137,7 @@ struct PassArgs {
   double* whist;
   int* rb_clear;             // rebuild flag consumed before this evaluation
   int* rb_set;               // rebuild flag for the next evaluation
+  unsigned* part;            // per-block counters: collisions, pairs, walls, stored
   Ctl* ctl;
 };
 
@@ -129,44 +147,151 @@ __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
   o[2] = make_double2(b.y, b.z);
 }
 
-template <int MODE, bool UPDATE>
-__global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
-  __shared__ double4 sh_pos[DEM_BLOCK];
-  __shared__ double2 sh_vw[DEM_BLOCK * 3];
-  __shared__ int sh_rc[DEM_BLOCK];
-  __shared__ int sh_st[DEM_BLOCK];
-  __shared__ int sh_id[DEM_BLOCK];
-  __shared__ int sh_off[DEM_BLOCK + 1];
-  __shared__ double sh_c[9 * DEM_BLOCK];   // per edge: FN, FT, torque contribution
-  __shared__ unsigned char sh_ok[DEM_BLOCK];
+// Component tables of up to DEM_LDS_NC components are staged in LDS by every
+// force pass, so the per-edge lookups of the pair constants (indexed by the
+// partner's component) cost an LDS read instead of a cache round trip.
+#define DEM_LDS_NC 16
 
+// Pair contacts of one particle, in list order (src/MForces.f90:203-271).
+// Software pipeline over the edges: while edge k is evaluated, the partner
+// records and history of edge k+1 and the index and flag of edge k+2 are in
+// flight, so the memory latency overlaps the arithmetic.
+struct EdgeAcc {
+  d3 F, T;
+  unsigned long long n_col, n_pair, n_stored;
+  int err;
+};
+
+struct EdgeData {
+  double4 Q;
+  Kin K;
+  double2 h0, h1, h2;
+};
+
+__device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
+                                          const double2* __restrict__ hist, long long ecap,
+                                          long long e, int j, unsigned char fl) {
+  d.Q = a.pos_in[j];
+  d.K = load_kin(a.kin_in + j);
+  if (fl & 1) {
+    d.h0 = hist[e];
+    d.h1 = hist[ecap + e];
+    d.h2 = hist[2 * ecap + e];
+  } else {
+    d.h0 = d.h1 = d.h2 = make_double2(0.0, 0.0);
+  }
+}
+
+template <bool UPDATE>
+__device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
+                                          const PairConst* ptab, const CompConst* ctab,
+                                          const int* __restrict__ nbr,
+                                          unsigned char* __restrict__ flag,
+                                          double2* __restrict__ hist, long long ecap,
+                                          long long e, int deg, const double4& P, d3 vi, d3 wi,
+                                          int ci, double ri, int cell_p, int st_p, int id_p,
+                                          EdgeAcc& acc) {
+  if (deg <= 0) return;
+  int j_n = 0;
+  unsigned char fl_c = flag[e], fl_n = 0;
+  EdgeData nx;
+  load_edge(nx, a, hist, ecap, e, nbr[e], fl_c);
+  if (deg > 1) {
+    j_n = nbr[e + 64];
+    fl_n = flag[e + 64];
+  }
+  for (int k = 0; k < deg; ++k, e += 64) {
+    const EdgeData cur = nx;
+    const unsigned char fl = fl_c;
+    if (k + 1 < deg) {
+      load_edge(nx, a, hist, ecap, e + 64, j_n, fl_n);
+      fl_c = fl_n;
+    }
+    if (k + 2 < deg) {
+      j_n = nbr[e + 128];
+      fl_n = flag[e + 128];
+    }
+    const double4 Q = cur.Q;
+    const Kin& KJ = cur.K;
+    const double2 h0 = cur.h0, h1 = cur.h1, h2 = cur.h2;
+    const int cj = (int)Q.w;
+    const d3 rij = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
+    const double dsq = dot3(rij, rij);
+    const PairConst& pc = ptab[ci + cj * s.nc];
+    const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
+    if (delta > 0.0) {
+      int role;
+      if (s.hist_mode == 0) {
+        role = ref_role(s, cell_p, KJ.rc, st_p, KJ.st, id_p, KJ.id);
+        if (role < 0) { acc.err |= DEM_KERR_ROLE; role = id_p < KJ.id; }
+      } else {
+        role = id_p < KJ.id;
+      }
+      const double sg = role ? 1.0 : -1.0;
+      ContactIn in;
+      in.contacting = fl & 1;
+      in.ft0 = scl3(sg, mk3(h0.x, h0.y, h1.x));
+      in.vt0 = scl3(sg, mk3(h1.y, h2.x, h2.y));
+      if (!in.contacting && role) ++acc.n_col;
+      if (role) ++acc.n_pair;
+      const double rj = ctab[cj].radius;
+      const double dij = sqrt(dsq);
+      in.nij = div3(rij, dij);
+      in.disp_n = scl3(dij - pc.mindij, in.nij);
+      in.vij = sub3(kin_v(KJ), vi);
+      in.vijc = sub3(in.vij, cross3(add3(scl3(ri, wi), scl3(rj, kin_w(KJ))), in.nij));
+      in.delta_dsq = delta;
+      in.half_dt = a.half_dt;
+      const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
+      // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381); torque of
+      // "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
+      acc.F = add3(add3(acc.F, o.fn), o.ft);
+      acc.T = add3(acc.T, role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft)));
+      if (UPDATE) {
+        const d3 f = scl3(sg, o.ft), g = scl3(sg, o.vt);
+        hist[e] = make_double2(f.x, f.y);
+        hist[ecap + e] = make_double2(f.z, g.x);
+        hist[2 * ecap + e] = make_double2(g.y, g.z);
+        if (!(fl & 1)) flag[e] = 1;
+        ++acc.n_stored;
+      }
+    } else if (UPDATE && (fl & 1)) {
+      // The reference removes a stored pair only when its cell loop visits
+      // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
+      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, KJ.rc);
+      if (keep) ++acc.n_stored;
+      else flag[e] = 0;
+    }
+  }
+}
+
+template <int MODE, bool UPDATE>
+__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, PassArgs a) {
+  __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
+  __shared__ CompConst sh_comp[DEM_LDS_NC];
   Ctl* ctl = a.ctl;
-  if (*(volatile int*)&ctl->escaped) return;
+  // Escapes are published between passes (k_pass_reduce), so every block of
+  // a pass reads the same value here.
+  if (ctl->escaped) return;
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t == 0) {
-    *a.rb_clear = 0;
-    ctl->pairs[a.par ^ 1] = 0;
-    ctl->wallc[a.par ^ 1] = 0;
-    if (UPDATE) ctl->stored[a.par ^ 1] = 0;
+  if (blockIdx.x == 0 && t == 0) *a.rb_clear = 0;
+  const bool lds_tab = s.nc <= DEM_LDS_NC;
+  if (lds_tab) {
+    for (int q = t; q < s.nc * s.nc; q += DEM_BLOCK) sh_pair[q] = s.pair[q];
+    if (t < s.nc) sh_comp[t] = s.comp[t];
   }
   const size_t N = (size_t)s.np;
-  const int ep = *(volatile const int*)a.E.epar;
-  const int* __restrict__ off = a.E.off[ep];
-  const int* __restrict__ nbr = a.E.nbr[ep];
-  unsigned char* __restrict__ flag = a.E.flag[ep];
-  double2* __restrict__ hist = a.E.hist[ep];
+  const int ep = *a.E.epar;
 
-  const int p0 = xcd_block() * DEM_BLOCK;
-  const int nb = min(DEM_BLOCK, s.np - p0);
-  const int i = p0 + t;
-  const bool active = t < nb;
-  unsigned long long n_col = 0, n_pair = 0, n_wall = 0, n_stored = 0;
-  int err = 0, need_rb = 0, n_esc = 0;
+  const int i = xcd_block() * DEM_BLOCK + t;
+  const bool active = i < s.np;
+  unsigned long long n_wall = 0;
+  int need_rb = 0, n_esc = 0;
 
-  // ---- stage the block's particles
   double4 P = make_double4(0, 0, 0, 0);
   d3 vi = mk3(0, 0, 0), wi = vi;
-  int cell_p = 0, st_p = 0, id_p = 0;
+  int cell_p = 0, st_p = 0, id_p = 0, deg = 0;
+  long long e = 0;
   if (active) {
     P = a.pos_in[i];
     const Kin K = load_kin(a.kin_in + i);
@@ -175,121 +300,29 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
     cell_p = K.rc;
     st_p = K.st;
     id_p = K.id;
-    sh_pos[t] = P;
-    sh_vw[3 * t] = K.v0;
-    sh_vw[3 * t + 1] = K.v1;
-    sh_vw[3 * t + 2] = K.v2;
-    sh_rc[t] = cell_p;
-    sh_st[t] = st_p;
-    sh_id[t] = id_p;
-    sh_off[t] = off[i];
+    deg = a.E.cnt[ep][i];
+    e = sell_edge(a.E.soff[ep], i, 0);
   }
-  if (t == 0) sh_off[nb] = off[p0 + nb];
   const int ci = (int)P.w;
   const CompConst cc = s.comp[active ? ci : 0];
   const double ri = cc.radius;
+  EdgeAcc acc;
   // iForce = EffectiveMass*Gravity (src/MForces.f90:190)
-  d3 F = scl3(cc.eff_mass, mk3(s.grav[0], s.grav[1], s.grav[2]));
-  d3 T = mk3(0.0, 0.0, 0.0);
-  __syncthreads();
-
-  // ---- particle-pair contacts, one lane per directed edge
-  const int e_begin = sh_off[0], e_end = sh_off[nb];
-  for (int c0 = e_begin; c0 < e_end; c0 += DEM_BLOCK) {
-    const int e = c0 + t;
-    unsigned char ok = 0;
-    if (e < e_end) {
-      int lo = 0, hi = nb - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sh_off[mid] <= e) lo = mid;
-        else hi = mid - 1;
-      }
-      const int li = lo;
-      // Everything the edge may need is requested before the overlap test:
-      // the edge record and its history (coalesced), then both records of
-      // the partner together, so a contact costs two memory round trips.
-      const int j = nbr[e];
-      const unsigned char fl = flag[e];
-      const double2* hp = hist + 3 * (size_t)e;
-      const double2 h0 = hp[0], h1 = hp[1], h2 = hp[2];
-      const double4 Q = a.pos_in[j];
-      const Kin KJ = load_kin(a.kin_in + j);
-      const double4 Pi = sh_pos[li];
-      const int cie = (int)Pi.w;
-      const int cj = (int)Q.w;
-      const d3 rij = min_image3(s, mk3(Q.x - Pi.x, Q.y - Pi.y, Q.z - Pi.z));
-      const double dsq = dot3(rij, rij);
-      const PairConst& pc = s.pair[cie + cj * s.nc];
-      const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
-      if (delta > 0.0) {
-        const int idi = sh_id[li];
-        int role;
-        if (s.hist_mode == 0) {
-          role = ref_role(s, sh_rc[li], KJ.rc, sh_st[li], KJ.st, idi, KJ.id);
-          if (role < 0) { err |= DEM_KERR_ROLE; role = idi < KJ.id; }
-        } else {
-          role = idi < KJ.id;
-        }
-        const double sg = role ? 1.0 : -1.0;
-        ContactIn in;
-        in.contacting = fl & 1;
-        if (in.contacting) {
-          in.ft0 = scl3(sg, mk3(h0.x, h0.y, h1.x));
-          in.vt0 = scl3(sg, mk3(h1.y, h2.x, h2.y));
-        } else {
-          in.ft0 = mk3(0.0, 0.0, 0.0);
-          in.vt0 = in.ft0;
-          if (role) ++n_col;
-        }
-        if (role) ++n_pair;
-        const double rip = s.comp[cie].radius;
-        const double rj = s.comp[cj].radius;
-        const double2 s0 = sh_vw[3 * li], s1 = sh_vw[3 * li + 1], s2 = sh_vw[3 * li + 2];
-        const d3 vie = mk3(s0.x, s0.y, s1.x), wie = mk3(s1.y, s2.x, s2.y);
-        const double dij = sqrt(dsq);
-        in.nij = div3(rij, dij);
-        in.disp_n = scl3(dij - pc.mindij, in.nij);
-        in.vij = sub3(kin_v(KJ), vie);
-        in.vijc = sub3(in.vij, cross3(add3(scl3(rip, wie), scl3(rj, kin_w(KJ))), in.nij));
-        in.delta_dsq = delta;
-        in.half_dt = a.half_dt;
-        const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
-        // torque of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
-        const d3 tq = role ? cross3(scl3(rip, in.nij), o.ft) : scl3(rip, cross3(in.nij, o.ft));
-        double* c = sh_c + 9 * t;
-        c[0] = o.fn.x; c[1] = o.fn.y; c[2] = o.fn.z;
-        c[3] = o.ft.x; c[4] = o.ft.y; c[5] = o.ft.z;
-        c[6] = tq.x; c[7] = tq.y; c[8] = tq.z;
-        ok = 1;
-        if (UPDATE) {
-          store_d3x2(hist + 3 * (size_t)e, scl3(sg, o.ft), scl3(sg, o.vt));
-          if (!(fl & 1)) flag[e] = 1;
-          ++n_stored;
-        }
-      } else if (UPDATE && (fl & 1)) {
-        // The reference removes a stored pair only when its cell loop visits
-        // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
-        const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, sh_rc[li], KJ.rc);
-        if (keep) ++n_stored;
-        else flag[e] = 0;
-      }
-    }
-    sh_ok[t] = ok;
+  acc.F = scl3(cc.eff_mass, mk3(s.grav[0], s.grav[1], s.grav[2]));
+  acc.T = mk3(0.0, 0.0, 0.0);
+  acc.n_col = acc.n_pair = acc.n_stored = 0;
+  acc.err = 0;
+  if (lds_tab) {
     __syncthreads();
-    if (active) {
-      const int b = max(sh_off[t], c0), f = min(sh_off[t + 1], c0 + DEM_BLOCK);
-      for (int q = b; q < f; ++q) {
-        const int k = q - c0;
-        if (!sh_ok[k]) continue;
-        const double* c = sh_c + 9 * k;
-        // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381)
-        F = add3(add3(F, mk3(c[0], c[1], c[2])), mk3(c[3], c[4], c[5]));
-        T = add3(T, mk3(c[6], c[7], c[8]));
-      }
-    }
-    __syncthreads();
+    pair_loop<UPDATE>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+                      e, deg, P, vi, wi, ci, ri, cell_p, st_p, id_p, acc);
+  } else {
+    pair_loop<UPDATE>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+                      e, deg, P, vi, wi, ci, ri, cell_p, st_p, id_p, acc);
   }
+  d3 F = acc.F, T = acc.T;
+  unsigned long long n_col = acc.n_col, n_pair = acc.n_pair, n_stored = acc.n_stored;
+  int err = acc.err;
 
   if (active) {
     const d3 xi = mk3(P.x, P.y, P.z);
@@ -434,24 +467,59 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_pass(SysConst s, PassArgs a) {
       if (!(x.x == x.x && x.y == x.y && x.z == x.z)) err |= DEM_KERR_NAN;
     }
   }
-  // block reductions -> one atomic per wavefront
-  n_col = wave_sum_u64(n_col);
-  n_pair = wave_sum_u64(n_pair);
-  n_wall = wave_sum_u64(n_wall);
-  n_stored = wave_sum_u64(n_stored);
+  // Counters: block sums to a per-block slot (k_pass_reduce adds them up);
+  // flags, which are rare, by atomics.
+  __shared__ unsigned sh_cnt[DEM_BLOCK / 64][4];
+  const unsigned c0 = (unsigned)wave_sum_u64(n_col), c1 = (unsigned)wave_sum_u64(n_pair);
+  const unsigned c2 = (unsigned)wave_sum_u64(n_wall), c3 = (unsigned)wave_sum_u64(n_stored);
   err = wave_or(err);
   need_rb = wave_or(need_rb);
   n_esc = (int)wave_sum_u64((unsigned long long)n_esc);
   if ((t & 63) == 0) {
-    if (n_col) atomicAdd(&ctl->ncol, n_col);
-    if (n_pair) atomicAdd(&ctl->pairs[a.par], n_pair);
-    if (n_wall) atomicAdd(&ctl->wallc[a.par], n_wall);
-    if (UPDATE && n_stored) atomicAdd(&ctl->stored[a.par], n_stored);
+    sh_cnt[t >> 6][0] = c0;
+    sh_cnt[t >> 6][1] = c1;
+    sh_cnt[t >> 6][2] = c2;
+    sh_cnt[t >> 6][3] = c3;
     if (err) atomicOr(&ctl->error, err);
     if (need_rb) atomicOr(a.rb_set, 1);
     if (MODE == MODE_CORR && n_esc) {
-      atomicAdd(&ctl->escaped, n_esc);
+      atomicAdd(&ctl->esc_pending, n_esc);
       atomicMin(&ctl->stop_step, a.step_idx);
+    }
+  }
+  __syncthreads();
+  if (t < 4) {
+    unsigned v = 0;
+    for (int w = 0; w < DEM_BLOCK / 64; ++w) v += sh_cnt[w][t];
+    a.part[4 * (size_t)blockIdx.x + t] = v;
+  }
+}
+
+// Sums the per-block counters of the pass just run into the control block
+// and publishes the escapes its corrector found.  Runs as one block.
+__global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* part, int nblk,
+                                                     int par, int update) {
+  if (ctl->escaped) return;  // the pass did not run
+  __shared__ unsigned long long sh[1024 / 64][4];
+  const int t = threadIdx.x;
+  unsigned long long v[4] = {0, 0, 0, 0};
+  for (int b = t; b < nblk; b += 1024)
+    for (int c = 0; c < 4; ++c) v[c] += part[4 * (size_t)b + c];
+  for (int c = 0; c < 4; ++c) v[c] = wave_sum_u64(v[c]);
+  if ((t & 63) == 0)
+    for (int c = 0; c < 4; ++c) sh[t >> 6][c] = v[c];
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long s[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 1024 / 64; ++w)
+      for (int c = 0; c < 4; ++c) s[c] += sh[w][c];
+    ctl->ncol += s[0];
+    ctl->pairs[par] = s[1];
+    ctl->wallc[par] = s[2];
+    if (update) ctl->stored[par] = s[3];
+    if (ctl->esc_pending) {
+      ctl->escaped = ctl->esc_pending;
+      ctl->esc_pending = 0;
     }
   }
 }
@@ -500,8 +568,8 @@ __global__ void k_check_disp(SysConst s, const double4* pos, const double4* xref
 // rebuild can be enqueued every step without a host round trip.
 
 __device__ __forceinline__ bool gate_closed(const int* gate, int force, const Ctl* ctl) {
-  if (*(volatile const int*)&ctl->escaped) return true;
-  return !force && !*(volatile const int*)gate;
+  if (ctl->escaped) return true;
+  return !force && !*gate;
 }
 
 __global__ void k_fill_i32(int* p, int n, int v, const int* gate, int force, const Ctl* ctl) {
@@ -580,14 +648,14 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int nti
 
 // Exclusive scan of in[0..n) into start[0..n] (start[n] = total); cursor (may
 // be null) receives a copy of start[0..n).  With out_sel != null the output
-// is the offsets array of the edge set being built (1 - *epar).
+// is the slice offsets of the edge set being built (1 - *epar).
 __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
                                                           const int* tile_off, int* start,
                                                           int* cursor, const EdgeSets* out_sel,
                                                           const int* gate, int force,
                                                           const Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
-  if (out_sel) start = out_sel->off[1 - *(volatile const int*)out_sel->epar];
+  if (out_sel) start = out_sel->soff[1 - *out_sel->epar];
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
@@ -679,12 +747,16 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
   }
 }
 
-// Pass 1 of the list build: neighbours per particle (capped at max_nb).
+// Pass 1 of the list build: neighbours per particle (capped at max_nb) into
+// cnt of the set being built, and 64 * (widest list) of each 64-particle
+// slice into slice_w.
 __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const double4* pos,
                                                          const int* key, const int* start,
-                                                         const int* cell_idx, int* cnt,
-                                                         const int* gate, int force, Ctl* ctl) {
+                                                         const int* cell_idx, EdgeSets E,
+                                                         int* slice_w, const int* gate, int force,
+                                                         Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
+  const int q = 1 - *E.epar;
   const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
   int n = 0, err = 0;
   if (i < s.np) {
@@ -694,11 +766,13 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
       err = DEM_KERR_NBR_OVERFLOW;
       n = s.max_nb;
     }
-    cnt[i] = n;
+    E.cnt[q][i] = n;
   }
   err = wave_or(err);
   const int hw = wave_max(n);
+  const int w0 = i & ~63;
   if ((threadIdx.x & 63) == 0) {
+    if (w0 < s.np) slice_w[w0 >> 6] = 64 * hw;
     if (err) atomicOr(&ctl->error, err);
     atomicMax(&ctl->max_nnb, hw);
   }
@@ -717,44 +791,51 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
                                                         const Kin* kin_cur, double4* xref,
                                                         const int* gate, int force, Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
-  const int p = *(volatile const int*)E.epar, q = 1 - p;
+  const int p = *E.epar, q = 1 - p;
   const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
+  const long long ecap = E.ecap;
   unsigned long long nedges = 0, dropped = 0;
   if (i < s.np) {
     const double4 P = pos[i];
     const int o = src ? src[i] : i;
-    const int ob = E.off[p][o], oe = E.off[p][o + 1];
+    const int odeg = E.cnt[p][o];
+    const long long ob = sell_edge(E.soff[p], o, 0);
     const int* onbr = E.nbr[p];
     const unsigned char* oflag = E.flag[p];
     const double2* ohist = E.hist[p];
-    const int nb0 = E.off[q][i], nb1 = E.off[q][i + 1];
-    int e = nb0;
+    const int ndeg = E.cnt[q][i];
+    long long e = sell_edge(E.soff[q], i, 0);
+    int k = 0;
     unsigned long long found = 0;
     for_each_neighbour(s, pos, i, P, key[i], start, cell_idx, [&](int j) {
-      if (e >= nb1) return;
+      if (k >= ndeg) return;
       E.nbr[q][e] = j;
       const int jo = src ? src[j] : j;
       unsigned char fl = 0;
-      for (int m = ob; m < oe; ++m) {
-        if (onbr[m] == jo) {
-          fl = oflag[m];
+      for (int m = 0; m < odeg; ++m) {
+        const long long om = ob + 64LL * m;
+        if (onbr[om] == jo) {
+          fl = oflag[om];
           if (fl & 1) {
-            const double2* h = ohist + 3 * (size_t)m;
-            double2* g = E.hist[q] + 3 * (size_t)e;
-            g[0] = h[0]; g[1] = h[1]; g[2] = h[2];
+            double2* g = E.hist[q];
+            g[e] = ohist[om];
+            g[ecap + e] = ohist[ecap + om];
+            g[2 * ecap + e] = ohist[2 * ecap + om];
           }
-          if (m - ob < 64) found |= 1ull << (m - ob);
+          found |= 1ull << m;
           break;
         }
       }
       E.flag[q][e] = fl;
-      ++e;
+      ++k;
+      e += 64;
     });
-    nedges = (unsigned long long)(nb1 - nb0);
-    for (int m = ob; m < oe; ++m) {
-      if (!(oflag[m] & 1)) continue;
-      if (m - ob < 64 && (found >> (m - ob)) & 1ull) continue;
-      const int jn = o2n ? o2n[onbr[m]] : onbr[m];
+    nedges = (unsigned long long)ndeg;
+    for (int m = 0; m < odeg; ++m) {
+      const long long om = ob + 64LL * m;
+      if (!(oflag[om] & 1)) continue;
+      if ((found >> m) & 1ull) continue;
+      const int jn = o2n ? o2n[onbr[om]] : onbr[om];
       if (s.hist_mode == 0 && !ref_adjacent(s, kin_cur[i].rc, kin_cur[jn].rc)) ++dropped;
     }
     xref[i] = P;

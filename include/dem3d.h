@@ -86,7 +86,7 @@ typedef struct {
                                InitCellStructure3D does (src/MCells.f90:155)  */
   int32_t precision;        /* DEM_FP64 or DEM_MIXED                          */
   int32_t history_mode;     /* DEM_HIST_REFERENCE or DEM_HIST_CANONICAL       */
-  int32_t max_neighbours;   /* neighbour-list capacity per particle, 0 = 64   */
+  int32_t max_neighbours;   /* neighbour-list capacity per particle (1..64), 0 = 48 */
   int32_t max_contacts;     /* contact-table capacity per particle, 0 = 24    */
   double lbox[3];           /* Lbox                                            */
   double gravity[3];        /* Gravity                                         */
