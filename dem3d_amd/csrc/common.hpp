@@ -71,11 +71,14 @@ struct SysConst {
   int max_nb, max_c;
   // reference grid (src/MCells.f90:155-227)
   int rn[3];
-  int pad0;
+  int rc_pack;  // every axis has <= 1024 cells: Kin carries packed coordinates
   double rn_by_l[3], rn_by2[3];
-  // neighbour-search grid
+  // neighbour-search grid: gn cells per axis, gb = ceil(gn/8) bricks,
+  // gncell = 512 * bricks keys (see grid_key)
   int gn[3];
+  int gb[3];
   int gncell;
+  int pad1;
   double g_by_l[3], g_by2[3];
   double lbox[3], invl[3], phalf[3];
   double grav[3];

@@ -231,21 +231,23 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
                   "(with fewer the reference visits pairs twice)", rn[d], d + 1);
     }
     s.rn[d] = rn[d];
+    s.rc_pack = (d == 0 ? 1 : s.rc_pack) && rn[d] <= 1024;
     s.rn_by_l[d] = (double)rn[d] / p->lbox[d];
     s.rn_by2[d] = (double)rn[d] * 0.5;
   }
   // neighbour-search grid: cells of at least 2*rmax + skin
   const double rcmax = 2.0 * rmax + h->skin;
-  long long gn = 1;
+  long long gn = 512;  // keys: 8x8x8 cells per brick
   for (int d = 0; d < 3; ++d) {
     s.gn[d] = std::max(1, (int)(p->lbox[d] / rcmax));
+    s.gb[d] = (s.gn[d] + 7) / 8;
     s.g_by_l[d] = (double)s.gn[d] / p->lbox[d];
     s.g_by2[d] = (double)s.gn[d] * 0.5;
-    gn *= s.gn[d];
+    gn *= s.gb[d];
   }
   if (gn > (1LL << 30)) {
     delete h;
-    return fail(nullptr, DEM_ERR_ARG, "neighbour grid too large (%lld cells)", gn);
+    return fail(nullptr, DEM_ERR_ARG, "neighbour grid too large (%lld cell keys)", gn);
   }
   s.gncell = (int)gn;
   h->gncell = (int)gn;
@@ -508,8 +510,8 @@ static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
   const double4* pos = h->pos[b];
   enqueue_binning(h, pos, gate, force);
   const int nb = nblocks(s.np);
-  hipLaunchKernelGGL(k_nbr_count, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key,
-                     h->cell_start, h->cell_idx, h->es, h->slice_w, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_nbr_count, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->cell_start,
+                     h->cell_idx, h->es, h->slice_w, gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
                      h->nslices, h->tile_sum, gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles_np,
@@ -517,7 +519,7 @@ static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
   hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
                      h->nslices, h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, gate, force,
                      h->ctl);
-  hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->key, h->cell_start,
+  hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->cell_start,
                      h->cell_idx, h->es, src, o2n, (const Kin*)h->kin[b], h->xref, gate, force,
                      h->ctl);
   hipLaunchKernelGGL(k_nbr_flip, dim3(1), dim3(64), 0, st, h->es, gate, force, h->ctl);
@@ -634,7 +636,7 @@ int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const doub
     k.id = (int)i;
     k.rc = 0;
     k.st = cell_rank ? cell_rank[i] : 0;
-    k.pad = 0;
+    k.rxyz = 0;
     for (int d = 0; d < 3; ++d) {
       if (a) ha[6 * i + d] = a[d * N + i];
       if (alpha) ha[6 * i + 3 + d] = alpha[d * N + i];

@@ -34,7 +34,7 @@
 #include "physics.hpp"
 
 #ifndef DEM_PASS_WAVES
-#define DEM_PASS_WAVES 2
+#define DEM_PASS_WAVES 3
 #endif
 // Occupancy target of the force passes (wavefronts per SIMD): the register
 // budget is 512 / DEM_PASS_WAVES VGPRs.
@@ -47,7 +47,7 @@ struct __align__(64) Kin {
   int id;              // reference id, 0-based
   int rc;              // reference-grid cell (0-based) of the positions
   int st;              // entry stamp into that cell's list
-  int pad;
+  int rxyz;            // packed coordinates of rc (when SysConst.rc_pack)
 };
 
 __device__ __forceinline__ Kin load_kin(const Kin* k) {
@@ -60,16 +60,16 @@ __device__ __forceinline__ Kin load_kin(const Kin* k) {
   r.id = d.x;
   r.rc = d.y;
   r.st = d.z;
-  r.pad = d.w;
+  r.rxyz = d.w;
   return r;
 }
 
-__device__ __forceinline__ void store_kin(Kin* k, d3 v, d3 w, int id, int rc, int st) {
+__device__ __forceinline__ void store_kin(Kin* k, d3 v, d3 w, int id, RefCell rc, int st) {
   double2* q = reinterpret_cast<double2*>(k);
   q[0] = make_double2(v.x, v.y);
   q[1] = make_double2(v.z, w.x);
   q[2] = make_double2(w.y, w.z);
-  reinterpret_cast<int4*>(k)[3] = make_int4(id, rc, st, 0);
+  reinterpret_cast<int4*>(k)[3] = make_int4(id, rc.lin, st, rc.xyz);
 }
 
 __device__ __forceinline__ d3 kin_v(const Kin& k) { return mk3(k.v0.x, k.v0.y, k.v1.x); }
@@ -153,9 +153,11 @@ __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
 #define DEM_LDS_NC 16
 
 // Pair contacts of one particle, in list order (src/MForces.f90:203-271).
-// Software pipeline over the edges: while edge k is evaluated, the partner
-// records and history of edge k+1 and the index and flag of edge k+2 are in
-// flight, so the memory latency overlaps the arithmetic.
+// The index and flag of edge k+1 are requested before the partner records
+// and history of edge k, so an edge waits for one memory round trip.  (A
+// deeper pipeline, with the partner records of edge k+1 in flight during
+// edge k, needs ~40 more VGPRs, drops to 2 waves per SIMD and measured
+// slower.)
 struct EdgeAcc {
   d3 F, T;
   unsigned long long n_col, n_pair, n_stored;
@@ -189,28 +191,23 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
                                           unsigned char* __restrict__ flag,
                                           double2* __restrict__ hist, long long ecap,
                                           long long e, int deg, const double4& P, d3 vi, d3 wi,
-                                          int ci, double ri, int cell_p, int st_p, int id_p,
-                                          EdgeAcc& acc) {
-  if (deg <= 0) return;
+                                          int ci, double ri, int cell_p, int xyz_p, int st_p,
+                                          int id_p, EdgeAcc& acc) {
   int j_n = 0;
-  unsigned char fl_c = flag[e], fl_n = 0;
-  EdgeData nx;
-  load_edge(nx, a, hist, ecap, e, nbr[e], fl_c);
-  if (deg > 1) {
-    j_n = nbr[e + 64];
-    fl_n = flag[e + 64];
+  unsigned char fl_n = 0;
+  if (deg > 0) {
+    j_n = nbr[e];
+    fl_n = flag[e];
   }
   for (int k = 0; k < deg; ++k, e += 64) {
-    const EdgeData cur = nx;
-    const unsigned char fl = fl_c;
+    const int j = j_n;
+    const unsigned char fl = fl_n;
     if (k + 1 < deg) {
-      load_edge(nx, a, hist, ecap, e + 64, j_n, fl_n);
-      fl_c = fl_n;
+      j_n = nbr[e + 64];
+      fl_n = flag[e + 64];
     }
-    if (k + 2 < deg) {
-      j_n = nbr[e + 128];
-      fl_n = flag[e + 128];
-    }
+    EdgeData cur;
+    load_edge(cur, a, hist, ecap, e, j, fl);
     const double4 Q = cur.Q;
     const Kin& KJ = cur.K;
     const double2 h0 = cur.h0, h1 = cur.h1, h2 = cur.h2;
@@ -222,7 +219,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     if (delta > 0.0) {
       int role;
       if (s.hist_mode == 0) {
-        role = ref_role(s, cell_p, KJ.rc, st_p, KJ.st, id_p, KJ.id);
+        role = ref_role(s, cell_p, xyz_p, KJ.rc, KJ.rxyz, st_p, KJ.st, id_p, KJ.id);
         if (role < 0) { acc.err |= DEM_KERR_ROLE; role = id_p < KJ.id; }
       } else {
         role = id_p < KJ.id;
@@ -258,7 +255,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     } else if (UPDATE && (fl & 1)) {
       // The reference removes a stored pair only when its cell loop visits
       // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
-      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, KJ.rc);
+      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, xyz_p, KJ.rc, KJ.rxyz);
       if (keep) ++acc.n_stored;
       else flag[e] = 0;
     }
@@ -290,7 +287,7 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
 
   double4 P = make_double4(0, 0, 0, 0);
   d3 vi = mk3(0, 0, 0), wi = vi;
-  int cell_p = 0, st_p = 0, id_p = 0, deg = 0;
+  int cell_p = 0, xyz_p = 0, st_p = 0, id_p = 0, deg = 0;
   long long e = 0;
   if (active) {
     P = a.pos_in[i];
@@ -298,6 +295,7 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
     vi = kin_v(K);
     wi = kin_w(K);
     cell_p = K.rc;
+    xyz_p = K.rxyz;
     st_p = K.st;
     id_p = K.id;
     deg = a.E.cnt[ep][i];
@@ -315,10 +313,10 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
   if (lds_tab) {
     __syncthreads();
     pair_loop<UPDATE>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
-                      e, deg, P, vi, wi, ci, ri, cell_p, st_p, id_p, acc);
+                      e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   } else {
     pair_loop<UPDATE>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
-                      e, deg, P, vi, wi, ci, ri, cell_p, st_p, id_p, acc);
+                      e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   }
   d3 F = acc.F, T = acc.T;
   unsigned long long n_col = acc.n_col, n_pair = acc.n_pair, n_stored = acc.n_stored;
@@ -379,8 +377,8 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
       const d3 v = add3(vi, scl3(a.h, acc));
       const d3 wv = add3(wi, scl3(a.h, alp));
       a.pos_out[i] = make_double4(x.x, x.y, x.z, P.w);
-      const int rc = ref_cell(s, x.x, x.y, x.z);
-      store_kin(a.kin_out + i, v, wv, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
+      const RefCell rc = ref_cell(s, x.x, x.y, x.z);
+      store_kin(a.kin_out + i, v, wv, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
       store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       const double4 R = a.xref[i];
       const d3 dd = min_image3(s, mk3(x.x - R.x, x.y - R.y, x.z - R.z));
@@ -405,8 +403,8 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
       if (s.pbc[1]) xp.y = xp.y - s.lbox[1] * fnint(xp.y * s.invl[1]);
       if (s.pbc[2]) xp.z = xp.z - s.lbox[2] * fnint(xp.z * s.invl[2]);
       a.pos_out[i] = make_double4(xp.x, xp.y, xp.z, P.w);
-      const int rc = ref_cell(s, xp.x, xp.y, xp.z);
-      store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
+      const RefCell rc = ref_cell(s, xp.x, xp.y, xp.z);
+      store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
       if (s.nprev > 0) {
         double* r = a.slot_wr + i;
         r[0] = vi.x; r[N] = vi.y; r[2 * N] = vi.z;
@@ -447,8 +445,8 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
       if (s.pbc[1]) x.y = x.y - s.lbox[1] * fnint(x.y * s.invl[1]);
       if (s.pbc[2]) x.z = x.z - s.lbox[2] * fnint(x.z * s.invl[2]);
       a.pos[i] = make_double4(x.x, x.y, x.z, X.w);
-      const int rc = ref_cell(s, x.x, x.y, x.z);
-      store_kin(a.kin + i, v, wv, id_p, rc, (rc != cell_p) ? a.stamp_next : st_p);
+      const RefCell rc = ref_cell(s, x.x, x.y, x.z);
+      store_kin(a.kin + i, v, wv, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
       // escape scan of UpdateParticleList (src/MDEM.f90:624-625)
       if ((x.x > s.phalf[0]) || (x.x < -s.phalf[0]) || (x.y > s.phalf[1]) ||
           (x.y < -s.phalf[1]) || (x.z > s.phalf[2]) || (x.z < -s.phalf[2]))
@@ -544,9 +542,10 @@ __global__ void k_refcell(SysConst s, double4* pos, const Kin* prev, Kin* kin, i
     pos[i] = P;
   }
   const int rc_prev = prev[i].rc, st_prev = prev[i].st;
-  const int rc = ref_cell(s, P.x, P.y, P.z);
-  kin[i].rc = rc;
-  kin[i].st = (wrap < 0 || rc == rc_prev) ? st_prev : stamp_next;
+  const RefCell rc = ref_cell(s, P.x, P.y, P.z);
+  kin[i].rc = rc.lin;
+  kin[i].rxyz = rc.xyz;
+  kin[i].st = (wrap < 0 || rc.lin == rc_prev) ? st_prev : stamp_next;
 }
 
 // Displacement check of uploaded positions against the neighbour-list origin.
@@ -707,11 +706,12 @@ __global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const in
 // scanning the 27 cells around i's cell in a fixed order.
 template <typename F>
 __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const double4* pos,
-                                                   int i, const double4& P, int k,
+                                                   int i, const double4& P,
                                                    const int* start, const int* cell_idx,
                                                    F&& fn) {
   const int ci = (int)P.w;
-  const int gx = k % s.gn[0], gy = (k / s.gn[0]) % s.gn[1], gz = k / (s.gn[0] * s.gn[1]);
+  const Int3 g = grid_coords(s, P.x, P.y, P.z);
+  const int gx = g.x, gy = g.y, gz = g.z;
   for (int dz = -1; dz <= 1; ++dz) {
     int cz = gz + dz;
     if (s.pbc[2]) {
@@ -733,7 +733,7 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
           if (s.gn[0] == 2 && dx == 1) continue;
           cx = (cx + s.gn[0]) % s.gn[0];
         } else if (cx < 0 || cx >= s.gn[0]) continue;
-        const int c = cx + (cy + cz * s.gn[1]) * s.gn[0];
+        const int c = grid_key(s, cx, cy, cz);
         const int e = start[c + 1];
         for (int p = start[c]; p < e; ++p) {
           const int j = cell_idx[p];
@@ -751,7 +751,7 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
 // cnt of the set being built, and 64 * (widest list) of each 64-particle
 // slice into slice_w.
 __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const double4* pos,
-                                                         const int* key, const int* start,
+                                                         const int* start,
                                                          const int* cell_idx, EdgeSets E,
                                                          int* slice_w, const int* gate, int force,
                                                          Ctl* ctl) {
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
   int n = 0, err = 0;
   if (i < s.np) {
     const double4 P = pos[i];
-    for_each_neighbour(s, pos, i, P, key[i], start, cell_idx, [&](int) { ++n; });
+    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int) { ++n; });
     if (n > s.max_nb) {
       err = DEM_KERR_NBR_OVERFLOW;
       n = s.max_nb;
@@ -785,7 +785,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
 // permutation, and o2n its inverse).  Stored pairs that drop out of the list
 // are counted when the reference would have kept them (cells not adjacent).
 __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
-                                                        const int* key, const int* start,
+                                                        const int* start,
                                                         const int* cell_idx, EdgeSets E,
                                                         const int* src, const int* o2n,
                                                         const Kin* kin_cur, double4* xref,
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
     long long e = sell_edge(E.soff[q], i, 0);
     int k = 0;
     unsigned long long found = 0;
-    for_each_neighbour(s, pos, i, P, key[i], start, cell_idx, [&](int j) {
+    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int j) {
       if (k >= ndeg) return;
       E.nbr[q][e] = j;
       const int jo = src ? src[j] : j;
@@ -836,7 +836,9 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
       if (!(oflag[om] & 1)) continue;
       if ((found >> m) & 1ull) continue;
       const int jn = o2n ? o2n[onbr[om]] : onbr[om];
-      if (s.hist_mode == 0 && !ref_adjacent(s, kin_cur[i].rc, kin_cur[jn].rc)) ++dropped;
+      if (s.hist_mode == 0 &&
+          !ref_adjacent(s, kin_cur[i].rc, kin_cur[i].rxyz, kin_cur[jn].rc, kin_cur[jn].rxyz))
+        ++dropped;
     }
     xref[i] = P;
   }

@@ -32,8 +32,10 @@ __device__ __forceinline__ d3 cross3(d3 a, d3 b) {
 }
 __device__ __forceinline__ d3 vget(const double* p) { return d3{p[0], p[1], p[2]}; }
 
-// Fortran nint of real(8) to default integer: half away from zero.
-__device__ __forceinline__ double fnint(double x) { return (double)(int)__builtin_round(x); }
+// Fortran nint of real(8) to default integer, back to real(8): half away
+// from zero; "+ 0.0" turns the -0.0 that round() gives in (-0.5, 0] into the
+// +0.0 of the integer round trip.
+__device__ __forceinline__ double fnint(double x) { return __builtin_round(x) + 0.0; }
 
 // Minimum image of one component (src/MForces.f90:212-214).
 __device__ __forceinline__ double min_image(double r, double l, double invl) {
@@ -57,25 +59,50 @@ __device__ __forceinline__ int cell_coord(double x, double n_by_l, double n_by2,
   return (int)t;
 }
 
-// CurrentCell3D, 0-based linear index, x fastest.
-__device__ __forceinline__ int ref_cell(const SysConst& s, double x, double y, double z) {
+// CurrentCell3D, 0-based linear index, x fastest; with rc_pack also the
+// coordinates packed 10 bits each (x | y << 10 | z << 20), so the role and
+// adjacency tests of a pair need no integer division.
+struct RefCell {
+  int lin, xyz;
+};
+__device__ __forceinline__ RefCell ref_cell(const SysConst& s, double x, double y, double z) {
   int ix = cell_coord(x, s.rn_by_l[0], s.rn_by2[0], s.rn[0]);
   int iy = cell_coord(y, s.rn_by_l[1], s.rn_by2[1], s.rn[1]);
   int iz = cell_coord(z, s.rn_by_l[2], s.rn_by2[2], s.rn[2]);
-  return ix + (iy + iz * s.rn[1]) * s.rn[0];
+  return RefCell{ix + (iy + iz * s.rn[1]) * s.rn[0], s.rc_pack ? (ix | (iy << 10) | (iz << 20)) : 0};
 }
 
-// Neighbour-search cell.  Periodic coordinates are wrapped first so that a
-// particle slightly outside the box (between Euler sub-steps, which wrap only
-// at the end, src/MDEM.f90:279) lands in the cell its minimum images see.
-__device__ __forceinline__ int grid_cell(const SysConst& s, double x, double y, double z) {
+// Neighbour-search cell coordinates.  Periodic coordinates are wrapped
+// first so that a particle slightly outside the box (between Euler sub-steps,
+// which wrap only at the end, src/MDEM.f90:279) lands in the cell its minimum
+// images see.
+struct Int3 {
+  int x, y, z;
+};
+__device__ __forceinline__ Int3 grid_coords(const SysConst& s, double x, double y, double z) {
   if (s.pbc[0]) x = min_image(x, s.lbox[0], s.invl[0]);
   if (s.pbc[1]) y = min_image(y, s.lbox[1], s.invl[1]);
   if (s.pbc[2]) z = min_image(z, s.lbox[2], s.invl[2]);
-  int ix = cell_coord(x, s.g_by_l[0], s.g_by2[0], s.gn[0]);
-  int iy = cell_coord(y, s.g_by_l[1], s.g_by2[1], s.gn[1]);
-  int iz = cell_coord(z, s.g_by_l[2], s.g_by2[2], s.gn[2]);
-  return ix + (iy + iz * s.gn[1]) * s.gn[0];
+  return Int3{cell_coord(x, s.g_by_l[0], s.g_by2[0], s.gn[0]),
+              cell_coord(y, s.g_by_l[1], s.g_by2[1], s.gn[1]),
+              cell_coord(z, s.g_by_l[2], s.g_by2[2], s.gn[2])};
+}
+
+// Key of a neighbour-search cell: 8x8x8-cell bricks in x-fastest order,
+// Morton order inside a brick.  Particles are kept sorted by key, so the 27
+// cells around a particle are close to it in memory in all three directions
+// (with x-fastest cells, a z neighbour is a whole xy layer away).
+__device__ __forceinline__ int morton3_3bit(int x, int y, int z) {
+  auto spread = [](int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); };
+  return spread(x) | (spread(y) << 1) | (spread(z) << 2);
+}
+__device__ __forceinline__ int grid_key(const SysConst& s, int x, int y, int z) {
+  const int b = ((z >> 3) * s.gb[1] + (y >> 3)) * s.gb[0] + (x >> 3);
+  return (b << 9) | morton3_3bit(x & 7, y & 7, z & 7);
+}
+__device__ __forceinline__ int grid_cell(const SysConst& s, double x, double y, double z) {
+  const Int3 g = grid_coords(s, x, y, z);
+  return grid_key(s, g.x, g.y, g.z);
 }
 
 // Signed offset from cell coordinate a to b along one reference-grid axis,
@@ -93,7 +120,8 @@ __device__ __forceinline__ int axis_offset(int a, int b, int n, int periodic) {
 struct CellXYZ {
   int x, y, z;
 };
-__device__ __forceinline__ CellXYZ ref_cell_xyz(const SysConst& s, int c) {
+__device__ __forceinline__ CellXYZ ref_cell_xyz(const SysConst& s, int c, int packed) {
+  if (s.rc_pack) return CellXYZ{packed & 1023, (packed >> 10) & 1023, packed >> 20};
   int nxy = s.rn[0] * s.rn[1];
   int z = c / nxy;
   int r = c - z * nxy;
@@ -102,10 +130,10 @@ __device__ __forceinline__ CellXYZ ref_cell_xyz(const SysConst& s, int c) {
 }
 
 // Are two reference cells the same or neighbours (i.e. does the reference's
-// cell loop visit the pair)?
-__device__ __forceinline__ bool ref_adjacent(const SysConst& s, int ca, int cb) {
+// cell loop visit the pair)?  c = linear index, x = packed coordinates.
+__device__ __forceinline__ bool ref_adjacent(const SysConst& s, int ca, int xa, int cb, int xb) {
   if (ca == cb) return true;
-  CellXYZ a = ref_cell_xyz(s, ca), b = ref_cell_xyz(s, cb);
+  CellXYZ a = ref_cell_xyz(s, ca, xa), b = ref_cell_xyz(s, cb, xb);
   return axis_offset(a.x, b.x, s.rn[0], s.pbc[0]) != 2 &&
          axis_offset(a.y, b.y, s.rn[1], s.pbc[1]) != 2 &&
          axis_offset(a.z, b.z, s.rn[2], s.pbc[2]) != 2;
@@ -118,13 +146,13 @@ __device__ __forceinline__ bool ref_adjacent(const SysConst& s, int ca, int cb) 
 // the one whose cell lists the other cell among its 13 half-stencil cells
 // (kx + Nx*(ky + Ny*kz) > 0, src/MCells.f90:454-456).
 // Returns 1 if p is "i", 0 if q is, -1 if the cells are not adjacent.
-__device__ __forceinline__ int ref_role(const SysConst& s, int cp, int cq, int sp, int sq,
-                                        int idp, int idq) {
+__device__ __forceinline__ int ref_role(const SysConst& s, int cp, int xp, int cq, int xq, int sp,
+                                        int sq, int idp, int idq) {
   if (cp == cq) {
     if (sp != sq) return sp > sq;
     return idp > idq;
   }
-  CellXYZ a = ref_cell_xyz(s, cp), b = ref_cell_xyz(s, cq);
+  CellXYZ a = ref_cell_xyz(s, cp, xp), b = ref_cell_xyz(s, cq, xq);
   int dx = axis_offset(a.x, b.x, s.rn[0], s.pbc[0]);
   int dy = axis_offset(a.y, b.y, s.rn[1], s.pbc[1]);
   int dz = axis_offset(a.z, b.z, s.rn[2], s.pbc[2]);
