@@ -138,11 +138,25 @@ def cells_to_rank(cells, np_):
 # runs in a fresh process.
 
 def _ref_job(job, q):
-    try:
-        q.put(("ok", _ref_job_body(job)))
-    except BaseException as e:  # pragma: no cover - reported to the parent
-        import traceback
-        q.put(("err", traceback.format_exc()))
+    # The Fortran modules keep array temporaries on the stack; beyond ~200k
+    # particles they overflow a default 8 MiB main-thread stack, so the job
+    # runs on a thread with a 1 GiB stack.
+    import threading
+
+    res = {}
+
+    def body():
+        try:
+            res["v"] = ("ok", _ref_job_body(job))
+        except BaseException:  # pragma: no cover - reported to the parent
+            import traceback
+            res["v"] = ("err", traceback.format_exc())
+
+    threading.stack_size(1 << 30)
+    th = threading.Thread(target=body)
+    th.start()
+    th.join()
+    q.put(res.get("v", ("err", "reference job ended without a result")))
 
 
 def _d(a):
@@ -266,8 +280,19 @@ def run_reference(job: dict, timeout: Optional[float] = 600, method: str = "fork
     q = ctx.Queue()
     p = ctx.Process(target=_ref_job, args=(job, q))
     p.start()
+    import queue as _queue
+    import time as _time
+    t_end = None if timeout is None else _time.monotonic() + timeout
     try:
-        status, payload = q.get(timeout=timeout)
+        while True:
+            try:
+                status, payload = q.get(timeout=1.0)
+                break
+            except _queue.Empty:
+                if not p.is_alive():
+                    raise RuntimeError(f"reference job died (exit code {p.exitcode})")
+                if t_end is not None and _time.monotonic() > t_end:
+                    raise TimeoutError(f"reference job exceeded {timeout} s")
     finally:
         p.join(timeout=30)
         if p.is_alive():
