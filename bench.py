@@ -40,7 +40,7 @@ def cpu_baseline(cells, steps):
     from checkers import ref_available, run_reference
     if not ref_available(big=True):
         return None
-    wl = workloads.config2(cells=cells)
+    wl = workloads.bidisperse_sheared_4m(cells=cells)
     res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, big=True,
                              actions=[("time", 1, steps)]), timeout=900)
     sec = res["seconds"]
@@ -117,7 +117,7 @@ def main(argv=None):
 
     from dem3d_amd import workloads
     from dem3d_amd.native import DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, System, load_hip
-    wl = workloads.config2(cells=cells)
+    wl = workloads.bidisperse_sheared_4m(cells=cells)
     print(f"workload: {wl.name} np={wl.np} {wl.note}", file=sys.stderr, flush=True)
     hmode = DEM_HIST_REFERENCE if args.history == "reference" else DEM_HIST_CANONICAL
     g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode)
@@ -144,7 +144,7 @@ def main(argv=None):
     achieved = dom_b / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     traffic = traffic_from_profiles(wl.name)
     out = {
-        "metric": "particle-steps/sec at 4M spheres (FP64)",
+        "metric": "particle-steps/sec at 4M spheres (FP64 vs mixed) + batched p50/p99 step latency",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -69,7 +69,7 @@ def _floor(lx, ly, lz, kn, top: Optional[float] = None, top_velocity=(0.0, 0.0, 
     return walls
 
 
-def config0(seed: int = 20240001, steps: int = 20000) -> Workload:
+def mono_settling_2k(seed: int = 20240001, steps: int = 20000) -> Workload:
     """2 000 monodisperse spheres (d = 1 mm, 2500 kg/m^3) on a 10x10x20 simple
     cubic lattice at spacing 1.2 d with +-0.05 d jitter, box 12 x 12 x 48 mm,
     periodic x,y, floor at z = -Lz/2, velocities U(+-0.05 m/s), kn = 1e3 N/m,
@@ -89,11 +89,11 @@ def config0(seed: int = 20240001, steps: int = 20000) -> Workload:
     vel = rng.uniform(-0.05, 0.05, pos.shape)
     L, T = red.length_scale, red.time_scale
     n = pos.shape[1]
-    return Workload("config0-mono-2k", spec, red, np.ones(n, np.int32), pos / L, vel * T / L,
+    return Workload("mono-settling-2k", spec, red, np.ones(n, np.int32), pos / L, vel * T / L,
                     np.zeros_like(pos), steps)
 
 
-def config1(seed: int = 20240002, steps: int = 5000, n: int = 200_000, nbins: int = 16) -> Workload:
+def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000, nbins: int = 16) -> Workload:
     """200 000 polydisperse spheres, radii U[0.4, 0.6] mm discretised into 16
     equal-width components, at volume fraction ~0.45 (jittered FCC lattice),
     base 40 x 40 mm, periodic x,y, floor, settling under gravity."""
@@ -124,7 +124,7 @@ def config1(seed: int = 20240002, steps: int = 5000, n: int = 200_000, nbins: in
     red = reduce_spec(spec)
     L, T = red.length_scale, red.time_scale
     vel = rng.uniform(-0.05, 0.05, pos.shape)
-    return Workload("config1-poly-200k", spec, red, (comp + 1).astype(np.int32), pos / L,
+    return Workload("poly-settling-200k", spec, red, (comp + 1).astype(np.int32), pos / L,
                     vel * T / L, np.zeros_like(pos), steps,
                     note=f"FCC a={a*1e3:.4f} mm, phi~0.45, Lz={lz*1e3:.2f} mm")
 
@@ -141,7 +141,7 @@ def _fcc(nx, ny, nz, a):
     return (pts * a + 0.5 * a * 0.5).T.copy()
 
 
-def config2(seed: int = 20240003, steps: int = 2000, cells=(100, 100, 50),
+def bidisperse_sheared_4m(seed: int = 20240003, steps: int = 2000, cells=(100, 100, 50),
             shear_rate: float = 10.0) -> Workload:
     """4 000 000 bidisperse spheres (d = 1.0 / 1.4 mm, 50/50 by number) in a
     dense bed: rock-salt arrangement (large spheres on an FCC lattice, small
@@ -177,7 +177,7 @@ def config2(seed: int = 20240003, steps: int = 2000, cells=(100, 100, 50),
     comp = np.where(large, 2, 1).astype(np.int32)
     L = red.length_scale
     n = comp.shape[0]
-    return Workload("config2-bidisperse-sheared", spec, red, comp, pos / L,
+    return Workload("bidisperse-sheared-4M", spec, red, comp, pos / L,
                     np.zeros((3, n)), np.zeros((3, n)), steps,
                     note=f"rock-salt a={a*1e3:.5f} mm, {nx}x{ny}x{nz} cells, phi=0.567, "
                          f"top wall v_x={vtop:.4f} m/s")

@@ -6,7 +6,7 @@ stores inputs and outputs as .npz (numpy arrays only, no pickles):
 
   ref_small.npz     216 spheres, 2 components, floor, periodic x,y: one
                     ComputeAccelerations(Update) + Euler start-up + 20 steps
-  ref_config0.npz   the 2 000-sphere monodisperse system of the benchmark
+  ref_mono_settling_2k.npz   the 2 000-sphere monodisperse system of the benchmark
                     suite (jittered lattice): start-up + 60 steps
   ref_capped.npz    3 000 spheres in a box whose natural cell count exceeds
                     MaxNCell = 30 000: CellIndex on the capped grid
@@ -136,11 +136,11 @@ def main():
              [("compute", 1), ("snapshot", "c0"), ("euler",), ("euler",), ("euler",),
               ("snapshot", "e"), ("step", 4, 20), ("snapshot", "s")],
              "small_box(n_side=(6,6,6), nc=2, seed=3, vel=0.3)")
-    wl = workloads.config0(steps=20000)
-    run_case("ref_config0.npz", wl,
+    wl = workloads.mono_settling_2k(steps=20000)
+    run_case("ref_mono_settling_2k.npz", wl,
              [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 60),
               ("snapshot", "s")],
-             "config0(seed=20240001): start-up + 60 steps")
+             "mono_settling_2k(seed=20240001): start-up + 60 steps")
     wl = capped_workload()
     run_case("ref_capped.npz", wl, [("compute", 1), ("snapshot", "c0")],
              "3000 spheres, 40 mm periodic box: natural grid 40^3 > MaxNCell")

@@ -95,7 +95,7 @@ def run_actions(o, actions):
 FIXTURES = {
     "ref_small.npz": [("compute", 1), ("snapshot", "c0"), ("euler",), ("euler",), ("euler",),
                       ("snapshot", "e"), ("step", 4, 20), ("snapshot", "s")],
-    "ref_config0.npz": [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 60),
+    "ref_mono_settling_2k.npz": [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 60),
                         ("snapshot", "s")],
     "ref_capped.npz": [("compute", 1), ("snapshot", "c0")],
     "ref_cone.npz": [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
