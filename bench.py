@@ -14,6 +14,10 @@ Prints one JSON line:
   cpu_baseline the reference Fortran modules (oracle/_ref, capacity
                constants raised) timed on one host core on a bounded sample
                of the same bed
+  mixed        the same run in the mixed-precision mode (FP32 force
+               arithmetic, FP64 state and spring history): rate, speed-up over
+               FP64, and its kinetic energy and packing fraction against the
+               FP64 run's after the same steps
 """
 from __future__ import annotations
 
@@ -77,6 +81,31 @@ def traffic_from_profiles(workload):
         return json.load(f).get("pass1_bytes_per_launch")
 
 
+def timed_run(wl, args, precision, hmode):
+    """Upload, Euler start-up, warm-up, then exactly args.steps timed steps
+    bracketed by device synchronisations.  Returns the rate, the stats of the
+    library (pass timings from HIP events on its stream) and the final
+    positions."""
+    from dem3d_amd.native import System, load_hip
+    g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode, precision=precision)
+    g.upload(wl.comp, wl.x, wl.v, wl.w)
+    g.startup()
+    if args.warmup:
+        g.step(args.warmup)
+    g.lib.set_option(g.h, b"timing", 1.0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    st = g.stats()
+    sec = t1 - t0
+    out = {"ms": 1e3 * sec / args.steps, "value": wl.np * args.steps / sec, "stats": st,
+           "x": g.download()["x"] if args.precision == "both" else None}
+    g.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +116,9 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0)
+    ap.add_argument("--precision", choices=("fp64", "both"), default="both",
+                    help="fp64: the FP64 path only; both: also time the mixed-precision mode and "
+                         "compare its energy and packing fraction with the FP64 run")
     ap.add_argument("--history", choices=("reference", "canonical"), default="reference",
                     help="orientation of stored tangential history (see include/dem3d.h)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -116,25 +148,34 @@ def main(argv=None):
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     from dem3d_amd import workloads
-    from dem3d_amd.native import DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, System, load_hip
+    from dem3d_amd.native import DEM_FP64, DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, DEM_MIXED
+    from dem3d_amd.observables import packing_fraction
     wl = workloads.bidisperse_sheared_4m(cells=cells)
     print(f"workload: {wl.name} np={wl.np} {wl.note}", file=sys.stderr, flush=True)
     hmode = DEM_HIST_REFERENCE if args.history == "reference" else DEM_HIST_CANONICAL
-    g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode)
-    g.upload(wl.comp, wl.x, wl.v, wl.w)
-    g.startup()
-    if args.warmup:
-        g.step(args.warmup)
-    g.lib.set_option(g.h, b"timing", 1.0)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.synchronize()
-    t1 = time.perf_counter()
-    st = g.stats()
-    sec = t1 - t0
-    ms = 1e3 * sec / args.steps
-    value = wl.np * args.steps / sec
+    r64 = timed_run(wl, args, DEM_FP64, hmode)
+    print(f"fp64: {r64['ms']:.4f} ms/step", file=sys.stderr, flush=True)
+    st, ms, value = r64["stats"], r64["ms"], r64["value"]
+    mixed = None
+    if args.precision == "both":
+        rmx = timed_run(wl, args, DEM_MIXED, hmode)
+        print(f"mixed: {rmx['ms']:.4f} ms/step", file=sys.stderr, flush=True)
+        phi64 = packing_fraction(wl.red, r64["x"], wl.comp)
+        phimx = packing_fraction(wl.red, rmx["x"], wl.comp)
+        ke64, kemx = r64["stats"]["kinetic_energy"], rmx["stats"]["kinetic_energy"]
+        mixed = {
+            "value": rmx["value"],
+            "ms_per_step": rmx["ms"],
+            "speedup_vs_fp64": rmx["value"] / value,
+            "ms_pass1": rmx["stats"]["ms_pass1"],
+            "ms_pass2": rmx["stats"]["ms_pass2"],
+            "pair_contacts": rmx["stats"]["pair_contacts"],
+            "after_steps": args.warmup + args.steps,
+            "kinetic_energy_rel_diff_vs_fp64": (kemx - ke64) / ke64 if ke64 else None,
+            "packing_fraction_fp64": phi64,
+            "packing_fraction_mixed": phimx,
+            "packing_fraction_diff": phimx - phi64,
+        }
     P, W = st["pair_contacts"], st["wall_contacts"]
     b1 = PASS1_B_PER_PARTICLE * wl.np + PASS1_B_PER_CONTACT * (P + W)
     b2 = PASS2_B_PER_PARTICLE * wl.np + PASS2_B_PER_CONTACT * (P + W)
@@ -182,6 +223,7 @@ def main(argv=None):
             "step_frac": (b1 + b2) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -103,6 +103,7 @@ struct dem_system {
   bool keep_pred_acc = false;
   int64_t uploaded_unlisted = 0;  // uploaded pairs with no edge in the list
   bool timing = false;
+  bool mixed = false;        // DEM_MIXED force arithmetic
   Timer timer;
   double ms_pass1 = 0, ms_pass2 = 0, ms_rebuild = 0, ms_step = 0;
 };
@@ -173,8 +174,8 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     return fail(nullptr, DEM_ERR_ARG, "method_order must be 1..4 and n_euler >= 1");
   if (p->max_neighbours < 0 || p->max_neighbours > 64)
     return fail(nullptr, DEM_ERR_ARG, "max_neighbours must be 0 (default 48) or 1..64");
-  if (p->precision != DEM_FP64)
-    return fail(nullptr, DEM_ERR_UNSUPPORTED, "precision mode %d is not built", p->precision);
+  if (p->precision != DEM_FP64 && p->precision != DEM_MIXED)
+    return fail(nullptr, DEM_ERR_ARG, "precision must be DEM_FP64 or DEM_MIXED");
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
     return fail(nullptr, DEM_ERR_DEVICE, "no HIP device available");
@@ -185,6 +186,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   s.nc = nc;
   s.nw = nw;
   s.hist_mode = h->hist_mode = p->history_mode;
+  h->mixed = p->precision == DEM_MIXED;
   h->order = p->method_order;
   h->nprev = s.nprev = p->method_order - 1;
   h->n_euler = p->n_euler;
@@ -1012,7 +1014,12 @@ static PassArgs base_args(dem_handle_t h) {
 template <int MODE, bool UPDATE>
 static void launch_pass(dem_handle_t h, const PassArgs& a) {
   const int nb = nblocks(h->np);
-  hipLaunchKernelGGL((k_pass<MODE, UPDATE>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream, h->sc, a);
+  if (h->mixed)
+    hipLaunchKernelGGL((k_pass<MODE, UPDATE, true>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream,
+                       h->sc, a);
+  else
+    hipLaunchKernelGGL((k_pass<MODE, UPDATE, false>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream,
+                       h->sc, a);
   hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl,
                      (const unsigned*)a.part, nb, a.par, UPDATE ? 1 : 0);
 }

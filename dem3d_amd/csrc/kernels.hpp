@@ -32,6 +32,7 @@
 #pragma clang fp contract(off)
 
 #include "physics.hpp"
+#include "mixed.hpp"
 
 #ifndef DEM_PASS_WAVES
 #define DEM_PASS_WAVES 3
@@ -159,7 +160,8 @@ __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
 // edge k, needs ~40 more VGPRs, drops to 2 waves per SIMD and measured
 // slower.)
 struct EdgeAcc {
-  d3 F, T;
+  d3 F, T;          // FP64 sums (and gravity)
+  KahanF3 Ff, Tf;   // compensated FP32 sums of the mixed mode
   unsigned long long n_col, n_pair, n_stored;
   int err;
 };
@@ -184,7 +186,7 @@ __device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
   }
 }
 
-template <bool UPDATE>
+template <bool UPDATE, bool MIXED>
 __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
                                           const PairConst* ptab, const CompConst* ctab,
                                           const int* __restrict__ nbr,
@@ -232,6 +234,24 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
       if (!in.contacting && role) ++acc.n_col;
       if (role) ++acc.n_pair;
       const double rj = ctab[cj].radius;
+      if (MIXED) {
+        const MixedOut o = contact_mixed(
+            in.contacting, rij, dsq, delta, pc.mindij, sub3(kin_v(KJ), vi), wi, kin_w(KJ), ri, rj,
+            in.ft0, in.vt0, (float)a.half_dt, (float)pc.kn, (float)pc.yn, (float)pc.kt,
+            (float)pc.yt, pc.musq);
+        kahan_add3(acc.Ff, o.fn);
+        kahan_add3(acc.Ff, o.ft);
+        kahan_add3(acc.Tf, sclf((float)ri, o.tq_dir));
+        if (UPDATE) {
+          const d3 f = scl3(sg, o.ft_store), g = scl3(sg, tod(o.vt));
+          hist[e] = make_double2(f.x, f.y);
+          hist[ecap + e] = make_double2(f.z, g.x);
+          hist[2 * ecap + e] = make_double2(g.y, g.z);
+          if (!(fl & 1)) flag[e] = 1;
+          ++acc.n_stored;
+        }
+        continue;
+      }
       const double dij = sqrt(dsq);
       in.nij = div3(rij, dij);
       in.disp_n = scl3(dij - pc.mindij, in.nij);
@@ -262,7 +282,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
   }
 }
 
-template <int MODE, bool UPDATE>
+template <int MODE, bool UPDATE, bool MIXED>
 __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, PassArgs a) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
@@ -308,17 +328,22 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
   // iForce = EffectiveMass*Gravity (src/MForces.f90:190)
   acc.F = scl3(cc.eff_mass, mk3(s.grav[0], s.grav[1], s.grav[2]));
   acc.T = mk3(0.0, 0.0, 0.0);
+  acc.Ff.s = acc.Ff.c = acc.Tf.s = acc.Tf.c = mkf(0.0f, 0.0f, 0.0f);
   acc.n_col = acc.n_pair = acc.n_stored = 0;
   acc.err = 0;
   if (lds_tab) {
     __syncthreads();
-    pair_loop<UPDATE>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+    pair_loop<UPDATE, MIXED>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
                       e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   } else {
-    pair_loop<UPDATE>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+    pair_loop<UPDATE, MIXED>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
                       e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   }
   d3 F = acc.F, T = acc.T;
+  if (MIXED) {
+    F = add3(F, kahan_value(acc.Ff));
+    T = add3(T, kahan_value(acc.Tf));
+  }
   unsigned long long n_col = acc.n_col, n_pair = acc.n_pair, n_stored = acc.n_stored;
   int err = acc.err;
 

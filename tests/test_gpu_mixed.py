@@ -1,0 +1,97 @@
+"""The mixed-precision mode (DEM_MIXED) against the FP64 oracle.
+
+Bar: contact sets identical to the FP64 ones (the overlap test stays FP64);
+per-particle forces and torques within 1e-5 of the particle's force scale
+(|m_eff g| + sum of |F_N| + |F_T| over its contacts, from the oracle); after
+a start-up and 200 ABM steps, kinetic energy within 1e-4 relative and the
+packing fraction within 1e-4 absolute of the FP64 oracle's."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dem3d_amd import workloads
+from dem3d_amd.native import DEM_MIXED, System
+from dem3d_amd.observables import packing_fraction
+from checkers import oracle_system
+from helpers import accel_tolerance_ok, force_scale, full_state, upload_same
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+KE_TOL = 1e-4
+PHI_TOL = 1e-4
+
+CASES = {
+    "bidisperse-floor": dict(n_side=(6, 6, 6), nc=2, seed=3, vel=0.3),
+    "tri-periodic": dict(n_side=(8, 8, 5), nc=3, ratio=1.4, seed=5, walls=False,
+                         pbc=(True, True, True), vel=0.5, spacing=1.0),
+}
+
+
+def _inv(wl):
+    comp = wl.comp - 1
+    return wl.red.inv_mass[comp], wl.red.inv_inertia[comp]
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_mixed_single_evaluation_with_history(hip, case):
+    wl = workloads.small_box(**CASES[case])
+    o0 = oracle_system(wl.red)
+    o0.upload(wl.comp, wl.x, wl.v, wl.w)
+    o0.startup()
+    o0.step(40)
+    st = full_state(o0, with_rank=True)
+    con = o0.download_contacts()
+    g = System(hip, wl.red, precision=DEM_MIXED)
+    o = oracle_system(wl.red)
+    upload_same([g, o], wl, state=st, contacts=con)
+    for s in (g, o):
+        s.compute_accelerations(0, True)
+    sg, so = g.download(), o.download()
+    fs, ts = force_scale(o)
+    im, ii = _inv(wl)
+    ok_a, ra = accel_tolerance_ok(sg["a"], so["a"], fs, im, FORCE_TOL)
+    ok_b, rb = accel_tolerance_ok(sg["alpha"], so["alpha"], ts + 1e-300, ii, FORCE_TOL)
+    assert ok_a, f"acceleration error {ra:.3g} x tolerance"
+    assert ok_b, f"angular acceleration error {rb:.3g} x tolerance"
+    cg, co = g.download_contacts(), o.download_contacts()
+    assert np.array_equal(cg["keys"], co["keys"])
+    assert np.array_equal(cg["wkeys"], co["wkeys"])
+    scale = np.abs(co["ft"]).max()
+    assert np.abs(cg["ft"] - co["ft"]).max() <= FORCE_TOL * scale
+    assert np.array_equal(g.download_cells(), o.download_cells())
+    assert g.stats()["pair_contacts"] == o.stats()["pair_contacts"]
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_mixed_trajectory_energy_and_packing(hip, case):
+    wl = workloads.small_box(**CASES[case])
+    g = System(hip, wl.red, precision=DEM_MIXED)
+    o = oracle_system(wl.red)
+    upload_same([g, o], wl)
+    for s in (g, o):
+        s.startup()
+        s.step(200)
+    kg, ko = g.stats()["kinetic_energy"], o.stats()["kinetic_energy"]
+    assert abs(kg - ko) <= KE_TOL * abs(ko), (kg, ko)
+    sg, so = g.download(), o.download()
+    pg = packing_fraction(wl.red, sg["x"], wl.comp)
+    po = packing_fraction(wl.red, so["x"], wl.comp)
+    assert abs(pg - po) <= PHI_TOL, (pg, po)
+    # the bulk state stays close; contacts may differ only by pairs at the edge
+    pc_g, pc_o = g.stats()["pair_contacts"], o.stats()["pair_contacts"]
+    assert abs(pc_g - pc_o) <= max(2, 0.01 * pc_o), (pc_g, pc_o)
+
+
+def test_mixed_run_to_run_bitwise_deterministic(hip):
+    wl = workloads.small_box(**CASES["tri-periodic"])
+    out = []
+    for _ in range(2):
+        g = System(hip, wl.red, precision=DEM_MIXED)
+        g.upload(wl.comp, wl.x, wl.v, wl.w)
+        g.startup()
+        g.step(50)
+        out.append(g.download()["x"].tobytes())
+        g.close()
+    assert out[0] == out[1]
