@@ -1,0 +1,162 @@
+// batch.hpp -- many independent systems stepped together.
+//
+// Every system keeps its own buffers (one dem_system each).  A batch adds a
+// device table with one BatchRec per system -- its constants and the
+// arguments of the two passes and of the neighbour-list rebuild of the
+// current step, written by the host into pinned memory and copied over once
+// per step -- and block maps that assign each block of a launch to
+// (system, block within that system).  Each phase of a step is then one
+// launch for the whole batch, whatever the number and sizes of its systems,
+// and the kernel bodies are the single-system ones.
+#pragma once
+
+#include "kernels.hpp"
+
+struct RebuildArgs {
+  const double4* pos;
+  const Kin* kin;
+  double4* xref;
+  int* key;
+  int* cell_count;
+  int* cell_start;
+  int* cursor;
+  int* cell_idx;
+  int* tile_sum;
+  int* slice_w;
+  const EdgeSets* d_es;
+  EdgeSets es;
+  const int* gate;
+  Ctl* ctl;
+  int gncell, ntiles, nslices, ntiles_np;
+};
+
+struct BatchRec {
+  SysConst s;
+  PassArgs p[2];  // pass 1 (forces at x_n + predictor), pass 2 (forces at the prediction + corrector)
+  RebuildArgs r;
+  int nblk;       // particle blocks of the system
+  int pad[3];
+};
+
+// Block maps: (system, block within the system) per block of a launch.
+enum { MAP_PART = 0, MAP_CELL = 1, MAP_CTILE = 2, MAP_STILE = 3, MAP_KINDS = 4 };
+
+__device__ __forceinline__ int cell_blocks(const RebuildArgs& r) {
+  return (r.gncell + 1 + DEM_BLOCK - 1) / DEM_BLOCK;
+}
+
+template <int MODE, bool UPDATE, bool MIXED, int PI>
+__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass_batch(
+    const BatchRec* __restrict__ rec, const int2* __restrict__ map) {
+  const int2 m = map[xcd_block()];
+  const BatchRec& r = rec[m.x];
+  pass_body<MODE, UPDATE, MIXED>(r.s, r.p[PI], m.y);
+}
+
+// One block per system.
+__global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __restrict__ rec,
+                                                            int pi, int update) {
+  const BatchRec& r = rec[blockIdx.x];
+  const PassArgs& a = r.p[pi];
+  pass_reduce_body(a.ctl, a.part, r.nblk, a.par, update);
+}
+
+__global__ void k_fill_cells_batch(const BatchRec* __restrict__ rec,
+                                   const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const RebuildArgs& r = rec[m.x].r;
+  fill_i32_body(r.cell_count, r.gncell + 1, 0, r.gate, 0, r.ctl, m.y, cell_blocks(r));
+}
+
+__global__ void k_bin_count_batch(const BatchRec* __restrict__ rec,
+                                  const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const BatchRec& b = rec[m.x];
+  bin_count_body(b.s, b.r.pos, b.r.key, b.r.cell_count, b.r.gate, 0, b.r.ctl, m.y, b.nblk);
+}
+
+// which = 0: the cell counts (map of cell tiles); 1: the slice widths (map
+// of slice tiles).
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_tiles_batch(const BatchRec* __restrict__ rec,
+                                                                const int2* __restrict__ map,
+                                                                int which) {
+  const int2 m = map[blockIdx.x];
+  const RebuildArgs& r = rec[m.x].r;
+  if (which == 0)
+    scan_tiles_body(r.cell_count, r.gncell, r.tile_sum, r.gate, 0, r.ctl, m.y);
+  else
+    scan_tiles_body(r.slice_w, r.nslices, r.tile_sum, r.gate, 0, r.ctl, m.y);
+}
+
+// One block per system.
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine_batch(const BatchRec* __restrict__ rec,
+                                                                int which) {
+  const RebuildArgs& r = rec[blockIdx.x].r;
+  scan_spine_body(r.tile_sum, which == 0 ? r.ntiles : r.ntiles_np, r.gate, 0, r.ctl);
+}
+
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply_batch(const BatchRec* __restrict__ rec,
+                                                                const int2* __restrict__ map,
+                                                                int which) {
+  const int2 m = map[blockIdx.x];
+  const RebuildArgs& r = rec[m.x].r;
+  if (which == 0)
+    scan_apply_body(r.cell_count, r.gncell, r.tile_sum, r.cell_start, r.cursor, nullptr, r.gate,
+                    0, r.ctl, m.y);
+  else
+    scan_apply_body(r.slice_w, r.nslices, r.tile_sum, nullptr, nullptr, r.d_es, r.gate, 0,
+                    r.ctl, m.y);
+}
+
+__global__ void k_bin_scatter_batch(const BatchRec* __restrict__ rec,
+                                    const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const BatchRec& b = rec[m.x];
+  bin_scatter_body(b.s.np, b.r.key, b.r.cursor, b.r.cell_idx, b.r.gate, 0, b.r.ctl, m.y, b.nblk);
+}
+
+__global__ void k_cell_sort_batch(const BatchRec* __restrict__ rec,
+                                  const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const RebuildArgs& r = rec[m.x].r;
+  cell_sort_body(r.gncell, r.cell_start, r.cell_idx, r.gate, 0, r.ctl, m.y, cell_blocks(r));
+}
+
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count_batch(const BatchRec* __restrict__ rec,
+                                                               const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const BatchRec& b = rec[m.x];
+  nbr_count_body(b.s, b.r.pos, b.r.cell_start, b.r.cell_idx, b.r.es, b.r.slice_w, b.r.gate, 0,
+                 b.r.ctl, m.y);
+}
+
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill_batch(const BatchRec* __restrict__ rec,
+                                                              const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const BatchRec& b = rec[m.x];
+  nbr_fill_body(b.s, b.r.pos, b.r.cell_start, b.r.cell_idx, b.r.es, nullptr, nullptr, b.r.kin,
+                b.r.xref, b.r.gate, 0, b.r.ctl, m.y);
+}
+
+// One block per system.
+__global__ void k_nbr_flip_batch(const BatchRec* __restrict__ rec) {
+  const RebuildArgs& r = rec[blockIdx.x].r;
+  nbr_flip_body(r.es, r.gate, 0, r.ctl);
+}
+
+// Control blocks of all systems into one array (one copy back per call).
+__global__ void k_gather_ctl(const BatchRec* __restrict__ rec, int n, Ctl* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = *rec[k].r.ctl;
+}
+
+// Reset of the per-call escape bookkeeping of every system.
+__global__ void k_reset_escape(const BatchRec* __restrict__ rec, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    Ctl* c = rec[k].r.ctl;
+    c->stop_step = 0x7fffffff;
+    c->escaped = 0;
+    c->esc_pending = 0;
+  }
+}

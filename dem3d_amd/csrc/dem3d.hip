@@ -6,7 +6,7 @@
 // upload/download calls and once at the end of dem_step (to read the control
 // block: errors, escapes).
 #include "../../include/dem3d.h"
-#include "kernels.hpp"
+#include "batch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -104,6 +104,9 @@ struct dem_system {
   int64_t uploaded_unlisted = 0;  // uploaded pairs with no edge in the list
   bool timing = false;
   bool mixed = false;        // DEM_MIXED force arithmetic
+  bool in_batch = false;     // stepped by a dem_batch (its own stream must stay idle)
+  dem_batch_t batch = nullptr;
+  bool batch_first = false;  // next batched step is the first since joining
   Timer timer;
   double ms_pass1 = 0, ms_pass2 = 0, ms_rebuild = 0, ms_step = 0;
 };
@@ -334,6 +337,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
 }
 
 void dem_destroy(dem_handle_t h) {
+  if (h && h->batch) dem_batch_remove(h->batch, h);
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_state(h);
@@ -1137,6 +1141,69 @@ int dem_euler_startup(dem_handle_t h) {
   return check_errors(h);
 }
 
+}  // extern "C"
+
+// Arguments of the two passes of one MoveParticles step (src/MDEM.f90:285-369),
+// s = index of the step within the current call; advances the host-side
+// bookkeeping (evaluation count, ipc rotation) as the launches will.
+static void step_args(dem_handle_t h, int64_t s, bool first, PassArgs& a, PassArgs& b) {
+  const size_t N = (size_t)h->np;
+  // pass 1: forces at x_n (Update), AB predictor, ring store
+  a = base_args(h);
+  a.pos_in = h->pos[h->cur];
+  a.kin_in = h->kin[h->cur];
+  a.pos_out = h->pos[h->cur ^ 1];
+  a.kin_out = h->kin[h->cur ^ 1];
+  a.acc_out = h->acc;
+  a.coef0 = h->ab0;
+  for (int k = 0; k < h->nprev; ++k) {
+    a.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
+    a.coef[k] = h->ab[k];
+  }
+  a.slot_wr = h->nprev > 0 ? h->ring + (size_t)(h->ipc[h->nprev - 1] - 1) * 9 * N : nullptr;
+  a.check_pred = first ? 1 : 0;
+  after_eval(h, true, h->cur);
+  if (h->nprev > 0) {
+    const int last = h->ipc[h->nprev - 1];
+    for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+    h->ipc[0] = last;
+  }
+  // pass 2: forces at the predicted state, AM corrector, look-ahead
+  b = base_args(h);
+  b.pos_in = h->pos[h->cur ^ 1];
+  b.kin_in = h->kin[h->cur ^ 1];
+  b.pos = h->pos[h->cur];
+  b.kin = h->kin[h->cur];
+  b.acc_out = h->keep_pred_acc ? h->acc_p : nullptr;
+  b.coef0 = h->am0;
+  for (int k = 0; k < h->nprev; ++k) {
+    b.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
+    b.coef[k] = h->am[k];
+    b.ab[k] = h->ab[k];
+  }
+  b.ab0 = h->ab0;
+  b.step_idx = (int)s;
+  after_eval(h, false, h->cur ^ 1);
+  h->acc_in_ring = h->nprev > 0;
+}
+
+// Undo the host-side bookkeeping of the steps enqueued after the one in
+// which particles escaped: their kernels returned at once.
+static void rollback_steps(dem_handle_t h, int64_t done, int64_t enqueued) {
+  for (int64_t k = done; k < enqueued; ++k) {
+    if (h->nprev > 0) {
+      const int first_slot = h->ipc[0];
+      for (int q = 0; q + 1 < h->nprev; ++q) h->ipc[q] = h->ipc[q + 1];
+      h->ipc[h->nprev - 1] = first_slot;
+    }
+    h->eval -= 2;
+    h->steps_since_resort -= 1;
+  }
+  h->last_update_eval = h->eval - 2;
+}
+
+extern "C" {
+
 int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_done,
              int32_t* escaped) {
   (void)first_step;
@@ -1144,8 +1211,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
   if (escaped) *escaped = 0;
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
   if (nsteps <= 0 || h->np == 0) return DEM_OK;
-  const int np = h->np;
-  const size_t N = (size_t)np;
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
   hipStream_t st = h->stream;
   {
     int big = 0x7fffffff;
@@ -1172,49 +1238,14 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
       rc = resort(h);
       if (rc) return rc;
     }
-    // pass 1: forces at x_n (Update), AB predictor, ring store
-    PassArgs a = base_args(h);
-    a.pos_in = h->pos[h->cur];
-    a.kin_in = h->kin[h->cur];
-    a.pos_out = h->pos[h->cur ^ 1];
-    a.kin_out = h->kin[h->cur ^ 1];
-    a.acc_out = h->acc;
-    a.coef0 = h->ab0;
-    for (int k = 0; k < h->nprev; ++k) {
-      a.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
-      a.coef[k] = h->ab[k];
-    }
-    a.slot_wr = h->nprev > 0 ? h->ring + (size_t)(h->ipc[h->nprev - 1] - 1) * 9 * N : nullptr;
-    a.check_pred = first ? 1 : 0;
+    PassArgs a, b;
+    step_args(h, s, first, a, b);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     launch_pass<MODE_PRED, true>(h, a);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    after_eval(h, true, h->cur);
-    if (h->nprev > 0) {
-      const int last = h->ipc[h->nprev - 1];
-      for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
-      h->ipc[0] = last;
-    }
-    // pass 2: forces at the predicted state, AM corrector, look-ahead
-    PassArgs b = base_args(h);
-    b.pos_in = h->pos[h->cur ^ 1];
-    b.kin_in = h->kin[h->cur ^ 1];
-    b.pos = h->pos[h->cur];
-    b.kin = h->kin[h->cur];
-    b.acc_out = h->keep_pred_acc ? h->acc_p : nullptr;
-    b.coef0 = h->am0;
-    for (int k = 0; k < h->nprev; ++k) {
-      b.slot_rd[k] = h->ring + (size_t)(h->ipc[k] - 1) * 9 * N;
-      b.coef[k] = h->am[k];
-      b.ab[k] = h->ab[k];
-    }
-    b.ab0 = h->ab0;
-    b.step_idx = (int)s;
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     launch_pass<MODE_CORR, false>(h, b);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
-    after_eval(h, false, h->cur ^ 1);
-    h->acc_in_ring = h->nprev > 0;
     // neighbour list, when pass 2 asked for it
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     enqueue_rebuild(h, h->cur, h->rb + (h->eval & 1), 0);
@@ -1246,16 +1277,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     // The kernels of the steps after the escape returned at once: undo their
     // host-side bookkeeping (ring rotation, evaluation count).
     done = (int64_t)c.stop_step + 1;
-    for (int64_t k = done; k < enqueued; ++k) {
-      if (h->nprev > 0) {
-        const int first_slot = h->ipc[0];
-        for (int q = 0; q + 1 < h->nprev; ++q) h->ipc[q] = h->ipc[q + 1];
-        h->ipc[h->nprev - 1] = first_slot;
-      }
-      h->eval -= 2;
-      h->steps_since_resort -= 1;
-    }
-    h->last_update_eval = h->eval - 2;
+    rollback_steps(h, done, enqueued);
   }
   h->steps += done;
   if (steps_done) *steps_done = done;
@@ -1313,6 +1335,343 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->ms_rebuild = h->ms_rebuild;
   out->ms_step = h->ms_step;
   return DEM_OK;
+}
+
+}  // extern "C"
+
+// ---- batches ---------------------------------------------------------------
+
+struct dem_batch {
+  char err[512];
+  std::vector<dem_handle_t> sys;
+  hipStream_t stream = nullptr;
+  int precision = -1;  // DEM_FP64 / DEM_MIXED of its systems
+  static const int K = 4;  // pinned staging slots (a step's table is copied from one)
+  BatchRec* h_rec[K] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[K] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_used[K] = {false, false, false, false};
+  size_t rec_cap = 0;
+  BatchRec* d_rec = nullptr;
+  int2* d_map[MAP_KINDS] = {nullptr, nullptr, nullptr, nullptr};
+  size_t map_cap[MAP_KINDS] = {0, 0, 0, 0};
+  int map_len[MAP_KINDS] = {0, 0, 0, 0};
+  bool dirty = true;
+  Ctl* d_ctl = nullptr;
+  Ctl* h_ctl = nullptr;
+  size_t ctl_cap = 0;
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  std::vector<double> step_ms;
+};
+
+static int bfail(dem_batch_t b, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(b ? b->err : g_err, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define BHIPCHK(b, expr)                                                                 \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return bfail((b), DEM_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Arguments of the neighbour-list rebuild of system h around pos[b] (the
+// kernels of enqueue_rebuild), gated on *gate.
+static RebuildArgs rebuild_args(dem_handle_t h, int b, const int* gate) {
+  RebuildArgs r;
+  memset(&r, 0, sizeof r);
+  r.pos = h->pos[b];
+  r.kin = h->kin[b];
+  r.xref = h->xref;
+  r.key = h->key;
+  r.cell_count = h->cell_count;
+  r.cell_start = h->cell_start;
+  r.cursor = h->cursor;
+  r.cell_idx = h->cell_idx;
+  r.tile_sum = h->tile_sum;
+  r.slice_w = h->slice_w;
+  r.d_es = h->d_es;
+  r.es = h->es;
+  r.gate = gate;
+  r.ctl = h->ctl;
+  r.gncell = h->gncell;
+  r.ntiles = h->ntiles;
+  r.nslices = h->nslices;
+  r.ntiles_np = h->ntiles_np;
+  return r;
+}
+
+// Block maps of the current systems: particles (256 per block), cell keys
+// (256 per block), cell-scan tiles, slice-scan tiles.
+static int batch_maps(dem_batch_t b) {
+  std::vector<int2> m[MAP_KINDS];
+  for (size_t i = 0; i < b->sys.size(); ++i) {
+    const dem_handle_t h = b->sys[i];
+    const int cnt[MAP_KINDS] = {nblocks(h->np), nblocks((long long)h->gncell + 1), h->ntiles,
+                                h->ntiles_np};
+    for (int k = 0; k < MAP_KINDS; ++k)
+      for (int j = 0; j < cnt[k]; ++j) m[k].push_back(make_int2((int)i, j));
+  }
+  for (int k = 0; k < MAP_KINDS; ++k) {
+    if (m[k].size() > b->map_cap[k]) {
+      if (b->d_map[k]) (void)hipFree(b->d_map[k]);
+      b->d_map[k] = nullptr;
+      BHIPCHK(b, hipMalloc((void**)&b->d_map[k], m[k].size() * sizeof(int2)));
+      b->map_cap[k] = m[k].size();
+    }
+    if (!m[k].empty())
+      BHIPCHK(b, hipMemcpyAsync(b->d_map[k], m[k].data(), m[k].size() * sizeof(int2),
+                                hipMemcpyHostToDevice, b->stream));
+    b->map_len[k] = (int)m[k].size();
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  return DEM_OK;
+}
+
+static int batch_capacity(dem_batch_t b, size_t n) {
+  if (n <= b->rec_cap) return DEM_OK;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (int k = 0; k < dem_batch::K; ++k) {
+    if (b->h_rec[k]) (void)hipHostFree(b->h_rec[k]);
+    b->h_rec[k] = nullptr;
+    BHIPCHK(b, hipHostMalloc((void**)&b->h_rec[k], n * sizeof(BatchRec), hipHostMallocDefault));
+    b->ev_used[k] = false;
+  }
+  if (b->d_rec) (void)hipFree(b->d_rec);
+  b->d_rec = nullptr;
+  BHIPCHK(b, hipMalloc((void**)&b->d_rec, n * sizeof(BatchRec)));
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  b->d_ctl = nullptr;
+  b->h_ctl = nullptr;
+  BHIPCHK(b, hipMalloc((void**)&b->d_ctl, n * sizeof(Ctl)));
+  BHIPCHK(b, hipHostMalloc((void**)&b->h_ctl, n * sizeof(Ctl), hipHostMallocDefault));
+  b->rec_cap = n;
+  return DEM_OK;
+}
+
+extern "C" {
+
+int dem_batch_create(dem_batch_t* out) {
+  if (!out) return bfail(nullptr, DEM_ERR_ARG, "null argument");
+  *out = nullptr;
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+    return bfail(nullptr, DEM_ERR_DEVICE, "no HIP device available");
+  dem_batch_t b = new dem_batch();
+  b->err[0] = 0;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  for (int k = 0; k < dem_batch::K && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+  if (e != hipSuccess) {
+    dem_batch_destroy(b);
+    return bfail(nullptr, DEM_ERR_DEVICE, "batch setup failed: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return DEM_OK;
+}
+
+void dem_batch_destroy(dem_batch_t b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto h : b->sys) {
+    h->in_batch = false;
+    h->batch = nullptr;
+  }
+  for (int k = 0; k < dem_batch::K; ++k) {
+    if (b->h_rec[k]) (void)hipHostFree(b->h_rec[k]);
+    if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+  }
+  for (int k = 0; k < MAP_KINDS; ++k)
+    if (b->d_map[k]) (void)hipFree(b->d_map[k]);
+  if (b->d_rec) (void)hipFree(b->d_rec);
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  for (auto e : b->tev) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* dem_batch_last_error(dem_batch_t b) { return b ? b->err : g_err; }
+
+int dem_batch_add(dem_batch_t b, dem_handle_t h) {
+  if (!b || !h) return bfail(b, DEM_ERR_ARG, "null argument");
+  if (!h->has_state || h->np == 0) return bfail(b, DEM_ERR_STATE, "upload a non-empty state first");
+  if (h->in_batch) return bfail(b, DEM_ERR_STATE, "the system already belongs to a batch");
+  const int prec = h->mixed ? DEM_MIXED : DEM_FP64;
+  if (b->precision >= 0 && prec != b->precision)
+    return bfail(b, DEM_ERR_ARG, "all systems of a batch use the same precision mode");
+  if (!h->rc_ready) {
+    enqueue_refcell(h, h->cur);
+    h->rc_ready = true;
+  }
+  BHIPCHK(b, hipStreamSynchronize(h->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->precision = prec;
+  h->in_batch = true;
+  h->batch = b;
+  h->batch_first = true;
+  b->sys.push_back(h);
+  b->dirty = true;
+  return DEM_OK;
+}
+
+int dem_batch_remove(dem_batch_t b, dem_handle_t h) {
+  if (!b || !h) return bfail(b, DEM_ERR_ARG, "null argument");
+  auto it = std::find(b->sys.begin(), b->sys.end(), h);
+  if (it == b->sys.end()) return bfail(b, DEM_ERR_ARG, "the system is not in this batch");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->sys.erase(it);
+  h->in_batch = false;
+  h->batch = nullptr;
+  b->dirty = true;
+  if (b->sys.empty()) b->precision = -1;
+  return DEM_OK;
+}
+
+int dem_batch_size(dem_batch_t b) { return b ? (int)b->sys.size() : 0; }
+
+int dem_batch_set_option(dem_batch_t b, const char* name, double value) {
+  if (!b || !name) return bfail(b, DEM_ERR_ARG, "null argument");
+  if (!strcmp(name, "timing")) b->timing = value != 0.0;
+  else return bfail(b, DEM_ERR_ARG, "unknown option '%s'", name);
+  return DEM_OK;
+}
+
+int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* escaped) {
+  if (!b) return bfail(nullptr, DEM_ERR_ARG, "null batch");
+  const size_t n = b->sys.size();
+  for (size_t i = 0; i < n; ++i) {
+    if (steps_done) steps_done[i] = 0;
+    if (escaped) escaped[i] = 0;
+  }
+  if (n == 0 || nsteps <= 0) return DEM_OK;
+  int rc = batch_capacity(b, n);
+  if (rc) return rc;
+  if (b->dirty) {
+    rc = batch_maps(b);
+    if (rc) return rc;
+  }
+  hipStream_t st = b->stream;
+  const bool mixed = b->precision == DEM_MIXED;
+  if (b->timing) {
+    while (b->tev.size() < (size_t)nsteps + 1) {
+      hipEvent_t e;
+      BHIPCHK(b, hipEventCreate(&e));
+      b->tev.push_back(e);
+    }
+  }
+  const int gp = b->map_len[MAP_PART], gc = b->map_len[MAP_CELL];
+  const int gt = b->map_len[MAP_CTILE], gs = b->map_len[MAP_STILE];
+  const int ns = (int)n;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int slot = (int)(s % dem_batch::K);
+    if (b->ev_used[slot]) BHIPCHK(b, hipEventSynchronize(b->ev[slot]));
+    BatchRec* R = b->h_rec[slot];
+    for (size_t i = 0; i < n; ++i) {
+      dem_handle_t h = b->sys[i];
+      BatchRec& r = R[i];
+      r.s = h->sc;
+      step_args(h, s, h->batch_first, r.p[0], r.p[1]);
+      h->batch_first = false;
+      h->steps_since_resort += 1;
+      r.r = rebuild_args(h, h->cur, h->rb + (h->eval & 1));
+      r.nblk = nblocks(h->np);
+    }
+    BHIPCHK(b, hipMemcpyAsync(b->d_rec, R, n * sizeof(BatchRec), hipMemcpyHostToDevice, st));
+    BHIPCHK(b, hipEventRecord(b->ev[slot], st));
+    b->ev_used[slot] = true;
+    const BatchRec* rec = b->d_rec;
+    if (s == 0)
+      hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
+    if (b->timing) BHIPCHK(b, hipEventRecord(b->tev[s], st));
+    // pass 1, pass 2 (one launch each for all systems) and their counters
+    if (mixed)
+      hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, true, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
+                         st, rec, (const int2*)b->d_map[MAP_PART]);
+    else
+      hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, false, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
+                         st, rec, (const int2*)b->d_map[MAP_PART]);
+    hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 0, 1);
+    if (mixed)
+      hipLaunchKernelGGL((k_pass_batch<MODE_CORR, false, true, 1>), dim3(gp), dim3(DEM_BLOCK), 0,
+                         st, rec, (const int2*)b->d_map[MAP_PART]);
+    else
+      hipLaunchKernelGGL((k_pass_batch<MODE_CORR, false, false, 1>), dim3(gp), dim3(DEM_BLOCK),
+                         0, st, rec, (const int2*)b->d_map[MAP_PART]);
+    hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 1, 0);
+    // neighbour lists of the systems whose pass 2 asked for one
+    const int2* mp = b->d_map[MAP_PART];
+    hipLaunchKernelGGL(k_fill_cells_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_CELL]);
+    hipLaunchKernelGGL(k_bin_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+    hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_CTILE], 0);
+    hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 0);
+    hipLaunchKernelGGL(k_scan_apply_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_CTILE], 0);
+    hipLaunchKernelGGL(k_bin_scatter_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+    hipLaunchKernelGGL(k_cell_sort_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_CELL]);
+    hipLaunchKernelGGL(k_nbr_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+    hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_STILE], 1);
+    hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 1);
+    hipLaunchKernelGGL(k_scan_apply_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
+                       (const int2*)b->d_map[MAP_STILE], 1);
+    hipLaunchKernelGGL(k_nbr_fill_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+    hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(64), 0, st, rec);
+  }
+  if (b->timing) BHIPCHK(b, hipEventRecord(b->tev[nsteps], st));
+  BHIPCHK(b, hipGetLastError());
+  hipLaunchKernelGGL(k_gather_ctl, dim3(nblocks(ns, 64)), dim3(64), 0, st,
+                     (const BatchRec*)b->d_rec, ns, b->d_ctl);
+  BHIPCHK(b, hipMemcpyAsync(b->h_ctl, b->d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  BHIPCHK(b, hipStreamSynchronize(st));
+  if (b->timing) {
+    b->step_ms.assign((size_t)nsteps, 0.0);
+    float ms;
+    for (int64_t s = 0; s < nsteps; ++s)
+      if (hipEventElapsedTime(&ms, b->tev[s], b->tev[s + 1]) == hipSuccess) b->step_ms[s] = ms;
+  }
+  int result = DEM_OK;
+  for (size_t i = 0; i < n; ++i) {
+    dem_handle_t h = b->sys[i];
+    const Ctl& c = b->h_ctl[i];
+    int64_t done = nsteps;
+    if (c.escaped > 0) {
+      done = (int64_t)c.stop_step + 1;
+      rollback_steps(h, done, nsteps);
+      if (result == DEM_OK) {
+        result = DEM_ERR_ESCAPE;
+        bfail(b, DEM_ERR_ESCAPE, "system %zu: %d particles left the box in step %lld", i,
+              c.escaped, (long long)done);
+      }
+    }
+    h->steps += done;
+    if (steps_done) steps_done[i] = done;
+    if (escaped) escaped[i] = c.escaped;
+    if (c.error) {
+      const int erc = check_errors(h);
+      if (erc && (result == DEM_OK || result == DEM_ERR_ESCAPE)) {
+        result = erc;
+        bfail(b, erc, "system %zu: %s", i, h->err);
+      }
+    }
+  }
+  return result;
+}
+
+// Device time of each step of the last dem_batch_step (option "timing").
+int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms) {
+  if (!b) return 0;
+  const int64_t n = (int64_t)b->step_ms.size();
+  for (int64_t k = 0; k < n && k < cap && ms; ++k) ms[k] = b->step_ms[k];
+  return n;
 }
 
 }  // extern "C"

@@ -282,8 +282,9 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
   }
 }
 
+// One force pass over the particles [blk*256, blk*256+256) of one system.
 template <int MODE, bool UPDATE, bool MIXED>
-__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, PassArgs a) {
+__device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
   Ctl* ctl = a.ctl;
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
   // a pass reads the same value here.
   if (ctl->escaped) return;
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t == 0) *a.rb_clear = 0;
+  if (blk == 0 && t == 0) *a.rb_clear = 0;
   const bool lds_tab = s.nc <= DEM_LDS_NC;
   if (lds_tab) {
     for (int q = t; q < s.nc * s.nc; q += DEM_BLOCK) sh_pair[q] = s.pair[q];
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
   const size_t N = (size_t)s.np;
   const int ep = *a.E.epar;
 
-  const int i = xcd_block() * DEM_BLOCK + t;
+  const int i = blk * DEM_BLOCK + t;
   const bool active = i < s.np;
   unsigned long long n_wall = 0;
   int need_rb = 0, n_esc = 0;
@@ -514,14 +515,19 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, Pa
   if (t < 4) {
     unsigned v = 0;
     for (int w = 0; w < DEM_BLOCK / 64; ++w) v += sh_cnt[w][t];
-    a.part[4 * (size_t)blockIdx.x + t] = v;
+    a.part[4 * (size_t)blk + t] = v;
   }
+}
+
+template <int MODE, bool UPDATE, bool MIXED>
+__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, PassArgs a) {
+  pass_body<MODE, UPDATE, MIXED>(s, a, xcd_block());
 }
 
 // Sums the per-block counters of the pass just run into the control block
 // and publishes the escapes its corrector found.  Runs as one block.
-__global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* part, int nblk,
-                                                     int par, int update) {
+__device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part, int nblk,
+                                                 int par, int update) {
   if (ctl->escaped) return;  // the pass did not run
   __shared__ unsigned long long sh[1024 / 64][4];
   const int t = threadIdx.x;
@@ -545,6 +551,10 @@ __global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* 
       ctl->esc_pending = 0;
     }
   }
+}
+__global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* part, int nblk,
+                                                     int par, int update) {
+  pass_reduce_body(ctl, part, nblk, par, update);
 }
 
 // ---------------------------------------------------------------------------
@@ -596,20 +606,33 @@ __device__ __forceinline__ bool gate_closed(const int* gate, int force, const Ct
   return !force && !*gate;
 }
 
-__global__ void k_fill_i32(int* p, int n, int v, const int* gate, int force, const Ctl* ctl) {
+// Kernel bodies take their block index blk and block count nblk explicitly:
+// the single-system kernels pass blockIdx.x / gridDim.x, the batched ones
+// (batch.hpp) the block's position within its own system.
+
+__device__ __forceinline__ void fill_i32_body(int* p, int n, int v, const int* gate, int force,
+                                              const Ctl* ctl, int blk, int nblk) {
   if (gate_closed(gate, force, ctl)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+  for (int i = blk * DEM_BLOCK + threadIdx.x; i < n; i += nblk * DEM_BLOCK) p[i] = v;
+}
+__global__ void k_fill_i32(int* p, int n, int v, const int* gate, int force, const Ctl* ctl) {
+  fill_i32_body(p, n, v, gate, force, ctl, blockIdx.x, gridDim.x);
 }
 
-__global__ void k_bin_count(SysConst s, const double4* pos, int* key, int* count, const int* gate,
-                            int force, const Ctl* ctl) {
+__device__ __forceinline__ void bin_count_body(const SysConst& s, const double4* pos, int* key,
+                                               int* count, const int* gate, int force,
+                                               const Ctl* ctl, int blk, int nblk) {
   if (gate_closed(gate, force, ctl)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.np; i += gridDim.x * blockDim.x) {
+  for (int i = blk * DEM_BLOCK + threadIdx.x; i < s.np; i += nblk * DEM_BLOCK) {
     const double4 P = pos[i];
     const int c = grid_cell(s, P.x, P.y, P.z);
     key[i] = c;
     atomicAdd(&count[c], 1);
   }
+}
+__global__ void k_bin_count(SysConst s, const double4* pos, int* key, int* count, const int* gate,
+                            int force, const Ctl* ctl) {
+  bin_count_body(s, pos, key, count, gate, force, ctl, blockIdx.x, gridDim.x);
 }
 
 #define SCAN_ITEMS 8
@@ -643,21 +666,25 @@ __device__ __forceinline__ int block_scan_excl(int v, int* total) {
   return off + inc - v;
 }
 
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_tiles(const int* in, int n, int* tile_sum,
-                                                          const int* gate, int force,
-                                                          const Ctl* ctl) {
+__device__ __forceinline__ void scan_tiles_body(const int* in, int n, int* tile_sum,
+                                                const int* gate, int force, const Ctl* ctl,
+                                                int blk) {
   if (gate_closed(gate, force, ctl)) return;
-  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const int base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k) s += (base + k < n) ? in[base + k] : 0;
   int tot;
   block_scan_excl(s, &tot);
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) tile_sum[blk] = tot;
 }
-
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int ntiles,
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_tiles(const int* in, int n, int* tile_sum,
                                                           const int* gate, int force,
                                                           const Ctl* ctl) {
+  scan_tiles_body(in, n, tile_sum, gate, force, ctl, blockIdx.x);
+}
+
+__device__ __forceinline__ void scan_spine_body(int* tile_sum, int ntiles, const int* gate,
+                                                int force, const Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
   int carry = 0;
   for (int b = 0; b < ntiles; b += DEM_BLOCK) {
@@ -669,18 +696,22 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int nti
     carry += tot;
   }
 }
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int ntiles,
+                                                          const int* gate, int force,
+                                                          const Ctl* ctl) {
+  scan_spine_body(tile_sum, ntiles, gate, force, ctl);
+}
 
 // Exclusive scan of in[0..n) into start[0..n] (start[n] = total); cursor (may
 // be null) receives a copy of start[0..n).  With out_sel != null the output
 // is the slice offsets of the edge set being built (1 - *epar).
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
-                                                          const int* tile_off, int* start,
-                                                          int* cursor, const EdgeSets* out_sel,
-                                                          const int* gate, int force,
-                                                          const Ctl* ctl) {
+__device__ __forceinline__ void scan_apply_body(const int* in, int n, const int* tile_off,
+                                                int* start, int* cursor,
+                                                const EdgeSets* out_sel, const int* gate,
+                                                int force, const Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
   if (out_sel) start = out_sel->soff[1 - *out_sel->epar];
-  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const int base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -688,7 +719,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
     s += v[k];
   }
   int tot;
-  int ex = block_scan_excl(s, &tot) + tile_off[blockIdx.x];
+  int ex = block_scan_excl(s, &tot) + tile_off[blk];
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) {
       start[base + k] = ex;
@@ -698,22 +729,35 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
   }
   if (base <= n - 1 && n - 1 < base + SCAN_ITEMS) start[n] = ex;
 }
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
+                                                          const int* tile_off, int* start,
+                                                          int* cursor, const EdgeSets* out_sel,
+                                                          const int* gate, int force,
+                                                          const Ctl* ctl) {
+  scan_apply_body(in, n, tile_off, start, cursor, out_sel, gate, force, ctl, blockIdx.x);
+}
 
-__global__ void k_bin_scatter(int np, const int* key, int* cursor, int* cell_idx,
-                              const int* gate, int force, const Ctl* ctl) {
+__device__ __forceinline__ void bin_scatter_body(int np, const int* key, int* cursor,
+                                                 int* cell_idx, const int* gate, int force,
+                                                 const Ctl* ctl, int blk, int nblk) {
   if (gate_closed(gate, force, ctl)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+  for (int i = blk * DEM_BLOCK + threadIdx.x; i < np; i += nblk * DEM_BLOCK) {
     const int p = atomicAdd(&cursor[key[i]], 1);
     cell_idx[p] = i;
   }
 }
+__global__ void k_bin_scatter(int np, const int* key, int* cursor, int* cell_idx,
+                              const int* gate, int force, const Ctl* ctl) {
+  bin_scatter_body(np, key, cursor, cell_idx, gate, force, ctl, blockIdx.x, gridDim.x);
+}
 
 // Sort each cell's slice by particle index (insertion sort; cells hold a few
 // particles), so the order does not depend on atomic arrival order.
-__global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const int* gate,
-                            int force, const Ctl* ctl) {
+__device__ __forceinline__ void cell_sort_body(int ncell, const int* start, int* cell_idx,
+                                               const int* gate, int force, const Ctl* ctl,
+                                               int blk, int nblk) {
   if (gate_closed(gate, force, ctl)) return;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+  for (int c = blk * DEM_BLOCK + threadIdx.x; c < ncell; c += nblk * DEM_BLOCK) {
     const int b = start[c], e = start[c + 1];
     for (int p = b + 1; p < e; ++p) {
       const int v = cell_idx[p];
@@ -725,6 +769,10 @@ __global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const in
       cell_idx[q + 1] = v;
     }
   }
+}
+__global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const int* gate,
+                            int force, const Ctl* ctl) {
+  cell_sort_body(ncell, start, cell_idx, gate, force, ctl, blockIdx.x, gridDim.x);
 }
 
 // Visits every j != i with |r_ij|^2 < (r_i + r_j + skin)^2 (minimum image),
@@ -775,14 +823,13 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
 // Pass 1 of the list build: neighbours per particle (capped at max_nb) into
 // cnt of the set being built, and 64 * (widest list) of each 64-particle
 // slice into slice_w.
-__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const double4* pos,
-                                                         const int* start,
-                                                         const int* cell_idx, EdgeSets E,
-                                                         int* slice_w, const int* gate, int force,
-                                                         Ctl* ctl) {
+__device__ __forceinline__ void nbr_count_body(const SysConst& s, const double4* pos,
+                                               const int* start, const int* cell_idx,
+                                               const EdgeSets& E, int* slice_w,
+                                               const int* gate, int force, Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
   const int q = 1 - *E.epar;
-  const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
+  const int i = blk * DEM_BLOCK + threadIdx.x;
   int n = 0, err = 0;
   if (i < s.np) {
     const double4 P = pos[i];
@@ -801,7 +848,14 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
     if (err) atomicOr(&ctl->error, err);
     atomicMax(&ctl->max_nnb, hw);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nbpairs = 0;
+  if (blk == 0 && threadIdx.x == 0) ctl->nbpairs = 0;
+}
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const double4* pos,
+                                                         const int* start,
+                                                         const int* cell_idx, EdgeSets E,
+                                                         int* slice_w, const int* gate, int force,
+                                                         Ctl* ctl) {
+  nbr_count_body(s, pos, start, cell_idx, E, slice_w, gate, force, ctl, blockIdx.x);
 }
 
 // Pass 2 of the list build: fill the new edge set and carry the history of
@@ -809,15 +863,14 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
 // the current edge set uses (null = identity; a resort passes its
 // permutation, and o2n its inverse).  Stored pairs that drop out of the list
 // are counted when the reference would have kept them (cells not adjacent).
-__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
-                                                        const int* start,
-                                                        const int* cell_idx, EdgeSets E,
-                                                        const int* src, const int* o2n,
-                                                        const Kin* kin_cur, double4* xref,
-                                                        const int* gate, int force, Ctl* ctl) {
+__device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* pos,
+                                              const int* start, const int* cell_idx,
+                                              const EdgeSets& E, const int* src, const int* o2n,
+                                              const Kin* kin_cur, double4* xref,
+                                              const int* gate, int force, Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
   const int p = *E.epar, q = 1 - p;
-  const int i = blockIdx.x * DEM_BLOCK + threadIdx.x;
+  const int i = blk * DEM_BLOCK + threadIdx.x;
   const long long ecap = E.ecap;
   unsigned long long nedges = 0, dropped = 0;
   if (i < s.np) {
@@ -874,14 +927,27 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
     if (nedges) atomicAdd(&ctl->nbpairs, nedges);
   }
 }
+__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
+                                                        const int* start,
+                                                        const int* cell_idx, EdgeSets E,
+                                                        const int* src, const int* o2n,
+                                                        const Kin* kin_cur, double4* xref,
+                                                        const int* gate, int force, Ctl* ctl) {
+  nbr_fill_body(s, pos, start, cell_idx, E, src, o2n, kin_cur, xref, gate, force, ctl,
+                blockIdx.x);
+}
 
 // Make the freshly built edge set current.
-__global__ void k_nbr_flip(EdgeSets E, const int* gate, int force, Ctl* ctl) {
+__device__ __forceinline__ void nbr_flip_body(const EdgeSets& E, const int* gate, int force,
+                                              Ctl* ctl) {
   if (gate_closed(gate, force, ctl)) return;
   if (threadIdx.x == 0) {
     *E.epar = 1 - *E.epar;
     ctl->rebuilds += 1;
   }
+}
+__global__ void k_nbr_flip(EdgeSets E, const int* gate, int force, Ctl* ctl) {
+  nbr_flip_body(E, gate, force, ctl);
 }
 
 // ---------------------------------------------------------------------------

@@ -124,6 +124,15 @@ class Library:
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+        "batch_destroy": (None, [C.c_void_p]),
+        "batch_last_error": (C.c_char_p, [C.c_void_p]),
+        "batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "batch_remove": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "batch_size": (C.c_int, [C.c_void_p]),
+        "batch_step": (C.c_int, [C.c_void_p, C.c_int64, _lp, _ip]),
+        "batch_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+        "batch_step_times": (C.c_int64, [C.c_void_p, C.c_int64, _dp]),
     }
 
     def __init__(self, path: str, prefix: str):
@@ -366,3 +375,63 @@ class System:
         s = Stats()
         self._check(self.lib.stats(self.h, C.byref(s)))
         return s.asdict()
+
+
+class Batch:
+    """Independent systems stepped together (dem_batch_*): every phase of a
+    step is one kernel launch for all of them."""
+
+    def __init__(self, lib: Library):
+        if not hasattr(lib, "batch_create"):
+            raise RuntimeError(f"{lib.path} has no batch entry points")
+        self.lib = lib
+        b = C.c_void_p()
+        rc = lib.batch_create(C.byref(b))
+        if rc != DEM_OK:
+            raise DemError(rc, lib.batch_last_error(None).decode())
+        self.b = b
+        self.systems = []
+
+    def _check(self, rc):
+        if rc != DEM_OK:
+            raise DemError(rc, self.lib.batch_last_error(self.b).decode())
+        return rc
+
+    def add(self, sysm: "System"):
+        self._check(self.lib.batch_add(self.b, sysm.h))
+        self.systems.append(sysm)
+
+    def remove(self, sysm: "System"):
+        self._check(self.lib.batch_remove(self.b, sysm.h))
+        self.systems.remove(sysm)
+
+    def set_timing(self, on=True):
+        self._check(self.lib.batch_set_option(self.b, b"timing", 1.0 if on else 0.0))
+
+    def step(self, nsteps, allow_escape=False):
+        n = len(self.systems)
+        done = np.zeros(max(n, 1), np.int64)
+        esc = np.zeros(max(n, 1), np.int32)
+        rc = self.lib.batch_step(self.b, int(nsteps), done.ctypes.data_as(_lp),
+                                 esc.ctypes.data_as(_ip))
+        if not (rc == DEM_ERR_ESCAPE and allow_escape):
+            self._check(rc)
+        return done[:n], esc[:n]
+
+    def step_times(self):
+        k = self.lib.batch_step_times(self.b, 0, None)
+        out = np.zeros(max(k, 1))
+        self.lib.batch_step_times(self.b, k, out.ctypes.data_as(_dp))
+        return out[:k]
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.batch_destroy(self.b)
+            self.b = None
+            self.systems = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -224,6 +224,33 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *                       particle and step); default 0 */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
+/* ---- batches ------------------------------------------------------------
+ * No counterpart in the reference (it runs one system per process).  For
+ * callers that run many independent systems: a batch steps them together,
+ * every phase of a step (the two force passes, their counter sums, each
+ * kernel of the neighbour-list rebuild) being one launch for all of its
+ * systems.  Systems are created, uploaded and started (dem_euler_startup)
+ * as usual and then added; while in a batch they are stepped only by
+ * dem_batch_step (dem_step on them fails), and downloads and dem_stats work
+ * between batch steps.  All systems of a batch use one precision mode;
+ * batched systems are not re-sorted. */
+typedef struct dem_batch* dem_batch_t;
+int dem_batch_create(dem_batch_t* out);
+void dem_batch_destroy(dem_batch_t b);  /* leaves the systems alive */
+const char* dem_batch_last_error(dem_batch_t b);
+int dem_batch_add(dem_batch_t b, dem_handle_t h);
+int dem_batch_remove(dem_batch_t b, dem_handle_t h);
+int dem_batch_size(dem_batch_t b);
+/* nsteps MoveParticles steps of every system; steps_done[i] / escaped[i]
+ * (may be NULL) per system in the order of addition.  A system in which
+ * particles escaped stops after that step (DEM_ERR_ESCAPE is returned). */
+int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
+                   int32_t* escaped);
+/* "timing": 1 = record the device time of every step of a dem_batch_step
+ * call, read back with dem_batch_step_times (returns the count). */
+int dem_batch_set_option(dem_batch_t b, const char* name, double value);
+int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
