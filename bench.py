@@ -18,6 +18,12 @@ Prints one JSON line:
                arithmetic, FP64 state and spring history): rate, speed-up over
                FP64, and its kinetic energy and packing fraction against the
                FP64 run's after the same steps
+  ensemble     256 independent systems (500..20 000 spheres each) stepped as
+               one batch (one launch per phase) in mixed and in FP64, against
+               the plain FP64 path that steps each system alone
+  serving      a prefix of the Poisson request trace served with continuous
+               batching: p50/p99 step time, waiting and completion latency,
+               aggregate particle-steps/s
 """
 from __future__ import annotations
 
@@ -26,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -106,6 +114,93 @@ def timed_run(wl, args, precision, hmode):
     return out
 
 
+def ensemble_run(args):
+    """The 256-system ensemble (sizes log-uniform in [500, 20 000]): batched
+    mixed precision for --ens-steps steps, batched FP64, and the plain FP64
+    path (each system stepped alone, on a --alone-steps sample)."""
+    from dem3d_amd import workloads
+    from dem3d_amd.native import DEM_FP64, DEM_MIXED, Batch, System, load_hip
+    lib = load_hip()
+    members = workloads.ensemble_256()
+    ntot = sum(w.np for w in members)
+
+    def build(precision):
+        out = []
+        for w in members:
+            s = System(lib, w.red, precision=precision)
+            s.upload(w.comp, w.x, w.v, w.w)
+            s.startup()
+            out.append(s)
+        for s in out:
+            s.synchronize()
+        return out
+
+    def batched(systems, steps):
+        b = Batch(lib)
+        b.set_timing(True)
+        for s in systems:
+            b.add(s)
+        b.step(20)  # warm-up
+        t0 = time.perf_counter()
+        b.step(steps)
+        sec = time.perf_counter() - t0
+        st = b.step_times()
+        b.close()
+        return ntot * steps / sec, st
+
+    res = {"systems": len(members), "spheres": ntot,
+           "sizes": [int(min(w.np for w in members)), int(max(w.np for w in members))]}
+    sysm = build(DEM_MIXED)
+    v_mx, st_mx = batched(sysm, args.ens_steps)
+    for s in sysm:
+        s.close()
+    print(f"ensemble mixed batched: {v_mx:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    sys64 = build(DEM_FP64)
+    v_64b, st_64 = batched(sys64, args.ens_steps)
+    t_alone = 0.0
+    ps_alone = 0.0
+    for s, w in zip(sys64, members):
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(args.alone_steps)
+        s.synchronize()
+        t_alone += time.perf_counter() - t0
+        ps_alone += w.np * args.alone_steps
+    for s in sys64:
+        s.close()
+    v_alone = ps_alone / t_alone
+    print(f"ensemble fp64 batched: {v_64b:.4g}, alone: {v_alone:.4g}", file=sys.stderr,
+          flush=True)
+    res.update({
+        "mixed_batched_particle_steps_per_s": v_mx,
+        "mixed_batched_steps": args.ens_steps,
+        "step_ms_p50": float(np.percentile(st_mx, 50)),
+        "step_ms_p99": float(np.percentile(st_mx, 99)),
+        "fp64_batched_particle_steps_per_s": v_64b,
+        "fp64_batched_step_ms_p50": float(np.percentile(st_64, 50)),
+        "fp64_alone_particle_steps_per_s": v_alone,
+        "fp64_alone_sample_steps": args.alone_steps,
+        "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
+        "speedup_fp64_batched_vs_fp64_alone": v_64b / v_alone,
+    })
+    return res
+
+
+def serving_run(args):
+    """The first --serve-requests requests of the Poisson serving trace
+    (50 req/s, sizes log-uniform in [1 000, 500 000], steps U[1 000, 20 000]),
+    served in real time with continuous batching in mixed precision."""
+    from dem3d_amd import serve, workloads
+    from dem3d_amd.native import DEM_MIXED, load_hip
+    trace = workloads.serving_trace()[: args.serve_requests]
+    rep = serve.serve(load_hip(), trace, precision=DEM_MIXED, chunk=100,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = rep.summary()
+    out["trace"] = (f"first {len(trace)} of 200 requests, Poisson 50/s, n log-uniform "
+                    f"[1000, 500000], steps U[1000, 20000], mixed precision")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +214,12 @@ def main(argv=None):
     ap.add_argument("--precision", choices=("fp64", "both"), default="both",
                     help="fp64: the FP64 path only; both: also time the mixed-precision mode and "
                          "compare its energy and packing fraction with the FP64 run")
+    ap.add_argument("--ens-steps", type=int, default=10000,
+                    help="steps of the batched 256-system ensemble (0 = skip the ensemble)")
+    ap.add_argument("--alone-steps", type=int, default=100,
+                    help="steps per system of the one-at-a-time FP64 sample")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving trace to run (0 = skip serving)")
     ap.add_argument("--history", choices=("reference", "canonical"), default="reference",
                     help="orientation of stored tangential history (see include/dem3d.h)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -176,6 +277,8 @@ def main(argv=None):
             "packing_fraction_mixed": phimx,
             "packing_fraction_diff": phimx - phi64,
         }
+    ensemble = ensemble_run(args) if args.ens_steps > 0 else None
+    serving = serving_run(args) if args.serve_requests > 0 else None
     P, W = st["pair_contacts"], st["wall_contacts"]
     b1 = PASS1_B_PER_PARTICLE * wl.np + PASS1_B_PER_CONTACT * (P + W)
     b2 = PASS2_B_PER_PARTICLE * wl.np + PASS2_B_PER_CONTACT * (P + W)
@@ -224,6 +327,8 @@ def main(argv=None):
         },
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "ensemble": ensemble,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -1,0 +1,160 @@
+"""Continuous batching of simulation requests on one GPU.
+
+Requests (workloads.Request: arrival time, size, steps, contact parameters)
+arrive on their own clock.  The server admits a waiting request as soon as
+its estimated device footprint fits the budget next to the systems already
+resident, builds and starts it on its own stream (upload + Euler start-up)
+while the batch keeps running, and then adds it to the batch.  The batch
+advances all resident systems together in chunks of at most `chunk` steps
+(bounded by the fewest remaining steps, so a finished system leaves at once
+and its memory is freed for the next admission).  Every system's state
+evolves exactly as it would alone (the batched kernels run the same bodies
+with the same arguments), which tests/test_gpu_serve.py checks.
+
+Reported: per-step device time of the batch (p50/p99), per-request waiting
+time (arrival to admission) and completion latency (arrival to the end of
+its last step), and the aggregate particle-steps/s over the run.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import workloads
+from .native import DEM_MIXED, Batch, Library, System
+
+# Device bytes per particle of one system (two state buffers, two edge sets
+# of max_nb slots with 4 + 1 + 48 bytes each, ring and staging) and a fixed
+# part per system; an upper estimate used for admission.
+BYTES_PER_PARTICLE = 2 * (32 + 64 + 4 + 4 + 48) + 2 * 48 * (4 + 1 + 48) + 2 * 216 + 200
+BYTES_PER_SYSTEM = 32 << 20
+
+
+def estimate_device_bytes(n: int) -> int:
+    return BYTES_PER_SYSTEM + BYTES_PER_PARTICLE * int(n)
+
+
+@dataclass
+class Record:
+    request: workloads.Request
+    index: int
+    admitted: float = float("nan")
+    started: float = float("nan")
+    finished: float = float("nan")
+    remaining: int = 0
+    system: Optional[System] = None
+    result: Optional[dict] = None
+
+
+@dataclass
+class Report:
+    records: List[Record]
+    step_ms: np.ndarray
+    wall_s: float
+    particle_steps: float
+    chunks: int
+    max_resident: int
+    peak_bytes: int
+
+    def summary(self) -> dict:
+        lat = np.array([r.finished - r.request.arrival for r in self.records])
+        wait = np.array([r.admitted - r.request.arrival for r in self.records])
+        st = self.step_ms if self.step_ms.size else np.zeros(1)
+        return {
+            "requests": len(self.records),
+            "particle_steps": self.particle_steps,
+            "wall_s": self.wall_s,
+            "particle_steps_per_s": self.particle_steps / self.wall_s if self.wall_s else 0.0,
+            "step_ms_p50": float(np.percentile(st, 50)),
+            "step_ms_p99": float(np.percentile(st, 99)),
+            "completion_s_p50": float(np.percentile(lat, 50)),
+            "completion_s_p99": float(np.percentile(lat, 99)),
+            "wait_s_p50": float(np.percentile(wait, 50)),
+            "wait_s_p99": float(np.percentile(wait, 99)),
+            "batch_steps": int(st.size),
+            "chunks": self.chunks,
+            "max_resident": self.max_resident,
+            "peak_device_bytes_est": self.peak_bytes,
+        }
+
+
+def build_system(lib: Library, req: workloads.Request, precision: int) -> System:
+    wl = workloads.ensemble_member(req.n, req.mu, req.e, req.seed, steps=req.steps)
+    s = System(lib, wl.red, precision=precision)
+    s.upload(wl.comp, wl.x, wl.v, wl.w)
+    s.startup()
+    return s
+
+
+def serve(lib: Library, trace: List[workloads.Request], precision: int = DEM_MIXED,
+          budget_bytes: int = 200 << 30, chunk: int = 100, realtime: bool = True,
+          keep: Optional[Callable[[Record], dict]] = None, log=None) -> Report:
+    """Runs the trace to completion.  realtime=False admits requests as if
+    each had already arrived when the previous chunk ended (a closed loop,
+    for tests).  keep(record) may extract results before a finished system
+    is destroyed."""
+    recs = [Record(q, k) for k, q in enumerate(sorted(trace, key=lambda q: q.arrival))]
+    pending = list(recs)
+    active: List[Record] = []
+    batch = Batch(lib)
+    batch.set_timing(True)
+    step_ms: List[np.ndarray] = []
+    used = 0
+    peak = 0
+    max_res = 0
+    chunks = 0
+    psteps = 0.0
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0
+    try:
+        while pending or active:
+            # admit everything that has arrived and fits
+            while pending and (not realtime or pending[0].request.arrival <= now()):
+                need = estimate_device_bytes(pending[0].request.n)
+                if active and used + need > budget_bytes:
+                    break
+                r = pending.pop(0)
+                r.admitted = now()
+                r.system = build_system(lib, r.request, precision)
+                r.remaining = r.request.steps
+                batch.add(r.system)
+                r.started = now()
+                active.append(r)
+                used += need
+                peak = max(peak, used)
+            if not active:
+                if pending:  # idle until the next arrival
+                    time.sleep(max(0.0, min(0.05, pending[0].request.arrival - now())))
+                continue
+            max_res = max(max_res, len(active))
+            k = min(chunk, min(r.remaining for r in active))
+            n_part = sum(r.request.n for r in active)
+            done, esc = batch.step(k)
+            chunks += 1
+            step_ms.append(batch.step_times())
+            psteps += float(n_part) * k
+            t = now()
+            still = []
+            for r in active:
+                r.remaining -= k
+                if r.remaining <= 0:
+                    r.finished = t
+                    if keep is not None:
+                        r.result = keep(r)
+                    batch.remove(r.system)
+                    r.system.close()
+                    r.system = None
+                    used -= estimate_device_bytes(r.request.n)
+                else:
+                    still.append(r)
+            active = still
+            if log is not None:
+                log(f"t={t:.2f}s chunk={k} resident={len(active)} pending={len(pending)}")
+    finally:
+        batch.close()
+    wall = now()
+    sm = np.concatenate(step_ms) if step_ms else np.zeros(0)
+    return Report(recs, sm, wall, psteps, chunks, max_res, peak)

@@ -213,3 +213,80 @@ def small_box(n_side=(6, 6, 6), seed=1, jitter=0.05, spacing=1.02, nc=2, ratio=1
     v = rng.uniform(-vel, vel, pos.shape) * T / L
     w = rng.uniform(-vel, vel, pos.shape) * T / L / 0.5
     return Workload(f"small-{n}", spec, red, comp, pos / L, v, w, steps)
+
+
+def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,
+                    steps: int = 10000, phi: float = 0.5) -> Workload:
+    """One system of an ensemble: n spheres with radii U[0.9, 1.1] r0
+    discretised into nbins equal-width components, on a jittered FCC lattice
+    at volume fraction ~phi in a box periodic in x,y with a floor, settling
+    under gravity; friction mu and restitution e for every pair and the
+    floor, kn = 1e3 N/m."""
+    kn = 1e3
+    rng = np.random.default_rng(seed)
+    edges = np.linspace(0.9 * r0, 1.1 * r0, nbins + 1)
+    centres = 0.5 * (edges[1:] + edges[:-1])
+    r = rng.uniform(0.9 * r0, 1.1 * r0, n)
+    comp = np.clip(np.searchsorted(edges, r) - 1, 0, nbins - 1)
+    rr = centres[comp]
+    mean_vol = float(np.mean(4.0 / 3.0 * math.pi * rr ** 3))
+    a = (4.0 * mean_vol / phi) ** (1.0 / 3.0)          # FCC constant for phi
+    nxy = max(3, int(round((n / 4.0) ** (1.0 / 3.0))))  # roughly cubic
+    nz = int(math.ceil(n / (4 * nxy * nxy)))
+    lx = ly = nxy * a
+    lz = nz * a + 6.0 * r0 + 0.5 * a
+    pos = _fcc(nxy, nxy, nz, a)[:, :n]
+    pos[0] -= 0.5 * lx
+    pos[1] -= 0.5 * ly
+    pos[2] += -0.5 * lz + 1.1 * r0
+    gap = a / math.sqrt(2.0) - 2.2 * r0
+    pos = pos + rng.uniform(-0.45 * gap, 0.45 * gap, pos.shape)
+    spec = _spec((lx, ly, lz), list(2.0 * centres), 2500.0, kn, e, mu,
+                 walls=_floor(lx, ly, lz, kn), total_steps=steps)
+    red = reduce_spec(spec)
+    L, T = red.length_scale, red.time_scale
+    vel = rng.uniform(-0.05, 0.05, pos.shape)
+    return Workload(f"member-{n}", spec, red, (comp + 1).astype(np.int32), pos / L,
+                    vel * T / L, np.zeros_like(pos), steps,
+                    note=f"n={n} mu={mu:.3f} e={e:.3f} FCC a={a*1e3:.4f} mm phi~{phi}")
+
+
+def ensemble_256(seed: int = 20240004, n_systems: int = 256, steps: int = 10000):
+    """The ensemble of 256 independent systems: sizes log-uniform in
+    [500, 20 000] spheres, mu ~ U[0.1, 0.7], e ~ U[0.6, 0.95], radii
+    U[0.9, 1.1] r0 in 8 components, each in its own periodic box with a floor
+    at phi ~ 0.5.  Returns the list of member workloads."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(500), math.log(20000), n_systems))).astype(int)
+    mus = rng.uniform(0.1, 0.7, n_systems)
+    es = rng.uniform(0.6, 0.95, n_systems)
+    seeds = rng.integers(0, 2 ** 31 - 1, n_systems)
+    return [ensemble_member(int(n), float(m), float(e), int(sd), steps=steps)
+            for n, m, e, sd in zip(sizes, mus, es, seeds)]
+
+
+@dataclass
+class Request:
+    arrival: float      # seconds after the start of the trace
+    n: int
+    steps: int
+    mu: float
+    e: float
+    seed: int
+
+
+def serving_trace(seed: int = 20240005, n_requests: int = 200, rate: float = 50.0,
+                  n_range=(1000, 500000), step_range=(1000, 20000)):
+    """Requests of the serving stream: Poisson arrivals at `rate` per second,
+    sizes log-uniform in n_range, step counts uniform in step_range, contact
+    parameters as in the ensemble."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    ns = np.round(np.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1]),
+                                     n_requests))).astype(int)
+    st = rng.integers(step_range[0], step_range[1] + 1, n_requests)
+    mus = rng.uniform(0.1, 0.7, n_requests)
+    es = rng.uniform(0.6, 0.95, n_requests)
+    seeds = rng.integers(0, 2 ** 31 - 1, n_requests)
+    return [Request(float(a), int(n), int(s), float(m), float(e), int(sd))
+            for a, n, s, m, e, sd in zip(t, ns, st, mus, es, seeds)]
