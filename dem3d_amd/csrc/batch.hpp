@@ -1,13 +1,15 @@
 // batch.hpp -- many independent systems stepped together.
 //
 // Every system keeps its own buffers (one dem_system each).  A batch adds a
-// device table with one BatchRec per system -- its constants and the
-// arguments of the two passes and of the neighbour-list rebuild of the
-// current step, written by the host into pinned memory and copied over once
-// per step -- and block maps that assign each block of a launch to
+// device table with one BatchRec per system -- its constants, the arguments
+// of its two passes and of its neighbour-list rebuild, and the step
+// bookkeeping (evaluation count, ABM ring order) -- uploaded once per
+// dem_batch_step call and advanced on the device by k_batch_prepare at the
+// start of every step, and block maps that assign each block of a launch to
 // (system, block within that system).  Each phase of a step is then one
 // launch for the whole batch, whatever the number and sizes of its systems,
-// and the kernel bodies are the single-system ones.
+// the launches of every step are the same, and the kernel bodies are the
+// single-system ones.
 #pragma once
 
 #include "kernels.hpp"
@@ -35,8 +37,59 @@ struct BatchRec {
   PassArgs p[2];  // pass 1 (forces at x_n + predictor), pass 2 (forces at the prediction + corrector)
   RebuildArgs r;
   int nblk;       // particle blocks of the system
-  int pad[3];
+  // step bookkeeping, advanced by k_batch_prepare
+  int nprev;
+  int first;      // 1 until the first step after the system joined the batch
+  int step;       // steps of the current dem_batch_step call so far
+  int ipc[3];     // ring slot order, 1-based (ipc(1) = most recent)
+  int pad;
+  long long eval, stamp_base;
+  double* ring;
+  int* rb;
 };
+
+// Arguments of the next MoveParticles step of every system, from its
+// bookkeeping; the same as the host's step_args would give (evaluation
+// parity, entry stamps, ring slots rotated as ipc = cshift(ipc, -1) between
+// the passes, src/MDEM.f90:329-334).  One thread per system.
+__global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  BatchRec& r = rec[k];
+  const size_t N = (size_t)r.s.np;
+  const int np = r.nprev;
+  int ipc[3] = {r.ipc[0], r.ipc[1], r.ipc[2]};
+  long long ev = r.eval;
+  PassArgs& a = r.p[0];
+  int par = (int)(ev & 1);
+  a.stamp_next = (int)(r.stamp_base + ev + 1);
+  a.par = par;
+  a.rb_clear = r.rb + par;
+  a.rb_set = r.rb + (par ^ 1);
+  for (int q = 0; q < np; ++q) a.slot_rd[q] = r.ring + (size_t)(ipc[q] - 1) * 9 * N;
+  a.slot_wr = np > 0 ? r.ring + (size_t)(ipc[np - 1] - 1) * 9 * N : nullptr;
+  a.check_pred = r.first;
+  r.first = 0;
+  ev += 1;
+  if (np > 0) {
+    const int last = ipc[np - 1];
+    for (int q = np - 1; q > 0; --q) ipc[q] = ipc[q - 1];
+    ipc[0] = last;
+  }
+  PassArgs& b = r.p[1];
+  par = (int)(ev & 1);
+  b.stamp_next = (int)(r.stamp_base + ev + 1);
+  b.par = par;
+  b.rb_clear = r.rb + par;
+  b.rb_set = r.rb + (par ^ 1);
+  for (int q = 0; q < np; ++q) b.slot_rd[q] = r.ring + (size_t)(ipc[q] - 1) * 9 * N;
+  b.step_idx = r.step;
+  r.step += 1;
+  ev += 1;
+  r.r.gate = r.rb + (int)(ev & 1);
+  r.eval = ev;
+  for (int q = 0; q < 3; ++q) r.ipc[q] = ipc[q];
+}
 
 // Block maps: (system, block within the system) per block of a launch.
 enum { MAP_PART = 0, MAP_CELL = 1, MAP_CTILE = 2, MAP_STILE = 3, MAP_KINDS = 4 };

@@ -1405,6 +1405,63 @@ static RebuildArgs rebuild_args(dem_handle_t h, int b, const int* gate) {
   return r;
 }
 
+// Table entry of system h at the start of a dem_batch_step call: constants,
+// the step-invariant arguments of its passes and rebuild (those step_args
+// and enqueue_rebuild give) and its bookkeeping; k_batch_prepare fills in
+// the per-step arguments.
+static void batch_rec_init(dem_handle_t h, BatchRec& r) {
+  memset(&r, 0, sizeof r);
+  r.s = h->sc;
+  PassArgs a = base_args(h);
+  a.pos_in = h->pos[h->cur];
+  a.kin_in = h->kin[h->cur];
+  a.pos_out = h->pos[h->cur ^ 1];
+  a.kin_out = h->kin[h->cur ^ 1];
+  a.acc_out = h->acc;
+  a.coef0 = h->ab0;
+  for (int k = 0; k < h->nprev; ++k) a.coef[k] = h->ab[k];
+  PassArgs b = base_args(h);
+  b.pos_in = h->pos[h->cur ^ 1];
+  b.kin_in = h->kin[h->cur ^ 1];
+  b.pos = h->pos[h->cur];
+  b.kin = h->kin[h->cur];
+  b.acc_out = h->keep_pred_acc ? h->acc_p : nullptr;
+  b.coef0 = h->am0;
+  for (int k = 0; k < h->nprev; ++k) {
+    b.coef[k] = h->am[k];
+    b.ab[k] = h->ab[k];
+  }
+  b.ab0 = h->ab0;
+  r.p[0] = a;
+  r.p[1] = b;
+  r.r = rebuild_args(h, h->cur, h->rb);
+  r.nblk = nblocks(h->np);
+  r.nprev = h->nprev;
+  r.first = h->batch_first ? 1 : 0;
+  r.step = 0;
+  for (int k = 0; k < 3; ++k) r.ipc[k] = h->ipc[k];
+  r.eval = h->eval;
+  r.stamp_base = h->stamp_base;
+  r.ring = h->ring;
+  r.rb = h->rb;
+}
+
+// Host-side bookkeeping of n batched steps (what step_args does per step).
+static void batch_advance_host(dem_handle_t h, int64_t n) {
+  for (int64_t s = 0; s < n; ++s) {
+    after_eval(h, true, h->cur);
+    if (h->nprev > 0) {
+      const int last = h->ipc[h->nprev - 1];
+      for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+      h->ipc[0] = last;
+    }
+    after_eval(h, false, h->cur ^ 1);
+  }
+  h->acc_in_ring = h->nprev > 0;
+  h->batch_first = false;
+  h->steps_since_resort += n;
+}
+
 // Block maps of the current systems: particles (256 per block), cell keys
 // (256 per block), cell-scan tiles, slice-scan tiles.
 static int batch_maps(dem_batch_t b) {
@@ -1568,27 +1625,19 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   const int gp = b->map_len[MAP_PART], gc = b->map_len[MAP_CELL];
   const int gt = b->map_len[MAP_CTILE], gs = b->map_len[MAP_STILE];
   const int ns = (int)n;
+  // the table of this call, once; every step then advances it on the device
+  BatchRec* R = b->h_rec[0];
+  if (b->ev_used[0]) BHIPCHK(b, hipEventSynchronize(b->ev[0]));
+  for (size_t i = 0; i < n; ++i) batch_rec_init(b->sys[i], R[i]);
+  BHIPCHK(b, hipMemcpyAsync(b->d_rec, R, n * sizeof(BatchRec), hipMemcpyHostToDevice, st));
+  BHIPCHK(b, hipEventRecord(b->ev[0], st));
+  b->ev_used[0] = true;
+  BatchRec* rec = b->d_rec;
+  hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
+                     ns);
   for (int64_t s = 0; s < nsteps; ++s) {
-    const int slot = (int)(s % dem_batch::K);
-    if (b->ev_used[slot]) BHIPCHK(b, hipEventSynchronize(b->ev[slot]));
-    BatchRec* R = b->h_rec[slot];
-    for (size_t i = 0; i < n; ++i) {
-      dem_handle_t h = b->sys[i];
-      BatchRec& r = R[i];
-      r.s = h->sc;
-      step_args(h, s, h->batch_first, r.p[0], r.p[1]);
-      h->batch_first = false;
-      h->steps_since_resort += 1;
-      r.r = rebuild_args(h, h->cur, h->rb + (h->eval & 1));
-      r.nblk = nblocks(h->np);
-    }
-    BHIPCHK(b, hipMemcpyAsync(b->d_rec, R, n * sizeof(BatchRec), hipMemcpyHostToDevice, st));
-    BHIPCHK(b, hipEventRecord(b->ev[slot], st));
-    b->ev_used[slot] = true;
-    const BatchRec* rec = b->d_rec;
-    if (s == 0)
-      hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
     if (b->timing) BHIPCHK(b, hipEventRecord(b->tev[s], st));
+    hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
     // pass 1, pass 2 (one launch each for all systems) and their counters
     if (mixed)
       hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, true, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
@@ -1641,6 +1690,7 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   int result = DEM_OK;
   for (size_t i = 0; i < n; ++i) {
     dem_handle_t h = b->sys[i];
+    batch_advance_host(h, nsteps);
     const Ctl& c = b->h_ctl[i];
     int64_t done = nsteps;
     if (c.escaped > 0) {
