@@ -211,5 +211,7 @@ __global__ void k_reset_escape(const BatchRec* __restrict__ rec, int n) {
     c->stop_step = 0x7fffffff;
     c->escaped = 0;
     c->esc_pending = 0;
+    c->pairs_upd_sum = 0;
+    c->upd_evals = 0;
   }
 }

@@ -50,6 +50,8 @@ struct Ctl {
   unsigned long long stored[2];     // stored pair entries after an update eval
   unsigned long long nbpairs;       // directed neighbour entries of last build
   unsigned long long stale_dropped; // stored pairs lost from the list at a rebuild
+  unsigned long long pairs_upd_sum; // pair contacts summed over updating passes
+  unsigned long long upd_evals;     // updating passes summed
 };
 
 enum {

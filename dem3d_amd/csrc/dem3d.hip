@@ -1218,6 +1218,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     HIPCHK(h, hipMemcpyAsync(&h->ctl->stop_step, &big, sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(h, hipMemsetAsync(&h->ctl->escaped, 0, sizeof(int), st));
     HIPCHK(h, hipMemsetAsync(&h->ctl->esc_pending, 0, sizeof(int), st));
+    HIPCHK(h, hipMemsetAsync(&h->ctl->pairs_upd_sum, 0, 2 * sizeof(unsigned long long), st));
   }
   if (!h->rc_ready) {
     enqueue_refcell(h, h->cur);
@@ -1334,6 +1335,7 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->ms_pass2 = h->ms_pass2;
   out->ms_rebuild = h->ms_rebuild;
   out->ms_step = h->ms_step;
+  out->mean_pair_contacts = c.upd_evals ? (double)c.pairs_upd_sum / (double)c.upd_evals : 0.0;
   return DEM_OK;
 }
 

@@ -545,7 +545,11 @@ __device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part,
     ctl->ncol += s[0];
     ctl->pairs[par] = s[1];
     ctl->wallc[par] = s[2];
-    if (update) ctl->stored[par] = s[3];
+    if (update) {
+      ctl->stored[par] = s[3];
+      ctl->pairs_upd_sum += s[1];
+      ctl->upd_evals += 1;
+    }
     if (ctl->esc_pending) {
       ctl->escaped = ctl->esc_pending;
       ctl->esc_pending = 0;

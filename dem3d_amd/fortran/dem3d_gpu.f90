@@ -63,6 +63,7 @@ module dem3d_gpu
     integer(c_int64_t) :: ref_ncell(3), nb_ncell(3)
     real(c_double) :: kinetic_energy, max_overlap
     real(c_double) :: ms_pass1, ms_pass2, ms_rebuild, ms_step
+    real(c_double) :: mean_pair_contacts
   end type dem_stats_t
 
   public :: dem_create, dem_destroy, dem_last_error, dem_set_walls

@@ -71,6 +71,7 @@ class Stats(C.Structure):
         ("stale_dropped", C.c_int64), ("ref_ncell", C.c_int64 * 3), ("nb_ncell", C.c_int64 * 3),
         ("kinetic_energy", C.c_double), ("max_overlap", C.c_double), ("ms_pass1", C.c_double),
         ("ms_pass2", C.c_double), ("ms_rebuild", C.c_double), ("ms_step", C.c_double),
+        ("mean_pair_contacts", C.c_double),
     ]
 
     def asdict(self):

@@ -153,6 +153,8 @@ typedef struct {
   double ms_pass2;
   double ms_rebuild;        /* mean over the rebuilds that ran                 */
   double ms_step;           /* mean device time per step                       */
+  double mean_pair_contacts;/* P averaged over the first passes of the last
+                               dem_step call                                   */
 } dem_stats_t;
 
 /* ---- lifetime ------------------------------------------------------------ */
