@@ -46,6 +46,14 @@ struct BatchRec {
   long long eval, stamp_base;
   double* ring;
   int* rb;
+  // MoveWalls of systems with WallsInMovement (nw_move > 0), at
+  // Time = (next_step + step)*TimeStep before pass 2 (src/MDEM.f90:333-337)
+  WallGeom* walls;
+  const WallMotion* wm;
+  int nw_move;
+  int pad2;
+  long long next_step;
+  double dt, time;
 };
 
 // Arguments of the next MoveParticles step of every system, from its
@@ -84,6 +92,7 @@ __global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
   b.rb_set = r.rb + (par ^ 1);
   for (int q = 0; q < np; ++q) b.slot_rd[q] = r.ring + (size_t)(ipc[q] - 1) * 9 * N;
   b.step_idx = r.step;
+  r.time = (double)(r.next_step + r.step) * r.dt;
   r.step += 1;
   ev += 1;
   r.r.gate = r.rb + (int)(ev & 1);
@@ -104,6 +113,12 @@ __global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass_batch(
   const int2 m = map[xcd_block()];
   const BatchRec& r = rec[m.x];
   pass_body<MODE, UPDATE, MIXED>(r.s, r.p[PI], m.y);
+}
+
+// One block per system.
+__global__ void k_move_walls_batch(const BatchRec* __restrict__ rec) {
+  const BatchRec& r = rec[blockIdx.x];
+  if (r.nw_move > 0) move_walls_body(r.walls, r.wm, r.nw_move, r.time, r.r.ctl);
 }
 
 // One block per system.

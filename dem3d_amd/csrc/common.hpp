@@ -33,6 +33,11 @@ struct WallGeom {
   double rap, sin_t, cos_t;
 };
 
+// TWallMovement (src/MDEM.f90:39-45), reduced units.
+struct WallMotion {
+  double cf[3], amp[3], vamp[3], tpf[3], phase[3];
+};
+
 // Device-side control block: flags and counters written by kernels.
 struct Ctl {
   int rebuild;          // neighbour list must be rebuilt before the next pass

@@ -44,6 +44,7 @@ struct dem_system {
   double dt = 0.0, skin = 0.0;
   double ab0 = 0, am0 = 0, ab[3] = {0, 0, 0}, am[3] = {0, 0, 0};
   std::vector<WallGeom> walls_h;
+  std::vector<dem_wall_t> walls_in;  // as given to dem_set_walls
   std::vector<WallConst> wallc_h;
   std::vector<double> radius_h;
   // device constant tables
@@ -51,6 +52,10 @@ struct dem_system {
   CompConst* d_comp = nullptr;
   WallConst* d_wallc = nullptr;
   WallGeom* d_walls = nullptr;
+  WallMotion* d_wm = nullptr;  // WM(1:nw) when WallsInMovement
+  int nw_move = 0;             // nw when WallsInMovement, else 0
+  double time = 0.0;           // Time (src/MConfig.f90:61)
+  int64_t next_step = 0;       // step number of the next batched step
   // state: buffer cur is Particle, cur^1 Predicted (or Euler scratch)
   int np = 0;
   bool has_state = false;
@@ -194,6 +199,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   h->nprev = s.nprev = p->method_order - 1;
   h->n_euler = p->n_euler;
   h->dt = p->time_step;
+  h->next_step = p->method_order;  // InitialStep after the start-up (src/DEM.f90:87-92)
   s.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 48;
   s.max_c = 0;
   double rmax = 0.0, rmin = 1e300;
@@ -312,6 +318,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   if (e == hipSuccess) e = dalloc(&h->d_comp, comp.size());
   if (e == hipSuccess) e = dalloc(&h->d_wallc, h->wallc_h.size());
   if (e == hipSuccess) e = dalloc(&h->d_walls, h->walls_h.size());
+  if (e == hipSuccess) e = dalloc(&h->d_wm, h->walls_h.size());
   if (e == hipSuccess) e = dalloc(&h->ctl, 1);
   if (e == hipSuccess) e = dalloc(&h->rb, 2);
   if (e == hipSuccess)
@@ -341,7 +348,7 @@ void dem_destroy(dem_handle_t h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_state(h);
-  dfree(h->d_pair); dfree(h->d_comp); dfree(h->d_wallc); dfree(h->d_walls);
+  dfree(h->d_pair); dfree(h->d_comp); dfree(h->d_wallc); dfree(h->d_walls); dfree(h->d_wm);
   dfree(h->ctl); dfree(h->rb);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -352,6 +359,8 @@ int dem_set_option(dem_handle_t h, const char* name, double value) {
   if (!strcmp(name, "timing")) h->timing = value != 0.0;
   else if (!strcmp(name, "resort_interval")) h->resort_interval = (int)value;
   else if (!strcmp(name, "keep_predicted_acc")) h->keep_pred_acc = value != 0.0;
+  else if (!strcmp(name, "time")) h->time = value;
+  else if (!strcmp(name, "next_step")) h->next_step = (int64_t)value;
   else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
   return DEM_OK;
 }
@@ -360,7 +369,10 @@ int dem_set_option(dem_handle_t h, const char* name, double value) {
 int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* w) {
   if (!h) return DEM_ERR_ARG;
   if (nw != h->sc.nw) return fail(h, DEM_ERR_ARG, "nw=%d but the system has %d walls", nw, h->sc.nw);
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   const int nc = h->sc.nc;
+  h->walls_in.assign(w, w + nw);
   for (int j = 0; j < nw; ++j) {
     if (w[j].shape != DEM_SHAPE_RECTANGLE && w[j].shape != DEM_SHAPE_CONE)
       return fail(h, DEM_ERR_ARG, "wall %d has invalid shape %d", j + 1, w[j].shape);
@@ -398,6 +410,57 @@ int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* w) {
                       hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->d_wallc, h->wallc_h.data(), h->wallc_h.size() * sizeof(WallConst),
                       hipMemcpyHostToDevice));
+  return DEM_OK;
+}
+
+// WallsInMovement and WM(1:NW) (src/DEM.f90:716-732, scaled as :841-846).
+int dem_set_wall_motion(dem_handle_t h, int32_t nw, const dem_wall_motion_t* m) {
+  if (!h) return DEM_ERR_ARG;
+  if (nw != 0 && nw != h->sc.nw)
+    return fail(h, DEM_ERR_ARG, "nw=%d but the system has %d walls", nw, h->sc.nw);
+  if (nw > 0 && !m) return fail(h, DEM_ERR_ARG, "null wall motion");
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (nw == 0) {
+    h->nw_move = 0;
+    return DEM_OK;
+  }
+  std::vector<WallMotion> wm((size_t)nw);
+  for (int j = 0; j < nw; ++j) {
+    for (int d = 0; d < 3; ++d) {
+      // the specification refuses negative frequencies (src/DEM.f90:723-728)
+      if (!(m[j].two_pi_frequency[d] >= 0.0))
+        return fail(h, DEM_ERR_ARG, "wall %d: two_pi_frequency must be >= 0", j + 1);
+      wm[j].cf[d] = m[j].constant_factor[d];
+      wm[j].amp[d] = m[j].amplitude[d];
+      wm[j].vamp[d] = m[j].velocity_amplitude[d];
+      wm[j].tpf[d] = m[j].two_pi_frequency[d];
+      wm[j].phase[d] = m[j].phase_angle[d];
+    }
+  }
+  HIPCHK(h, hipMemcpy(h->d_wm, wm.data(), wm.size() * sizeof(WallMotion), hipMemcpyHostToDevice));
+  h->nw_move = nw;
+  return DEM_OK;
+}
+
+int dem_download_walls(dem_handle_t h, int32_t nw, dem_wall_t* w) {
+  if (!h) return DEM_ERR_ARG;
+  if (nw != h->sc.nw) return fail(h, DEM_ERR_ARG, "nw=%d but the system has %d walls", nw, h->sc.nw);
+  if (nw == 0) return DEM_OK;
+  if (!w) return fail(h, DEM_ERR_ARG, "null output");
+  std::vector<WallGeom> g((size_t)nw);
+  hipStream_t st = h->in_batch ? nullptr : h->stream;
+  if (st) HIPCHK(h, hipStreamSynchronize(st));
+  else HIPCHK(h, hipDeviceSynchronize());
+  HIPCHK(h, hipMemcpy(g.data(), h->d_walls, g.size() * sizeof(WallGeom), hipMemcpyDeviceToHost));
+  if (h->walls_in.size() != (size_t)nw) return fail(h, DEM_ERR_STATE, "walls were never set");
+  for (int j = 0; j < nw; ++j) {
+    w[j] = h->walls_in[j];
+    for (int d = 0; d < 3; ++d) {
+      w[j].position[d] = g[j].pos[d];
+      w[j].velocity[d] = g[j].vel[d];
+    }
+  }
   return DEM_OK;
 }
 
@@ -1098,6 +1161,13 @@ int dem_compute_accelerations(dem_handle_t h, int32_t which, int32_t update) {
   return check_errors(h);
 }
 
+// MoveWalls(time) on the handle's stream when WallsInMovement.
+static void enqueue_move_walls(dem_handle_t h, double time) {
+  if (h->nw_move > 0)
+    hipLaunchKernelGGL(k_move_walls, dim3(1), dim3(64), 0, h->stream, h->d_walls,
+                       (const WallMotion*)h->d_wm, h->nw_move, time, (const Ctl*)h->ctl);
+}
+
 int dem_euler_startup(dem_handle_t h) {
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
   if (h->nprev == 0) return fail(h, DEM_ERR_STATE, "Euler start-up needs MethodOrder > 1");
@@ -1128,6 +1198,9 @@ int dem_euler_startup(dem_handle_t h) {
     a.hsq2 = 0.5 * (hs * hs);
     a.half_dt = 0.5 * hs;
     launch_pass<MODE_EULER, true>(h, a);
+    // Time = Time + NewTimeStep; MoveWalls(Time)  (src/MDEM.f90:275-276)
+    h->time = h->time + hs;
+    enqueue_move_walls(h, h->time);
     after_eval(h, true, h->cur);
     h->cur ^= 1;
     enqueue_rebuild(h, h->cur, h->rb + (h->eval & 1), 0);
@@ -1206,7 +1279,6 @@ extern "C" {
 
 int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_done,
              int32_t* escaped) {
-  (void)first_step;
   if (steps_done) *steps_done = 0;
   if (escaped) *escaped = 0;
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
@@ -1244,6 +1316,8 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     launch_pass<MODE_PRED, true>(h, a);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
+    // Time = step*TimeStep; MoveWalls(Time)  (src/MDEM.f90:333-337)
+    enqueue_move_walls(h, (double)(first_step + s) * h->dt);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     launch_pass<MODE_CORR, false>(h, b);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
@@ -1281,6 +1355,8 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     rollback_steps(h, done, enqueued);
   }
   h->steps += done;
+  if (done > 0) h->time = (double)(first_step + done - 1) * h->dt;
+  h->next_step = first_step + done;
   if (steps_done) *steps_done = done;
   if (escaped) *escaped = c.escaped;
   rc = check_errors(h);
@@ -1305,6 +1381,8 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
     out->ref_ncell[d] = h->sc.rn[d];
     out->nb_ncell[d] = h->sc.gn[d];
   }
+  out->time = h->time;
+  out->next_step = h->next_step;
   if (!h->has_state) return DEM_OK;
   Ctl c;
   int rc = read_ctl(h, &c);
@@ -1446,6 +1524,11 @@ static void batch_rec_init(dem_handle_t h, BatchRec& r) {
   r.stamp_base = h->stamp_base;
   r.ring = h->ring;
   r.rb = h->rb;
+  r.walls = h->d_walls;
+  r.wm = h->d_wm;
+  r.nw_move = h->nw_move;
+  r.next_step = h->next_step;
+  r.dt = h->dt;
 }
 
 // Host-side bookkeeping of n batched steps (what step_args does per step).
@@ -1635,6 +1718,8 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   BHIPCHK(b, hipEventRecord(b->ev[0], st));
   b->ev_used[0] = true;
   BatchRec* rec = b->d_rec;
+  bool moving = false;
+  for (size_t i = 0; i < n; ++i) moving = moving || b->sys[i]->nw_move > 0;
   hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
                      ns);
   for (int64_t s = 0; s < nsteps; ++s) {
@@ -1648,6 +1733,7 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
       hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, false, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
                          st, rec, (const int2*)b->d_map[MAP_PART]);
     hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 0, 1);
+    if (moving) hipLaunchKernelGGL(k_move_walls_batch, dim3(ns), dim3(64), 0, st, rec);
     if (mixed)
       hipLaunchKernelGGL((k_pass_batch<MODE_CORR, false, true, 1>), dim3(gp), dim3(DEM_BLOCK), 0,
                          st, rec, (const int2*)b->d_map[MAP_PART]);
@@ -1705,6 +1791,8 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
       }
     }
     h->steps += done;
+    if (done > 0) h->time = (double)(h->next_step + done - 1) * h->dt;
+    h->next_step += done;
     if (steps_done) steps_done[i] = done;
     if (escaped) escaped[i] = c.escaped;
     if (c.error) {

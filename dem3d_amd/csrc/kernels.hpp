@@ -561,6 +561,29 @@ __global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* 
   pass_reduce_body(ctl, part, nblk, par, update);
 }
 
+// MoveWalls(Time) (src/MDEM.f90:392-408): every axis of every wall, one
+// thread per wall.  Phase is reduced by 2 pi with truncation toward zero as
+// int(Phase/TwoPi) does.  Does nothing once a step had escapes (the passes
+// after it do not run either).
+__device__ __forceinline__ void move_walls_body(WallGeom* walls, const WallMotion* wm, int nw,
+                                                double time, const Ctl* ctl) {
+  if (ctl->escaped) return;
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  for (int j = threadIdx.x; j < nw; j += blockDim.x) {
+    const WallMotion m = wm[j];
+    for (int d = 0; d < 3; ++d) {
+      double ph = m.tpf[d] * time + m.phase[d];
+      ph = ph - two_pi * trunc(ph / two_pi);
+      walls[j].pos[d] = m.cf[d] + m.amp[d] * cos(ph);
+      walls[j].vel[d] = m.vamp[d] * sin(ph);
+    }
+  }
+}
+__global__ void k_move_walls(WallGeom* walls, const WallMotion* wm, int nw, double time,
+                             const Ctl* ctl) {
+  move_walls_body(walls, wm, nw, time, ctl);
+}
+
 // ---------------------------------------------------------------------------
 // Cell state of positions set from outside (upload, the ComputeAccelerations
 // drop-in, the PBC wrap after the Euler start-up): CurrentCell3D plus the

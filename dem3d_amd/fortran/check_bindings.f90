@@ -10,6 +10,7 @@ program check_bindings
   type(dem_wall_t) :: w
   type(dem_particle_t) :: q
   type(dem_stats_t) :: s
+  type(dem_wall_motion_t) :: m
   type(c_ptr) :: h
   real(c_double), target :: radius(1), tab(1)
   integer(c_int) :: rc
@@ -18,6 +19,7 @@ program check_bindings
   write(*, '(a, i0)') 'sizeof dem_wall_t ', c_sizeof(w)
   write(*, '(a, i0)') 'sizeof dem_particle_t ', c_sizeof(q)
   write(*, '(a, i0)') 'sizeof dem_stats_t ', c_sizeof(s)
+  write(*, '(a, i0)') 'sizeof dem_wall_motion_t ', c_sizeof(m)
 
   radius = 0.5d0
   tab = 1.0d0

@@ -50,6 +50,12 @@ module dem3d_gpu
     real(c_double) :: position(3) = 0, velocity(3) = 0, omega(3) = 0
   end type dem_wall_t
 
+  ! TWallMovement (src/MDEM.f90:39-45), reduced units.
+  type, bind(c), public :: dem_wall_motion_t
+    real(c_double) :: constant_factor(3) = 0, amplitude(3) = 0, velocity_amplitude(3) = 0
+    real(c_double) :: two_pi_frequency(3) = 0, phase_angle(3) = 0
+  end type dem_wall_motion_t
+
   ! Byte image of TParticle (src/MDimension3D.f90:28-35): 128 bytes.
   type, bind(c), public :: dem_particle_t
     integer(c_int32_t) :: index = 0, pad_ = 0
@@ -63,10 +69,12 @@ module dem3d_gpu
     integer(c_int64_t) :: ref_ncell(3), nb_ncell(3)
     real(c_double) :: kinetic_energy, max_overlap
     real(c_double) :: ms_pass1, ms_pass2, ms_rebuild, ms_step
-    real(c_double) :: mean_pair_contacts
+    real(c_double) :: mean_pair_contacts, time
+    integer(c_int64_t) :: next_step
   end type dem_stats_t
 
   public :: dem_create, dem_destroy, dem_last_error, dem_set_walls
+  public :: dem_set_wall_motion, dem_download_walls
   public :: dem_upload_state, dem_download_state, dem_download_predicted
   public :: dem_upload_contacts, dem_download_contacts, dem_download_cells
   public :: dem_compute_accelerations_aos, dem_compute_accelerations
@@ -92,6 +100,20 @@ module dem3d_gpu
       type(c_ptr), value :: h
       integer(c_int32_t), value :: nw
       type(dem_wall_t), intent(in) :: walls(*)
+    end function
+    ! WM(1:nw) of MDEM; nw = 0 stops the walls.
+    integer(c_int) function dem_set_wall_motion(h, nw, motion) &
+        bind(c, name="dem_set_wall_motion")
+      import :: c_int, c_int32_t, c_ptr, dem_wall_motion_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nw
+      type(dem_wall_motion_t), intent(in) :: motion(*)
+    end function
+    integer(c_int) function dem_download_walls(h, nw, walls) bind(c, name="dem_download_walls")
+      import :: c_int, c_int32_t, c_ptr, dem_wall_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nw
+      type(dem_wall_t), intent(inout) :: walls(*)
     end function
     ! SoA arrays are Fortran X(np,3); optional arrays are passed as c_null_ptr.
     integer(c_int) function dem_upload_state(h, np, comp, x, v, w, a, alpha, ring, ipc, &

@@ -54,6 +54,12 @@ class WallT(C.Structure):
     ]
 
 
+class WallMotionT(C.Structure):
+    _fields_ = [("constant_factor", C.c_double * 3), ("amplitude", C.c_double * 3),
+                ("velocity_amplitude", C.c_double * 3), ("two_pi_frequency", C.c_double * 3),
+                ("phase_angle", C.c_double * 3)]
+
+
 class ParticleT(C.Structure):
     _fields_ = [("index", C.c_int32), ("pad_", C.c_int32), ("position", C.c_double * 3),
                 ("velocity", C.c_double * 3), ("acceleration", C.c_double * 3),
@@ -71,7 +77,7 @@ class Stats(C.Structure):
         ("stale_dropped", C.c_int64), ("ref_ncell", C.c_int64 * 3), ("nb_ncell", C.c_int64 * 3),
         ("kinetic_energy", C.c_double), ("max_overlap", C.c_double), ("ms_pass1", C.c_double),
         ("ms_pass2", C.c_double), ("ms_rebuild", C.c_double), ("ms_step", C.c_double),
-        ("mean_pair_contacts", C.c_double),
+        ("mean_pair_contacts", C.c_double), ("time", C.c_double), ("next_step", C.c_int64),
     ]
 
     def asdict(self):
@@ -122,6 +128,8 @@ class Library:
     }
     OPTIONAL = {
         "set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+        "set_wall_motion": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallMotionT)]),
+        "download_walls": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallT)]),
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
@@ -234,6 +242,18 @@ def make_walls(red: Reduced):
     return arr
 
 
+def make_wall_motion(motion):
+    """dem_wall_motion_t[nw] from a list of dicts with the TWallMovement
+    fields (constant_factor, amplitude, velocity_amplitude, two_pi_frequency,
+    phase_angle), reduced units."""
+    arr = (WallMotionT * max(len(motion), 1))()
+    for j, m in enumerate(motion):
+        for name, _ in WallMotionT._fields_:
+            for d in range(3):
+                getattr(arr[j], name)[d] = float(m[name][d])
+    return arr
+
+
 class System:
     """One granular system resident in a library (GPU product or a checker)."""
 
@@ -248,9 +268,29 @@ class System:
         self.h = h
         self.np = 0
         self.nprev = red.method_order - 1
+        self.next_step = red.method_order  # InitialStep after the start-up (src/DEM.f90:87-92)
         if red.nw:
             self.walls = make_walls(red)
             self._check(lib.set_walls(self.h, red.nw, self.walls))
+            if getattr(red, "wall_motion", None):
+                self.set_wall_motion(red.wall_motion)
+
+    def set_wall_motion(self, motion):
+        """WallsInMovement with WM = motion (list of dicts, one per wall);
+        an empty list stops the walls."""
+        self._wm = make_wall_motion(motion)
+        self._check(self.lib.set_wall_motion(self.h, len(motion), self._wm))
+
+    def download_walls(self):
+        nw = self.red.nw
+        arr = (WallT * max(nw, 1))()
+        self._check(self.lib.download_walls(self.h, nw, arr))
+        pos = np.array([[arr[j].position[d] for d in range(3)] for j in range(nw)]).reshape(nw, 3)
+        vel = np.array([[arr[j].velocity[d] for d in range(3)] for j in range(nw)]).reshape(nw, 3)
+        return dict(position=pos, velocity=vel)
+
+    def set_time(self, t):
+        self._check(self.lib.set_option(self.h, b"time", float(t)))
 
     def _check(self, rc):
         if rc != DEM_OK:
@@ -361,8 +401,10 @@ class System:
     def step(self, nsteps, first_step=None, allow_escape=False):
         done = C.c_int64()
         esc = C.c_int32()
-        first = self.red.method_order if first_step is None else first_step
+        first = self.next_step if first_step is None else first_step
         rc = self.lib.step(self.h, int(nsteps), int(first), C.byref(done), C.byref(esc))
+        if rc in (DEM_OK, DEM_ERR_ESCAPE):
+            self.next_step = int(first) + done.value
         if rc == DEM_ERR_ESCAPE and allow_escape:
             return done.value, esc.value
         self._check(rc)
@@ -415,6 +457,9 @@ class Batch:
         esc = np.zeros(max(n, 1), np.int32)
         rc = self.lib.batch_step(self.b, int(nsteps), done.ctypes.data_as(_lp),
                                  esc.ctypes.data_as(_ip))
+        if rc in (DEM_OK, DEM_ERR_ESCAPE):
+            for sysm, d in zip(self.systems, done[:n]):
+                sysm.next_step += int(d)
         if not (rc == DEM_ERR_ESCAPE and allow_escape):
             self._check(rc)
         return done[:n], esc[:n]

@@ -57,6 +57,11 @@ class Wall:
     w: Sequence[float]
     velocity: Sequence[float] = (0.0, 0.0, 0.0)
     omega: Sequence[float] = (0.0, 0.0, 0.0)
+    # movement line (src/DEM.f90:722): amplitude (m), frequency (Hz) and phase
+    # (fraction of a period) per axis; used when the system's walls move
+    amplitude: Sequence[float] = (0.0, 0.0, 0.0)
+    frequency: Sequence[float] = (0.0, 0.0, 0.0)
+    phase: Sequence[float] = (0.0, 0.0, 0.0)
 
 
 @dataclass
@@ -79,6 +84,7 @@ class SystemSpec:
     method_order: int = 4
     n_euler: int = 10
     fluid_density: float = 0.0
+    walls_in_movement: bool = False  # WallsInMovement (src/DEM.f90:716)
 
 
 @dataclass
@@ -121,6 +127,7 @@ class Reduced:
     time_scale: float
     max_ncell: int = 30000
     ref_grid: Optional[Sequence[int]] = None
+    wall_motion: Optional[list] = None  # WM(1:NW), reduced (list of dicts) or None
 
 
 def _gram_schmidt(u, w):
@@ -227,6 +234,25 @@ def reduce_spec(s: SystemSpec, max_ncell: int = 30000) -> Reduced:
     ytw_r = ytw * T / M
     gravity = np.array([g * t2 / L for g in s.gravity])
     dt_r = dt / T
+    wall_motion = None
+    if s.walls_in_movement and nw:
+        wall_motion = []
+        for wl in s.walls:
+            # src/DEM.f90:722-732 in SI, then ScaleVariables (src/DEM.f90:841-846)
+            for f in wl.frequency:
+                if f < 0.0:
+                    raise ValueError("wall movement frequencies cannot be negative")
+            tpf = [TWO_PI * f for f in wl.frequency]
+            phase = [TWO_PI * p for p in wl.phase]
+            vamp = [-t * a for t, a in zip(tpf, wl.amplitude)]
+            cf = [p - a * math.cos(ph) for p, a, ph in zip(wl.position, wl.amplitude, phase)]
+            wall_motion.append(dict(
+                constant_factor=np.array([c / L for c in cf]),
+                amplitude=np.array([a / L for a in wl.amplitude]),
+                velocity_amplitude=np.array([v * T / L for v in vamp]),
+                two_pi_frequency=np.array([t * T for t in tpf]),
+                phase_angle=np.array(phase),
+            ))
     walls = []
     for wl in s.walls:
         u, w = _gram_schmidt(wl.u, wl.w)
@@ -248,6 +274,7 @@ def reduce_spec(s: SystemSpec, max_ncell: int = 30000) -> Reduced:
         mu=np.asarray(s.mu, float).reshape(nc, nc), musq=None, knw=knw_r, ynw=ynw_r,
         ktw=ktw_r, ytw=ytw_r, muw=np.asarray(s.muw, float).reshape(nc, nw), muwsq=None,
         walls=walls, length_scale=L, mass_scale=M, time_scale=T, max_ncell=max_ncell,
+        wall_motion=wall_motion,
     )
     initialize_force(r, s.fluid_density * ((L * L) * L) / M)
     return r

@@ -20,7 +20,7 @@ from typing import Optional
 
 import numpy as np
 
-from .spec import RECTANGLE, Component, Reduced, SystemSpec, Wall, reduce_spec
+from .spec import CONE, RECTANGLE, Component, Reduced, SystemSpec, Wall, reduce_spec
 
 
 @dataclass
@@ -213,6 +213,54 @@ def small_box(n_side=(6, 6, 6), seed=1, jitter=0.05, spacing=1.02, nc=2, ratio=1
     v = rng.uniform(-vel, vel, pos.shape) * T / L
     w = rng.uniform(-vel, vel, pos.shape) * T / L / 0.5
     return Workload(f"small-{n}", spec, red, comp, pos / L, v, w, steps)
+
+
+def ball_bearings(seed: int = 20240006, steps: int = 2000, n: int = 39,
+                  shake: bool = True) -> Workload:
+    """The reference's example input (examples/ball-bearings/ball-bearings.inp):
+    39 steel balls (10 mm, 7800 kg/m^3, stiffness 1e5 N/m) in a dish made of
+    five Cone walls -- an inner cylinder of radius 35 mm and its bottom disc,
+    an outer cylinder of radius 36 mm and its disc, and the annulus that joins
+    their rims -- shaken on a 10 mm circle at 1.5 Hz (amplitude (10, 10, 0)
+    mm, phases (0, -1/4, 0) of a period), e = 0.7, mu = 0.3, 10 steps per
+    collision, 100 Euler intervals, closed box 0.11 x 0.11 x 0.05 m.  The
+    example pours the balls (ball-bearings.prg); here they start at rest on two
+    square layers (10.5 mm pitch, jittered by up to 0.1 mm) inside the inner
+    cylinder."""
+    rng = np.random.default_rng(seed)
+    k = 1.0e5
+    up, side = (0.0, 0.0, 1.0), (0.0, 1.0, 0.0)
+    amp, freq, phase = (0.01, 0.01, 0.0), (1.5, 1.5, 1.5), (0.0, -0.25, 0.0)
+    mv = dict(amplitude=amp, frequency=freq, phase=phase) if shake else {}
+    geo = [(0.015, 0.070, 0.070, -0.0165), (0.000, 0.000, 0.070, -0.0240),
+           (0.016, 0.072, 0.072, -0.0170), (0.000, 0.000, 0.072, -0.0250),
+           (0.000, 0.070, 0.072, -0.0090)]
+    walls = [Wall(CONE, l1, l2, l3, k, k, (0.01, 0.0, z), up, side, **mv) for l1, l2, l3, z in geo]
+    nw = len(walls)
+    spec = SystemSpec(
+        lbox=[0.11, 0.11, 0.05], components=[Component(0.01, 7.8e3, k, k)], walls=walls,
+        en=np.full((1, 1), 0.7), et=np.full((1, 1), 0.7), mu=np.full((1, 1), 0.3),
+        enw=np.full((1, nw), 0.7), etw=np.full((1, nw), 0.7), muw=np.full((1, nw), 0.3),
+        pbc=[False, False, False], gravity=[0.0, 0.0, -9.8], total_time=30.0,
+        min_steps_per_collision=10.0, method_order=4, n_euler=100, walls_in_movement=shake)
+    red = reduce_spec(spec)
+    L = red.length_scale
+    a, rmax = 0.0105, 0.035 - 0.005 - 0.0005
+    cx, cy, z0 = 0.01 + 3.7e-4, 2.1e-4, -0.024 + 0.005 + 2e-4
+    pts = []
+    for layer in range(3):
+        off = 0.5 * a * (layer % 2)
+        for i in range(-4, 5):
+            for j in range(-4, 5):
+                px, py = i * a + off, j * a + off
+                if math.hypot(px, py) <= rmax:
+                    pts.append((cx + px, cy + py, z0 + layer * a))
+    if len(pts) < n:
+        raise ValueError(f"at most {len(pts)} balls fit the start layers")
+    pos = np.array(pts[:n]).T + rng.uniform(-1e-4, 1e-4, (3, n))
+    zero = np.zeros((3, n))
+    return Workload(f"ball-bearings-{n}", spec, red, np.ones(n, np.int32), pos / L, zero.copy(),
+                    zero.copy(), steps, note="example input, balls placed instead of poured")
 
 
 def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,

@@ -126,6 +126,20 @@ typedef struct {
   double position[3], velocity[3], omega[3];
 } dem_wall_t;
 
+/* TWallMovement (src/MDEM.f90:39-45), one per wall, in reduced units (as
+ * ReadSpecifications and ScaleVariables leave it, src/DEM.f90:719-732 and
+ * :841-846).  MoveWalls(Time) (src/MDEM.f90:392-408) sets, per axis,
+ *   Phase    = two_pi_frequency*Time + phase_angle, reduced by 2 pi
+ *   Position = constant_factor + amplitude*cos(Phase)
+ *   Velocity = velocity_amplitude*sin(Phase). */
+typedef struct {
+  double constant_factor[3];
+  double amplitude[3];
+  double velocity_amplitude[3];
+  double two_pi_frequency[3];
+  double phase_angle[3];
+} dem_wall_motion_t;
+
 /* TParticle (src/MDimension3D.f90:28-35) byte layout: 128 B records. */
 typedef struct {
   int32_t index;            /* component, 1-based */
@@ -155,6 +169,13 @@ typedef struct {
   double ms_step;           /* mean device time per step                       */
   double mean_pair_contacts;/* P averaged over the first passes of the last
                                dem_step call                                   */
+  double time;              /* Time (src/MConfig.f90:61) of the last MoveWalls
+                               point reached: step*TimeStep after a step,
+                               advanced by TimeStep/NumberOfEulerIntervals per
+                               Euler sub-step                                  */
+  int64_t next_step;        /* step number a batch gives the system's next
+                               MoveParticles (first_step + steps of the last
+                               dem_step; MethodOrder initially)                */
 } dem_stats_t;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -166,6 +187,15 @@ const char* dem_last_error(dem_handle_t h);
 /* ---- state in / out ------------------------------------------------------ */
 /* Walls; replaces Wall(:) (src/MConfig.f90:70) and runs ConfigureWalls. */
 int dem_set_walls(dem_handle_t h, int32_t nw, const dem_wall_t* walls);
+/* WallsInMovement = .true. with WM(1:nw) = motion (src/DEM.f90:716-732);
+ * nw = 0 (motion ignored) sets WallsInMovement = .false.  The walls then move
+ * where the reference calls MoveWalls: before the corrector pass of every
+ * step, at Time = step*TimeStep (src/MDEM.f90:333-337), and after every Euler
+ * sub-step, at the accumulated Time (src/MDEM.f90:273-277).  Until the first
+ * move they keep the positions and velocities of dem_set_walls. */
+int dem_set_wall_motion(dem_handle_t h, int32_t nw, const dem_wall_motion_t* motion);
+/* Wall(1:nw) as they stand now (positions and velocities moved). */
+int dem_download_walls(dem_handle_t h, int32_t nw, dem_wall_t* walls);
 
 /* Particle state.  a/alpha/ring/ipc/cell_rank may be NULL (zeros; fresh ring;
  * fresh cell lists as DistributeParticles3D builds them).  cell_rank gives the
@@ -223,7 +253,10 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *                       cell order (0 = only at upload); default 500
  *   "keep_predicted_acc" 1 = keep Predicted%Acceleration/Alpha for
  *                       dem_download_predicted (one extra 48 B write per
- *                       particle and step); default 0 */
+ *                       particle and step); default 0
+ *   "time"              sets Time (src/MConfig.f90:61), e.g. from a saved
+ *                       configuration before an Euler start-up
+ *   "next_step"         sets the step number of the next batched step */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
 /* ---- batches ------------------------------------------------------------
@@ -245,7 +278,9 @@ int dem_batch_remove(dem_batch_t b, dem_handle_t h);
 int dem_batch_size(dem_batch_t b);
 /* nsteps MoveParticles steps of every system; steps_done[i] / escaped[i]
  * (may be NULL) per system in the order of addition.  A system in which
- * particles escaped stops after that step (DEM_ERR_ESCAPE is returned). */
+ * particles escaped stops after that step (DEM_ERR_ESCAPE is returned).
+ * The steps of a system are numbered from its next_step (dem_stats), which
+ * sets Time for its moving walls. */
 int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
                    int32_t* escaped);
 /* "timing": 1 = record the device time of every step of a dem_batch_step

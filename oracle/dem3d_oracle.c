@@ -68,6 +68,9 @@ struct orc_system {
   double ab0, am0, ab[3], am[3];
   dem_wall_t wall[MAXW];
   int wall_order[MAXW];
+  int walls_in_movement;           /* WallsInMovement (src/MDEM.f90:36) */
+  dem_wall_motion_t wm[MAXW];      /* WM(1:NW) */
+  int64_t next_step;
   cone_t cone[MAXW];
   cone_comp_t cone_comp[MAXW][MAXC];
   /* reference cell structure */
@@ -688,6 +691,21 @@ static void store_ring(orc_handle_t h) {
   }
 }
 
+/* MoveWalls (src/MDEM.f90:392-408).  Phase - TwoPi*real(int(Phase/TwoPi))
+ * reduces the phase with truncation toward zero. */
+static void move_walls(orc_handle_t h, double time) {
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  for (int j = 0; j < h->nw; j++) {
+    const dem_wall_motion_t* m = &h->wm[j];
+    for (int d = 0; d < 3; d++) {
+      double phase = m->two_pi_frequency[d] * time + m->phase_angle[d];
+      phase = phase - two_pi * (double)(int32_t)(phase / two_pi);
+      h->wall[j].position[d] = m->constant_factor[d] + m->amplitude[d] * cos(phase);
+      h->wall[j].velocity[d] = m->velocity_amplitude[d] * sin(phase);
+    }
+  }
+}
+
 /* MoveParticles (src/MDEM.f90:285-369) */
 static int move_particles(orc_handle_t h, int64_t step) {
   int rc = compute_accelerations(h, h->part, h->dt, 1);
@@ -714,6 +732,7 @@ static int move_particles(orc_handle_t h, int64_t step) {
     store_ring(h);
   }
   h->time = (double)step * h->dt;
+  if (h->walls_in_movement) move_walls(h, h->time);
   rc = compute_accelerations(h, h->pred, h->dt, 0);
   if (rc) return rc;
   for (int i = 0; i < h->np; i++) {
@@ -754,6 +773,7 @@ static int move_particles_euler(orc_handle_t h) {
       for (int d = 0; d < 3; d++) q->omega[d] = q->omega[d] + q->alpha[d] * hstep;
     }
     h->time = h->time + hstep;
+    if (h->walls_in_movement) move_walls(h, h->time);
   }
   apply_pbc_particles(h, h->part);
   return DEM_OK;
@@ -801,6 +821,7 @@ int orc_create(const dem_params_t* p, orc_handle_t* out) {
   h->order = p->method_order;
   h->nprev = p->method_order - 1;
   h->n_euler = p->n_euler;
+  h->next_step = p->method_order;
   const int nc = p->nc, nw = p->nw;
   for (int c = 0; c < nc; c++) {
     h->radius[c] = p->radius[c];
@@ -891,6 +912,37 @@ int orc_set_walls(orc_handle_t h, int32_t nw, const dem_wall_t* walls) {
     h->wall[j] = walls[j];
   }
   configure_walls(h);
+  return DEM_OK;
+}
+
+/* WallsInMovement and WM(1:NW) (src/DEM.f90:716-732) */
+int orc_set_wall_motion(orc_handle_t h, int32_t nw, const dem_wall_motion_t* m) {
+  if (nw != 0 && nw != h->nw)
+    return fail(h, DEM_ERR_ARG, "nw=%d but the system was created with %d", nw, h->nw);
+  if (nw == 0) {
+    h->walls_in_movement = 0;
+    return DEM_OK;
+  }
+  for (int j = 0; j < nw; j++) {
+    for (int d = 0; d < 3; d++)
+      if (!(m[j].two_pi_frequency[d] >= 0.0))
+        return fail(h, DEM_ERR_ARG, "wall %d: two_pi_frequency must be >= 0", j + 1);
+    h->wm[j] = m[j];
+  }
+  h->walls_in_movement = 1;
+  return DEM_OK;
+}
+
+int orc_download_walls(orc_handle_t h, int32_t nw, dem_wall_t* w) {
+  if (nw != h->nw) return fail(h, DEM_ERR_ARG, "nw mismatch");
+  for (int j = 0; j < nw; j++) w[j] = h->wall[j];
+  return DEM_OK;
+}
+
+int orc_set_option(orc_handle_t h, const char* name, double value) {
+  if (!strcmp(name, "time")) h->time = value;
+  else if (!strcmp(name, "next_step")) h->next_step = (int64_t)value;
+  else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
   return DEM_OK;
 }
 
@@ -1099,6 +1151,7 @@ int orc_step(orc_handle_t h, int64_t nsteps, int64_t first_step,
     esc = count_escaped(h);
     if (esc) break;
   }
+  h->next_step = first_step + done;
   if (steps_done) *steps_done = done;
   if (escaped) *escaped = esc;
   return esc ? fail(h, DEM_ERR_ESCAPE, "%d particles left the box", esc) : DEM_OK;
@@ -1124,6 +1177,8 @@ int orc_stats(orc_handle_t h, dem_stats_t* s) {
   }
   s->kinetic_energy = 0.5 * ek;
   s->max_overlap = h->max_overlap;
+  s->time = h->time;
+  s->next_step = h->next_step;
   return DEM_OK;
 }
 

@@ -28,6 +28,9 @@ int orc_create(const dem_params_t* params, orc_handle_t* out);
 void orc_destroy(orc_handle_t h);
 const char* orc_last_error(orc_handle_t h);
 int orc_set_walls(orc_handle_t h, int32_t nw, const dem_wall_t* walls);
+int orc_set_wall_motion(orc_handle_t h, int32_t nw, const dem_wall_motion_t* motion);
+int orc_download_walls(orc_handle_t h, int32_t nw, dem_wall_t* walls);
+int orc_set_option(orc_handle_t h, const char* name, double value);
 int orc_upload_state(orc_handle_t h, int32_t np, const int32_t* comp,
                      const double* x, const double* v, const double* w,
                      const double* a, const double* alpha, const double* ring,

@@ -118,6 +118,40 @@ contains
     call ConfigureWalls
   end subroutine ref_configure_walls
 
+  ! WallsInMovement = .true. and WM(1:NW) in reduced units (the values
+  ! ReadSpecifications and ScaleVariables leave, src/DEM.f90:719-732,841-846).
+  subroutine ref_set_wall_motion( nw_, cf, amp, vamp, tpf, phase ) &
+      bind(C, name="ref_set_wall_motion")
+    integer(c_int32_t), value :: nw_
+    real(c_double), intent(in) :: cf(3,nw_), amp(3,nw_), vamp(3,nw_), tpf(3,nw_), phase(3,nw_)
+    integer :: j
+    if (allocated(WM)) deallocate(WM)
+    allocate( WM(nw_) )
+    do j = 1, nw_
+      WM(j)%ConstantFactor = cf(:,j)
+      WM(j)%Amplitude = amp(:,j)
+      WM(j)%VelocityAmplitude = vamp(:,j)
+      WM(j)%TwoPiFrequency = tpf(:,j)
+      WM(j)%PhaseAngle = phase(:,j)
+    end do
+    WallsInMovement = .true.
+  end subroutine ref_set_wall_motion
+
+  subroutine ref_get_walls( nw_, pos, vel ) bind(C, name="ref_get_walls")
+    integer(c_int32_t), value :: nw_
+    real(c_double), intent(out) :: pos(3,nw_), vel(3,nw_)
+    integer :: j
+    do j = 1, nw_
+      pos(:,j) = Wall(j)%Position
+      vel(:,j) = Wall(j)%Velocity
+    end do
+  end subroutine ref_get_walls
+
+  function ref_get_time() result( t ) bind(C, name="ref_get_time")
+    real(c_double) :: t
+    t = Time
+  end function ref_get_time
+
   ! Tables derived by InitializeForce (src/MForces.f90:54-87).
   subroutine ref_get_tables( mindij_, mindijsq_, invmass_, invinertia_, effmass_, &
                              musq_, muwsq_ ) bind(C, name="ref_get_tables")

@@ -196,6 +196,13 @@ def _ref_job_body(job):
                          C.c_double(w["l2"]), C.c_double(w["l3"]), vec("u"), vec("w"), vec("n"),
                          vec("position"), vec("velocity"), vec("omega"))
     lib.ref_configure_walls()
+    wm = getattr(red, "wall_motion", None)
+    if wm:
+        fields = ("constant_factor", "amplitude", "velocity_amplitude", "two_pi_frequency",
+                  "phase_angle")
+        cols_wm = [tab(np.array([m[f] for m in wm], np.float64).T, (3, nw)) for f in fields]
+        lib.ref_set_wall_motion(C.c_int32(nw), *cols_wm)
+    lib.ref_get_time.restype = C.c_double
     out = {}
     md, mdsq, musq = (np.zeros((nc, nc), order="F") for _ in range(3))
     invm, invi, effm = (np.zeros(nc) for _ in range(3))
@@ -238,9 +245,13 @@ def _ref_job_body(job):
         lib.ref_save_state(C.c_char_p(path), C.c_int32(len(path)))
         st = parse_state_file(path, n, nprev)
         os.remove(path)
+        wpos, wvel = np.zeros((3, max(nw, 1)), order="F"), np.zeros((3, max(nw, 1)), order="F")
+        if nw:
+            lib.ref_get_walls(C.c_int32(nw), P(wpos), P(wvel))
         return dict(x=xs.T.copy(), v=vs.T.copy(), w=ws.T.copy(), a=as_.T.copy(),
                     alpha=als.T.copy(), cell_index=ci, state=st, ncol=int(lib.ref_ncol()),
-                    ke=float(lib.ref_kinetic_energy()))
+                    ke=float(lib.ref_kinetic_energy()), wall_pos=wpos[:, :nw].T.copy(),
+                    wall_vel=wvel[:, :nw].T.copy(), time=float(lib.ref_get_time()))
 
     out["snapshots"] = []
     for act in job["actions"]:

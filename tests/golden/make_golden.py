@@ -12,9 +12,13 @@ stores inputs and outputs as .npz (numpy arrays only, no pickles):
                     MaxNCell = 30 000: CellIndex on the capped grid
   ref_cone.npz      spheres inside a cone (hopper) wall plus a floor:
                     start-up + 30 steps (Overlap's Cone branch)
+  ref_ball_bearings.npz  the reference's example input (five Cone walls
+                    shaken on a circle, MoveWalls) with its 39 balls placed
+                    instead of poured: start-up + 1500 steps, wall positions,
+                    velocities and Time at each snapshot
 
 Usage (in a container that has the reference checkout and the built library):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture.npz ...]   (default: all)
 """
 from __future__ import annotations
 
@@ -36,6 +40,8 @@ RED_FIELDS = ("radius", "mass", "inertia", "density", "inv_mass", "inv_inertia",
               "min_dij", "min_dij_sq", "kn", "yn", "kt", "yt", "mu", "musq", "knw", "ynw", "ktw",
               "ytw", "muw", "muwsq", "lbox", "gravity")
 WALL_FIELDS = ("u", "w", "n", "position", "velocity", "omega")
+WM_FIELDS = ("constant_factor", "amplitude", "velocity_amplitude", "two_pi_frequency",
+             "phase_angle")
 
 
 def pack_reduced(red, out):
@@ -50,6 +56,9 @@ def pack_reduced(red, out):
                                             w["stiffness"], w["shear_stiffness"]])
         for k in WALL_FIELDS:
             out[f"wall{j}_{k}"] = np.asarray(w[k], np.float64)
+    for j, m in enumerate(red.wall_motion or []):
+        for k in WM_FIELDS:
+            out[f"wm{j}_{k}"] = np.asarray(m[k], np.float64)
 
 
 def snap_arrays(prefix, snap, out):
@@ -68,6 +77,9 @@ def snap_arrays(prefix, snap, out):
     out[f"{prefix}_cell_members"] = np.array([q for c in st["cells"] for q in c], np.int32)
     out[f"{prefix}_ncol"] = np.array(snap["ncol"])
     out[f"{prefix}_ke"] = np.array(snap["ke"])
+    out[f"{prefix}_wall_pos"] = snap["wall_pos"]
+    out[f"{prefix}_wall_vel"] = snap["wall_vel"]
+    out[f"{prefix}_time"] = np.array(snap["time"])
 
 
 def run_case(name, wl, actions, note):
@@ -130,26 +142,39 @@ def capped_workload(seed=13):
     return workloads.Workload("capped", spec, red, np.ones(n, np.int32), x, v, v.copy(), 0)
 
 
-def main():
-    wl = workloads.small_box(n_side=(6, 6, 6), nc=2, seed=3, vel=0.3)
-    run_case("ref_small.npz", wl,
-             [("compute", 1), ("snapshot", "c0"), ("euler",), ("euler",), ("euler",),
-              ("snapshot", "e"), ("step", 4, 20), ("snapshot", "s")],
-             "small_box(n_side=(6,6,6), nc=2, seed=3, vel=0.3)")
-    wl = workloads.mono_settling_2k(steps=20000)
-    run_case("ref_mono_settling_2k.npz", wl,
-             [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 60),
-              ("snapshot", "s")],
-             "mono_settling_2k(seed=20240001): start-up + 60 steps")
-    wl = capped_workload()
-    run_case("ref_capped.npz", wl, [("compute", 1), ("snapshot", "c0")],
-             "3000 spheres, 40 mm periodic box: natural grid 40^3 > MaxNCell")
-    wl = cone_workload()
-    run_case("ref_cone.npz", wl,
-             [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
-              ("snapshot", "s")],
-             "cone hopper + floor, 216 spheres")
+def main(names):
+    def want(name):
+        return not names or name in names
+    if want("ref_small.npz"):
+        wl = workloads.small_box(n_side=(6, 6, 6), nc=2, seed=3, vel=0.3)
+        run_case("ref_small.npz", wl,
+                 [("compute", 1), ("snapshot", "c0"), ("euler",), ("euler",), ("euler",),
+                  ("snapshot", "e"), ("step", 4, 20), ("snapshot", "s")],
+                 "small_box(n_side=(6,6,6), nc=2, seed=3, vel=0.3)")
+    if want("ref_mono_settling_2k.npz"):
+        wl = workloads.mono_settling_2k(steps=20000)
+        run_case("ref_mono_settling_2k.npz", wl,
+                 [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 60),
+                  ("snapshot", "s")],
+                 "mono_settling_2k(seed=20240001): start-up + 60 steps")
+    if want("ref_capped.npz"):
+        wl = capped_workload()
+        run_case("ref_capped.npz", wl, [("compute", 1), ("snapshot", "c0")],
+                 "3000 spheres, 40 mm periodic box: natural grid 40^3 > MaxNCell")
+    if want("ref_cone.npz"):
+        wl = cone_workload()
+        run_case("ref_cone.npz", wl,
+                 [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
+                  ("snapshot", "s")],
+                 "cone hopper + floor, 216 spheres")
+    if want("ref_ball_bearings.npz"):
+        wl = workloads.ball_bearings()
+        run_case("ref_ball_bearings.npz", wl,
+                 [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 1500),
+                  ("snapshot", "s")],
+                 "ball_bearings(seed=20240006): example input, moving cone walls, "
+                 "start-up + 1500 steps")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

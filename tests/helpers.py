@@ -59,3 +59,22 @@ def compare_contacts(cg, co, exact=True, rtol=0.0):
         else:
             scale = np.abs(co[k]).max() + 1e-300
             assert np.abs(cg[k] - co[k]).max() <= rtol * scale, k
+
+
+def shaken_floor(seed=41):
+    """A bed of two components settled by strong gravity (2000 m/s^2) on a
+    floor that MoveWalls shakes along x, y and z at 40, 55 and 70 Hz with
+    phases 0, -0.3 and 0.6 of a period."""
+    from dem3d_amd import workloads
+    wl = workloads.small_box(n_side=(5, 5, 3), nc=2, ratio=1.2, seed=seed, vel=0.05, e=0.3,
+                             gravity=-2000.0)
+    red = wl.red
+    T, L = red.time_scale, red.length_scale
+    amp = np.array([0.05e-3, 0.03e-3, 0.02e-3]) / L
+    tpf = 2.0 * np.pi * np.array([40.0, 55.0, 70.0]) * T
+    phase = 2.0 * np.pi * np.array([0.0, -0.3, 0.6])
+    pos = np.asarray(red.walls[0]["position"])
+    red.wall_motion = [dict(constant_factor=pos - amp * np.cos(phase), amplitude=amp,
+                            velocity_amplitude=-tpf * amp, two_pi_frequency=tpf,
+                            phase_angle=phase)]
+    return wl

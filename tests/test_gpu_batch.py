@@ -134,3 +134,41 @@ def test_step_on_batched_system_is_refused(hip):
     s.close()          # destroying a batched system takes it out of the batch
     assert b.lib.batch_size(b.b) == 0
     b.close()
+
+
+def test_batch_with_moving_walls_equals_alone(hip):
+    """Systems with moving walls (the shaken example dish, a shaken floor)
+    batched with a static one: two calls in a row give each the state, wall
+    positions and Time of the same two calls alone."""
+    from helpers import shaken_floor
+
+    def make(kind):
+        wl = workloads.ball_bearings() if kind == 0 else (
+            shaken_floor() if kind == 1 else workloads.small_box(**SYSTEMS[0]))
+        s = System(hip, wl.red)
+        s.upload(wl.comp, wl.x, wl.v, wl.w)
+        s.startup()
+        return s
+
+    def snap(s):
+        return _state(s) | {"walls": s.download_walls()["position"].copy(),
+                            "wvel": s.download_walls()["velocity"].copy(),
+                            "time": np.array(s.stats()["time"])}
+
+    refs = []
+    for kind in range(3):
+        s = make(kind)
+        s.step(120)
+        s.step(80)
+        refs.append(snap(s))
+        s.close()
+    systems = [make(kind) for kind in range(3)]
+    b = Batch(hip)
+    for s in systems:
+        b.add(s)
+    b.step(120)
+    b.step(80)
+    for s, ref in zip(systems, refs):
+        _assert_same(snap(s), ref)
+        assert s.stats()["next_step"] == s.red.method_order + 200
+    b.close()

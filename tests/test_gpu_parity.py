@@ -192,3 +192,45 @@ def test_escape_is_reported(hip):
     done_o, esc_o = o.step(50, allow_escape=True)
     assert esc == esc_o == 1
     assert done == done_o < 50
+
+
+def _moving_cases():
+    from helpers import shaken_floor
+    return {"ball-bearings": (workloads.ball_bearings(), 1500),
+            "shaken-floor": (shaken_floor(), 600)}
+
+
+@pytest.mark.parametrize("case", ["ball-bearings", "shaken-floor"])
+def test_moving_walls_match_oracle(hip, case):
+    """MoveWalls before every corrector and after every Euler sub-step: the
+    example input (five Cone walls shaken on a circle) and a floor shaken
+    along three axes.  Wall positions and velocities agree to the last bits
+    of the device cos/sin, Time exactly, states to round-off growth, contact
+    sets exactly."""
+    wl, nsteps = _moving_cases()[case]
+    g, o = _pair(hip, wl)
+    upload_same([g, o], wl)
+    for s in (g, o):
+        s.startup()
+    wg, wo = g.download_walls(), o.download_walls()
+    for k in ("position", "velocity"):
+        assert np.abs(wg[k] - wo[k]).max() <= 1e-14 * (np.abs(wo[k]).max() + 1.0), k
+    assert g.stats()["time"] == o.stats()["time"]
+    for s in (g, o):
+        s.step(nsteps)
+    sg, so = g.download(), o.download()
+    for k in ("x", "v", "w"):
+        scale = np.abs(so[k]).max() + 1e-300
+        err = np.abs(sg[k] - so[k]).max() / scale
+        print(f"{case} {k}: max error {err:.3g} of the largest value")
+        assert err <= 1e-8, k
+    wg, wo = g.download_walls(), o.download_walls()
+    for k in ("position", "velocity"):
+        assert np.abs(wg[k] - wo[k]).max() <= 1e-13 * (np.abs(wo[k]).max() + 1.0), k
+    stg, sto = g.stats(), o.stats()
+    assert stg["time"] == sto["time"]
+    assert stg["next_step"] == sto["next_step"] == wl.red.method_order + nsteps
+    cg, co = g.download_contacts(), o.download_contacts()
+    assert np.array_equal(cg["keys"], co["keys"])
+    assert np.array_equal(cg["wkeys"], co["wkeys"])
+    assert len(co["wkeys"]) > 3

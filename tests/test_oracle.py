@@ -40,6 +40,11 @@ def reduced_from_fixture(f) -> Reduced:
         for k in ("u", "w", "n", "position", "velocity", "omega"):
             w[k] = f[f"wall{j}_{k}"]
         walls.append(w)
+    wm = None
+    if nw and "wm0_amplitude" in f:
+        wm = [{k: f[f"wm{j}_{k}"] for k in ("constant_factor", "amplitude", "velocity_amplitude",
+                                            "two_pi_frequency", "phase_angle")}
+              for j in range(nw)]
     g = lambda k: f["red_" + k]
     return Reduced(nc=nc, nw=nw, pbc=pbc, method_order=order, n_euler=neuler, lbox=g("lbox"),
                    gravity=g("gravity"), time_step=float(sc[0]), number_of_steps=int(ints[7]),
@@ -48,7 +53,8 @@ def reduced_from_fixture(f) -> Reduced:
                    min_dij=g("min_dij"), min_dij_sq=g("min_dij_sq"), kn=g("kn"), yn=g("yn"),
                    kt=g("kt"), yt=g("yt"), mu=g("mu"), musq=g("musq"), knw=g("knw"), ynw=g("ynw"),
                    ktw=g("ktw"), ytw=g("ytw"), muw=g("muw"), muwsq=g("muwsq"), walls=walls,
-                   length_scale=float(sc[1]), mass_scale=float(sc[2]), time_scale=float(sc[3]))
+                   length_scale=float(sc[1]), mass_scale=float(sc[2]), time_scale=float(sc[3]),
+                   wall_motion=wm)
 
 
 def assert_snapshot(o, f, prefix):
@@ -67,6 +73,12 @@ def assert_snapshot(o, f, prefix):
     s = o.stats()
     assert s["ncol"] == int(f[f"{prefix}_ncol"]), prefix
     assert s["kinetic_energy"] == float(f[f"{prefix}_ke"]), prefix
+    if f"{prefix}_time" in f:
+        assert s["time"] == float(f[f"{prefix}_time"]), prefix
+        if o.red.nw:
+            wl = o.download_walls()
+            assert np.array_equal(wl["position"], f[f"{prefix}_wall_pos"]), prefix
+            assert np.array_equal(wl["velocity"], f[f"{prefix}_wall_vel"]), prefix
     # cell-list order (the reference's head-pushed lists)
     import ctypes as C
     r = np.zeros(o.np, np.int32)
@@ -100,6 +112,8 @@ FIXTURES = {
     "ref_capped.npz": [("compute", 1), ("snapshot", "c0")],
     "ref_cone.npz": [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
                      ("snapshot", "s")],
+    "ref_ball_bearings.npz": [("euler",), ("euler",), ("euler",), ("snapshot", "e"),
+                              ("step", 4, 1500), ("snapshot", "s")],
 }
 
 
@@ -126,6 +140,14 @@ def test_fixtures_exercise_their_features():
     assert np.prod(natural) > 30000 and np.prod(cap["grid"]) <= 30000
     cone = load_fixture("ref_cone.npz")
     assert np.any((cone["s_wkeys"] & 0xffffffff) == 2), "no contact with the cone"
+    bb = load_fixture("ref_ball_bearings.npz")
+    red = reduced_from_fixture(bb)
+    assert red.wall_motion is not None and len(bb["s_keys"]) > 10
+    touched = set(int(k) for k in (bb["s_wkeys"] & 0xffffffff))
+    assert {1, 2} <= touched, touched          # the inner cylinder and its disc
+    moved = bb["s_wall_pos"] - np.array([w["position"] for w in red.walls])
+    assert np.abs(moved[:, :2]).max() > 0.1    # shaken by more than 1 mm
+    assert np.abs(bb["s_wall_vel"]).max() > 0.0
 
 
 def test_derived_tables_match_initialize_force():
@@ -203,3 +225,35 @@ def test_resume_from_state_file_matches():
     for k in ("x", "v", "w", "a", "alpha"):
         assert np.array_equal(s[k], post[k]), k
     assert np.array_equal(o.download_contacts()["keys"], contacts_to_keys(post["state"]["pairs"])[0])
+
+
+@live
+def test_oracle_matches_live_reference_moving_floor():
+    """MoveWalls on a Rectangle floor shaken along all three axes with
+    different frequencies and phases (one negative): positions, velocities,
+    wall state and Time stay bitwise equal through the Euler start-up and the
+    steps."""
+    from helpers import shaken_floor
+    wl = shaken_floor()
+    red = wl.red
+    actions = [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 600),
+               ("snapshot", "s")]
+    res = run_reference(dict(red=red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, actions=actions))
+    o = oracle_system(red)
+    o.red = red
+    o.upload(wl.comp, wl.x, wl.v, wl.w)
+    snaps = iter(res["snapshots"])
+    for _ in run_actions(o, actions):
+        snap = next(snaps)
+        st = o.download()
+        for k in ("x", "v", "w", "a", "alpha"):
+            assert np.array_equal(st[k], snap[k]), k
+        walls = o.download_walls()
+        assert np.array_equal(walls["position"], snap["wall_pos"])
+        assert np.array_equal(walls["velocity"], snap["wall_vel"])
+        assert o.stats()["time"] == snap["time"]
+        wk, wft, _ = contacts_to_keys(snap["state"]["walls"])
+        con = o.download_contacts()
+        assert np.array_equal(con["wkeys"], wk)
+        assert np.array_equal(con["wft"].reshape(wft.shape), wft)
+    assert len(wk) > 10

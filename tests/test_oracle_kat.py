@@ -9,6 +9,9 @@ happens not to exercise it):
 * a sphere resting on the floor: overlap m g / Knw;
 * a sphere sliding on the floor: Coulomb-capped friction decelerates it at
   mu g (|FT| = mu |FN|);
+* a sphere riding a floor that MoveWalls shakes slowly up and down: the
+  overlap follows m (g + a_floor(t)) / Knw less the dashpot's share, with the
+  floor where A cos(omega t) puts it at t = step * TimeStep;
 * the contact set equals an O(N^2) all-pairs search with minimum image;
 * Newton's third law: internal pair forces sum to zero.
 """
@@ -109,6 +112,42 @@ def test_sliding_sphere_decelerates_at_mu_g():
     assert abs(st["a"][0, 0] + mu * g) <= 1e-6 * mu * g, (st["a"][0, 0], -mu * g)
     # the friction torque spins it up about +y at 5/2 mu g / r (I = 2/5 m r^2)
     assert abs(st["alpha"][1, 0] - 2.5 * mu * g / r) <= 1e-6 * 2.5 * mu * g / r
+
+
+def test_sphere_rides_a_shaken_floor():
+    red = _system((10, 10, 20), 1.0, e=0.3)
+    r, m = red.radius[0], red.eff_mass[0]
+    knw, ynw = red.knw[0, 0], red.ynw[0, 0]
+    g = abs(red.gravity[2])
+    zf = -0.5 * red.lbox[2]
+    delta0 = m * g / knw
+    omega = 2.0 * np.pi / 400.0          # 400 collision times per period
+    amp = 0.3 * g / omega ** 2           # peak floor acceleration 0.3 g
+    red.wall_motion = [dict(constant_factor=np.array([0.0, 0.0, zf - amp]),
+                            amplitude=np.array([0.0, 0.0, amp]),
+                            velocity_amplitude=np.array([0.0, 0.0, -omega * amp]),
+                            two_pi_frequency=np.array([0.0, 0.0, omega]),
+                            phase_angle=np.zeros(3))]
+    x = np.array([[0.0], [0.0], [zf + r - delta0]])
+    o = _run(red, [1], x, np.zeros((3, 1)), np.zeros((3, 1)), 0,
+             a=np.array([[0.0], [0.0], -g * np.ones(1)]))
+    step = red.method_order - 1
+    worst = 0.0
+    for k in range(12):
+        o.step(1500)
+        step += 1500
+        t = step * red.time_step
+        st = o.download()
+        z_floor = zf - amp + amp * np.cos(omega * t)
+        a_floor = -amp * omega ** 2 * np.cos(omega * t)
+        jerk = amp * omega ** 3 * np.sin(omega * t)
+        # quasi-static: Knw d + Ynw d' = m (g + a_floor), d' = m jerk / Knw
+        expect = m * (g + a_floor) / knw - ynw * m * jerk / knw ** 2
+        delta = r - (st["x"][2, 0] - z_floor)
+        worst = max(worst, abs(delta - expect))
+    # the overlap swings by 0.3 delta0; one step of error in the floor's
+    # time would move the floor by ~2 delta0
+    assert worst <= 5e-5 * delta0, worst / delta0
 
 
 def _min_image(d, L, periodic):
