@@ -7,6 +7,7 @@
 // block: errors, escapes).
 #include "../../include/dem3d.h"
 #include "batch.hpp"
+#include "files.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -46,7 +47,7 @@ struct dem_system {
   std::vector<WallGeom> walls_h;
   std::vector<dem_wall_t> walls_in;  // as given to dem_set_walls
   std::vector<WallConst> wallc_h;
-  std::vector<double> radius_h;
+  std::vector<double> radius_h, mass_h, inertia_h;
   // device constant tables
   PairConst* d_pair = nullptr;
   CompConst* d_comp = nullptr;
@@ -207,6 +208,8 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     rmax = std::max(rmax, p->radius[c]);
     rmin = std::min(rmin, p->radius[c]);
     h->radius_h.push_back(p->radius[c]);
+    h->mass_h.push_back(p->mass ? p->mass[c] : 0.0);
+    h->inertia_h.push_back(p->inertia ? p->inertia[c] : 0.0);
   }
   h->skin = p->skin > 0.0 ? p->skin : 0.2 * rmin;  // 0.1 of the smallest diameter
   s.skin_half_sq = 0.25 * h->skin * h->skin;
@@ -905,6 +908,337 @@ int dem_download_cells(dem_handle_t h, int32_t np, int32_t* cell_index) {
   HIPCHK(h, hipMemcpyAsync(cell_index, h->d_tmp, (size_t)np * sizeof(int), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+// The reference's list order within each cell: head-pushed lists put the
+// particle that entered last first (larger entry stamp), and particles that
+// entered in the same evaluation in decreasing index order (the
+// redistribution loop runs i = 1..NP, src/MCells.f90:607-643).
+int dem_download_cell_rank(dem_handle_t h, int32_t np, int32_t* rank) {
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "no state");
+  if (np != h->np) return fail(h, DEM_ERR_ARG, "np mismatch");
+  if (np == 0) return DEM_OK;
+  if (!rank) return fail(h, DEM_ERR_ARG, "null output");
+  int rc = ensure_tmp(h, (size_t)np * sizeof(int2));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cell_stamps_to_ref, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, h->stream, np,
+                     (const Kin*)h->kin[h->last_eval_buf], h->id, (int2*)h->d_tmp);
+  std::vector<int2> cs((size_t)np);
+  HIPCHK(h, hipMemcpyAsync(cs.data(), h->d_tmp, (size_t)np * sizeof(int2), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<int> order((size_t)np);
+  for (int i = 0; i < np; ++i) order[i] = i;
+  // by cell, then list position (stamp descending, index descending)
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    if (cs[a].x != cs[b].x) return cs[a].x < cs[b].x;
+    if (cs[a].y != cs[b].y) return cs[a].y > cs[b].y;
+    return a > b;
+  });
+  for (int k = 0; k < np;) {
+    int e = k;
+    while (e < np && cs[order[e]].x == cs[order[k]].x) ++e;
+    for (int q = k; q < e; ++q) rank[order[q]] = e - 1 - q;
+    k = e;
+  }
+  return DEM_OK;
+}
+
+// ---- files -------------------------------------------------------------------
+
+int dem_config_write(const char* path, const dem_config_t* cfg) {
+  if (!path || !cfg) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  demfile::Writer o;
+  const std::string e = demfile::write_config(o, *cfg);
+  if (!e.empty()) return fail(nullptr, DEM_ERR_ARG, "configuration: %s", e.c_str());
+  if (!demfile::save(path, o)) return fail(nullptr, DEM_ERR_IO, "cannot write %s", path);
+  return DEM_OK;
+}
+
+int dem_config_read(const char* path, dem_config_t* cfg) {
+  if (!path || !cfg) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  demfile::Reader r;
+  if (!demfile::load(path, r)) return fail(nullptr, DEM_ERR_IO, "cannot read %s", path);
+  const std::string e = demfile::read_config(r, *cfg);
+  if (!e.empty()) return fail(nullptr, DEM_ERR_IO, "%s: %s", path, e.c_str());
+  return DEM_OK;
+}
+
+int dem_state_file_write(const char* path, const dem_run_t* run, const dem_config_t* cfg,
+                         const dem_state_sections_t* sec) {
+  if (!path || !run || !cfg || !sec) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  if (cfg->form != DEM_CFG_COMPLETE)
+    return fail(nullptr, DEM_ERR_ARG, "a state file holds the Complete form");
+  demfile::Writer o;
+  std::string e = demfile::write_run_head(o, *run);
+  if (e.empty()) e = demfile::write_config(o, *cfg);
+  if (e.empty()) e = demfile::write_sections(o, *sec, cfg->np);
+  if (!e.empty()) return fail(nullptr, DEM_ERR_ARG, "state file: %s", e.c_str());
+  demfile::write_run_tail(o, *run);
+  if (!demfile::save(path, o)) return fail(nullptr, DEM_ERR_IO, "cannot write %s", path);
+  return DEM_OK;
+}
+
+int dem_state_file_read(const char* path, dem_run_t* run, dem_config_t* cfg,
+                        dem_state_sections_t* sec) {
+  if (!path || !cfg || !sec) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  if (sec->nprev < 0 || sec->nprev > 3) return fail(nullptr, DEM_ERR_ARG, "nprev must be 0..3");
+  demfile::Reader r;
+  if (!demfile::load(path, r)) return fail(nullptr, DEM_ERR_IO, "cannot read %s", path);
+  dem_run_t tmp;
+  memset(&tmp, 0, sizeof tmp);
+  dem_run_t& rr = run ? *run : tmp;
+  demfile::read_run_head(r, rr);
+  if (!r.ok) return fail(nullptr, DEM_ERR_IO, "%s: file ends inside the run records", path);
+  std::string e = demfile::read_config(r, *cfg);
+  if (e.empty() && cfg->form != DEM_CFG_COMPLETE) e = "configuration is not in the Complete form";
+  if (e.empty()) e = demfile::read_sections(r, *sec, cfg->np);
+  if (e.empty()) {
+    demfile::read_run_tail(r, rr);
+    if (!r.ok) e = "file ends inside the run records";
+  }
+  if (!e.empty()) return fail(nullptr, DEM_ERR_IO, "%s: %s", path, e.c_str());
+  return DEM_OK;
+}
+
+// SaveState (src/DEM.f90:947-1014) of the handle's state.
+int dem_save_state_file(dem_handle_t h, const char* path, const dem_run_t* run,
+                        const dem_config_t* meta) {
+  if (!h || !path || !meta) return fail(h, DEM_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  const int np = h->np, nc = h->sc.nc, nw = h->sc.nw;
+  if (nc > 0 && (!meta->density || !meta->stiffness || !meta->shear_stiffness))
+    return fail(h, DEM_ERR_ARG, "meta needs density, stiffness and shear_stiffness");
+  const size_t N = (size_t)np;
+  std::vector<int32_t> comp(N), cells(N), rank(N), number((size_t)nc, 0), ipc(3, 0);
+  std::vector<double> x(3 * N), v(3 * N), w(3 * N), ring((size_t)std::max(h->nprev, 1) * 9 * N);
+  int rc = dem_download_state(h, np, comp.data(), x.data(), v.data(), w.data(), nullptr, nullptr,
+                              ring.data(), ipc.data());
+  if (!rc && np) rc = dem_download_cells(h, np, cells.data());
+  if (!rc && np) rc = dem_download_cell_rank(h, np, rank.data());
+  int64_t npairs = 0, nwall = 0;
+  if (!rc)
+    rc = dem_download_contacts(h, 0, &npairs, nullptr, nullptr, nullptr, 0, &nwall, nullptr,
+                               nullptr, nullptr);
+  std::vector<uint64_t> keys((size_t)npairs), wkeys((size_t)nwall);
+  std::vector<double> ft(3 * (size_t)npairs), vt(3 * (size_t)npairs), wft(3 * (size_t)nwall),
+      wvt(3 * (size_t)nwall);
+  if (!rc)
+    rc = dem_download_contacts(h, npairs, &npairs, keys.data(), ft.data(), vt.data(), nwall,
+                               &nwall, wkeys.data(), wft.data(), wvt.data());
+  std::vector<dem_wall_t> walls((size_t)std::max(nw, 1));
+  if (!rc && nw) rc = dem_download_walls(h, nw, walls.data());
+  dem_stats_t st;
+  if (!rc) rc = dem_stats(h, &st);
+  if (rc) return rc;
+  // cell lists (SaveCellStructure): cells 1..Nt, each in list order
+  const int ncell = h->sc.rn[0] * h->sc.rn[1] * h->sc.rn[2];
+  std::vector<int32_t> len((size_t)ncell, 0), members(N);
+  for (size_t i = 0; i < N; ++i) {
+    if (cells[i] < 1 || cells[i] > ncell) return fail(h, DEM_ERR_STATE, "cell index out of range");
+    len[cells[i] - 1] += 1;
+  }
+  {
+    std::vector<int64_t> start((size_t)ncell + 1, 0);
+    for (int c = 0; c < ncell; ++c) start[c + 1] = start[c] + len[c];
+    for (size_t i = 0; i < N; ++i) {
+      const int c = cells[i] - 1;
+      members[(size_t)(start[c] + (len[c] - 1 - rank[i]))] = (int32_t)(i + 1);
+    }
+  }
+  for (size_t i = 0; i < N; ++i) number[comp[i] - 1] += 1;
+  dem_config_t cfg = *meta;
+  cfg.form = DEM_CFG_COMPLETE;
+  cfg.version = 2;
+  cfg.time = h->time;
+  for (int d = 0; d < 3; ++d) cfg.lbox[d] = h->sc.lbox[d];
+  cfg.nc = nc;
+  cfg.nw = nw;
+  cfg.np = np;
+  cfg.number = number.data();
+  cfg.radius = h->radius_h.data();
+  cfg.mass = h->mass_h.data();
+  cfg.inertia = h->inertia_h.data();
+  cfg.walls = walls.data();
+  cfg.comp = comp.data();
+  cfg.x = x.data();
+  cfg.v = v.data();
+  cfg.w = w.data();
+  dem_state_sections_t sec;
+  memset(&sec, 0, sizeof sec);
+  sec.nprev = h->nprev;
+  for (int k = 0; k < h->nprev; ++k) sec.ipc[k] = ipc[k];
+  sec.ring = ring.data();
+  sec.ncell = ncell;
+  sec.cell_len = len.data();
+  sec.cell_members = members.data();
+  sec.npairs = npairs;
+  sec.keys = keys.data();
+  sec.ft = ft.data();
+  sec.vt = vt.data();
+  sec.nwall = nwall;
+  sec.wkeys = wkeys.data();
+  sec.wft = wft.data();
+  sec.wvt = wvt.data();
+  dem_run_t r;
+  if (run) r = *run;
+  else memset(&r, 0, sizeof r);
+  r.ncol = (int32_t)st.ncol;
+  r.next_step = (int32_t)h->next_step;
+  demfile::Writer o;
+  std::string e = demfile::write_run_head(o, r);
+  if (e.empty()) e = demfile::write_config(o, cfg);
+  if (e.empty()) e = demfile::write_sections(o, sec, np);
+  if (!e.empty()) return fail(h, DEM_ERR_ARG, "state file: %s", e.c_str());
+  demfile::write_run_tail(o, r);
+  if (!demfile::save(path, o)) return fail(h, DEM_ERR_IO, "cannot write %s", path);
+  return DEM_OK;
+}
+
+// ReadState (src/DEM.f90:1016-1110) into the handle.
+int dem_load_state_file(dem_handle_t h, const char* path, dem_run_t* run, dem_config_t* meta) {
+  if (!h || !path) return fail(h, DEM_ERR_ARG, "null argument");
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  demfile::Reader r;
+  if (!demfile::load(path, r)) return fail(h, DEM_ERR_IO, "cannot read %s", path);
+  dem_run_t rr;
+  memset(&rr, 0, sizeof rr);
+  demfile::read_run_head(r, rr);
+  if (!r.ok) return fail(h, DEM_ERR_IO, "%s: file ends inside the run records", path);
+  // sizes first, then the arrays, from the same offset
+  const size_t at = r.off;
+  dem_config_t c;
+  memset(&c, 0, sizeof c);
+  std::string e = demfile::read_config(r, c);
+  if (e.empty() && c.form != DEM_CFG_COMPLETE) e = "configuration is not in the Complete form";
+  if (!e.empty()) return fail(h, DEM_ERR_IO, "%s: %s", path, e.c_str());
+  if (c.nc != h->sc.nc || c.nw != h->sc.nw)
+    return fail(h, DEM_ERR_ARG, "%s holds %d components and %d walls, the system %d and %d", path,
+                c.nc, c.nw, h->sc.nc, h->sc.nw);
+  const int np = c.np, nc = c.nc, nw = c.nw;
+  const size_t N = (size_t)np;
+  std::vector<int32_t> number((size_t)nc), comp(N);
+  std::vector<double> rad((size_t)nc), den((size_t)nc), stf((size_t)nc), shr((size_t)nc),
+      ms((size_t)nc), in((size_t)nc), x(3 * N), v(3 * N), w(3 * N);
+  std::vector<dem_wall_t> walls((size_t)std::max(nw, 1));
+  c.number = number.data();
+  c.radius = rad.data();
+  c.density = den.data();
+  c.stiffness = stf.data();
+  c.shear_stiffness = shr.data();
+  c.mass = ms.data();
+  c.inertia = in.data();
+  c.walls = walls.data();
+  c.comp = comp.data();
+  c.x = x.data();
+  c.v = v.data();
+  c.w = w.data();
+  r.off = at;
+  r.ok = true;
+  e = demfile::read_config(r, c);
+  const size_t at2 = r.off;
+  dem_state_sections_t sec;
+  memset(&sec, 0, sizeof sec);
+  sec.nprev = h->nprev;
+  if (e.empty()) e = demfile::read_sections(r, sec, np);
+  if (!e.empty()) return fail(h, DEM_ERR_IO, "%s: %s", path, e.c_str());
+  std::vector<double> ring((size_t)std::max(h->nprev, 1) * 9 * N);
+  std::vector<int32_t> len((size_t)sec.ncell), members(N);
+  std::vector<uint64_t> keys((size_t)sec.npairs), wkeys((size_t)sec.nwall);
+  std::vector<double> ft(3 * (size_t)sec.npairs), vt(3 * (size_t)sec.npairs),
+      wft(3 * (size_t)sec.nwall), wvt(3 * (size_t)sec.nwall);
+  sec.ring = ring.data();
+  sec.cell_len = len.data();
+  sec.cell_members = members.data();
+  sec.keys = keys.data();
+  sec.ft = ft.data();
+  sec.vt = vt.data();
+  sec.wkeys = wkeys.data();
+  sec.wft = wft.data();
+  sec.wvt = wvt.data();
+  r.off = at2;
+  r.ok = true;
+  e = demfile::read_sections(r, sec, np);
+  if (e.empty()) {
+    demfile::read_run_tail(r, rr);
+    if (!r.ok) e = "file ends inside the run records";
+  }
+  if (!e.empty()) return fail(h, DEM_ERR_IO, "%s: %s", path, e.c_str());
+  const int ncell = h->sc.rn[0] * h->sc.rn[1] * h->sc.rn[2];
+  if (sec.ncell != ncell)
+    return fail(h, DEM_ERR_ARG, "%s has %d cells, the system's grid %d", path, sec.ncell, ncell);
+  // list position -> rank (the head of a list of n has rank n-1)
+  std::vector<int32_t> rank(N, -1);
+  {
+    size_t m = 0;
+    for (int cc = 0; cc < sec.ncell; ++cc)
+      for (int k = 0; k < len[cc]; ++k, ++m) {
+        const int q = members[m];
+        if (q < 1 || q > np || rank[q - 1] >= 0)
+          return fail(h, DEM_ERR_IO, "%s: cell lists do not hold every particle once", path);
+        rank[q - 1] = len[cc] - 1 - k;
+      }
+  }
+  int rc = DEM_OK;
+  if (nw) rc = dem_set_walls(h, nw, walls.data());
+  if (!rc)
+    rc = dem_upload_state(h, np, comp.data(), x.data(), v.data(), w.data(), nullptr, nullptr,
+                          h->nprev > 0 ? ring.data() : nullptr,
+                          h->nprev > 0 ? sec.ipc : nullptr, rank.data());
+  if (!rc)
+    rc = dem_upload_contacts(h, sec.npairs, keys.data(), ft.data(), vt.data(), sec.nwall,
+                             wkeys.data(), wft.data(), wvt.data());
+  if (rc) return rc;
+  h->time = c.time;
+  h->next_step = rr.next_step;
+  {
+    const unsigned long long ncol = (unsigned long long)(uint32_t)rr.ncol;
+    HIPCHK(h, hipMemcpyAsync(&h->ctl->ncol, &ncol, sizeof ncol, hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (run) {
+    char* inter = run->interruption;
+    char* ret = run->retaking;
+    const int cap = run->n_interruptions;
+    *run = rr;
+    run->interruption = inter;
+    run->retaking = ret;
+    // the strings of the interruptions, when the caller gave room for them
+    if ((inter || ret) && cap >= rr.n_interruptions) {
+      dem_run_t again;
+      memset(&again, 0, sizeof again);
+      again.interruption = inter;
+      again.retaking = ret;
+      r.off = 0;
+      r.ok = true;
+      demfile::read_run_head(r, again);
+    }
+  }
+  if (meta) {
+    meta->form = c.form;
+    meta->version = c.version;
+    meta->length_scale = c.length_scale;
+    meta->time_scale = c.time_scale;
+    meta->mass_scale = c.mass_scale;
+    meta->time = c.time;
+    for (int d = 0; d < 3; ++d) meta->lbox[d] = c.lbox[d];
+    meta->nc = nc;
+    meta->nw = nw;
+    meta->np = np;
+    for (int k = 0; k < nc; ++k) {
+      if (meta->number) meta->number[k] = number[k];
+      if (meta->radius) meta->radius[k] = rad[k];
+      if (meta->density) meta->density[k] = den[k];
+      if (meta->stiffness) meta->stiffness[k] = stf[k];
+      if (meta->shear_stiffness) meta->shear_stiffness[k] = shr[k];
+      if (meta->mass) meta->mass[k] = ms[k];
+      if (meta->inertia) meta->inertia[k] = in[k];
+    }
+    for (int j = 0; j < nw; ++j)
+      if (meta->walls) meta->walls[j] = walls[j];
+  }
   return DEM_OK;
 }
 

@@ -1055,6 +1055,13 @@ __global__ void k_cells_to_ref(int np, const Kin* kin, const int* id, int* out) 
   out[id[i]] = kin[i].rc + 1;
 }
 
+// (cell, entry stamp) of every particle, by reference id.
+__global__ void k_cell_stamps_to_ref(int np, const Kin* kin, const int* id, int2* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  out[id[i]] = make_int2(kin[i].rc, kin[i].st);
+}
+
 // Deterministic partial sums for the kinetic energy (src/MDEM.f90:199-213):
 // each block sums its particles in index order, the host sums the blocks.
 __global__ __launch_bounds__(DEM_BLOCK) void k_energy(SysConst s, const double4* pos,

@@ -11,6 +11,9 @@ program check_bindings
   type(dem_particle_t) :: q
   type(dem_stats_t) :: s
   type(dem_wall_motion_t) :: m
+  type(dem_config_t) :: cf
+  type(dem_run_t) :: rn
+  type(dem_state_sections_t) :: sc
   type(c_ptr) :: h
   real(c_double), target :: radius(1), tab(1)
   integer(c_int) :: rc
@@ -20,6 +23,9 @@ program check_bindings
   write(*, '(a, i0)') 'sizeof dem_particle_t ', c_sizeof(q)
   write(*, '(a, i0)') 'sizeof dem_stats_t ', c_sizeof(s)
   write(*, '(a, i0)') 'sizeof dem_wall_motion_t ', c_sizeof(m)
+  write(*, '(a, i0)') 'sizeof dem_config_t ', c_sizeof(cf)
+  write(*, '(a, i0)') 'sizeof dem_run_t ', c_sizeof(rn)
+  write(*, '(a, i0)') 'sizeof dem_state_sections_t ', c_sizeof(sc)
 
   radius = 0.5d0
   tab = 1.0d0

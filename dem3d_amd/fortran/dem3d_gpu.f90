@@ -16,7 +16,9 @@ module dem3d_gpu
   private
 
   integer(c_int), parameter, public :: DEM_OK = 0, DEM_ERR_ARG = -1, DEM_ERR_DEVICE = -2, &
-       DEM_ERR_CAPACITY = -3, DEM_ERR_STATE = -4, DEM_ERR_UNSUPPORTED = -5, DEM_ERR_ESCAPE = -6
+       DEM_ERR_CAPACITY = -3, DEM_ERR_STATE = -4, DEM_ERR_UNSUPPORTED = -5, DEM_ERR_ESCAPE = -6, &
+       DEM_ERR_IO = -7
+  integer(c_int32_t), parameter, public :: DEM_CFG_COMPLETE = 72, DEM_CFG_COMPACT = 28
   integer(c_int32_t), parameter, public :: DEM_FP64 = 0, DEM_MIXED = 1
   integer(c_int32_t), parameter, public :: DEM_SHAPE_RECTANGLE = 1, DEM_SHAPE_CONE = 2
   integer(c_int32_t), parameter, public :: DEM_HIST_REFERENCE = 0, DEM_HIST_CANONICAL = 1
@@ -73,7 +75,45 @@ module dem3d_gpu
     integer(c_int64_t) :: next_step
   end type dem_stats_t
 
+  ! dem_config_t: a configuration file (SaveConfiguration, src/MConfig.f90:109-236).
+  type, bind(c), public :: dem_config_t
+    integer(c_int32_t) :: form = DEM_CFG_COMPLETE, version = 2
+    real(c_double) :: length_scale = 0, time_scale = 0, mass_scale = 0, time = 0
+    real(c_double) :: lbox(3) = 0
+    integer(c_int32_t) :: nc = 0, nw = 0, np = 0
+    type(c_ptr) :: number = c_null_ptr, radius = c_null_ptr, density = c_null_ptr
+    type(c_ptr) :: stiffness = c_null_ptr, shear_stiffness = c_null_ptr
+    type(c_ptr) :: mass = c_null_ptr, inertia = c_null_ptr, walls = c_null_ptr
+    type(c_ptr) :: comp = c_null_ptr, x = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+  end type dem_config_t
+
+  ! dem_run_t: the driver's records of a state file (SaveState, src/DEM.f90:947-1014).
+  type, bind(c), public :: dem_run_t
+    integer(c_int32_t) :: finished = 0
+    character(kind=c_char) :: start_time(28) = ' '
+    integer(c_int32_t) :: n_interruptions = 0
+    type(c_ptr) :: interruption = c_null_ptr, retaking = c_null_ptr
+    character(kind=c_char) :: finish_time(28) = ' '
+    integer(c_int32_t) :: hours = 0, minutes = 0, seconds = 0, milisecs = 0
+    real(c_double) :: aam = 0
+    integer(c_int32_t) :: dummy = 0, iix = 0, iiy = 0, iik = 0, conf = 0, next_step = 0, ncol = 0
+  end type dem_run_t
+
+  ! dem_state_sections_t: cell lists, ABM ring and contact lists of a state file.
+  type, bind(c), public :: dem_state_sections_t
+    integer(c_int32_t) :: nprev = 0, ipc(3) = 0
+    type(c_ptr) :: ring = c_null_ptr
+    integer(c_int32_t) :: ncell = 0
+    type(c_ptr) :: cell_len = c_null_ptr, cell_members = c_null_ptr
+    integer(c_int64_t) :: npairs = 0
+    type(c_ptr) :: keys = c_null_ptr, ft = c_null_ptr, vt = c_null_ptr
+    integer(c_int64_t) :: nwall = 0
+    type(c_ptr) :: wkeys = c_null_ptr, wft = c_null_ptr, wvt = c_null_ptr
+  end type dem_state_sections_t
+
   public :: dem_create, dem_destroy, dem_last_error, dem_set_walls
+  public :: dem_config_write, dem_config_read, dem_state_file_write, dem_state_file_read
+  public :: dem_save_state_file, dem_load_state_file, dem_download_cell_rank
   public :: dem_set_wall_motion, dem_download_walls
   public :: dem_upload_state, dem_download_state, dem_download_predicted
   public :: dem_upload_contacts, dem_download_contacts, dem_download_cells
@@ -202,6 +242,56 @@ module dem3d_gpu
       type(c_ptr), value :: h
       character(kind=c_char), intent(in) :: name(*)
       real(c_double), value :: value
+    end function
+    ! File names are C strings: pass trim(name)//c_null_char.
+    integer(c_int) function dem_config_write(path, cfg) bind(c, name="dem_config_write")
+      import :: c_int, c_char, dem_config_t
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_config_t), intent(in) :: cfg
+    end function
+    integer(c_int) function dem_config_read(path, cfg) bind(c, name="dem_config_read")
+      import :: c_int, c_char, dem_config_t
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_config_t), intent(inout) :: cfg
+    end function
+    integer(c_int) function dem_state_file_write(path, run, cfg, sec) &
+        bind(c, name="dem_state_file_write")
+      import :: c_int, c_char, dem_run_t, dem_config_t, dem_state_sections_t
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_run_t), intent(in) :: run
+      type(dem_config_t), intent(in) :: cfg
+      type(dem_state_sections_t), intent(in) :: sec
+    end function
+    integer(c_int) function dem_state_file_read(path, run, cfg, sec) &
+        bind(c, name="dem_state_file_read")
+      import :: c_int, c_char, dem_run_t, dem_config_t, dem_state_sections_t
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_run_t), intent(inout) :: run
+      type(dem_config_t), intent(inout) :: cfg
+      type(dem_state_sections_t), intent(inout) :: sec
+    end function
+    integer(c_int) function dem_save_state_file(h, path, run, meta) &
+        bind(c, name="dem_save_state_file")
+      import :: c_int, c_char, c_ptr, dem_run_t, dem_config_t
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_run_t), intent(in) :: run
+      type(dem_config_t), intent(in) :: meta
+    end function
+    integer(c_int) function dem_load_state_file(h, path, run, meta) &
+        bind(c, name="dem_load_state_file")
+      import :: c_int, c_char, c_ptr, dem_run_t, dem_config_t
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      type(dem_run_t), intent(inout) :: run
+      type(dem_config_t), intent(inout) :: meta
+    end function
+    integer(c_int) function dem_download_cell_rank(h, np, rank) &
+        bind(c, name="dem_download_cell_rank")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: np
+      integer(c_int32_t), intent(out) :: rank(*)
     end function
     integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
       import :: c_size_t, c_ptr

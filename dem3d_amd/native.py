@@ -88,6 +88,37 @@ class Stats(C.Structure):
         return out
 
 
+class ConfigT(C.Structure):
+    _fields_ = [
+        ("form", C.c_int32), ("version", C.c_int32), ("length_scale", C.c_double),
+        ("time_scale", C.c_double), ("mass_scale", C.c_double), ("time", C.c_double),
+        ("lbox", C.c_double * 3), ("nc", C.c_int32), ("nw", C.c_int32), ("np", C.c_int32),
+        ("number", _ip), ("radius", _dp), ("density", _dp), ("stiffness", _dp),
+        ("shear_stiffness", _dp), ("mass", _dp), ("inertia", _dp), ("walls", C.POINTER(WallT)),
+        ("comp", _ip), ("x", _dp), ("v", _dp), ("w", _dp),
+    ]
+
+
+class RunT(C.Structure):
+    _fields_ = [
+        ("finished", C.c_int32), ("start_time", C.c_char * 28), ("n_interruptions", C.c_int32),
+        ("interruption", C.c_void_p), ("retaking", C.c_void_p), ("finish_time", C.c_char * 28),
+        ("hours", C.c_int32), ("minutes", C.c_int32), ("seconds", C.c_int32),
+        ("milisecs", C.c_int32), ("aam", C.c_double), ("dummy", C.c_int32), ("iix", C.c_int32),
+        ("iiy", C.c_int32), ("iik", C.c_int32), ("conf", C.c_int32), ("next_step", C.c_int32),
+        ("ncol", C.c_int32),
+    ]
+
+
+class SectionsT(C.Structure):
+    _fields_ = [
+        ("nprev", C.c_int32), ("ipc", C.c_int32 * 3), ("ring", _dp), ("ncell", C.c_int32),
+        ("cell_len", _ip), ("cell_members", _ip), ("npairs", C.c_int64), ("keys", _up),
+        ("ft", _dp), ("vt", _dp), ("nwall", C.c_int64), ("wkeys", _up), ("wft", _dp),
+        ("wvt", _dp),
+    ]
+
+
 def _ptr(a: Optional[np.ndarray], typ=_dp):
     if a is None:
         return C.cast(None, typ)
@@ -133,6 +164,16 @@ class Library:
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "config_write": (C.c_int, [C.c_char_p, C.POINTER(ConfigT)]),
+        "config_read": (C.c_int, [C.c_char_p, C.POINTER(ConfigT)]),
+        "state_file_write": (C.c_int, [C.c_char_p, C.POINTER(RunT), C.POINTER(ConfigT),
+                                       C.POINTER(SectionsT)]),
+        "state_file_read": (C.c_int, [C.c_char_p, C.POINTER(RunT), C.POINTER(ConfigT),
+                                      C.POINTER(SectionsT)]),
+        "save_state_file": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(RunT),
+                                      C.POINTER(ConfigT)]),
+        "load_state_file": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(RunT),
+                                      C.POINTER(ConfigT)]),
         "batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
         "batch_destroy": (None, [C.c_void_p]),
         "batch_last_error": (C.c_char_p, [C.c_void_p]),

@@ -52,7 +52,9 @@ enum {
   DEM_ERR_CAPACITY = -3,   /* a fixed capacity was exceeded (see message)     */
   DEM_ERR_STATE = -4,      /* call out of order (e.g. step before upload)     */
   DEM_ERR_UNSUPPORTED = -5,/* feature outside the implemented path            */
-  DEM_ERR_ESCAPE = -6      /* particles left the box: host must remove them   */
+  DEM_ERR_ESCAPE = -6,     /* particles left the box: host must remove them   */
+  DEM_ERR_IO = -7          /* a file could not be read or written, or is not
+                              in the expected format                          */
 };
 
 enum { DEM_FP64 = 0, DEM_MIXED = 1 };               /* force arithmetic          */
@@ -258,6 +260,109 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *                       configuration before an Euler start-up
  *   "next_step"         sets the step number of the next batched step */
 int dem_set_option(dem_handle_t h, const char* name, double value);
+
+/* ---- files in the reference's binary formats ----------------------------
+ * BinFiles byte streams (src/BinFiles.f90): every value as its raw bytes in
+ * the order written, strings without a length, no record markers.  The
+ * functions without a handle work on host arrays and need no device.  Their
+ * errors are read with dem_last_error(NULL). */
+#define DEM_CFG_COMPLETE 72   /* Complete (src/MConfig.f90:20) */
+#define DEM_CFG_COMPACT 28    /* Compact */
+
+/* A configuration: what SaveConfiguration writes (src/MConfig.f90:109-236)
+ * and ReadConfiguration reads (:266-448).  Arrays belong to the caller.
+ * dem_config_read sets form, version, the scalars and nc, nw, np, then fills
+ * every array that is not NULL, so a call with NULL arrays queries the sizes.
+ * The Compact form stores positions as 16-bit integers and wall directions as
+ * two 16-bit angles; it has no mass scale, component properties beyond
+ * Number and Radius, wall velocities or particle velocities (those read back
+ * as zeros).  Files with orientation vectors (Direction) are read with the
+ * vectors skipped and written without them. */
+typedef struct {
+  int32_t form;             /* DEM_CFG_COMPLETE or DEM_CFG_COMPACT            */
+  int32_t version;          /* file version: 2 when written; 1 or 2 read      */
+  double length_scale, time_scale, mass_scale;
+  double time;              /* Time                                            */
+  double lbox[3];
+  int32_t nc, nw, np;
+  int32_t* number;          /* [nc] Component%Number (counts of each comp.)   */
+  double* radius;           /* [nc] Component%Radius ... %Inertia             */
+  double* density;
+  double* stiffness;
+  double* shear_stiffness;
+  double* mass;
+  double* inertia;
+  dem_wall_t* walls;        /* [nw] Wall(:) with lengths halved               */
+  int32_t* comp;            /* [np] Particle%Index (1-based)                  */
+  double* x;                /* [3][np] Position                               */
+  double* v;                /* [3][np] Velocity                               */
+  double* w;                /* [3][np] Omega                                  */
+} dem_config_t;
+int dem_config_write(const char* path, const dem_config_t* cfg);
+int dem_config_read(const char* path, dem_config_t* cfg);
+
+/* The driver's own records of a state file (SaveState, src/DEM.f90:947-1014):
+ * run bookkeeping before the configuration, random-number state and step
+ * counters after the contact list. */
+typedef struct {
+  int32_t finished;         /* Finished (2-byte logical: 1 = true)            */
+  char start_time[28];      /* StartTime                                       */
+  int32_t n_interruptions;  /* NumberOfInterruptions                           */
+  char* interruption;       /* [28*n] Interruption(:); read: NULL to query    */
+  char* retaking;           /* [28*n] Retaking(:)                              */
+  char finish_time[28];     /* FinishTime                                      */
+  int32_t hours, minutes, seconds, milisecs;  /* Duration (RealTime)        */
+  double aam;               /* random-number generator state (MUtil):        */
+  int32_t dummy, iix, iiy, iik;   /* aam, Dummy, iix, iiy, iik               */
+  int32_t conf;             /* Conf                                            */
+  int32_t next_step;        /* Step + 1                                        */
+  int32_t ncol;             /* Ncol                                            */
+} dem_run_t;
+
+/* The hot path's records of a state file: the cell lists
+ * (SaveCellStructure, src/MCells.f90:960-983), the ABM ring
+ * (SaveIntegrationVariables, src/MDEM.f90:52-68) and the contact lists
+ * (SaveContactList, src/MContacts.f90:75-118).  Reading fills what is not
+ * NULL after setting the counts. */
+typedef struct {
+  int32_t nprev;            /* NPrev (MethodOrder - 1); must be set to read   */
+  int32_t ipc[3];
+  double* ring;             /* [nprev][9][np], as dem_upload_state            */
+  int32_t ncell;            /* Nt                                              */
+  int32_t* cell_len;        /* [ncell] particles in each cell list            */
+  int32_t* cell_members;    /* [np] the lists one after another, list order   */
+  int64_t npairs;           /* pair contacts, keys (imin<<32)|imax ascending  */
+  uint64_t* keys;
+  double* ft;               /* [npairs][3] Vector1                             */
+  double* vt;               /* [npairs][3] Vector2                             */
+  int64_t nwall;            /* wall contacts, keys (i<<32)|wall ascending     */
+  uint64_t* wkeys;
+  double* wft;
+  double* wvt;
+} dem_state_sections_t;
+int dem_state_file_write(const char* path, const dem_run_t* run, const dem_config_t* cfg,
+                         const dem_state_sections_t* sec);
+/* cfg and sec are filled as by dem_config_read (Complete form); sec->nprev
+ * must be set by the caller (the file does not record it). */
+int dem_state_file_read(const char* path, dem_run_t* run, dem_config_t* cfg,
+                        dem_state_sections_t* sec);
+
+/* The state of a handle as a state file: particles, walls, Time, cell
+ * lists, ABM ring and contacts come from the device; meta gives what the
+ * library does not hold (scales, Component Density and stiffnesses; its
+ * array members other than those, and its particle arrays, are ignored;
+ * Component%Number is counted from the particles). */
+int dem_save_state_file(dem_handle_t h, const char* path, const dem_run_t* run,
+                        const dem_config_t* meta);
+/* Resume from a state file (ReadState, src/DEM.f90:1016-1110): uploads the
+ * particles (accelerations zero: the next MoveParticles recomputes them),
+ * ring, cell-list order and contacts, sets Time and next_step = Step + 1.
+ * The file's components and walls must be those of the handle (nc, nw).
+ * run and meta may be NULL; meta's arrays are filled where not NULL. */
+int dem_load_state_file(dem_handle_t h, const char* path, dem_run_t* run, dem_config_t* meta);
+/* Cell-list order of the last evaluation: within each cell, the particle
+ * with the largest rank heads the list (ranks 0..n-1 per cell). */
+int dem_download_cell_rank(dem_handle_t h, int32_t np, int32_t* rank);
 
 /* ---- batches ------------------------------------------------------------
  * No counterpart in the reference (it runs one system per process).  For

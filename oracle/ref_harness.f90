@@ -152,6 +152,181 @@ contains
     t = Time
   end function ref_get_time
 
+  ! What ReadSpecifications and ScaleVariables leave in MConfig for
+  ! SaveConfiguration to write: the scales, Time, the component and wall
+  ! stiffnesses, and Component%Number counted from the particles.
+  subroutine ref_set_meta( L, T, M, time_, stiff, shear, wstiff, wshear ) &
+      bind(C, name="ref_set_meta")
+    real(c_double), value :: L, T, M, time_
+    real(c_double), intent(in) :: stiff(NC), shear(NC), wstiff(max(NW,1)), wshear(max(NW,1))
+    integer :: i
+    LengthScale = L ; TimeScale = T ; MassScale = M ; Time = time_
+    do i = 1, NC
+      Component(i)%Stiffness = stiff(i)
+      Component(i)%ShearStiffness = shear(i)
+      Component(i)%Number = count( Particle(1:NP)%Index == i )
+    end do
+    do i = 1, NW
+      Wall(i)%Stiffness = wstiff(i)
+      Wall(i)%ShearStiffness = wshear(i)
+    end do
+  end subroutine ref_set_meta
+
+  ! SaveConfiguration( File, Form ) (src/MConfig.f90:87-105).
+  subroutine ref_save_configuration( path, n, form ) bind(C, name="ref_save_configuration")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n, form
+    character(len=1024) :: f
+    call c2f_string( path, n, f )
+    call SaveConfiguration( trim(f), int(form,1) )
+  end subroutine ref_save_configuration
+
+  ! ReadConfiguration( File ) (src/MConfig.f90:250-262), then the module
+  ! variables it set through the getters below.
+  subroutine ref_read_configuration( path, n ) bind(C, name="ref_read_configuration")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n
+    character(len=1024) :: f
+    call c2f_string( path, n, f )
+    call ReadConfiguration( trim(f) )
+  end subroutine ref_read_configuration
+
+  subroutine ref_get_config_scalars( s, counts ) bind(C, name="ref_get_config_scalars")
+    real(c_double), intent(out) :: s(7)
+    integer(c_int32_t), intent(out) :: counts(3)
+    s(1) = LengthScale ; s(2) = TimeScale ; s(3) = MassScale ; s(4) = Time
+    s(5:7) = Lbox
+    counts = [NC, NW, NP]
+  end subroutine ref_get_config_scalars
+
+  subroutine ref_get_components( nc_, number, props ) bind(C, name="ref_get_components")
+    integer(c_int32_t), value :: nc_
+    integer(c_int32_t), intent(out) :: number(nc_)
+    real(c_double), intent(out) :: props(6,nc_)
+    integer :: i
+    do i = 1, nc_
+      number(i) = Component(i)%Number
+      props(:,i) = [Component(i)%Radius, Component(i)%Density, Component(i)%Stiffness, &
+                    Component(i)%ShearStiffness, Component(i)%Mass, Component(i)%Inertia]
+    end do
+  end subroutine ref_get_components
+
+  ! Wall(j): shape, then L1 L2 L3 Stiffness ShearStiffness U W N Position
+  ! Velocity Omega.
+  subroutine ref_get_wall( j, shape, v ) bind(C, name="ref_get_wall")
+    integer(c_int32_t), value :: j
+    integer(c_int32_t), intent(out) :: shape
+    real(c_double), intent(out) :: v(23)
+    shape = Wall(j)%Shape
+    v(1:5) = [Wall(j)%L1, Wall(j)%L2, Wall(j)%L3, Wall(j)%Stiffness, Wall(j)%ShearStiffness]
+    v(6:8) = Wall(j)%U ; v(9:11) = Wall(j)%W ; v(12:14) = Wall(j)%N
+    v(15:17) = Wall(j)%Position ; v(18:20) = Wall(j)%Velocity ; v(21:23) = Wall(j)%Omega
+  end subroutine ref_get_wall
+
+  function ref_get_np() result( n ) bind(C, name="ref_get_np")
+    integer(c_int32_t) :: n
+    n = NP
+  end function ref_get_np
+
+  ! SaveState (src/DEM.f90:947-1014), whose run records are variables of the
+  ! main program: here they come in as arguments; the configuration, cell
+  ! lists, ring and contact lists are the reference's own routines.
+  subroutine ref_save_state_file( path, n, fin, start_time, nint, inter, ret, finish_time, &
+                                  hours, minutes, seconds, milisecs, aam_, dummy_, iix_, &
+                                  iiy_, iik_, conf, next_step ) &
+      bind(C, name="ref_save_state_file")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n, fin, nint, hours, minutes, seconds, milisecs
+    integer(c_int32_t), value :: dummy_, iix_, iiy_, iik_, conf, next_step
+    real(c_double), value :: aam_
+    character(kind=c_char), intent(in) :: start_time(28), finish_time(28)
+    character(kind=c_char), intent(in) :: inter(28,max(nint,1)), ret(28,max(nint,1))
+    character(len=1024) :: f
+    character(28) :: s
+    logical(lb) :: finished_
+    integer :: u, i
+    call c2f_string( path, n, f )
+    u = BinOpen( trim(f), 'write' )
+    finished_ = fin /= 0
+    call BinWrite( u, finished_ )
+    s = transfer( start_time, s ) ; call BinWrite( u, s )
+    call BinWrite( u, int(nint,ib) )
+    do i = 1, nint
+      s = transfer( inter(:,i), s ) ; call BinWrite( u, s )
+      s = transfer( ret(:,i), s ) ; call BinWrite( u, s )
+    end do
+    s = transfer( finish_time, s ) ; call BinWrite( u, s )
+    call BinWrite( u, int(hours,4) )
+    call BinWrite( u, int(minutes,1) )
+    call BinWrite( u, int(seconds,1) )
+    call BinWrite( u, int(milisecs,2) )
+    call SaveConfiguration( u, Complete )
+    call SaveCellDistribution( u )
+    call SaveIntegrationVariables( u )
+    call SaveContactList( u )
+    call BinWrite( u, real(aam_,8) )
+    call BinWrite( u, int(dummy_,4) )
+    call BinWrite( u, int(iix_,4) )
+    call BinWrite( u, int(iiy_,4) )
+    call BinWrite( u, int(iik_,4) )
+    call BinWrite( u, int(conf,ib) )
+    call BinWrite( u, int(next_step,ib) )
+    call BinWrite( u, Ncol )
+    call BinClose( u )
+  end subroutine ref_save_state_file
+
+  ! ReadState (src/DEM.f90:1016-1110) after ref_init: the run records are
+  ! returned (ints: finished, n_interruptions, hours, minutes, seconds,
+  ! milisecs, dummy, iix, iiy, iik, conf, next_step; aam), the rest goes into
+  ! the modules through the reference's own routines.
+  subroutine ref_read_state_file( path, n, ints, aam_, start_time, finish_time ) &
+      bind(C, name="ref_read_state_file")
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n
+    integer(c_int32_t), intent(out) :: ints(12)
+    real(c_double), intent(out) :: aam_
+    character(kind=c_char), intent(out) :: start_time(28), finish_time(28)
+    character(len=1024) :: f
+    character(28) :: s
+    logical(lb) :: finished_
+    integer(ib) :: nint_, i4
+    integer(1) :: i1a, i1b
+    integer(2) :: i2
+    integer(4) :: d4(4)
+    real(8) :: a8
+    integer :: u, i
+    call c2f_string( path, n, f )
+    u = BinOpen( trim(f), 'read' )
+    call BinRead( u, finished_ )
+    call BinRead( u, s ) ; start_time = transfer( s, start_time )
+    call BinRead( u, nint_ )
+    do i = 1, nint_
+      call BinRead( u, s )
+      call BinRead( u, s )
+    end do
+    call BinRead( u, s ) ; finish_time = transfer( s, finish_time )
+    call BinRead( u, i4 ) ; call BinRead( u, i1a ) ; call BinRead( u, i1b ) ; call BinRead( u, i2 )
+    call ReadConfiguration( u )
+    MaxNP = NP
+    call ConfigureWalls
+    call ReadCellDistribution( u )
+    call ReadIntegrationVariables( u )
+    call ReadContactList( u )
+    call BinRead( u, a8 )
+    do i = 1, 4
+      call BinRead( u, d4(i) )
+    end do
+    call BinRead( u, ints(11) )
+    call BinRead( u, ints(12) )
+    call BinRead( u, Ncol )
+    call BinClose( u )
+    ints(1) = merge(1, 0, finished_)
+    ints(2) = nint_
+    ints(3) = i4 ; ints(4) = i1a ; ints(5) = i1b ; ints(6) = i2
+    ints(7:10) = d4
+    aam_ = a8
+  end subroutine ref_read_state_file
+
   ! Tables derived by InitializeForce (src/MForces.f90:54-87).
   subroutine ref_get_tables( mindij_, mindijsq_, invmass_, invinertia_, effmass_, &
                              musq_, muwsq_ ) bind(C, name="ref_get_tables")

@@ -213,18 +213,43 @@ def _ref_job_body(job):
     grid = np.zeros(3, np.int32)
     lib.ref_get_grid(grid.ctypes.data_as(ip))
     out["grid"] = grid
-    n = int(job["comp"].shape[0])
-    comp = np.ascontiguousarray(job["comp"], np.int32)
-    cols = lambda a: _d(np.asarray(a).reshape(3, n).T)
-    x, v, w = cols(job["x"]), cols(job["v"]), cols(job["w"])
-    a = cols(job["a"]) if job.get("a") is not None else np.zeros((n, 3), order="F")
-    al = cols(job["alpha"]) if job.get("alpha") is not None else np.zeros((n, 3), order="F")
-    lib.ref_set_particles(C.c_int32(n), comp.ctypes.data_as(ip), P(x), P(v), P(w), P(a), P(al))
     scratch = os.path.join(ROOT, ".scratch")
     os.makedirs(scratch, exist_ok=True)
     tmp = tempfile.mkdtemp(prefix="ref_", dir=scratch)
     nprev = red.method_order - 1
-    if job.get("state") is not None:
+    if job.get("state_file") is not None:
+        # ReadState: the configuration, cell lists, ring and contacts from a file
+        path = job["state_file"].encode()
+        ints = np.zeros(12, np.int32)
+        aam = C.c_double()
+        st_, ft_ = C.create_string_buffer(28), C.create_string_buffer(28)
+        lib.ref_read_state_file(C.c_char_p(path), C.c_int32(len(path)), ints.ctypes.data_as(ip),
+                                C.byref(aam), st_, ft_)
+        lib.ref_get_np.restype = C.c_int32
+        n = int(lib.ref_get_np())
+        out["read_run"] = dict(ints=ints.copy(), aam=aam.value, start_time=st_.raw,
+                               finish_time=ft_.raw)
+        comp = np.zeros(n, np.int32)
+        cols = lambda a: _d(np.asarray(a).reshape(3, n).T)
+    else:
+        n = int(job["comp"].shape[0])
+        comp = np.ascontiguousarray(job["comp"], np.int32)
+        cols = lambda a: _d(np.asarray(a).reshape(3, n).T)
+        x, v, w = cols(job["x"]), cols(job["v"]), cols(job["w"])
+        a = cols(job["a"]) if job.get("a") is not None else np.zeros((n, 3), order="F")
+        al = cols(job["alpha"]) if job.get("alpha") is not None else np.zeros((n, 3), order="F")
+        lib.ref_set_particles(C.c_int32(n), comp.ctypes.data_as(ip), P(x), P(v), P(w), P(a),
+                              P(al))
+    if job.get("meta") is not None:
+        m = job["meta"]
+        lib.ref_set_meta(C.c_double(m["length_scale"]), C.c_double(m["time_scale"]),
+                         C.c_double(m["mass_scale"]), C.c_double(m.get("time", 0.0)),
+                         tab(m["stiffness"], nc), tab(m["shear_stiffness"], nc),
+                         tab(m["wall_stiffness"] if nw else [0.0], max(nw, 1)),
+                         tab(m["wall_shear_stiffness"] if nw else [0.0], max(nw, 1)))
+    if job.get("state_file") is not None:
+        pass
+    elif job.get("state") is not None:
         st = job["state"]
         path = os.path.join(tmp, "in.stt").encode()
         write_state_file(path, st["cells"], st["ipc"], st["ring"], st["pairs"], st["walls"], n)
@@ -272,6 +297,29 @@ def _ref_job_body(job):
             lib.ref_move_particles(C.c_int32(int(act[1])), C.c_int32(int(act[2])))
         elif kind == "snapshot":
             out["snapshots"].append(snapshot(act[1]))
+        elif kind == "save_config":
+            path = act[1].encode()
+            lib.ref_save_configuration(C.c_char_p(path), C.c_int32(len(path)), C.c_int32(act[2]))
+        elif kind == "read_config":
+            path = act[1].encode()
+            lib.ref_read_configuration(C.c_char_p(path), C.c_int32(len(path)))
+            out.setdefault("configs", []).append(_ref_config(lib))
+        elif kind == "save_state_file":
+            _, fpath, run = act
+            path = fpath.encode()
+            pad = lambda s_: C.create_string_buffer(s_.encode()[:28].ljust(28), 28)
+            ninter = len(run.get("interruption", []))
+            inter = C.create_string_buffer(b"".join(q.encode()[:28].ljust(28)
+                                                    for q in run.get("interruption", [])) or b" " * 28)
+            ret = C.create_string_buffer(b"".join(q.encode()[:28].ljust(28)
+                                                  for q in run.get("retaking", [])) or b" " * 28)
+            i32 = lambda k: C.c_int32(int(run.get(k, 0)))
+            lib.ref_save_state_file(C.c_char_p(path), C.c_int32(len(path)), i32("finished"),
+                                    pad(run.get("start_time", "")), C.c_int32(ninter), inter, ret,
+                                    pad(run.get("finish_time", "")), i32("hours"), i32("minutes"),
+                                    i32("seconds"), i32("milisecs"),
+                                    C.c_double(run.get("aam", 0.0)), i32("dummy"), i32("iix"),
+                                    i32("iiy"), i32("iik"), i32("conf"), i32("next_step"))
         elif kind == "time":
             import time as _t
             t0 = _t.perf_counter()
@@ -280,6 +328,39 @@ def _ref_job_body(job):
         else:
             raise ValueError(kind)
     shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
+def _ref_config(lib):
+    """The MConfig variables ReadConfiguration set, as a configuration dict."""
+    s = np.zeros(7)
+    counts = np.zeros(3, np.int32)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    lib.ref_get_config_scalars(s.ctypes.data_as(dp), counts.ctypes.data_as(ip))
+    nc, nw, n = (int(q) for q in counts)
+    number = np.zeros(nc, np.int32)
+    props = np.zeros((nc, 6))
+    lib.ref_get_components(C.c_int32(nc), number.ctypes.data_as(ip), props.ctypes.data_as(dp))
+    walls = []
+    for j in range(nw):
+        shape = C.c_int32()
+        v = np.zeros(23)
+        lib.ref_get_wall(C.c_int32(j + 1), C.byref(shape), v.ctypes.data_as(dp))
+        walls.append(dict(shape=shape.value, l1=v[0], l2=v[1], l3=v[2], stiffness=v[3],
+                          shear_stiffness=v[4], u=v[5:8], w=v[8:11], n=v[11:14],
+                          position=v[14:17], velocity=v[17:20], omega=v[20:23]))
+    comp = np.zeros(n, np.int32)
+    xs, vs, ws, as_, als = (np.zeros((n, 3), order="F") for _ in range(5))
+    lib.ref_get_particles(C.c_int32(n), comp.ctypes.data_as(ip), xs.ctypes.data_as(dp),
+                          vs.ctypes.data_as(dp), ws.ctypes.data_as(dp), as_.ctypes.data_as(dp),
+                          als.ctypes.data_as(dp))
+    out = dict(length_scale=s[0], time_scale=s[1], mass_scale=s[2], time=s[3], lbox=s[4:7],
+               number=number, comp=comp, x=xs.T.copy(), v=vs.T.copy(), w=ws.T.copy(),
+               walls=walls)
+    for k, name in enumerate(("radius", "density", "stiffness", "shear_stiffness", "mass",
+                              "inertia")):
+        out[name] = props[:, k].copy()
     return out
 
 

@@ -16,6 +16,11 @@ stores inputs and outputs as .npz (numpy arrays only, no pickles):
                     shaken on a circle, MoveWalls) with its 39 balls placed
                     instead of poured: start-up + 1500 steps, wall positions,
                     velocities and Time at each snapshot
+  ref_resume.stt    a state file written by the reference's SaveState sequence
+                    (run records, Complete configuration, cell lists, ring,
+                    contact lists) after start-up + 20 steps of 48 spheres
+  ref_resume.npz    the reference's continuation from ref_resume.stt
+                    (ReadState + 15 MoveParticles)
 
 Usage (in a container that has the reference checkout and the built library):
     python tests/golden/make_golden.py [fixture.npz ...]   (default: all)
@@ -142,6 +147,39 @@ def capped_workload(seed=13):
     return workloads.Workload("capped", spec, red, np.ones(n, np.int32), x, v, v.copy(), 0)
 
 
+def resume_fixture():
+    wl = workloads.small_box(n_side=(4, 4, 3), nc=2, seed=61, vel=0.4)
+    red = wl.red
+    rng = np.random.default_rng(61)
+    meta = dict(length_scale=red.length_scale, time_scale=red.time_scale,
+                mass_scale=red.mass_scale, time=0.0,
+                stiffness=rng.uniform(1.0, 2.0, red.nc),
+                shear_stiffness=rng.uniform(0.2, 0.5, red.nc),
+                wall_stiffness=rng.uniform(1.0, 2.0, red.nw),
+                wall_shear_stiffness=rng.uniform(0.2, 0.5, red.nw))
+    stt = os.path.join(HERE, "ref_resume.stt")
+    run = dict(finished=0, start_time="Mon Dec  1 10:00:00 2026", n_interruptions=1,
+               interruption=["Mon Dec  1 11:00:00 2026"], retaking=["Mon Dec  1 11:05:00 2026"],
+               finish_time="", hours=1, minutes=2, seconds=3, milisecs=456, aam=0.25, dummy=-7,
+               iix=11, iiy=13, iik=17, conf=3, next_step=24)
+    run_reference(dict(red=red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, meta=meta,
+                       actions=[("euler",), ("euler",), ("euler",), ("step", 4, 20),
+                                ("save_state_file", stt, run)]))
+    res = run_reference(dict(red=red, state_file=stt, actions=[("step", 24, 15),
+                                                               ("snapshot", "r")]))
+    snap = res["snapshots"][0]
+    out = {"next_step": np.array(24), "resume_steps": np.array(15),
+           "note": np.array("small_box(n_side=(4,4,3), nc=2, seed=61, vel=0.4): start-up + 20 "
+                            "steps saved; the reference resumed 15 steps")}
+    pack_reduced(red, out)
+    for k in ("x", "v", "w", "a", "alpha"):
+        out[f"r_{k}"] = snap[k]
+    out["r_keys"] = contacts_to_keys(snap["state"]["pairs"])[0]
+    path = os.path.join(HERE, "ref_resume.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {stt} ({os.path.getsize(stt)} B) and {path}")
+
+
 def main(names):
     def want(name):
         return not names or name in names
@@ -167,6 +205,8 @@ def main(names):
                  [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
                   ("snapshot", "s")],
                  "cone hopper + floor, 216 spheres")
+    if want("ref_resume.npz"):
+        resume_fixture()
     if want("ref_ball_bearings.npz"):
         wl = workloads.ball_bearings()
         run_case("ref_ball_bearings.npz", wl,

@@ -30,7 +30,8 @@ def c_sizes(tmp_path):
     src.write_text('#include <stdio.h>\n#include "dem3d.h"\nint main(void){\n'
                    + "".join(f'printf("sizeof {t} %zu\\n", sizeof({t}));\n'
                              for t in ("dem_params_t", "dem_wall_t", "dem_particle_t",
-                                       "dem_stats_t", "dem_wall_motion_t"))
+                                       "dem_stats_t", "dem_wall_motion_t", "dem_config_t",
+                                       "dem_run_t", "dem_state_sections_t"))
                    + "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
@@ -56,6 +57,9 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert C.sizeof(native.WallT) == cs["dem_wall_t"]
     assert C.sizeof(native.Stats) == cs["dem_stats_t"]
     assert C.sizeof(native.WallMotionT) == cs["dem_wall_motion_t"]
+    assert C.sizeof(native.ConfigT) == cs["dem_config_t"]
+    assert C.sizeof(native.RunT) == cs["dem_run_t"]
+    assert C.sizeof(native.SectionsT) == cs["dem_state_sections_t"]
 
 
 @pytest.mark.skipif(shutil.which("amdflang") is None or not os.path.exists(LIB),

@@ -1,0 +1,218 @@
+"""Configuration (.cfg) and state (.stt) files in the reference's binary
+layout (dem_config_*, dem_state_file_*: host code of the product library,
+no device needed).
+
+Pinned to the reference's own writers and readers (SaveConfiguration,
+ReadConfiguration, and SaveState / ReadState around SaveCellStructure,
+SaveIntegrationVariables, SaveContactList) where oracle/_ref is built:
+the bytes we write equal the bytes the reference writes for the same data,
+each side reads the other's files to the same values, and a run resumed by
+the reference from a file we wrote continues bitwise like the oracle
+resumed from the same file.  Without the reference build, the committed
+fixture tests/golden/ref_resume.stt (written by the reference) still pins the
+reader: the oracle resumed from it reproduces the reference's continuation
+stored in tests/golden/ref_resume.npz bit for bit.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from dem3d_amd import files, workloads
+from dem3d_amd.native import load_hip
+from checkers import (cells_to_rank, contacts_to_keys, oracle_system, ref_available,
+                      run_reference)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+live = pytest.mark.skipif(not ref_available(), reason="reference build not present")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return load_hip()
+
+
+def _case(seed=51):
+    wl = workloads.small_box(n_side=(5, 4, 4), nc=2, seed=seed, vel=0.4)
+    red = wl.red
+    nc = red.nc
+    rng = np.random.default_rng(seed)
+    meta = dict(length_scale=red.length_scale, time_scale=red.time_scale,
+                mass_scale=red.mass_scale, time=1.25,
+                stiffness=rng.uniform(1.0, 2.0, nc), shear_stiffness=rng.uniform(0.2, 0.5, nc),
+                wall_stiffness=rng.uniform(1.0, 2.0, red.nw),
+                wall_shear_stiffness=rng.uniform(0.2, 0.5, red.nw))
+    return wl, meta
+
+
+def _cfg(wl, meta, form, x=None, v=None, w=None):
+    red = wl.red
+    walls = []
+    for j, wd in enumerate(red.walls):
+        d = dict(wd)
+        d["stiffness"] = meta["wall_stiffness"][j]
+        d["shear_stiffness"] = meta["wall_shear_stiffness"][j]
+        walls.append(d)
+    return dict(form=form, length_scale=meta["length_scale"], time_scale=meta["time_scale"],
+                mass_scale=meta["mass_scale"], time=meta["time"], lbox=red.lbox,
+                radius=red.radius, density=red.density, stiffness=meta["stiffness"],
+                shear_stiffness=meta["shear_stiffness"], mass=red.mass, inertia=red.inertia,
+                walls=walls, comp=wl.comp, x=wl.x if x is None else x,
+                v=wl.v if v is None else v, w=wl.w if w is None else w)
+
+
+def _same_config(a, b, full=True):
+    for k in ("length_scale", "time_scale", "time", "lbox", "number", "radius", "comp", "x"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    if full:
+        for k in ("mass_scale", "density", "stiffness", "shear_stiffness", "mass", "inertia",
+                  "v", "w"):
+            assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    assert len(a["walls"]) == len(b["walls"])
+    keys = ("u", "w", "n", "position") + (("velocity", "omega") if full else ())
+    for wa, wb in zip(a["walls"], b["walls"]):
+        assert wa["shape"] == wb["shape"]
+        for k in ("l1", "l2", "l3") + (("stiffness", "shear_stiffness") if full else ()):
+            assert wa[k] == wb[k], k
+        for k in keys:
+            assert np.array_equal(np.asarray(wa[k]), np.asarray(wb[k])), k
+
+
+def test_config_round_trip(lib, tmp_path):
+    wl, meta = _case()
+    cfg = _cfg(wl, meta, files.COMPLETE)
+    p = str(tmp_path / "a.cfg")
+    files.write_config(lib, p, cfg)
+    back = files.read_config(lib, p)
+    assert back["form"] == files.COMPLETE and back["version"] == 2
+    back_cfg = dict(back)
+    cfg["number"] = np.bincount(wl.comp, minlength=wl.red.nc + 1)[1:]
+    _same_config(back_cfg, cfg)
+    # Compact: 16-bit positions within half a quantum of the originals
+    q = str(tmp_path / "b.cfg")
+    files.write_config(lib, q, _cfg(wl, meta, files.COMPACT))
+    c = files.read_config(lib, q)
+    quantum = 0.5 * max(wl.red.lbox) / 32766.0
+    order = np.argsort(wl.comp, kind="stable")   # Compact groups the particles by component
+    assert np.abs(c["x"] - wl.x[:, order]).max() <= quantum
+    assert np.array_equal(c["comp"], wl.comp[order])
+
+
+def test_config_rejects_inconsistent_counts(lib, tmp_path):
+    wl, meta = _case()
+    cfg = _cfg(wl, meta, files.COMPLETE)
+    cfg["number"] = [1] * wl.red.nc
+    with pytest.raises(Exception, match="number"):
+        files.write_config(lib, str(tmp_path / "bad.cfg"), cfg)
+    with open(tmp_path / "junk.cfg", "wb") as f:
+        f.write(b"NOPE" + bytes(64))
+    with pytest.raises(Exception, match="not a configuration"):
+        files.read_config(lib, str(tmp_path / "junk.cfg"))
+
+
+@live
+@pytest.mark.parametrize("form", [files.COMPLETE, files.COMPACT])
+def test_config_bytes_equal_reference(lib, tmp_path, form):
+    wl, meta = _case()
+    ref_path = str(tmp_path / "ref.cfg")
+    ours = str(tmp_path / "ours.cfg")
+    run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, meta=meta,
+                       actions=[("save_config", ref_path, form)]))
+    files.write_config(lib, ours, _cfg(wl, meta, form))
+    a, b = open(ref_path, "rb").read(), open(ours, "rb").read()
+    assert len(a) == len(b) and a == b
+    # each side reads the other's file to the same values
+    res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w,
+                             actions=[("read_config", ours)]))
+    theirs = res["configs"][0]
+    mine = files.read_config(lib, ref_path)
+    _same_config(mine, theirs, full=form == files.COMPLETE)
+
+
+def _run_record():
+    return dict(finished=0, start_time="Mon Dec  1 10:00:00 2026", n_interruptions=2,
+                interruption=["Mon Dec  1 11:00:00 2026", "Tue Dec  2 09:30:00 2026"],
+                retaking=["Mon Dec  1 11:05:00 2026", "Tue Dec  2 09:31:00 2026"],
+                finish_time="Wed Dec  3 08:00:00 2026", hours=21, minutes=59, seconds=58,
+                milisecs=997, aam=4.656612873077392578125e-10, dummy=-20, iix=123456789,
+                iiy=987654321, iik=42, conf=7)
+
+
+@live
+def test_state_file_interchange_with_reference(lib, tmp_path):
+    wl, meta = _case()
+    red = wl.red
+    nprev = red.method_order - 1
+    ref_stt = str(tmp_path / "ref.stt")
+    run = dict(_run_record(), next_step=30)
+    res = run_reference(dict(red=red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, meta=meta,
+                             actions=[("euler",), ("euler",), ("euler",), ("step", 4, 26),
+                                      ("snapshot", "m"), ("save_state_file", ref_stt, run)]))
+    snap = res["snapshots"][0]
+    # our reader on the reference's file
+    r, cfg, sec = files.read_state_file(lib, ref_stt, nprev)
+    for k, v in run.items():
+        got = r[k]
+        if isinstance(v, list):
+            assert [s.rstrip() for s in got] == v, k
+        elif isinstance(v, str):
+            assert got.rstrip() == v, k
+        else:
+            assert got == v, k
+    assert r["ncol"] == snap["ncol"]
+    for k in ("x", "v", "w"):
+        assert np.array_equal(cfg[k], snap[k]), k
+    assert cfg["time"] == snap["time"]
+    st = snap["state"]
+    assert sec["cells"] == [list(c) for c in st["cells"]]
+    assert np.array_equal(sec["ipc"], st["ipc"])
+    assert np.array_equal(sec["ring"], st["ring"])
+    keys, ft, vt = contacts_to_keys(st["pairs"])
+    assert np.array_equal(sec["keys"], keys) and np.array_equal(sec["ft"], ft)
+    wk, wft, wvt = contacts_to_keys(st["walls"])
+    assert np.array_equal(sec["wkeys"], wk) and np.array_equal(sec["wvt"], wvt)
+    assert len(keys) > 10
+    # our writer reproduces the reference's bytes from what we read
+    ours = str(tmp_path / "ours.stt")
+    files.write_state_file(lib, ours, r, cfg, sec)
+    assert open(ours, "rb").read() == open(ref_stt, "rb").read()
+    # the reference resumes from our file exactly as the oracle does
+    post = run_reference(dict(red=red, state_file=ours,
+                              actions=[("step", 30, 15), ("snapshot", "r")]))
+    assert post["read_run"]["ints"][11] == 30                    # InitialStep
+    o = oracle_system(red)
+    o.red = red
+    o.upload(cfg["comp"], cfg["x"], cfg["v"], cfg["w"], None, None, sec["ring"], sec["ipc"],
+             files.cell_rank_from_lists(sec["cells"], cfg["comp"].shape[0]))
+    o.upload_contacts(sec["keys"], sec["ft"], sec["vt"], sec["wkeys"], sec["wft"], sec["wvt"])
+    o.step(15, 30)
+    so, sr = o.download(), post["snapshots"][0]
+    for k in ("x", "v", "w", "a", "alpha"):
+        assert np.array_equal(so[k], sr[k]), k
+    assert np.array_equal(o.download_contacts()["keys"],
+                          contacts_to_keys(sr["state"]["pairs"])[0])
+
+
+def test_resume_from_reference_state_fixture(lib):
+    """tests/golden/ref_resume.stt was written by the reference's SaveState
+    sequence; ref_resume.npz holds the reference's own continuation from it
+    (ReadState + 15 MoveParticles).  Read with our reader, the oracle
+    continues bit for bit."""
+    f = np.load(os.path.join(GOLDEN, "ref_resume.npz"), allow_pickle=False)
+    from test_oracle import reduced_from_fixture
+    red = reduced_from_fixture(f)
+    nprev = red.method_order - 1
+    r, cfg, sec = files.read_state_file(lib, os.path.join(GOLDEN, "ref_resume.stt"), nprev)
+    assert r["next_step"] == int(f["next_step"])
+    o = oracle_system(red)
+    o.red = red
+    o.upload(cfg["comp"], cfg["x"], cfg["v"], cfg["w"], None, None, sec["ring"], sec["ipc"],
+             files.cell_rank_from_lists(sec["cells"], cfg["comp"].shape[0]))
+    o.upload_contacts(sec["keys"], sec["ft"], sec["vt"], sec["wkeys"], sec["wft"], sec["wvt"])
+    o.step(int(f["resume_steps"]), r["next_step"])
+    st = o.download()
+    for k in ("x", "v", "w", "a", "alpha"):
+        assert np.array_equal(st[k], f[f"r_{k}"]), k
+    assert np.array_equal(o.download_contacts()["keys"], f["r_keys"])
