@@ -88,6 +88,23 @@ class Stats(C.Structure):
         return out
 
 
+class RngT(C.Structure):
+    _fields_ = [("aam", C.c_double), ("dummy", C.c_int32), ("iix", C.c_int32),
+                ("iiy", C.c_int32), ("iik", C.c_int32)]
+
+    @classmethod
+    def seeded(cls, seed: int):
+        """The generator as a run starts it: Dummy = the specification's seed
+        (negative), iix = iiy = -1 (src/MUtil.f90:10-11)."""
+        return cls(0.0, int(seed), -1, -1, 0)
+
+
+class PourT(C.Structure):
+    _fields_ = [("target_number", C.POINTER(C.c_int32)), ("shape", C.c_int32),
+                ("max_attempts", C.c_int32), ("size", C.c_double * 2),
+                ("length_scale", C.c_double), ("time_scale", C.c_double)]
+
+
 class ConfigT(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("version", C.c_int32), ("length_scale", C.c_double),
@@ -163,6 +180,9 @@ class Library:
         "download_walls": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallT)]),
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
+        "ranf": (C.c_double, [C.POINTER(RngT)]),
+        "pour": (C.c_int, [C.c_void_p, C.POINTER(PourT), C.POINTER(RngT), _ip]),
+        "remove_escaped": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
         "config_write": (C.c_int, [C.c_char_p, C.POINTER(ConfigT)]),
         "config_read": (C.c_int, [C.c_char_p, C.POINTER(ConfigT)]),
@@ -450,6 +470,26 @@ class System:
             return done.value, esc.value
         self._check(rc)
         return done.value, esc.value
+
+    def pour(self, target, shape, size, max_attempts, rng: "RngT", length_scale, time_scale):
+        """One PourParticle; returns 1 when a particle was inserted (rng advances)."""
+        tn = np.ascontiguousarray(target, np.int32)
+        p = PourT(tn.ctypes.data_as(_ip), int(shape), int(max_attempts),
+                  (C.c_double * 2)(*(list(size) + [0.0] * (2 - len(size)))),
+                  float(length_scale), float(time_scale))
+        ins = C.c_int32()
+        self._check(self.lib.pour(self.h, C.byref(p), C.byref(rng), C.byref(ins)))
+        self.np += ins.value
+        return ins.value
+
+    def remove_escaped(self):
+        """UpdateParticleList; returns the indices each removal freed."""
+        cap = max(self.np, 1)
+        out = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        self._check(self.lib.remove_escaped(self.h, cap, out.ctypes.data_as(_ip), C.byref(n)))
+        self.np -= n.value
+        return out[: n.value].copy()
 
     def synchronize(self):
         if hasattr(self.lib, "synchronize"):

@@ -257,10 +257,32 @@ def ball_bearings(seed: int = 20240006, steps: int = 2000, n: int = 39,
                     pts.append((cx + px, cy + py, z0 + layer * a))
     if len(pts) < n:
         raise ValueError(f"at most {len(pts)} balls fit the start layers")
-    pos = np.array(pts[:n]).T + rng.uniform(-1e-4, 1e-4, (3, n))
+    pos = np.array(pts[:n], dtype=np.float64).reshape(n, 3).T + rng.uniform(-1e-4, 1e-4, (3, n))
     zero = np.zeros((3, n))
     return Workload(f"ball-bearings-{n}", spec, red, np.ones(n, np.int32), pos / L, zero.copy(),
                     zero.copy(), steps, note="example input, balls placed instead of poured")
+
+
+def discharge(seed: int = 20240007, n_side=(8, 8, 6), gravity: float = -20000.0,
+              steps: int = 600) -> Workload:
+    """A bed with an open bottom (no floor, periodic in x and y) falling out of
+    the box under strong gravity, for the removal of escaped particles
+    (UpdateParticleList).  The particles of the two lowest layers, which
+    leave first, hold the last indices in random order, so the last particle
+    -- the one that takes a removed particle's place -- is usually a
+    neighbour of the one leaving, with contacts to rename."""
+    wl = small_box(n_side=n_side, nc=2, ratio=1.2, seed=seed, walls=False,
+                   pbc=(True, True, False), vel=0.2, spacing=1.01, gravity=gravity, e=0.5,
+                   steps=steps)
+    rng = np.random.default_rng(seed)
+    # the two lowest layers take the last indices, in random order; the rest
+    # are shuffled below them
+    z = wl.x[2]
+    low = np.nonzero(z < np.sort(np.unique(np.round(z, 3)))[0] + 1.5)[0]
+    high = np.setdiff1d(np.arange(wl.np), low)
+    perm = np.concatenate([rng.permutation(high), rng.permutation(low)])
+    return Workload(f"discharge-{wl.np}", wl.spec, wl.red, wl.comp[perm], wl.x[:, perm],
+                    wl.v[:, perm], wl.w[:, perm], steps, note="open bottom, shuffled indices")
 
 
 def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,

@@ -261,6 +261,43 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *   "next_step"         sets the step number of the next batched step */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
+/* ---- particle number changes --------------------------------------------- */
+/* State of the reference's random-number generator (MUtil: aam, Dummy, iix,
+ * iiy, iik).  A run starts with dummy = the (negative) seed of the
+ * specification and iix = iiy = -1. */
+typedef struct {
+  double aam;
+  int32_t dummy, iix, iiy, iik;
+} dem_rng_t;
+/* ranf(Dummy) (src/MUtil.f90:16-44): the next deviate in (0, 1). */
+double dem_ranf(dem_rng_t* rng);
+
+#define DEM_POUR_RECTANGLE 1
+#define DEM_POUR_CIRCLE 2
+/* A pouring specification (the .prg file, src/DEM.f90:749-803), reduced. */
+typedef struct {
+  const int32_t* target_number;  /* [nc] TargetNumber                         */
+  int32_t shape;                 /* DEM_POUR_RECTANGLE or DEM_POUR_CIRCLE     */
+  int32_t max_attempts;          /* MaxAttempts                               */
+  double size[2];                /* PouringSize (1 value for a circle)        */
+  double length_scale, time_scale;  /* for the inserted velocity            */
+} dem_pour_t;
+/* One PourParticle (src/MDEM.f90:412-536): picks a component short of its
+ * target and up to max_attempts positions on the pouring surface at the top
+ * of the box, and inserts the particle (InsertParticle, :540-562) as number
+ * NP+1 at the head of its cell list, when none of the tests fails.  rng
+ * advances as the reference's generator does.  *inserted = 1 when a particle
+ * was added.  The driver calls it before the steps with mod(Step, IPouring)
+ * == 0 (src/DEM.f90:115-123). */
+int dem_pour(dem_handle_t h, const dem_pour_t* pour, dem_rng_t* rng, int32_t* inserted);
+/* UpdateParticleList (src/MDEM.f90:612-671) on the resident state: every
+ * particle outside the box is removed, from the last index down, the last
+ * particle taking its place (cell-list position, ABM ring and contact
+ * lists renamed as the reference renames them).  removed[0..cap) receives
+ * the 1-based index each removal freed, in order; *nremoved the count.
+ * Call it after dem_step returned DEM_ERR_ESCAPE, then step on. */
+int dem_remove_escaped(dem_handle_t h, int32_t cap, int32_t* removed, int32_t* nremoved);
+
 /* ---- files in the reference's binary formats ----------------------------
  * BinFiles byte streams (src/BinFiles.f90): every value as its raw bytes in
  * the order written, strings without a length, no record markers.  The

@@ -81,6 +81,8 @@ struct orc_system {
   int* nbr_cells;   /* [nt*13] neighbour cells, list order */
   int* n_nbr_cells; /* [nt]                                */
   int* cell_index;  /* [np] 1-based CellIndex              */
+  int cap;          /* particles the arrays hold (>= max_np) */
+  int max_np;       /* MaxNP (src/MDEM.f90:22): size of MainParticle */
   /* particles and integrator */
   int np;
   particle_t* part;
@@ -679,7 +681,7 @@ static void rotate_ipc(orc_handle_t h) {
 }
 
 static double* prev_slot(orc_handle_t h, int slot1, int i) {
-  return &h->prev[((size_t)(slot1 - 1) * h->np + i) * 9];
+  return &h->prev[((size_t)i * 3 + (slot1 - 1)) * 9];
 }
 
 static void store_ring(orc_handle_t h) {
@@ -721,7 +723,9 @@ static int move_particles(orc_handle_t h, int64_t step) {
     }
     particle_t* q = &h->part[i];
     particle_t* r = &h->pred[i];
-    r->index = q->index;
+    /* Predicted(i)%Index is not assigned here: it keeps what InsertParticle,
+     * ResizeVariables or ReadIntegrationVariables last set, which after a
+     * removal can be the component of the particle that left slot i. */
     for (int d = 0; d < 3; d++) r->position[d] = q->position[d] + h->ab0 * q->velocity[d] + sv[d];
     for (int d = 0; d < 3; d++) r->velocity[d] = q->velocity[d] + h->ab0 * q->acceleration[d] + sa[d];
     for (int d = 0; d < 3; d++) r->omega[d] = q->omega[d] + h->ab0 * q->alpha[d] + sl[d];
@@ -789,6 +793,289 @@ static int count_escaped(orc_handle_t h) {
     n += out;
   }
   return n;
+}
+
+/* ---- particle number changes (MDEM, MCells, MContacts) --------------------- */
+
+/* Room for n particles in every per-particle array, contents kept. */
+static void reserve_particles(orc_handle_t h, int n) {
+  if (n <= h->cap) return;
+  const int old = h->cap;
+  const size_t c = (size_t)n;
+  h->part = (particle_t*)realloc(h->part, sizeof(particle_t) * c);
+  h->pred = (particle_t*)realloc(h->pred, sizeof(particle_t) * c);
+  h->prev = (double*)realloc(h->prev, sizeof(double) * 27 * c);
+  h->cell_next = (int*)realloc(h->cell_next, sizeof(int) * c);
+  h->cell_index = (int*)realloc(h->cell_index, sizeof(int) * c);
+  h->force = (double*)realloc(h->force, sizeof(double) * 3 * c);
+  h->torque = (double*)realloc(h->torque, sizeof(double) * 3 * c);
+  h->fscale = (double*)realloc(h->fscale, sizeof(double) * 2 * c);
+  h->pairs = (plist_t*)realloc(h->pairs, sizeof(plist_t) * c);
+  h->walls = (plist_t*)realloc(h->walls, sizeof(plist_t) * c);
+  memset(h->pairs + old, 0, sizeof(plist_t) * (c - old));
+  memset(h->walls + old, 0, sizeof(plist_t) * (c - old));
+  memset(h->part + old, 0, sizeof(particle_t) * (c - old));
+  memset(h->pred + old, 0, sizeof(particle_t) * (c - old));
+  memset(h->prev + 27 * (size_t)old, 0, sizeof(double) * 27 * (c - old));
+  h->cap = n;
+}
+
+/* ResizeVariables / ResizeContactList (src/MDEM.f90:566-608,
+ * src/MContacts.f90:53-71): MaxNP becomes m and MainParticle keeps the lists
+ * of particles 1..NP only (the new elements start null). */
+static void resize_variables(orc_handle_t h, int m) {
+  for (int i = h->np; i < h->cap; i++) {
+    plist_free(&h->pairs[i]);
+    plist_free(&h->walls[i]);
+  }
+  h->max_np = m;
+  reserve_particles(h, m);
+  for (int i = 0; i < h->np; i++) h->pred[i].index = h->part[i].index;
+}
+
+/* The Monte Carlo neighbour list of a cell (0-based): HinderNeighbourCellsList,
+ * NeighbourCells(iCell, MonteCarlo, ...) and AllNeighbourParticles
+ * (src/MCells.f90:340-374,424-476,833-860,1056-1074).  The cell's own list,
+ * then the particles of each of the (up to 26) cells around it, every kCell /= 0
+ * of AddAllNeighbours3D, in the head-pushed (reversed) order of AddNeighbour.
+ * 0-based particle indices into out (grown as needed); returns the count. */
+static int mc_neighbour_particles(orc_handle_t h, int icell, int** out, int* cap) {
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const int iz = icell / (nx * ny);
+  const int iy = icell / nx - iz * ny;
+  const int ix = icell - (iy + iz * ny) * nx;
+  int xi, xf, yi, yf, zi, zf;
+  verify_pbc(&xi, &xf, ix, nx, h->pbc[0]);
+  verify_pbc(&yi, &yf, iy, ny, h->pbc[1]);
+  verify_pbc(&zi, &zf, iz, nz, h->pbc[2]);
+  int cells[27], nc = 0;
+  for (int kz = zi; kz <= zf; kz++) {
+    const int jz = apply_pbc(iz + kz, nz, h->pbc[2]);
+    for (int ky = yi; ky <= yf; ky++) {
+      const int jy = apply_pbc(iy + ky, ny, h->pbc[1]);
+      for (int kx = xi; kx <= xf; kx++) {
+        if (kx + nx * (ky + ny * kz) == 0) continue;
+        const int jx = apply_pbc(ix + kx, nx, h->pbc[0]);
+        cells[nc++] = jx + nx * (jy + ny * jz);
+      }
+    }
+  }
+  int n = 0;
+  for (int k = -1; k < nc; k++) {
+    const int c = k < 0 ? icell : cells[nc - 1 - k];
+    for (int q = h->cell_head[c]; q >= 0; q = h->cell_next[q]) {
+      if (n == *cap) {
+        *cap = *cap ? 2 * *cap : 256;
+        *out = (int*)realloc(*out, sizeof(int) * *cap);
+      }
+      (*out)[n++] = q;
+    }
+  }
+  return n;
+}
+
+/* RemoveParticleFromCell (src/MCells.f90:702-724). */
+static void remove_from_cell(orc_handle_t h, int i, int icell) {
+  int* link = &h->cell_head[icell];
+  while (*link != i) link = &h->cell_next[*link];
+  *link = h->cell_next[i];
+}
+
+/* ReplaceParticleIndexInCell (src/MCells.f90:728-743): the entry keeps its
+ * place in the list. */
+static void replace_in_cell(orc_handle_t h, int old, int icell, int now) {
+  int* link = &h->cell_head[icell];
+  while (*link != old) link = &h->cell_next[*link];
+  *link = now;
+  h->cell_next[now] = h->cell_next[old];
+}
+
+/* RemoveParticleFromContactList (src/MContacts.f90:579-633). */
+static void remove_from_contacts(orc_handle_t h, int i, const int* nb, int n) {
+  for (int jj = 0; jj < n; jj++) {
+    plist_t* l = &h->pairs[nb[jj]];
+    if (l->n == 0) continue;
+    int pos = 0;
+    while (l->idx[pos] < i && pos + 1 < l->n) pos++;
+    if (l->idx[pos] == i) plist_remove(l, pos);
+  }
+  h->pairs[i].n = 0;
+  h->walls[i].n = 0;
+}
+
+/* ChangeIndexInContactList (src/MContacts.f90:173-252): particle NP (here
+ * old) becomes i.  In the lists of the neighbours j < i, a last entry old
+ * is renamed i and moved behind the last earlier entry below i; a last entry
+ * old in the list of a neighbour j > i moves to the list of i as partner j,
+ * in neighbour order.  Wall contacts move with the particle. */
+static void change_index_in_contacts(orc_handle_t h, int old, int i, const int* nb, int n) {
+  plist_t first;
+  memset(&first, 0, sizeof first);
+  for (int jj = 0; jj < n; jj++) {
+    const int j = nb[jj];
+    plist_t* l = &h->pairs[j];
+    if (l->n == 0) continue;
+    const int last = l->n - 1;
+    if (l->idx[last] != old) continue;
+    double v1[3], v2[3];
+    memcpy(v1, l->v1 + 3 * last, sizeof v1);
+    memcpy(v2, l->v2 + 3 * last, sizeof v2);
+    if (j < i) {
+      if (last == 0) {
+        l->idx[0] = i;
+      } else {
+        int point = -1;
+        for (int k = 0; k < last; k++)
+          if (l->idx[k] < i) point = k;
+        plist_remove(l, last);
+        plist_insert(l, point + 1, i, v1, v2);
+      }
+    } else {
+      plist_remove(l, last);
+      plist_insert(&first, first.n, j, v1, v2);
+    }
+  }
+  plist_free(&h->pairs[i]);
+  h->pairs[i] = first;
+  plist_free(&h->walls[i]);
+  h->walls[i] = h->walls[old];
+  memset(&h->walls[old], 0, sizeof(plist_t));
+}
+
+/* UpdateParticleList (src/MDEM.f90:612-671): particles outside the box are
+ * removed, i = NP down to 1; the last particle takes the place of each. */
+int orc_remove_escaped(orc_handle_t h, int32_t cap, int32_t* removed, int32_t* nremoved) {
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  int count = 0;
+  int* nb = NULL;
+  int nbcap = 0;
+  for (int i = h->np - 1; i >= 0; i--) {
+    const double* r = h->part[i].position;
+    int out = 0;
+    for (int d = 0; d < 3; d++) out |= (r[d] > h->phalf[d]) || (r[d] < h->mhalf[d]);
+    if (!out) continue;
+    const int icell = h->cell_index[i] - 1;
+    const int n = mc_neighbour_particles(h, icell, &nb, &nbcap);
+    remove_from_cell(h, i, icell);
+    remove_from_contacts(h, i, nb, n);
+    const int last = h->np - 1;
+    if (i != last) {
+      h->part[i] = h->part[last];
+      memcpy(&h->prev[(size_t)i * 27], &h->prev[(size_t)last * 27], sizeof(double) * 27);
+      replace_in_cell(h, last, h->cell_index[last] - 1, i);
+      h->cell_index[i] = h->cell_index[last];
+      change_index_in_contacts(h, last, i, nb, n);
+    }
+    h->np -= 1;
+    if (removed && count < cap) removed[count] = i + 1;
+    count++;
+    if (h->max_np >= (int)(1.1 * (double)h->np)) resize_variables(h, (int)(1.05 * (double)h->np));
+  }
+  free(nb);
+  if (nremoved) *nremoved = count;
+  return DEM_OK;
+}
+
+/* ranf (src/MUtil.f90:16-44): Park-Miller "minimal standard" generator by
+ * Schrage's method combined with a Marsaglia shift sequence.  aam is
+ * nearest(1.0,-1.0)/IM evaluated in default (single) precision. */
+double orc_ranf(dem_rng_t* r) {
+  const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+  if (r->dummy <= 0 || r->iiy < 0) {
+    r->aam = (double)(nextafterf(1.0f, 0.0f) / (float)im);
+    const int32_t a = r->dummy < 0 ? -r->dummy : r->dummy;
+    r->iiy = (888889999 ^ a) | 1;
+    r->iix = 777755555 ^ a;
+    r->dummy = a + 1;
+  }
+  uint32_t x = (uint32_t)r->iix;
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  r->iix = (int32_t)x;
+  r->iik = r->iiy / iq;
+  r->iiy = ia * (r->iiy - r->iik * iq) - ir * r->iik;
+  if (r->iiy < 0) r->iiy += im;
+  return r->aam * (double)((im & (r->iix ^ r->iiy)) | 1);
+}
+
+/* PourParticle and InsertParticle (src/MDEM.f90:412-562), as the reference
+ * runs them: the largest radius for the height, the spacing test and the
+ * wall test; the spacing test on the integer parts of the separations
+ * (Rij is declared integer); the inserted particle at rest but for a
+ * vertical velocity -1e-4*LengthScale/TimeScale. */
+int orc_pour(orc_handle_t h, const dem_pour_t* pr, dem_rng_t* rng, int32_t* inserted) {
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (!pr || !rng || !pr->target_number) return fail(h, DEM_ERR_ARG, "null argument");
+  if (inserted) *inserted = 0;
+  int number[MAXC] = {0}, notreached[MAXC], nnr = 0;
+  for (int i = 0; i < h->np; i++) number[h->part[i].index - 1]++;
+  for (int c = 0; c < h->nc; c++)
+    if (number[c] < pr->target_number[c]) notreached[nnr++] = c + 1;
+  if (nnr == 0) return DEM_OK;
+  particle_t ip;
+  memset(&ip, 0, sizeof ip);
+  const int j0 = (int)((double)nnr * orc_ranf(rng)) + 1;
+  ip.index = notreached[j0 - 1];
+  int ibig = 0;
+  for (int c = 1; c < h->nc; c++)
+    if (h->radius[c] > h->radius[ibig]) ibig = c;
+  const double ri = h->radius[ibig];
+  const double di = 2.0 * ri;
+  ip.position[2] = h->phalf[2] - ri;
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  int possible = 0, attempt = 0;
+  int* nb = NULL;
+  int nbcap = 0;
+  while (!possible && attempt < pr->max_attempts) {
+    possible = 1;
+    attempt++;
+    if (pr->shape == DEM_POUR_RECTANGLE) {
+      for (int d = 0; d < 2; d++) ip.position[d] = (pr->size[d] - di) * (orc_ranf(rng) - 0.5);
+    } else {
+      const double rad = 0.5 * orc_ranf(rng) * (pr->size[0] - di);
+      const double angle = two_pi * orc_ranf(rng);
+      ip.position[0] = rad * sin(angle);
+      ip.position[1] = rad * cos(angle);
+    }
+    const int icell = current_cell(h, ip.position) - 1;
+    const int n = mc_neighbour_particles(h, icell, &nb, &nbcap);
+    for (int k = 0; possible && k < n; k++) {
+      const particle_t* jp = &h->part[nb[k]];
+      int32_t s = 0;
+      for (int d = 0; d < 3; d++) {
+        const int32_t rij = (int32_t)(jp->position[d] - ip.position[d]);
+        s += rij * rij;
+      }
+      possible = (double)s > h->mindijsq[ibig + (jp->index - 1) * h->nc];
+    }
+    for (int jw = 0; possible && jw < h->nw; jw++) {
+      double dij, nij[3], qji[3];
+      const int ov = overlap(h, &ip, jw, ri, &dij, nij, qji);
+      if (ov < 0) {
+        free(nb);
+        return fail(h, DEM_ERR_UNSUPPORTED, "pouring position on the axis of cone wall %d", jw + 1);
+      }
+      possible = !ov;
+    }
+  }
+  free(nb);
+  if (!possible) return DEM_OK;
+  if (h->np == h->max_np) {
+    int m = (int)(1.05 * (double)h->max_np);
+    if (m < h->max_np + 1) m = h->max_np + 1;
+    resize_variables(h, m);
+  }
+  ip.velocity[2] = -(1.e-4 * pr->length_scale / pr->time_scale);
+  const int k = h->np;
+  h->np += 1;
+  h->part[k] = ip;
+  h->pred[k].index = ip.index;
+  memset(&h->prev[(size_t)k * 27], 0, sizeof(double) * 27);
+  h->cell_index[k] = current_cell(h, ip.position);
+  add_particle(h, k, h->cell_index[k]);
+  if (inserted) *inserted = 1;
+  return DEM_OK;
 }
 
 /* ---- ABI -------------------------------------------------------------------- */
@@ -881,9 +1168,9 @@ int orc_create(const dem_params_t* p, orc_handle_t* out) {
 
 static void free_particles(orc_handle_t h) {
   if (h->pairs)
-    for (int i = 0; i < h->np; i++) plist_free(&h->pairs[i]);
+    for (int i = 0; i < h->cap; i++) plist_free(&h->pairs[i]);
   if (h->walls)
-    for (int i = 0; i < h->np; i++) plist_free(&h->walls[i]);
+    for (int i = 0; i < h->cap; i++) plist_free(&h->walls[i]);
   free(h->pairs); free(h->walls); free(h->part); free(h->pred); free(h->prev);
   free(h->cell_next); free(h->cell_index); free(h->force); free(h->torque);
   free(h->fscale);
@@ -892,6 +1179,8 @@ static void free_particles(orc_handle_t h) {
   h->prev = h->force = h->torque = h->fscale = NULL;
   h->cell_next = h->cell_index = NULL;
   h->np = 0;
+  h->cap = 0;
+  h->max_np = 0;
   h->has_state = 0;
 }
 
@@ -961,10 +1250,12 @@ int orc_upload_state(orc_handle_t h, int32_t np, const int32_t* comp,
   if (np < 0 || !comp || !x || !v || !w) return fail(h, DEM_ERR_ARG, "bad state arguments");
   free_particles(h);
   h->np = np;
-  size_t n = np > 0 ? (size_t)np : 1;
+  h->max_np = np;
+  h->cap = np > 0 ? np : 1;
+  size_t n = (size_t)h->cap;
   h->part = (particle_t*)calloc(n, sizeof(particle_t));
   h->pred = (particle_t*)calloc(n, sizeof(particle_t));
-  h->prev = (double*)calloc((size_t)(h->nprev > 0 ? h->nprev : 1) * n * 9, sizeof(double));
+  h->prev = (double*)calloc(n * 27, sizeof(double));
   h->cell_next = (int*)malloc(sizeof(int) * n);
   h->cell_index = (int*)malloc(sizeof(int) * n);
   h->force = (double*)malloc(sizeof(double) * 3 * n);
@@ -989,7 +1280,7 @@ int orc_upload_state(orc_handle_t h, int32_t np, const int32_t* comp,
     for (int s = 0; s < h->nprev; s++)
       for (int c = 0; c < 9; c++)
         for (int i = 0; i < np; i++)
-          h->prev[((size_t)s * np + i) * 9 + c] = ring[((size_t)s * 9 + c) * np + i];
+          h->prev[((size_t)i * 3 + s) * 9 + c] = ring[((size_t)s * 9 + c) * np + i];
   }
   for (int k = 0; k < 3; k++) h->ipc[k] = k + 1;
   if (ipc)
@@ -1028,7 +1319,7 @@ int orc_download_state(orc_handle_t h, int32_t np, int32_t* comp, double* x,
     for (int s = 0; s < h->nprev; s++)
       for (int c = 0; c < 9; c++)
         for (int i = 0; i < np; i++)
-          ring[((size_t)s * 9 + c) * np + i] = h->prev[((size_t)s * np + i) * 9 + c];
+          ring[((size_t)s * 9 + c) * np + i] = h->prev[((size_t)i * 3 + s) * 9 + c];
   if (ipc)
     for (int k = 0; k < h->nprev; k++) ipc[k] = h->ipc[k];
   return DEM_OK;
@@ -1082,7 +1373,10 @@ int orc_download_contacts(orc_handle_t h, int64_t cap, int64_t* npairs,
                           int64_t* nwall, uint64_t* wkeys, double* wft,
                           double* wvt) {
   int64_t n = 0, m = 0;
-  for (int i = 0; i < h->np; i++) {
+  /* all MaxNP lists, as SaveContactList writes size(MainParticle) of them
+   * (src/MContacts.f90:82-84): after removals some lists past NP remain */
+  const int nl = h->max_np > h->np ? h->max_np : h->np;
+  for (int i = 0; i < nl; i++) {
     const plist_t* l = &h->pairs[i];
     for (int k = 0; k < l->n; k++, n++) {
       if (n >= cap) continue;

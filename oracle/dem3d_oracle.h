@@ -31,6 +31,9 @@ int orc_set_walls(orc_handle_t h, int32_t nw, const dem_wall_t* walls);
 int orc_set_wall_motion(orc_handle_t h, int32_t nw, const dem_wall_motion_t* motion);
 int orc_download_walls(orc_handle_t h, int32_t nw, dem_wall_t* walls);
 int orc_set_option(orc_handle_t h, const char* name, double value);
+double orc_ranf(dem_rng_t* rng);
+int orc_pour(orc_handle_t h, const dem_pour_t* pour, dem_rng_t* rng, int32_t* inserted);
+int orc_remove_escaped(orc_handle_t h, int32_t cap, int32_t* removed, int32_t* nremoved);
 int orc_upload_state(orc_handle_t h, int32_t np, const int32_t* comp,
                      const double* x, const double* v, const double* w,
                      const double* a, const double* alpha, const double* ring,

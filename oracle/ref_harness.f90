@@ -34,6 +34,7 @@ use MContacts
 use MFluid
 use MForces
 use MDEM
+use MUtil
 use BinFiles
 
 implicit none
@@ -222,6 +223,44 @@ contains
     v(6:8) = Wall(j)%U ; v(9:11) = Wall(j)%W ; v(12:14) = Wall(j)%N
     v(15:17) = Wall(j)%Position ; v(18:20) = Wall(j)%Velocity ; v(21:23) = Wall(j)%Omega
   end subroutine ref_get_wall
+
+  ! ranf (src/MUtil.f90:16-44) n times from the generator state (aam, Dummy,
+  ! iix, iiy, iik), which is returned advanced.
+  subroutine ref_ranf_seq( n, aam_, st, out ) bind(C, name="ref_ranf_seq")
+    integer(c_int32_t), value :: n
+    real(c_double), intent(inout) :: aam_
+    integer(c_int32_t), intent(inout) :: st(4)
+    real(c_double), intent(out) :: out(max(n,1))
+    integer :: k
+    aam = aam_ ; Dummy = st(1) ; iix = st(2) ; iiy = st(3) ; iik = st(4)
+    do k = 1, n
+      out(k) = ranf( Dummy )
+    end do
+    aam_ = aam ; st = [Dummy, iix, iiy, iik]
+  end subroutine ref_ranf_seq
+
+  ! PourParticle (src/MDEM.f90:412-536) with the generator state given and
+  ! returned, as PouringAttempt calls it (src/DEM.f90:115-123).
+  subroutine ref_pour( target, shape, psize, maxatt, aam_, st ) bind(C, name="ref_pour")
+    integer(c_int32_t), intent(in) :: target(NC)
+    integer(c_int32_t), value :: shape, maxatt
+    real(c_double), intent(in) :: psize(2)
+    real(c_double), intent(inout) :: aam_
+    integer(c_int32_t), intent(inout) :: st(4)
+    integer(ib) :: tn(NC)
+    tn = target
+    aam = aam_ ; Dummy = st(1) ; iix = st(2) ; iiy = st(3) ; iik = st(4)
+    if (shape == 1) then
+      call PourParticle( tn, int(shape,1), psize(1:2), int(maxatt,ib) )
+    else
+      call PourParticle( tn, int(shape,1), psize(1:1), int(maxatt,ib) )
+    end if
+    aam_ = aam ; st = [Dummy, iix, iiy, iik]
+  end subroutine ref_pour
+
+  subroutine ref_update_particle_list() bind(C, name="ref_update_particle_list")
+    call UpdateParticleList
+  end subroutine ref_update_particle_list
 
   function ref_get_np() result( n ) bind(C, name="ref_get_np")
     integer(c_int32_t) :: n

@@ -259,7 +259,10 @@ def _ref_job_body(job):
     lib.ref_kinetic_energy.restype = C.c_double
     lib.ref_ncol.restype = C.c_int32
 
+    lib.ref_get_np.restype = C.c_int32
+
     def snapshot(tag):
+        n = int(lib.ref_get_np())
         c2 = np.zeros(n, np.int32)
         xs, vs, ws, as_, als = (np.zeros((n, 3), order="F") for _ in range(5))
         lib.ref_get_particles(C.c_int32(n), c2.ctypes.data_as(ip), P(xs), P(vs), P(ws), P(as_),
@@ -320,6 +323,22 @@ def _ref_job_body(job):
                                     i32("seconds"), i32("milisecs"),
                                     C.c_double(run.get("aam", 0.0)), i32("dummy"), i32("iix"),
                                     i32("iiy"), i32("iik"), i32("conf"), i32("next_step"))
+        elif kind == "pour":
+            _, pr = act
+            rng = out.setdefault("rng", dict(aam=0.0, st=np.array([pr["seed"], -1, -1, 0],
+                                                                  np.int32)))
+            aam_ = C.c_double(rng["aam"])
+            st = np.ascontiguousarray(rng["st"], np.int32)
+            tn = np.ascontiguousarray(pr["target"], np.int32)
+            size = np.ascontiguousarray(list(pr["size"]) + [0.0] * (2 - len(pr["size"])),
+                                        np.float64)
+            lib.ref_pour(tn.ctypes.data_as(ip), C.c_int32(pr["shape"]), P(size),
+                         C.c_int32(pr["max_attempts"]), C.byref(aam_), st.ctypes.data_as(ip))
+            rng["aam"], rng["st"] = aam_.value, st.copy()
+            out.setdefault("np_after", []).append(int(lib.ref_get_np()))
+        elif kind == "remove":
+            lib.ref_update_particle_list()
+            out.setdefault("np_after", []).append(int(lib.ref_get_np()))
         elif kind == "time":
             import time as _t
             t0 = _t.perf_counter()

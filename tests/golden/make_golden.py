@@ -21,6 +21,8 @@ stores inputs and outputs as .npz (numpy arrays only, no pickles):
                     contact lists) after start-up + 20 steps of 48 spheres
   ref_resume.npz    the reference's continuation from ref_resume.stt
                     (ReadState + 15 MoveParticles)
+  ref_ranf.npz      1 000 deviates of the reference's generator ranf from the
+                    seed -20, and its state afterwards
 
 Usage (in a container that has the reference checkout and the built library):
     python tests/golden/make_golden.py [fixture.npz ...]   (default: all)
@@ -180,6 +182,22 @@ def resume_fixture():
     print(f"wrote {stt} ({os.path.getsize(stt)} B) and {path}")
 
 
+def ranf_fixture():
+    import ctypes as C
+    from checkers import REF_LIB
+    ref = C.CDLL(REF_LIB)
+    n, seed = 1000, -20
+    aam = C.c_double(0.0)
+    st = np.array([seed, -1, -1, 0], np.int32)
+    out = np.zeros(n)
+    ref.ref_ranf_seq(C.c_int32(n), C.byref(aam), st.ctypes.data_as(C.POINTER(C.c_int32)),
+                     out.ctypes.data_as(C.POINTER(C.c_double)))
+    path = os.path.join(HERE, "ref_ranf.npz")
+    np.savez_compressed(path, seed=np.array(seed), n=np.array(n), values=out, state=st,
+                        aam=np.array(aam.value))
+    print(f"wrote {path}")
+
+
 def main(names):
     def want(name):
         return not names or name in names
@@ -205,6 +223,8 @@ def main(names):
                  [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
                   ("snapshot", "s")],
                  "cone hopper + floor, 216 spheres")
+    if want("ref_ranf.npz"):
+        ranf_fixture()
     if want("ref_resume.npz"):
         resume_fixture()
     if want("ref_ball_bearings.npz"):
