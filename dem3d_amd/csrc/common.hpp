@@ -94,4 +94,10 @@ struct SysConst {
   const CompConst* comp;
   const WallConst* wallc;
   const WallGeom* walls;
+  // neighbour lists with the largest pair cutoff for every pair: set while
+  // some Predicted%Index differs from the particle's component (the
+  // predictor pass then evaluates it with another radius)
+  double rc2_all;
+  int cut_all;
+  int pad2;
 };

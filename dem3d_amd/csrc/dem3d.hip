@@ -56,6 +56,17 @@ struct dem_system {
   WallMotion* d_wm = nullptr;  // WM(1:nw) when WallsInMovement
   int nw_move = 0;             // nw when WallsInMovement, else 0
   double time = 0.0;           // Time (src/MConfig.f90:61)
+  // particle-number changes
+  int max_np = 0;              // MaxNP (src/MDEM.f90:22)
+  std::vector<double> pred_w_h;  // Predicted%Index (0-based, by id) when stale
+  double* d_pred_w = nullptr;
+  size_t pred_w_cap = 0;
+  struct Orphan {
+    uint64_t key;
+    double ft[3], vt[3];
+  };
+  std::vector<Orphan> orphans;  // stored pairs naming a particle >= NP
+  int pending_cut_all = 0;      // SysConst.cut_all for the next upload (host edits)
   int64_t next_step = 0;       // step number of the next batched step
   // state: buffer cur is Particle, cur^1 Predicted (or Euler scratch)
   int np = 0;
@@ -297,6 +308,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     q.mindij = p->min_dij[t]; q.mindijsq = p->min_dij_sq[t];
     const double rc = p->min_dij[t] + h->skin;
     q.rc2 = rc * rc;
+    s.rc2_all = std::max(s.rc2_all, q.rc2);
   }
   std::vector<CompConst> comp(nc);
   for (int c = 0; c < nc; ++c) {
@@ -352,6 +364,7 @@ void dem_destroy(dem_handle_t h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_state(h);
   dfree(h->d_pair); dfree(h->d_comp); dfree(h->d_wallc); dfree(h->d_walls); dfree(h->d_wm);
+  dfree(h->d_pred_w);
   dfree(h->ctl); dfree(h->rb);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -686,6 +699,11 @@ int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp, const doub
   h->cur = 0;
   h->last_eval_buf = 0;
   h->acc_in_ring = false;
+  h->max_np = np;
+  h->pred_w_h.clear();
+  h->orphans.clear();
+  s.cut_all = h->pending_cut_all;
+  h->pending_cut_all = 0;
   for (int k = 0; k < 3; ++k) h->ipc[k] = k + 1;
   if (ipc)
     for (int k = 0; k < h->nprev; ++k) {
@@ -1246,6 +1264,7 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
                         const double* vt, int64_t nwall, const uint64_t* wkeys, const double* wft,
                         const double* wvt) {
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  h->orphans.clear();
   const size_t N = (size_t)h->np;
   const SysConst& s = h->sc;
   std::vector<int> idh;
@@ -1370,6 +1389,13 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
       we.push_back(x);
     }
   }
+  for (const auto& o : h->orphans) {
+    Ent x;
+    x.key = o.key;
+    memcpy(x.f, o.ft, sizeof x.f);
+    memcpy(x.g, o.vt, sizeof x.g);
+    pe.push_back(x);
+  }
   auto by_key = [](const Ent& x, const Ent& y) { return x.key < y.key; };
   std::sort(pe.begin(), pe.end(), by_key);
   std::sort(we.begin(), we.end(), by_key);
@@ -1408,6 +1434,7 @@ static PassArgs base_args(dem_handle_t h) {
   a.part = h->part;
   a.ctl = h->ctl;
   a.half_dt = 0.5 * h->dt;
+  a.pred_w = h->pred_w_h.empty() ? nullptr : h->d_pred_w;
   return a;
 }
 
@@ -1507,7 +1534,20 @@ int dem_euler_startup(dem_handle_t h) {
   if (h->nprev == 0) return fail(h, DEM_ERR_STATE, "Euler start-up needs MethodOrder > 1");
   if (h->acc_in_ring)
     return fail(h, DEM_ERR_STATE, "Euler start-up after MoveParticles is not supported");
-  if (h->np == 0) return DEM_OK;
+  if (h->np == 0) {
+    // no particles yet (a box to be poured): the ring order, Time and the
+    // walls advance all the same
+    const int last = h->ipc[h->nprev - 1];
+    for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+    h->ipc[0] = last;
+    const double hs = h->dt / (double)h->n_euler;
+    for (int sub = 0; sub < h->n_euler; ++sub) {
+      h->time = h->time + hs;
+      enqueue_move_walls(h, h->time);
+    }
+    HIPCHK(h, hipGetLastError());
+    return DEM_OK;
+  }
   const int np = h->np;
   hipStream_t st = h->stream;
   // ipc = cshift(ipc, -1); Previous(ipc(1)) = (v, a, alpha)  (src/MDEM.f90:266-271)
@@ -1616,8 +1656,25 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
   if (steps_done) *steps_done = 0;
   if (escaped) *escaped = 0;
   if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
-  if (nsteps <= 0 || h->np == 0) return DEM_OK;
+  if (nsteps <= 0) return DEM_OK;
   if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  if (h->np == 0) {
+    // no particles: MoveParticles still turns the ring, sets Time and moves
+    // the walls (src/MDEM.f90:329-337)
+    for (int64_t s = 0; s < nsteps && h->nprev > 0; ++s) {
+      const int last = h->ipc[h->nprev - 1];
+      for (int k = h->nprev - 1; k > 0; --k) h->ipc[k] = h->ipc[k - 1];
+      h->ipc[0] = last;
+    }
+    h->eval += 2 * nsteps;
+    h->time = (double)(first_step + nsteps - 1) * h->dt;
+    enqueue_move_walls(h, h->time);
+    HIPCHK(h, hipGetLastError());
+    h->next_step = first_step + nsteps;
+    h->steps += nsteps;
+    if (steps_done) *steps_done = nsteps;
+    return DEM_OK;
+  }
   hipStream_t st = h->stream;
   {
     int big = 0x7fffffff;
@@ -1749,6 +1806,516 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->ms_step = h->ms_step;
   out->mean_pair_contacts = c.upd_evals ? (double)c.pairs_upd_sum / (double)c.upd_evals : 0.0;
   return DEM_OK;
+}
+
+}  // extern "C"
+
+// ---- particle number changes ---------------------------------------------------
+//
+// UpdateParticleList and PourParticle change the particle count between
+// steps, rarely.  They run on the host, on the reference's own structures
+// rebuilt from the device -- particles, ABM ring, cell lists in list order,
+// contact lists -- and the result goes back with the cell state of the last
+// evaluation kept (the next evaluation then transfers, as
+// RedistributeParticles3D does, the particles whose positions left their
+// cell).
+
+namespace {
+
+struct HEntry {
+  int idx;
+  double v1[3], v2[3];
+};
+
+struct HostSys {
+  int np = 0;
+  std::vector<int32_t> comp;                // 1-based
+  std::vector<double> x, v, w;              // [np][3]
+  std::vector<double> ring;                 // [np][27]: ring slot s at 9*s
+  std::vector<double> pred_w;               // Predicted%Index - 1, by id
+  std::vector<int> cell;                    // 0-based cell of the last evaluation
+  std::vector<int> head, next;              // cell lists (head-pushed order)
+  std::vector<std::vector<HEntry>> pairs;   // MainParticle(:)%FirstPartner
+  std::vector<std::vector<HEntry>> walls;   // MainParticle(:)%FirstWall
+};
+
+}  // namespace
+
+static int host_download(dem_handle_t h, HostSys& hs) {
+  const int np = h->np;
+  const size_t N = (size_t)np;
+  hs.np = np;
+  hs.comp.assign(N, 0);
+  hs.x.assign(3 * N, 0.0);
+  hs.v.assign(3 * N, 0.0);
+  hs.w.assign(3 * N, 0.0);
+  hs.ring.assign(27 * N, 0.0);
+  hs.cell.assign(N, 0);
+  const int ncell = h->sc.rn[0] * h->sc.rn[1] * h->sc.rn[2];
+  hs.head.assign((size_t)ncell, -1);
+  hs.next.assign(N, -1);
+  const size_t nl = (size_t)std::max(np, h->max_np);
+  hs.pairs.assign(nl, {});
+  hs.walls.assign(nl, {});
+  std::vector<uint64_t> keys, wkeys;
+  std::vector<double> ft, vt, wft, wvt;
+  if (np > 0) {
+    std::vector<double> xs(3 * N), vs(3 * N), ws(3 * N), ring((size_t)std::max(h->nprev, 1) * 9 * N);
+    std::vector<int32_t> ipc(3), cells(N), rank(N);
+    int rc = dem_download_state(h, np, hs.comp.data(), xs.data(), vs.data(), ws.data(), nullptr,
+                                nullptr, ring.data(), ipc.data());
+    if (!rc) rc = dem_download_cells(h, np, cells.data());
+    if (!rc) rc = dem_download_cell_rank(h, np, rank.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < N; ++i) {
+      for (int d = 0; d < 3; ++d) {
+        hs.x[3 * i + d] = xs[d * N + i];
+        hs.v[3 * i + d] = vs[d * N + i];
+        hs.w[3 * i + d] = ws[d * N + i];
+      }
+      for (int k = 0; k < h->nprev; ++k)
+        for (int q = 0; q < 9; ++q) hs.ring[27 * i + 9 * k + q] = ring[((size_t)k * 9 + q) * N + i];
+      hs.cell[i] = cells[i] - 1;
+    }
+    // lists: within a cell by rank, highest first (head)
+    std::vector<int> order(N);
+    for (size_t i = 0; i < N; ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      if (hs.cell[a] != hs.cell[b]) return hs.cell[a] < hs.cell[b];
+      return rank[a] < rank[b];
+    });
+    for (int q : order) {  // ascending rank: each push lands at the head
+      hs.next[q] = hs.head[hs.cell[q]];
+      hs.head[hs.cell[q]] = q;
+    }
+  }
+  int64_t npairs = 0, nwall = 0;
+  int rc = dem_download_contacts(h, 0, &npairs, nullptr, nullptr, nullptr, 0, &nwall, nullptr,
+                                 nullptr, nullptr);
+  if (rc) return rc;
+  keys.resize((size_t)npairs);
+  ft.resize(3 * (size_t)npairs);
+  vt.resize(3 * (size_t)npairs);
+  wkeys.resize((size_t)nwall);
+  wft.resize(3 * (size_t)nwall);
+  wvt.resize(3 * (size_t)nwall);
+  rc = dem_download_contacts(h, npairs, &npairs, keys.data(), ft.data(), vt.data(), nwall, &nwall,
+                             wkeys.data(), wft.data(), wvt.data());
+  if (rc) return rc;
+  for (int64_t k = 0; k < npairs; ++k) {
+    const size_t a = (size_t)(keys[k] >> 32) - 1;
+    HEntry e;
+    e.idx = (int)(keys[k] & 0xffffffffu) - 1;
+    memcpy(e.v1, &ft[3 * k], sizeof e.v1);
+    memcpy(e.v2, &vt[3 * k], sizeof e.v2);
+    if (a >= hs.pairs.size()) hs.pairs.resize(a + 1);
+    hs.pairs[a].push_back(e);
+  }
+  for (int64_t k = 0; k < nwall; ++k) {
+    const size_t a = (size_t)(wkeys[k] >> 32) - 1;
+    HEntry e;
+    e.idx = (int)(wkeys[k] & 0xffffffffu) - 1;
+    memcpy(e.v1, &wft[3 * k], sizeof e.v1);
+    memcpy(e.v2, &wvt[3 * k], sizeof e.v2);
+    hs.walls[a].push_back(e);
+  }
+  hs.pred_w.assign(N, 0.0);
+  for (size_t i = 0; i < N; ++i)
+    hs.pred_w[i] = h->pred_w_h.empty() ? (double)(hs.comp[i] - 1) : h->pred_w_h[i];
+  return DEM_OK;
+}
+
+static int host_upload(dem_handle_t h, const HostSys& hs, int max_np) {
+  Ctl c0;
+  int rc = read_ctl(h, &c0);
+  if (rc) return rc;
+  const int np = hs.np;
+  const size_t N = (size_t)np;
+  std::vector<double> xs(3 * N), vs(3 * N), ws(3 * N), ring((size_t)std::max(h->nprev, 1) * 9 * N);
+  std::vector<int32_t> rank(N, 0);
+  std::vector<int2> cr(N);
+  for (size_t i = 0; i < N; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      xs[d * N + i] = hs.x[3 * i + d];
+      vs[d * N + i] = hs.v[3 * i + d];
+      ws[d * N + i] = hs.w[3 * i + d];
+    }
+    for (int k = 0; k < h->nprev; ++k)
+      for (int q = 0; q < 9; ++q) ring[((size_t)k * 9 + q) * N + i] = hs.ring[27 * i + 9 * k + q];
+  }
+  for (size_t c = 0; c < hs.head.size(); ++c) {
+    int n = 0;
+    for (int q = hs.head[c]; q >= 0; q = hs.next[q]) ++n;
+    for (int q = hs.head[c]; q >= 0; q = hs.next[q]) rank[q] = --n;
+  }
+  for (size_t i = 0; i < N; ++i) cr[i] = make_int2(hs.cell[i], rank[i]);
+  int32_t ipc[3] = {h->ipc[0], h->ipc[1], h->ipc[2]};
+  const double time = h->time;
+  const int64_t next_step = h->next_step;
+  bool stale = false;
+  for (size_t i = 0; i < N; ++i) stale = stale || hs.pred_w[i] != (double)(hs.comp[i] - 1);
+  h->pending_cut_all = stale ? 1 : 0;  // the neighbour lists must cover the other radius
+  rc = dem_upload_state(h, np, hs.comp.data(), xs.data(), vs.data(), ws.data(), nullptr, nullptr,
+                        h->nprev > 0 ? ring.data() : nullptr, h->nprev > 0 ? ipc : nullptr,
+                        rank.data());
+  if (rc) return rc;
+  h->time = time;
+  h->next_step = next_step;
+  h->max_np = max_np;
+  hipStream_t st = h->stream;
+  // contacts: stored pairs between particles 1..NP go to the device, the
+  // others stay on the host until the particle count covers them
+  std::vector<uint64_t> keys, wkeys;
+  std::vector<double> ft, vt, wft, wvt;
+  std::vector<dem_system::Orphan> orphans;
+  for (size_t a = 0; a < hs.pairs.size(); ++a) {
+    std::vector<HEntry> l = hs.pairs[a];
+    std::stable_sort(l.begin(), l.end(), [](const HEntry& p, const HEntry& q) { return p.idx < q.idx; });
+    for (const HEntry& e : l) {
+      const uint64_t key = ((uint64_t)(a + 1) << 32) | (uint64_t)(e.idx + 1);
+      if ((int)a < np && e.idx < np) {
+        keys.push_back(key);
+        ft.insert(ft.end(), e.v1, e.v1 + 3);
+        vt.insert(vt.end(), e.v2, e.v2 + 3);
+      } else {
+        dem_system::Orphan o;
+        o.key = key;
+        memcpy(o.ft, e.v1, sizeof o.ft);
+        memcpy(o.vt, e.v2, sizeof o.vt);
+        orphans.push_back(o);
+      }
+    }
+  }
+  for (size_t a = 0; a < hs.walls.size() && (int)a < np; ++a)
+    for (const HEntry& e : hs.walls[a]) {
+      wkeys.push_back(((uint64_t)(a + 1) << 32) | (uint64_t)(e.idx + 1));
+      wft.insert(wft.end(), e.v1, e.v1 + 3);
+      wvt.insert(wvt.end(), e.v2, e.v2 + 3);
+    }
+  if (np > 0) {
+    rc = dem_upload_contacts(h, (int64_t)keys.size(), keys.data(), ft.data(), vt.data(),
+                             (int64_t)wkeys.size(), wkeys.data(), wft.data(), wvt.data());
+    if (rc) return rc;
+  }
+  h->orphans = orphans;
+  if (np > 0) {
+    // the cell state of the last evaluation, lists in the given order
+    int2* d_cr = nullptr;
+    HIPCHK(h, hipMalloc((void**)&d_cr, N * sizeof(int2)));
+    hipError_t e = hipMemcpyAsync(d_cr, cr.data(), N * sizeof(int2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_set_ref_cells, dim3(nblocks(np)), dim3(DEM_BLOCK), 0, st, h->sc,
+                         h->kin[h->cur], (const int2*)d_cr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_cr);
+    if (e != hipSuccess) return fail(h, DEM_ERR_DEVICE, "cell state upload: %s", hipGetErrorString(e));
+    h->last_eval_buf = h->cur;
+    h->rc_ready = false;
+  }
+  // Predicted%Index where it differs from the particle's component
+  h->pred_w_h.clear();
+  if (stale) {
+    h->pred_w_h = hs.pred_w;
+    if (h->pred_w_cap < N) {
+      dfree(h->d_pred_w);
+      HIPCHK(h, dalloc(&h->d_pred_w, N));
+      h->pred_w_cap = N;
+    }
+    HIPCHK(h, hipMemcpy(h->d_pred_w, h->pred_w_h.data(), N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // counters that run across the whole simulation
+  HIPCHK(h, hipMemcpyAsync(&h->ctl->ncol, &c0.ncol, 2 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipMemcpyAsync(&h->ctl->stale_dropped, &c0.stale_dropped, sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  return check_errors(h);
+}
+
+// CurrentCell3D (src/MCells.f90:498-513), 0-based.
+static int host_cell(const SysConst& s, const double* r) {
+  const RefCell c = ref_cell(s, r[0], r[1], r[2]);
+  return c.lin;
+}
+
+// The Monte Carlo neighbour list of a cell: its own list, then the lists of
+// the (up to 26) cells around it in the head-pushed order of AddNeighbour
+// (HinderNeighbourCellsList, NeighbourCells(iCell, MonteCarlo, ...),
+// AllNeighbourParticles: src/MCells.f90:340-374,424-476,833-860,1056-1074).
+static void host_mc_neighbours(const SysConst& s, const HostSys& hs, int icell,
+                               std::vector<int>& out) {
+  const int nx = s.rn[0], ny = s.rn[1], nz = s.rn[2];
+  const int iz = icell / (nx * ny);
+  const int iy = icell / nx - iz * ny;
+  const int ix = icell - (iy + iz * ny) * nx;
+  auto range = [](int i, int m, int pbc, int& lo, int& hi) {
+    lo = -1;
+    hi = 1;
+    if (!pbc) {
+      if (i == 0) lo = 0;
+      if (i == m - 1) hi = 0;
+    }
+  };
+  auto wrap = [](int j, int m, int pbc) {
+    if (pbc) {
+      if (j == m) j = 0;
+      if (j == -1) j = m - 1;
+    }
+    return j;
+  };
+  int xi, xf, yi, yf, zi, zf;
+  range(ix, nx, s.pbc[0], xi, xf);
+  range(iy, ny, s.pbc[1], yi, yf);
+  range(iz, nz, s.pbc[2], zi, zf);
+  std::vector<int> cells;
+  for (int kz = zi; kz <= zf; ++kz) {
+    const int jz = wrap(iz + kz, nz, s.pbc[2]);
+    for (int ky = yi; ky <= yf; ++ky) {
+      const int jy = wrap(iy + ky, ny, s.pbc[1]);
+      for (int kx = xi; kx <= xf; ++kx) {
+        if (kx + nx * (ky + ny * kz) == 0) continue;
+        const int jx = wrap(ix + kx, nx, s.pbc[0]);
+        cells.push_back(jx + nx * (jy + ny * jz));
+      }
+    }
+  }
+  out.clear();
+  for (int q = hs.head[icell]; q >= 0; q = hs.next[q]) out.push_back(q);
+  for (size_t k = cells.size(); k-- > 0;)
+    for (int q = hs.head[cells[k]]; q >= 0; q = hs.next[q]) out.push_back(q);
+}
+
+// ResizeVariables (src/MDEM.f90:566-608): MainParticle keeps the lists of
+// particles 1..NP, Predicted%Index is reset to the components.
+static void host_resize(HostSys& hs, int& max_np, int m) {
+  for (size_t a = (size_t)hs.np; a < hs.pairs.size(); ++a) {
+    hs.pairs[a].clear();
+    hs.walls[a].clear();
+  }
+  max_np = m;
+  if (hs.pairs.size() < (size_t)m) {
+    hs.pairs.resize((size_t)m);
+    hs.walls.resize((size_t)m);
+  }
+  for (int i = 0; i < hs.np; ++i) hs.pred_w[i] = (double)(hs.comp[i] - 1);
+}
+
+extern "C" {
+
+// ranf (src/MUtil.f90:16-44).
+double dem_ranf(dem_rng_t* r) {
+  const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+  if (r->dummy <= 0 || r->iiy < 0) {
+    r->aam = (double)(std::nextafter(1.0f, 0.0f) / (float)im);
+    const int32_t a = r->dummy < 0 ? -r->dummy : r->dummy;
+    r->iiy = (888889999 ^ a) | 1;
+    r->iix = 777755555 ^ a;
+    r->dummy = a + 1;
+  }
+  uint32_t x = (uint32_t)r->iix;
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  r->iix = (int32_t)x;
+  r->iik = r->iiy / iq;
+  r->iiy = ia * (r->iiy - r->iik * iq) - ir * r->iik;
+  if (r->iiy < 0) r->iiy += im;
+  return r->aam * (double)((im & (r->iix ^ r->iiy)) | 1);
+}
+
+// UpdateParticleList (src/MDEM.f90:612-671).
+int dem_remove_escaped(dem_handle_t h, int32_t cap, int32_t* removed, int32_t* nremoved) {
+  if (nremoved) *nremoved = 0;
+  if (!h || !h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  HostSys hs;
+  int rc = host_download(h, hs);
+  if (rc) return rc;
+  const SysConst& s = h->sc;
+  int max_np = h->max_np;
+  int count = 0;
+  std::vector<int> nb;
+  for (int i = hs.np - 1; i >= 0; --i) {
+    const double* r = &hs.x[3 * (size_t)i];
+    bool out = false;
+    for (int d = 0; d < 3; ++d) out = out || (r[d] > s.phalf[d]) || (r[d] < -s.phalf[d]);
+    if (!out) continue;
+    const int icell = hs.cell[i];
+    host_mc_neighbours(s, hs, icell, nb);
+    // RemoveParticleFromCell (src/MCells.f90:702-724)
+    {
+      int* link = &hs.head[icell];
+      while (*link != i) link = &hs.next[*link];
+      *link = hs.next[i];
+    }
+    // RemoveParticleFromContactList (src/MContacts.f90:579-633)
+    for (int j : nb) {
+      auto& l = hs.pairs[j];
+      if (l.empty()) continue;
+      size_t pos = 0;
+      while (l[pos].idx < i && pos + 1 < l.size()) ++pos;
+      if (l[pos].idx == i) l.erase(l.begin() + (long)pos);
+    }
+    hs.pairs[i].clear();
+    hs.walls[i].clear();
+    const int last = hs.np - 1;
+    if (i != last) {
+      hs.comp[i] = hs.comp[last];
+      for (int d = 0; d < 3; ++d) {
+        hs.x[3 * (size_t)i + d] = hs.x[3 * (size_t)last + d];
+        hs.v[3 * (size_t)i + d] = hs.v[3 * (size_t)last + d];
+        hs.w[3 * (size_t)i + d] = hs.w[3 * (size_t)last + d];
+      }
+      std::copy(&hs.ring[27 * (size_t)last], &hs.ring[27 * (size_t)last] + 27, &hs.ring[27 * (size_t)i]);
+      // ReplaceParticleIndexInCell: the entry keeps its place
+      {
+        int* link = &hs.head[hs.cell[last]];
+        while (*link != last) link = &hs.next[*link];
+        *link = i;
+        hs.next[i] = hs.next[last];
+      }
+      hs.cell[i] = hs.cell[last];
+      // ChangeIndexInContactList (src/MContacts.f90:173-252)
+      std::vector<HEntry> first;
+      for (int j : nb) {
+        auto& l = hs.pairs[j];
+        if (l.empty() || l.back().idx != last) continue;
+        HEntry e = l.back();
+        if (j < i) {
+          if (l.size() == 1) {
+            l[0].idx = i;
+          } else {
+            int point = -1;
+            for (size_t k = 0; k + 1 < l.size(); ++k)
+              if (l[k].idx < i) point = (int)k;
+            l.pop_back();
+            e.idx = i;
+            l.insert(l.begin() + (point + 1), e);
+          }
+        } else {
+          l.pop_back();
+          e.idx = j;
+          first.push_back(e);
+        }
+      }
+      hs.pairs[i] = first;
+      hs.walls[i] = hs.walls[last];
+      hs.walls[last].clear();
+    }
+    hs.np -= 1;
+    if (removed && count < cap) removed[count] = i + 1;
+    ++count;
+    if (max_np >= (int)(1.1 * (double)hs.np)) host_resize(hs, max_np, (int)(1.05 * (double)hs.np));
+  }
+  if (nremoved) *nremoved = count;
+  if (count == 0) return DEM_OK;
+  // the arrays by particle end at the new NP (Predicted%Index of the slots
+  // that received a particle keeps the old component)
+  const size_t N = (size_t)hs.np;
+  hs.comp.resize(N);
+  hs.x.resize(3 * N);
+  hs.v.resize(3 * N);
+  hs.w.resize(3 * N);
+  hs.ring.resize(27 * N);
+  hs.cell.resize(N);
+  hs.next.resize(N);
+  hs.pred_w.resize(N);
+  return host_upload(h, hs, max_np);
+}
+
+// PourParticle and InsertParticle (src/MDEM.f90:412-562).  The reference
+// tests the spacing with the integer parts of the separations (Rij is
+// declared integer) against MinDijSq of the largest component, and the
+// walls with the largest radius; the new particle falls at
+// -1e-4*LengthScale/TimeScale.
+int dem_pour(dem_handle_t h, const dem_pour_t* pr, dem_rng_t* rng, int32_t* inserted) {
+  if (inserted) *inserted = 0;
+  if (!h || !pr || !rng || !pr->target_number) return fail(h, DEM_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  if (pr->shape != DEM_POUR_RECTANGLE && pr->shape != DEM_POUR_CIRCLE)
+    return fail(h, DEM_ERR_ARG, "bad pouring shape %d", pr->shape);
+  const SysConst& s = h->sc;
+  const int nc = s.nc;
+  HostSys hs;
+  int rc = host_download(h, hs);
+  if (rc) return rc;
+  std::vector<int> number((size_t)nc, 0), notreached;
+  for (int i = 0; i < hs.np; ++i) number[hs.comp[i] - 1] += 1;
+  for (int c = 0; c < nc; ++c)
+    if (number[c] < pr->target_number[c]) notreached.push_back(c + 1);
+  if (notreached.empty()) return DEM_OK;
+  const int nnr = (int)notreached.size();
+  const int j0 = (int)((double)nnr * dem_ranf(rng)) + 1;
+  const int index = notreached[j0 - 1];
+  int ibig = 0;
+  for (int c = 1; c < nc; ++c)
+    if (h->radius_h[c] > h->radius_h[ibig]) ibig = c;
+  const double ri = h->radius_h[ibig], di = 2.0 * ri;
+  double p[3] = {0.0, 0.0, s.phalf[2] - ri};
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  // MinDijSq of the largest component with each component
+  std::vector<PairConst> pair((size_t)nc * nc);
+  HIPCHK(h, hipMemcpy(pair.data(), h->d_pair, pair.size() * sizeof(PairConst), hipMemcpyDeviceToHost));
+  bool possible = false;
+  int attempt = 0;
+  std::vector<int> nb;
+  while (!possible && attempt < pr->max_attempts) {
+    possible = true;
+    ++attempt;
+    if (pr->shape == DEM_POUR_RECTANGLE) {
+      for (int d = 0; d < 2; ++d) p[d] = (pr->size[d] - di) * (dem_ranf(rng) - 0.5);
+    } else {
+      const double rad = 0.5 * dem_ranf(rng) * (pr->size[0] - di);
+      const double angle = two_pi * dem_ranf(rng);
+      p[0] = rad * std::sin(angle);
+      p[1] = rad * std::cos(angle);
+    }
+    host_mc_neighbours(s, hs, host_cell(s, p), nb);
+    for (size_t k = 0; possible && k < nb.size(); ++k) {
+      const double* q = &hs.x[3 * (size_t)nb[k]];
+      int32_t sum = 0;
+      for (int d = 0; d < 3; ++d) {
+        const int32_t r = (int32_t)(q[d] - p[d]);
+        sum += r * r;
+      }
+      possible = (double)sum > pair[(size_t)ibig + (size_t)(hs.comp[nb[k]] - 1) * nc].mindijsq;
+    }
+    for (int jw = 0; possible && jw < s.nw; ++jw) {
+      const WallConst& wc = h->wallc_h[(size_t)(index - 1) + (size_t)jw * nc];
+      const WallHit hit = wall_overlap(s, h->walls_h[jw], mk3(p[0], p[1], p[2]), ri,
+                                       wc.max_alpha_u, wc.min_aw_sq, wc.max_aw_sq);
+      if (hit.ov < 0)
+        return fail(h, DEM_ERR_UNSUPPORTED, "pouring position on the axis of cone wall %d", jw + 1);
+      possible = hit.ov == 0;
+    }
+  }
+  if (!possible) return DEM_OK;
+  int max_np = h->max_np;
+  if (hs.np == max_np) host_resize(hs, max_np, std::max(max_np + 1, (int)(1.05 * (double)max_np)));
+  // InsertParticle (src/MDEM.f90:540-562)
+  const int k = hs.np;
+  hs.np += 1;
+  hs.comp.push_back(index);
+  for (int d = 0; d < 3; ++d) hs.x.push_back(p[d]);
+  hs.v.push_back(0.0);
+  hs.v.push_back(0.0);
+  hs.v.push_back(-(1.e-4 * pr->length_scale / pr->time_scale));
+  for (int d = 0; d < 3; ++d) hs.w.push_back(0.0);
+  hs.ring.resize(27 * (size_t)hs.np, 0.0);
+  hs.pred_w.push_back((double)(index - 1));
+  const int c = host_cell(s, p);
+  hs.cell.push_back(c);
+  hs.next.push_back(hs.head[c]);
+  hs.head[c] = k;
+  if (hs.pairs.size() < (size_t)hs.np) {
+    hs.pairs.resize((size_t)hs.np);
+    hs.walls.resize((size_t)hs.np);
+  }
+  if (inserted) *inserted = 1;
+  return host_upload(h, hs, max_np);
 }
 
 }  // extern "C"

@@ -140,6 +140,9 @@ struct PassArgs {
   int* rb_set;               // rebuild flag for the next evaluation
   unsigned* part;            // per-block counters: collisions, pairs, walls, stored
   Ctl* ctl;
+  // PRED: Predicted%Index by reference id when it differs from Particle%Index
+  // (after removals, until the next resize, src/MDEM.f90:636-640); else null
+  const double* pred_w;
 };
 
 __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
@@ -428,7 +431,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       if (s.pbc[0]) xp.x = xp.x - s.lbox[0] * fnint(xp.x * s.invl[0]);
       if (s.pbc[1]) xp.y = xp.y - s.lbox[1] * fnint(xp.y * s.invl[1]);
       if (s.pbc[2]) xp.z = xp.z - s.lbox[2] * fnint(xp.z * s.invl[2]);
-      a.pos_out[i] = make_double4(xp.x, xp.y, xp.z, P.w);
+      a.pos_out[i] = make_double4(xp.x, xp.y, xp.z, a.pred_w ? a.pred_w[id_p] : P.w);
       const RefCell rc = ref_cell(s, xp.x, xp.y, xp.z);
       store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
       if (s.nprev > 0) {
@@ -840,7 +843,7 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
           if (j == i) continue;
           const double4 Q = pos[j];
           const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
-          if (dot3(r, r) < s.pair[ci + (int)Q.w * s.nc].rc2) fn(j);
+          if (dot3(r, r) < (s.cut_all ? s.rc2_all : s.pair[ci + (int)Q.w * s.nc].rc2)) fn(j);
         }
       }
     }
@@ -1053,6 +1056,19 @@ __global__ void k_cells_to_ref(int np, const Kin* kin, const int* id, int* out) 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   out[id[i]] = kin[i].rc + 1;
+}
+
+// Reference cell and list rank of every particle, given by reference id: the
+// cell state of the last evaluation that host-side edits keep.
+__global__ void k_set_ref_cells(SysConst s, Kin* kin, const int2* cell_rank) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.np) return;
+  const int2 c = cell_rank[kin[i].id];
+  const int nxy = s.rn[0] * s.rn[1];
+  const int z = c.x / nxy, r = c.x - z * nxy, y = r / s.rn[0], x = r - y * s.rn[0];
+  kin[i].rc = c.x;
+  kin[i].st = c.y;
+  kin[i].rxyz = s.rc_pack ? (x | (y << 10) | (z << 20)) : 0;
 }
 
 // (cell, entry stamp) of every particle, by reference id.

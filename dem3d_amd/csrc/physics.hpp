@@ -1,4 +1,4 @@
-// physics.hpp -- per-contact arithmetic of the reference, as device functions.
+// physics.hpp -- per-contact arithmetic of the reference, for the kernels and the host.
 //
 // Every expression keeps the reference's association and the file is compiled
 // with floating-point contraction off, so each multiply and add rounds on its
@@ -8,18 +8,22 @@
 
 #include "common.hpp"
 
+// Host and device: the host-side particle-number changes (pouring tests
+// against walls, cell assignment) use the same arithmetic as the kernels.
+#define DEM_HD __host__ __device__ __forceinline__
+
 struct d3 {
   double x, y, z;
 };
 
-__device__ __forceinline__ d3 mk3(double a, double b, double c) { return d3{a, b, c}; }
-__device__ __forceinline__ d3 add3(d3 a, d3 b) { return d3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ d3 sub3(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ d3 scl3(double s, d3 a) { return d3{s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ d3 neg3(d3 a) { return d3{-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ d3 div3(d3 a, double s) { return d3{a.x / s, a.y / s, a.z / s}; }
+DEM_HD d3 mk3(double a, double b, double c) { return d3{a, b, c}; }
+DEM_HD d3 add3(d3 a, d3 b) { return d3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+DEM_HD d3 sub3(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+DEM_HD d3 scl3(double s, d3 a) { return d3{s * a.x, s * a.y, s * a.z}; }
+DEM_HD d3 neg3(d3 a) { return d3{-a.x, -a.y, -a.z}; }
+DEM_HD d3 div3(d3 a, double s) { return d3{a.x / s, a.y / s, a.z / s}; }
 // dot_product accumulates from zero in index order (src/MDimension3D.f90:75)
-__device__ __forceinline__ double dot3(d3 a, d3 b) {
+DEM_HD double dot3(d3 a, d3 b) {
   double s = 0.0;
   s = s + a.x * b.x;
   s = s + a.y * b.y;
@@ -27,22 +31,22 @@ __device__ __forceinline__ double dot3(d3 a, d3 b) {
   return s;
 }
 // CrossProduct (src/MDimension3D.f90:83-95)
-__device__ __forceinline__ d3 cross3(d3 a, d3 b) {
+DEM_HD d3 cross3(d3 a, d3 b) {
   return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ d3 vget(const double* p) { return d3{p[0], p[1], p[2]}; }
+DEM_HD d3 vget(const double* p) { return d3{p[0], p[1], p[2]}; }
 
 // Fortran nint of real(8) to default integer, back to real(8): half away
 // from zero; "+ 0.0" turns the -0.0 that round() gives in (-0.5, 0] into the
 // +0.0 of the integer round trip.
-__device__ __forceinline__ double fnint(double x) { return __builtin_round(x) + 0.0; }
+DEM_HD double fnint(double x) { return __builtin_round(x) + 0.0; }
 
 // Minimum image of one component (src/MForces.f90:212-214).
-__device__ __forceinline__ double min_image(double r, double l, double invl) {
+DEM_HD double min_image(double r, double l, double invl) {
   return r - l * fnint(r * invl);
 }
 
-__device__ __forceinline__ d3 min_image3(const SysConst& s, d3 r) {
+DEM_HD d3 min_image3(const SysConst& s, d3 r) {
   if (s.pbc[0]) r.x = min_image(r.x, s.lbox[0], s.invl[0]);
   if (s.pbc[1]) r.y = min_image(r.y, s.lbox[1], s.invl[1]);
   if (s.pbc[2]) r.z = min_image(r.z, s.lbox[2], s.invl[2]);
@@ -52,7 +56,7 @@ __device__ __forceinline__ d3 min_image3(const SysConst& s, d3 r) {
 // One coordinate of CurrentCell3D (src/MCells.f90:507-509):
 // max(0, min(int(n_by_l*x + n_by2), n-1)), with the float->int conversion
 // done only on in-range values.
-__device__ __forceinline__ int cell_coord(double x, double n_by_l, double n_by2, int n) {
+DEM_HD int cell_coord(double x, double n_by_l, double n_by2, int n) {
   double t = n_by_l * x + n_by2;
   if (!(t >= 1.0)) return 0;  // also catches NaN
   if (t >= (double)n) return n - 1;
@@ -65,7 +69,7 @@ __device__ __forceinline__ int cell_coord(double x, double n_by_l, double n_by2,
 struct RefCell {
   int lin, xyz;
 };
-__device__ __forceinline__ RefCell ref_cell(const SysConst& s, double x, double y, double z) {
+DEM_HD RefCell ref_cell(const SysConst& s, double x, double y, double z) {
   int ix = cell_coord(x, s.rn_by_l[0], s.rn_by2[0], s.rn[0]);
   int iy = cell_coord(y, s.rn_by_l[1], s.rn_by2[1], s.rn[1]);
   int iz = cell_coord(z, s.rn_by_l[2], s.rn_by2[2], s.rn[2]);
@@ -79,7 +83,7 @@ __device__ __forceinline__ RefCell ref_cell(const SysConst& s, double x, double 
 struct Int3 {
   int x, y, z;
 };
-__device__ __forceinline__ Int3 grid_coords(const SysConst& s, double x, double y, double z) {
+DEM_HD Int3 grid_coords(const SysConst& s, double x, double y, double z) {
   if (s.pbc[0]) x = min_image(x, s.lbox[0], s.invl[0]);
   if (s.pbc[1]) y = min_image(y, s.lbox[1], s.invl[1]);
   if (s.pbc[2]) z = min_image(z, s.lbox[2], s.invl[2]);
@@ -92,15 +96,15 @@ __device__ __forceinline__ Int3 grid_coords(const SysConst& s, double x, double 
 // Morton order inside a brick.  Particles are kept sorted by key, so the 27
 // cells around a particle are close to it in memory in all three directions
 // (with x-fastest cells, a z neighbour is a whole xy layer away).
-__device__ __forceinline__ int morton3_3bit(int x, int y, int z) {
+DEM_HD int morton3_3bit(int x, int y, int z) {
   auto spread = [](int v) { return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4); };
   return spread(x) | (spread(y) << 1) | (spread(z) << 2);
 }
-__device__ __forceinline__ int grid_key(const SysConst& s, int x, int y, int z) {
+DEM_HD int grid_key(const SysConst& s, int x, int y, int z) {
   const int b = ((z >> 3) * s.gb[1] + (y >> 3)) * s.gb[0] + (x >> 3);
   return (b << 9) | morton3_3bit(x & 7, y & 7, z & 7);
 }
-__device__ __forceinline__ int grid_cell(const SysConst& s, double x, double y, double z) {
+DEM_HD int grid_cell(const SysConst& s, double x, double y, double z) {
   const Int3 g = grid_coords(s, x, y, z);
   return grid_key(s, g.x, g.y, g.z);
 }
@@ -108,7 +112,7 @@ __device__ __forceinline__ int grid_cell(const SysConst& s, double x, double y, 
 // Signed offset from cell coordinate a to b along one reference-grid axis,
 // 2 if not adjacent.  Periodic axes wrap (VerifyPBC/ApplyPBC,
 // src/MCells.f90:1030-1052); the library requires >= 3 cells there.
-__device__ __forceinline__ int axis_offset(int a, int b, int n, int periodic) {
+DEM_HD int axis_offset(int a, int b, int n, int periodic) {
   int d = b - a;
   if (periodic) {
     if (d == n - 1) d = -1;
@@ -120,7 +124,7 @@ __device__ __forceinline__ int axis_offset(int a, int b, int n, int periodic) {
 struct CellXYZ {
   int x, y, z;
 };
-__device__ __forceinline__ CellXYZ ref_cell_xyz(const SysConst& s, int c, int packed) {
+DEM_HD CellXYZ ref_cell_xyz(const SysConst& s, int c, int packed) {
   if (s.rc_pack) return CellXYZ{packed & 1023, (packed >> 10) & 1023, packed >> 20};
   int nxy = s.rn[0] * s.rn[1];
   int z = c / nxy;
@@ -131,7 +135,7 @@ __device__ __forceinline__ CellXYZ ref_cell_xyz(const SysConst& s, int c, int pa
 
 // Are two reference cells the same or neighbours (i.e. does the reference's
 // cell loop visit the pair)?  c = linear index, x = packed coordinates.
-__device__ __forceinline__ bool ref_adjacent(const SysConst& s, int ca, int xa, int cb, int xb) {
+DEM_HD bool ref_adjacent(const SysConst& s, int ca, int xa, int cb, int xb) {
   if (ca == cb) return true;
   CellXYZ a = ref_cell_xyz(s, ca, xa), b = ref_cell_xyz(s, cb, xb);
   return axis_offset(a.x, b.x, s.rn[0], s.pbc[0]) != 2 &&
@@ -146,7 +150,7 @@ __device__ __forceinline__ bool ref_adjacent(const SysConst& s, int ca, int xa, 
 // the one whose cell lists the other cell among its 13 half-stencil cells
 // (kx + Nx*(ky + Ny*kz) > 0, src/MCells.f90:454-456).
 // Returns 1 if p is "i", 0 if q is, -1 if the cells are not adjacent.
-__device__ __forceinline__ int ref_role(const SysConst& s, int cp, int xp, int cq, int xq, int sp,
+DEM_HD int ref_role(const SysConst& s, int cp, int xp, int cq, int xq, int sp,
                                         int sq, int idp, int idq) {
   if (cp == cq) {
     if (sp != sq) return sp > sq;
@@ -161,7 +165,7 @@ __device__ __forceinline__ int ref_role(const SysConst& s, int cp, int xp, int c
 }
 
 // ChangePlane (src/MForces.f90:391-406)
-__device__ __forceinline__ d3 change_plane(d3 v, d3 n) {
+DEM_HD d3 change_plane(d3 v, d3 n) {
   d3 p = sub3(v, scl3(dot3(v, n), n));
   double nsq = dot3(p, p);
   if (nsq != 0.0) return scl3(sqrt(dot3(v, v) / nsq), p);
@@ -179,7 +183,7 @@ struct ContactOut {
 
 // CommonCalculations (src/MForces.f90:339-387), without the accumulation
 // into iForce/iTorque (done by the caller).
-__device__ __forceinline__ ContactOut common_calc(const ContactIn& c, d3 rij, double kn, double yn,
+DEM_HD ContactOut common_calc(const ContactIn& c, d3 rij, double kn, double yn,
                                                   double kt, double yt, double musq) {
   ContactOut o;
   d3 vijn = scl3(dot3(c.vijc, c.nij), c.nij);
@@ -217,7 +221,7 @@ struct WallHit {
   d3 nij, qji;
 };
 
-__device__ __forceinline__ WallHit wall_overlap(const SysConst& s, const WallGeom& w, d3 xi,
+DEM_HD WallHit wall_overlap(const SysConst& s, const WallGeom& w, d3 xi,
                                                 double ri, double max_alpha_u, double min_aw_sq,
                                                 double max_aw_sq) {
   WallHit h;

@@ -159,3 +159,15 @@ def test_removal_matches_reference(seed):
     assert nps == res["np_after"]
     assert len(removed) >= 15, len(removed)
     _compare(o, res["snapshots"][0], nps[-1])
+
+
+def test_product_ranf_equals_the_reference_vector():
+    """dem_ranf of the product library (host code, no device) against the
+    reference's own deviates."""
+    from dem3d_amd.native import load_hip
+    lib = load_hip()
+    f = np.load(os.path.join(GOLDEN, "ref_ranf.npz"), allow_pickle=False)
+    r = RngT.seeded(int(f["seed"]))
+    vals = np.array([lib.ranf(C.byref(r)) for _ in range(int(f["n"]))])
+    assert np.array_equal(vals, f["values"])
+    assert [r.dummy, r.iix, r.iiy, r.iik] == list(f["state"]) and r.aam == float(f["aam"])
