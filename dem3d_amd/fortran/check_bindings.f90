@@ -14,6 +14,8 @@ program check_bindings
   type(dem_config_t) :: cf
   type(dem_run_t) :: rn
   type(dem_state_sections_t) :: sc
+  type(dem_rng_t) :: rg
+  type(dem_pour_t) :: po
   type(c_ptr) :: h
   real(c_double), target :: radius(1), tab(1)
   integer(c_int) :: rc
@@ -26,6 +28,8 @@ program check_bindings
   write(*, '(a, i0)') 'sizeof dem_config_t ', c_sizeof(cf)
   write(*, '(a, i0)') 'sizeof dem_run_t ', c_sizeof(rn)
   write(*, '(a, i0)') 'sizeof dem_state_sections_t ', c_sizeof(sc)
+  write(*, '(a, i0)') 'sizeof dem_rng_t ', c_sizeof(rg)
+  write(*, '(a, i0)') 'sizeof dem_pour_t ', c_sizeof(po)
 
   radius = 0.5d0
   tab = 1.0d0

@@ -111,7 +111,22 @@ module dem3d_gpu
     type(c_ptr) :: wkeys = c_null_ptr, wft = c_null_ptr, wvt = c_null_ptr
   end type dem_state_sections_t
 
+  ! dem_rng_t: the generator state of MUtil (aam, Dummy, iix, iiy, iik).
+  type, bind(c), public :: dem_rng_t
+    real(c_double) :: aam = 0
+    integer(c_int32_t) :: dummy = 0, iix = -1, iiy = -1, iik = 0
+  end type dem_rng_t
+
+  integer(c_int32_t), parameter, public :: DEM_POUR_RECTANGLE = 1, DEM_POUR_CIRCLE = 2
+  ! dem_pour_t: the pouring specification (.prg), reduced.
+  type, bind(c), public :: dem_pour_t
+    type(c_ptr) :: target_number = c_null_ptr
+    integer(c_int32_t) :: shape = DEM_POUR_CIRCLE, max_attempts = 10
+    real(c_double) :: size(2) = 0, length_scale = 0, time_scale = 0
+  end type dem_pour_t
+
   public :: dem_create, dem_destroy, dem_last_error, dem_set_walls
+  public :: dem_ranf, dem_pour, dem_remove_escaped
   public :: dem_config_write, dem_config_read, dem_state_file_write, dem_state_file_read
   public :: dem_save_state_file, dem_load_state_file, dem_download_cell_rank
   public :: dem_set_wall_motion, dem_download_walls
@@ -242,6 +257,24 @@ module dem3d_gpu
       type(c_ptr), value :: h
       character(kind=c_char), intent(in) :: name(*)
       real(c_double), value :: value
+    end function
+    real(c_double) function dem_ranf(rng) bind(c, name="dem_ranf")
+      import :: c_double, dem_rng_t
+      type(dem_rng_t), intent(inout) :: rng
+    end function
+    integer(c_int) function dem_pour(h, pour, rng, inserted) bind(c, name="dem_pour")
+      import :: c_int, c_int32_t, c_ptr, dem_pour_t, dem_rng_t
+      type(c_ptr), value :: h
+      type(dem_pour_t), intent(in) :: pour
+      type(dem_rng_t), intent(inout) :: rng
+      integer(c_int32_t), intent(out) :: inserted
+    end function
+    integer(c_int) function dem_remove_escaped(h, cap, removed, nremoved) &
+        bind(c, name="dem_remove_escaped")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: removed(*), nremoved
     end function
     ! File names are C strings: pass trim(name)//c_null_char.
     integer(c_int) function dem_config_write(path, cfg) bind(c, name="dem_config_write")

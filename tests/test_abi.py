@@ -31,7 +31,8 @@ def c_sizes(tmp_path):
                    + "".join(f'printf("sizeof {t} %zu\\n", sizeof({t}));\n'
                              for t in ("dem_params_t", "dem_wall_t", "dem_particle_t",
                                        "dem_stats_t", "dem_wall_motion_t", "dem_config_t",
-                                       "dem_run_t", "dem_state_sections_t"))
+                                       "dem_run_t", "dem_state_sections_t", "dem_rng_t",
+                                       "dem_pour_t"))
                    + "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
@@ -60,6 +61,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert C.sizeof(native.ConfigT) == cs["dem_config_t"]
     assert C.sizeof(native.RunT) == cs["dem_run_t"]
     assert C.sizeof(native.SectionsT) == cs["dem_state_sections_t"]
+    assert C.sizeof(native.RngT) == cs["dem_rng_t"]
+    assert C.sizeof(native.PourT) == cs["dem_pour_t"]
 
 
 @pytest.mark.skipif(shutil.which("amdflang") is None or not os.path.exists(LIB),
