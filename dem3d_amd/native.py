@@ -88,6 +88,12 @@ class Stats(C.Structure):
         return out
 
 
+class FluidT(C.Structure):
+    _fields_ = [("fluid_density", C.c_double), ("viscosity", C.c_double),
+                ("flow_rate", C.c_double), ("nl", C.c_int32), ("pad_", C.c_int32),
+                ("density", _dp)]
+
+
 class RngT(C.Structure):
     _fields_ = [("aam", C.c_double), ("dummy", C.c_int32), ("iix", C.c_int32),
                 ("iiy", C.c_int32), ("iik", C.c_int32)]
@@ -181,6 +187,8 @@ class Library:
         "synchronize": (C.c_int, [C.c_void_p]),
         "download_cell_rank": (C.c_int, [C.c_void_p, C.c_int32, _ip]),
         "ranf": (C.c_double, [C.POINTER(RngT)]),
+        "set_fluid": (C.c_int, [C.c_void_p, C.POINTER(FluidT)]),
+        "download_fluid_field": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
         "pour": (C.c_int, [C.c_void_p, C.POINTER(PourT), C.POINTER(RngT), _ip]),
         "remove_escaped": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip]),
         "download_force_scale": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
@@ -335,6 +343,25 @@ class System:
             self._check(lib.set_walls(self.h, red.nw, self.walls))
             if getattr(red, "wall_motion", None):
                 self.set_wall_motion(red.wall_motion)
+        if getattr(red, "fluid", None):
+            self.set_fluid(red.fluid)
+
+    def set_fluid(self, fluid):
+        """FluidFlow with dict(density, viscosity, flow_rate, nl) (reduced);
+        None turns it off."""
+        if fluid is None:
+            self._check(self.lib.set_fluid(self.h, None))
+            return
+        self._dens = np.ascontiguousarray(self.red.density, np.float64)
+        f = FluidT(float(fluid["density"]), float(fluid["viscosity"]), float(fluid["flow_rate"]),
+                   int(fluid["nl"]), 0, self._dens.ctypes.data_as(_dp))
+        self._check(self.lib.set_fluid(self.h, C.byref(f)))
+
+    def download_fluid_field(self):
+        nl = int(self.red.fluid["nl"])
+        vf, ef = np.zeros(nl + 1), np.zeros(nl + 1)
+        self._check(self.lib.download_fluid_field(self.h, nl, _ptr(vf), _ptr(ef)))
+        return vf, ef
 
     def set_wall_motion(self, motion):
         """WallsInMovement with WM = motion (list of dicts, one per wall);

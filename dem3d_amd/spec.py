@@ -85,6 +85,12 @@ class SystemSpec:
     n_euler: int = 10
     fluid_density: float = 0.0
     walls_in_movement: bool = False  # WallsInMovement (src/DEM.f90:716)
+    # fluid block (src/mFluid.f90:29-57): FluidFlow, viscosity (Pa s), volume
+    # flow rate upward (m^3/s), number of layers NL
+    fluid_flow: bool = False
+    fluid_viscosity: float = 0.0
+    fluid_flow_rate: float = 0.0
+    fluid_layers: int = 0
 
 
 @dataclass
@@ -128,6 +134,7 @@ class Reduced:
     max_ncell: int = 30000
     ref_grid: Optional[Sequence[int]] = None
     wall_motion: Optional[list] = None  # WM(1:NW), reduced (list of dicts) or None
+    fluid: Optional[dict] = None  # FluidDensity, viscosity, flow_rate (reduced), nl
 
 
 def _gram_schmidt(u, w):
@@ -276,6 +283,13 @@ def reduce_spec(s: SystemSpec, max_ncell: int = 30000) -> Reduced:
         walls=walls, length_scale=L, mass_scale=M, time_scale=T, max_ncell=max_ncell,
         wall_motion=wall_motion,
     )
+    if s.fluid_flow:
+        # ScaleVariables and ScaleFluidProperties (src/DEM.f90:858,
+        # src/mFluid.f90:61-66)
+        r.fluid = dict(density=s.fluid_density * ((L * L) * L) / M,
+                       viscosity=s.fluid_viscosity * L * T / M,
+                       flow_rate=s.fluid_flow_rate * T / ((L * L) * L),
+                       nl=int(s.fluid_layers))
     initialize_force(r, s.fluid_density * ((L * L) * L) / M)
     return r
 

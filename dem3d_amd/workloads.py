@@ -285,6 +285,35 @@ def discharge(seed: int = 20240007, n_side=(8, 8, 6), gravity: float = -20000.0,
                     wl.v[:, perm], wl.w[:, perm], steps, note="open bottom, shuffled indices")
 
 
+def fluidized_bed(seed: int = 20240008, n_side=(6, 6, 6), velocity: float = 0.02,
+                  layers: int = 40, steps: int = 1000) -> Workload:
+    """Glass beads (1.0 / 1.2 mm, 2500 kg/m^3) in water (1000 kg/m^3,
+    1e-3 Pa s) flowing upward at `velocity` m/s superficial speed, in a box
+    periodic in x and y between a floor and a lid, with NL = `layers` fluid
+    layers: mFluid's porosity field and drag at every evaluation."""
+    wl = small_box(n_side=n_side, nc=2, ratio=1.2, seed=seed, vel=0.1, spacing=1.15, e=0.8)
+    spec = wl.spec
+    lx, ly, lz = spec.lbox
+    lz = 2.0 * lz
+    spec.lbox = [lx, ly, lz]
+    spec.walls = _floor(lx, ly, lz, 1e3, top=0.5 * lz)
+    nc, nw = len(spec.components), len(spec.walls)
+    spec.enw = np.full((nc, nw), 0.8)
+    spec.etw = np.full((nc, nw), 0.8)
+    spec.muw = np.full((nc, nw), 0.5)
+    spec.fluid_density = 1000.0
+    spec.fluid_flow = True
+    spec.fluid_viscosity = 1.0e-3
+    spec.fluid_flow_rate = velocity * lx * ly
+    spec.fluid_layers = layers
+    red = reduce_spec(spec)
+    L = red.length_scale
+    x = wl.x * wl.red.length_scale / L
+    x[2] = x[2] - 0.5 * (wl.spec.lbox[2] / L) * 0.5   # the bed in the lower half
+    return Workload(f"fluidized-{wl.np}", spec, red, wl.comp, x, wl.v, wl.w, steps,
+                    note="upward water flow, drag of src/mFluid.f90")
+
+
 def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,
                     steps: int = 10000, phi: float = 0.5) -> Workload:
     """One system of an ensemble: n spheres with radii U[0.9, 1.1] r0

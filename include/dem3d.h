@@ -261,6 +261,26 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *   "next_step"         sets the step number of the next batched step */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
+/* ---- fluid drag (mFluid) --------------------------------------------------- */
+/* An upward fluid flow through the box (src/mFluid.f90), reduced units.  At
+ * every force evaluation the porosity Ef and the fluid velocity Vf of NL+1
+ * horizontal layers are computed from the evaluated particles
+ * (UpdateVelocityField, :96-140), and every particle feels the drag of
+ * DragForce (:144-197) at the layer values interpolated to its height.  The
+ * buoyancy is in dem_params_t.eff_mass (EffectiveMass = m(rho - rho_f)/rho). */
+typedef struct {
+  double fluid_density;     /* FluidDensity                                    */
+  double viscosity;         /* FluidViscosity                                  */
+  double flow_rate;         /* FluidFlowRate (volume per time, upward)         */
+  int32_t nl;               /* NL: layers 0..NL over the box height (>= 1)     */
+  int32_t pad_;
+  const double* density;    /* [nc] Component%Density                          */
+} dem_fluid_t;
+/* FluidFlow = .true. with f; f = NULL turns it off. */
+int dem_set_fluid(dem_handle_t h, const dem_fluid_t* f);
+/* Vf(0:NL) and Ef(0:NL) of the last evaluation. */
+int dem_download_fluid_field(dem_handle_t h, int32_t nl, double* vf, double* ef);
+
 /* ---- particle number changes --------------------------------------------- */
 /* State of the reference's random-number generator (MUtil: aam, Dummy, iix,
  * iiy, iik).  A run starts with dummy = the (negative) seed of the

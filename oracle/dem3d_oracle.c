@@ -69,6 +69,12 @@ struct orc_system {
   dem_wall_t wall[MAXW];
   int wall_order[MAXW];
   int walls_in_movement;           /* WallsInMovement (src/MDEM.f90:36) */
+  /* fluid flow (mFluid) */
+  int fluid, nl;
+  double fluid_density, viscosity, flow_rate, area, delta_z, vf0, h_2, mh_2;
+  double density[MAXC];
+  double* vf;                      /* Vf(0:NL) */
+  double* ef;                      /* Ef(0:NL) */
   dem_wall_motion_t wm[MAXW];      /* WM(1:NW) */
   int64_t next_step;
   cone_t cone[MAXW];
@@ -470,6 +476,84 @@ static int overlap(const orc_handle_t h, const particle_t* ip, int jw, double ri
 
 /* ---- force evaluation (MForces) ------------------------------------------- */
 
+/* ---- fluid drag (mFluid) ------------------------------------------------------ */
+
+static const double PI_REF = 3.1415926535897932384626433832795; /* MUtil Pi */
+
+/* UpdateVelocityField (src/mFluid.f90:96-140): porosity and fluid velocity
+ * of the layers z_k = -H/2 + k*Delta_Z, k = 0..NL, from the evaluated
+ * particles (their Index: Predicted%Index in the corrector pass). */
+static void update_velocity_field(orc_handle_t h, const particle_t* p) {
+  for (int k = 0; k <= h->nl; k++) {
+    const double zk = h->mh_2 + (double)k * h->delta_z;
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+    for (int i = 0; i < h->np; i++) {
+      const int ind = p[i].index - 1;
+      const double d = zk - p[i].position[2];
+      const double diff2 = d * d;
+      const double r2 = h->radius[ind] * h->radius[ind];
+      if (diff2 < r2) {
+        const double ai = PI_REF * (r2 - diff2);
+        sa = sa + ai;
+        const double dai = ai * h->density[ind];
+        sb = sb + dai * p[i].velocity[2];
+        sc = sc + dai;
+      }
+    }
+    if (sa == 0.0) {
+      h->vf[k] = h->vf0;
+      h->ef[k] = 1.0;
+    } else {
+      const double es = sa / h->area;
+      h->ef[k] = 1.0 - es;
+      const double vs = sb / sc;
+      h->vf[k] = (h->vf0 - es * vs) / h->ef[k];
+    }
+  }
+}
+
+/* DragForce (src/mFluid.f90:144-197): Di Felice's drag with the porosity
+ * exponent found by fixed-point iteration.  Returns -1 when the particle is
+ * outside the layers (the reference would read past Vf). */
+static int drag_force(const orc_handle_t h, const particle_t* ip, double* fd) {
+  const double zi = ip->position[2];
+  const double jr = (zi + h->h_2) / h->delta_z;
+  const int j = (int)jr;
+  if (!(jr >= 0.0) || j + 1 > h->nl) return -1;
+  double vfz = h->vf[j];
+  vfz = vfz + (jr - (double)j) * (h->vf[j + 1] - vfz);
+  double efz = h->ef[j];
+  efz = efz + (jr - (double)j) * (h->ef[j + 1] - efz);
+  double vr[3];
+  for (int d = 0; d < 3; d++) vr[d] = -ip->velocity[d];
+  vr[2] = vfz + vr[2];
+  const double vra = sqrt(sumsq3(vr));
+  if (vra > 0.0) {
+    const double ri = h->radius[ip->index - 1];
+    const double re = 2.0 * efz * ri * vra * h->fluid_density / h->viscosity;
+    double n = 3.6, n0 = n + 1.0, x = 0.0;
+    while (fabs(n - n0) > 1.e-8) {
+      n0 = n;
+      x = re / pow(efz, n);
+      /* x**0.75 as the reference's compiler evaluates it: LLVM rewrites
+       * pow(x, 0.75) into sqrt(x)*sqrt(sqrt(x)) (within an ulp of pow) */
+      const double sx = sqrt(x);
+      const double x075 = sx * sqrt(sx);
+      n = (4.8 + 0.42 * x075) / (1.0 + 0.175 * x075);
+    }
+    double cd1 = 0.63 + 4.8 / sqrt(x);
+    cd1 = cd1 * cd1;
+    double cd = 0.63 + 4.8 / sqrt(re);
+    cd = cd * cd;
+    const double fi = (cd1 / cd) * pow(efz, 2.0 * (1.0 - n));
+    const double c = cd * (PI_REF / 2.0) * ri * ri * h->fluid_density * vra;
+    for (int d = 0; d < 3; d++) fd[d] = c * vr[d] * fi;
+  } else {
+    fd[0] = fd[1] = fd[2] = 0.0;
+  }
+  return 0;
+}
+
 /* ChangePlane (src/MForces.f90:391-406) */
 static void change_plane(const double* v, const double* n, double* out) {
   double vn = dot3(v, n);
@@ -552,6 +636,7 @@ static int compute_accelerations(orc_handle_t h, particle_t* p, double time_step
   memset(h->fscale, 0, sizeof(double) * 2 * np);
   int64_t npairs = 0, nwallc = 0;
   double max_ov = 0.0;
+  if (h->fluid) update_velocity_field(h, p);
   for (int icell = 0; icell < h->nt; icell++) {
     int npcc;
     int npsc = neighbour_particles(h, icell, &npcc);
@@ -640,6 +725,12 @@ static int compute_accelerations(orc_handle_t h, particle_t* p, double time_step
         } else {
           if (c.contacting && upd) remove_entry(&pr);
         }
+      }
+      if (h->fluid) {
+        double fd[3];
+        if (drag_force(h, ip, fd))
+          return fail(h, DEM_ERR_UNSUPPORTED, "particle %d is outside the fluid layers", i + 1);
+        for (int k = 0; k < 3; k++) c.i_force[k] = c.i_force[k] + fd[k];
       }
       double* fi = &h->force[3 * i];
       double* ti = &h->torque[3 * i];
@@ -1188,6 +1279,7 @@ void orc_destroy(orc_handle_t h) {
   if (!h) return;
   free_particles(h);
   free(h->cell_head); free(h->nbr_cells); free(h->n_nbr_cells); free(h->nbr_list);
+  free(h->vf); free(h->ef);
   free(h);
 }
 
@@ -1225,6 +1317,38 @@ int orc_set_wall_motion(orc_handle_t h, int32_t nw, const dem_wall_motion_t* m) 
 int orc_download_walls(orc_handle_t h, int32_t nw, dem_wall_t* w) {
   if (nw != h->nw) return fail(h, DEM_ERR_ARG, "nw mismatch");
   for (int j = 0; j < nw; j++) w[j] = h->wall[j];
+  return DEM_OK;
+}
+
+/* FluidFlow, the specification of ReadFluidSpecifications (scaled) and
+ * InitializeFluid (src/mFluid.f90:29-92). */
+int orc_set_fluid(orc_handle_t h, const dem_fluid_t* f) {
+  free(h->vf);
+  free(h->ef);
+  h->vf = h->ef = NULL;
+  h->fluid = 0;
+  if (!f) return DEM_OK;
+  if (f->nl < 1 || !f->density) return fail(h, DEM_ERR_ARG, "bad fluid specification");
+  h->fluid = 1;
+  h->nl = f->nl;
+  h->fluid_density = f->fluid_density;
+  h->viscosity = f->viscosity;
+  h->flow_rate = f->flow_rate;
+  for (int c = 0; c < h->nc; c++) h->density[c] = f->density[c];
+  h->area = h->lbox[0] * h->lbox[1];
+  h->delta_z = h->lbox[2] / (double)h->nl;
+  h->vf0 = h->flow_rate / h->area;
+  h->h_2 = h->lbox[2] / 2.0;
+  h->mh_2 = -h->h_2;
+  h->vf = (double*)calloc((size_t)h->nl + 1, sizeof(double));
+  h->ef = (double*)calloc((size_t)h->nl + 1, sizeof(double));
+  return DEM_OK;
+}
+
+int orc_download_fluid_field(orc_handle_t h, int32_t nl, double* vf, double* ef) {
+  if (!h->fluid || nl != h->nl) return fail(h, DEM_ERR_ARG, "no fluid with %d layers", nl);
+  memcpy(vf, h->vf, sizeof(double) * (size_t)(nl + 1));
+  memcpy(ef, h->ef, sizeof(double) * (size_t)(nl + 1));
   return DEM_OK;
 }
 

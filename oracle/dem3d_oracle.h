@@ -32,6 +32,8 @@ int orc_set_wall_motion(orc_handle_t h, int32_t nw, const dem_wall_motion_t* mot
 int orc_download_walls(orc_handle_t h, int32_t nw, dem_wall_t* walls);
 int orc_set_option(orc_handle_t h, const char* name, double value);
 double orc_ranf(dem_rng_t* rng);
+int orc_set_fluid(orc_handle_t h, const dem_fluid_t* f);
+int orc_download_fluid_field(orc_handle_t h, int32_t nl, double* vf, double* ef);
 int orc_pour(orc_handle_t h, const dem_pour_t* pour, dem_rng_t* rng, int32_t* inserted);
 int orc_remove_escaped(orc_handle_t h, int32_t cap, int32_t* removed, int32_t* nremoved);
 int orc_upload_state(orc_handle_t h, int32_t np, const int32_t* comp,

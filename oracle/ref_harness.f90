@@ -39,6 +39,9 @@ use BinFiles
 
 implicit none
 
+! set by ref_prepare_fluid: ref_init keeps FluidDensity and FluidFlow
+logical, private :: keep_fluid = .false.
+
 contains
 
   subroutine c2f_string( cstr, n, fstr )
@@ -86,8 +89,10 @@ contains
     InvLbox = 1.0_rb/Lbox
     PBC = pbc_ /= 0
     Gravity = gravity_
-    FluidDensity = 0.0_rb
-    FluidFlow = .false.
+    if (.not.keep_fluid) then
+      FluidDensity = 0.0_rb
+      FluidFlow = .false.
+    end if
     allocate( Kn(NC,NC), Yn(NC,NC), Kt(NC,NC), Yt(NC,NC), Mu(NC,NC) )
     Kn = kn_ ; Yn = yn_ ; Kt = kt_ ; Yt = yt_ ; Mu = mu_
     allocate( Knw(NC,NW), Ynw(NC,NW), Ktw(NC,NW), Ytw(NC,NW), Muw(NC,NW) )
@@ -101,6 +106,27 @@ contains
     WallsInMovement = .false.
     call InitializeAuxiliaries
   end subroutine ref_init
+
+  ! Fluid flow before ref_init, in the order of the driver: FluidDensity
+  ! (reduced), ReadFluidSpecifications on the text of the specification's
+  ! fluid block (SI), ScaleFluidProperties with the given scales
+  ! (src/DEM.f90:596,737,858-864).  ref_init then runs InitializeForce,
+  ! which calls InitializeFluid.
+  subroutine ref_prepare_fluid( rho, path, n, L, T, M ) bind(C, name="ref_prepare_fluid")
+    real(c_double), value :: rho, L, T, M
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int32_t), value :: n
+    character(len=1024) :: f
+    integer, parameter :: u = 74
+    call c2f_string( path, n, f )
+    FluidDensity = rho
+    LengthScale = L ; TimeScale = T ; MassScale = M
+    open( unit=u, file=trim(f), status='old' )
+    call ReadFluidSpecifications( u, 'Error reading the fluid block:' )
+    close( u )
+    if (FluidFlow) call ScaleFluidProperties
+    keep_fluid = .true.
+  end subroutine ref_prepare_fluid
 
   ! One wall (TWall, lengths already halved); ConfigureWalls after the last.
   subroutine ref_set_wall( j, shape, l1, l2, l3, u, w, n, pos, vel, omg ) &

@@ -177,6 +177,21 @@ def _ref_job_body(job):
         keep.append(a)
         return P(a)
 
+    fl = job.get("fluid_si")
+    if fl is not None:
+        # the specification's fluid block as ReadFluidSpecifications reads it
+        import tempfile as _tf
+        fd, fpath = _tf.mkstemp(suffix=".fluid", dir=os.path.join(ROOT, ".scratch")
+                                if os.path.isdir(os.path.join(ROOT, ".scratch")) else None)
+        with os.fdopen(fd, "w") as f:
+            f.write("[Simulate fluid flow?]\nyes\n[Fluid viscosity]\n%.17g\n"
+                    "[Flow rate]\n%.17g\n[Layers]\n%d\n" % (fl["viscosity"], fl["flow_rate"],
+                                                            int(fl["layers"])))
+        b = fpath.encode()
+        lib.ref_prepare_fluid(C.c_double(red.fluid["density"]), C.c_char_p(b), C.c_int32(len(b)),
+                              C.c_double(red.length_scale), C.c_double(red.time_scale),
+                              C.c_double(red.mass_scale))
+        os.remove(fpath)
     pbc = np.array(red.pbc, np.int32)
     lbox = _d(red.lbox)
     grav = _d(red.gravity)

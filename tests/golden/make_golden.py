@@ -23,6 +23,9 @@ stores inputs and outputs as .npz (numpy arrays only, no pickles):
                     (ReadState + 15 MoveParticles)
   ref_ranf.npz      1 000 deviates of the reference's generator ranf from the
                     seed -20, and its state afterwards
+  ref_fluid.npz     the fluidized bed of the benchmark suite (upward water
+                    flow, mFluid's layers and drag): one ComputeAccelerations,
+                    the start-up and 300 steps
 
 Usage (in a container that has the reference checkout and the built library):
     python tests/golden/make_golden.py [fixture.npz ...]   (default: all)
@@ -198,6 +201,24 @@ def ranf_fixture():
     print(f"wrote {path}")
 
 
+def fluid_fixture():
+    wl = workloads.fluidized_bed()
+    sp = wl.spec
+    actions = [("compute", 1), ("snapshot", "c"), ("euler",), ("euler",), ("euler",),
+               ("snapshot", "e"), ("step", 4, 300), ("snapshot", "s")]
+    res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, actions=actions,
+                             fluid_si=dict(viscosity=sp.fluid_viscosity,
+                                           flow_rate=sp.fluid_flow_rate,
+                                           layers=sp.fluid_layers)))
+    out = {"note": np.array("fluidized_bed(seed=20240008)")}
+    for tag, snap in zip(("c", "e", "s"), res["snapshots"]):
+        for k in ("x", "v", "w", "a", "alpha"):
+            out[f"{tag}_{k}"] = snap[k]
+    path = os.path.join(HERE, "ref_fluid.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path}")
+
+
 def main(names):
     def want(name):
         return not names or name in names
@@ -223,6 +244,8 @@ def main(names):
                  [("euler",), ("euler",), ("euler",), ("snapshot", "e"), ("step", 4, 30),
                   ("snapshot", "s")],
                  "cone hopper + floor, 216 spheres")
+    if want("ref_fluid.npz"):
+        fluid_fixture()
     if want("ref_ranf.npz"):
         ranf_fixture()
     if want("ref_resume.npz"):
