@@ -38,6 +38,12 @@ struct WallMotion {
   double cf[3], amp[3], vamp[3], tpf[3], phase[3];
 };
 
+// The fluid of mFluid (src/mFluid.f90:9-23, InitializeFluid :70-92), reduced.
+struct FluidConst {
+  double area, delta_z, vf0, h_2, mh_2, rho_f, mu;
+  int nl, pad;
+};
+
 // Device-side control block: flags and counters written by kernels.
 struct Ctl {
   int rebuild;          // neighbour list must be rebuilt before the next pass
@@ -66,6 +72,7 @@ enum {
   DEM_KERR_ROLE = 8,
   DEM_KERR_CONE_AXIS = 16,
   DEM_KERR_NAN = 32,
+  DEM_KERR_FLUID = 64,  // a particle outside the fluid layers
 };
 
 // Constants of one system, passed by value to every kernel.

@@ -67,6 +67,16 @@ struct dem_system {
   };
   std::vector<Orphan> orphans;  // stored pairs naming a particle >= NP
   int pending_cut_all = 0;      // SysConst.cut_all for the next upload (host edits)
+  // fluid flow (mFluid)
+  bool fluid = false;
+  FluidConst fc{};
+  double* d_dens = nullptr;     // Component%Density
+  double* d_vf = nullptr;       // Vf(0:NL), Ef(0:NL)
+  double* d_ef = nullptr;
+  double* d_fpart = nullptr;    // per-block layer sums
+  int2* d_frange = nullptr;     // per-block layer range
+  double* d_drag = nullptr;     // drag of every particle
+  size_t fpart_cap = 0, frange_cap = 0, drag_cap = 0;
   int64_t next_step = 0;       // step number of the next batched step
   // state: buffer cur is Particle, cur^1 Predicted (or Euler scratch)
   int np = 0;
@@ -365,6 +375,8 @@ void dem_destroy(dem_handle_t h) {
   free_state(h);
   dfree(h->d_pair); dfree(h->d_comp); dfree(h->d_wallc); dfree(h->d_walls); dfree(h->d_wm);
   dfree(h->d_pred_w);
+  dfree(h->d_dens); dfree(h->d_vf); dfree(h->d_ef); dfree(h->d_fpart); dfree(h->d_frange);
+  dfree(h->d_drag);
   dfree(h->ctl); dfree(h->rb);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -456,6 +468,49 @@ int dem_set_wall_motion(dem_handle_t h, int32_t nw, const dem_wall_motion_t* m) 
   }
   HIPCHK(h, hipMemcpy(h->d_wm, wm.data(), wm.size() * sizeof(WallMotion), hipMemcpyHostToDevice));
   h->nw_move = nw;
+  return DEM_OK;
+}
+
+// FluidFlow and InitializeFluid (src/mFluid.f90:29-92).
+int dem_set_fluid(dem_handle_t h, const dem_fluid_t* f) {
+  if (!h) return DEM_ERR_ARG;
+  if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (!f) {
+    h->fluid = false;
+    return DEM_OK;
+  }
+  if (f->nl < 1 || !f->density || !(f->viscosity > 0.0))
+    return fail(h, DEM_ERR_ARG, "fluid needs nl >= 1, component densities and viscosity > 0");
+  const SysConst& s = h->sc;
+  FluidConst c{};
+  c.area = s.lbox[0] * s.lbox[1];
+  c.delta_z = s.lbox[2] / (double)f->nl;
+  c.vf0 = f->flow_rate / c.area;
+  c.h_2 = s.lbox[2] / 2.0;
+  c.mh_2 = -c.h_2;
+  c.rho_f = f->fluid_density;
+  c.mu = f->viscosity;
+  c.nl = f->nl;
+  dfree(h->d_dens); dfree(h->d_vf); dfree(h->d_ef);
+  h->d_dens = h->d_vf = h->d_ef = nullptr;
+  HIPCHK(h, dalloc(&h->d_dens, (size_t)std::max(s.nc, 1)));
+  HIPCHK(h, dalloc(&h->d_vf, (size_t)f->nl + 1));
+  HIPCHK(h, dalloc(&h->d_ef, (size_t)f->nl + 1));
+  HIPCHK(h, hipMemcpy(h->d_dens, f->density, (size_t)s.nc * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(h->d_vf, 0, ((size_t)f->nl + 1) * sizeof(double)));
+  HIPCHK(h, hipMemset(h->d_ef, 0, ((size_t)f->nl + 1) * sizeof(double)));
+  h->fc = c;
+  h->fluid = true;
+  return DEM_OK;
+}
+
+int dem_download_fluid_field(dem_handle_t h, int32_t nl, double* vf, double* ef) {
+  if (!h || !vf || !ef) return DEM_ERR_ARG;
+  if (!h->fluid || nl != h->fc.nl) return fail(h, DEM_ERR_ARG, "no fluid with %d layers", nl);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(vf, h->d_vf, ((size_t)nl + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(ef, h->d_ef, ((size_t)nl + 1) * sizeof(double), hipMemcpyDeviceToHost));
   return DEM_OK;
 }
 
@@ -665,6 +720,7 @@ static int check_errors(dem_handle_t h) {
     if (c.error & DEM_KERR_ROLE) m += "overlapping pair in non-adjacent reference cells; ";
     if (c.error & DEM_KERR_CONE_AXIS) m += "particle on the axis of a cone wall; ";
     if (c.error & DEM_KERR_NAN) m += "non-finite position; ";
+    if (c.error & DEM_KERR_FLUID) m += "particle outside the fluid layers; ";
     int err = c.error;
     HIPCHK(h, hipMemsetAsync(&h->ctl->error, 0, sizeof(int), h->stream));
     return fail(h, (err & DEM_KERR_NBR_OVERFLOW) ? DEM_ERR_CAPACITY : DEM_ERR_STATE, "%s",
@@ -1438,9 +1494,44 @@ static PassArgs base_args(dem_handle_t h) {
   return a;
 }
 
+// UpdateVelocityField and DragForce of the evaluated state (pos, kin) into
+// h->d_drag (src/MForces.f90:170,316).
+static void enqueue_fluid(dem_handle_t h, const double4* pos, const Kin* kin) {
+  const int nb = nblocks(h->np);
+  const size_t rows = (size_t)nb * (size_t)(h->fc.nl + 1);
+  if (h->fpart_cap < 3 * rows) {
+    dfree(h->d_fpart);
+    h->d_fpart = nullptr;
+    h->fpart_cap = dalloc(&h->d_fpart, 3 * rows) == hipSuccess ? 3 * rows : 0;
+  }
+  if (h->frange_cap < (size_t)nb) {
+    dfree(h->d_frange);
+    h->d_frange = nullptr;
+    h->frange_cap = dalloc(&h->d_frange, (size_t)nb) == hipSuccess ? (size_t)nb : 0;
+  }
+  if (h->drag_cap < 3 * (size_t)h->np) {
+    dfree(h->d_drag);
+    h->d_drag = nullptr;
+    h->drag_cap = dalloc(&h->d_drag, 3 * (size_t)h->np) == hipSuccess ? 3 * (size_t)h->np : 0;
+  }
+  if (!h->fpart_cap || !h->frange_cap || !h->drag_cap) return;
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL(k_fluid_partial, dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, h->fc, pos, kin,
+                     (const double*)h->d_dens, h->d_fpart, h->d_frange);
+  hipLaunchKernelGGL(k_fluid_field, dim3(nblocks(h->fc.nl + 1, 64)), dim3(64), 0, st, h->fc, nb,
+                     (const double*)h->d_fpart, (const int2*)h->d_frange, h->d_vf, h->d_ef);
+  hipLaunchKernelGGL(k_fluid_drag, dim3(nb), dim3(DEM_BLOCK), 0, st, h->sc, h->fc, pos, kin,
+                     (const double*)h->d_vf, (const double*)h->d_ef, h->d_drag, h->ctl);
+}
+
 // One force evaluation: the pass kernel, then the sum of its counters.
 template <int MODE, bool UPDATE>
-static void launch_pass(dem_handle_t h, const PassArgs& a) {
+static void launch_pass(dem_handle_t h, const PassArgs& a0) {
+  PassArgs a = a0;
+  if (h->fluid) {
+    enqueue_fluid(h, a.pos_in, a.kin_in);
+    a.drag = h->drag_cap ? h->d_drag : nullptr;
+  }
   const int nb = nblocks(h->np);
   if (h->mixed)
     hipLaunchKernelGGL((k_pass<MODE, UPDATE, true>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream,
@@ -2545,6 +2636,7 @@ int dem_batch_add(dem_batch_t b, dem_handle_t h) {
   if (!b || !h) return bfail(b, DEM_ERR_ARG, "null argument");
   if (!h->has_state || h->np == 0) return bfail(b, DEM_ERR_STATE, "upload a non-empty state first");
   if (h->in_batch) return bfail(b, DEM_ERR_STATE, "the system already belongs to a batch");
+  if (h->fluid) return bfail(b, DEM_ERR_UNSUPPORTED, "systems with fluid flow are stepped alone");
   const int prec = h->mixed ? DEM_MIXED : DEM_FP64;
   if (b->precision >= 0 && prec != b->precision)
     return bfail(b, DEM_ERR_ARG, "all systems of a batch use the same precision mode");

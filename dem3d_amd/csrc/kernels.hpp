@@ -143,6 +143,8 @@ struct PassArgs {
   // PRED: Predicted%Index by reference id when it differs from Particle%Index
   // (after removals, until the next resize, src/MDEM.f90:636-640); else null
   const double* pred_w;
+  // fluid drag of every particle of this evaluation (k_fluid_drag), or null
+  const double* drag;
 };
 
 __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
@@ -395,6 +397,8 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       }
       if (UPDATE && wf_new != wf) a.wflag[i] = wf_new;
     }
+    // iForce = iForce + DragForce(iPart) (src/MForces.f90:316)
+    if (a.drag) F = add3(F, vget(a.drag + 3 * (size_t)i));
     // accelerations (src/MForces.f90:330-333)
     const d3 acc = scl3(cc.inv_mass, F);
     const d3 alp = scl3(cc.inv_inertia, T);
@@ -1056,6 +1060,171 @@ __global__ void k_cells_to_ref(int np, const Kin* kin, const int* id, int* out) 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   out[id[i]] = kin[i].rc + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Fluid drag (src/mFluid.f90).
+//
+// UpdateVelocityField (:96-140) sums, for every layer z_k = -H/2 + k*Delta_Z,
+// the cross-sections pi(r^2 - (z_k - z)^2) of the particles it cuts (and
+// their density-weighted vertical velocities).  Each block sums its own
+// particles into the layers they cut, in a fixed order; k_fluid_field adds
+// the blocks in index order.  The sums are deterministic but associate
+// differently from the reference's loop over particles (an ulp-level
+// difference).
+
+#define DEM_PI_REF 3.1415926535897932384626433832795
+
+__device__ __forceinline__ double block_sum_det(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (t == 0)
+    for (int w = 0; w < DEM_BLOCK / 64; ++w) r = r + sh[w];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(DEM_BLOCK) void k_fluid_partial(SysConst s, FluidConst f,
+                                                             const double4* pos, const Kin* kin,
+                                                             const double* dens, double* part,
+                                                             int2* range) {
+  __shared__ double sh[DEM_BLOCK / 64];
+  __shared__ int klo_sh[DEM_BLOCK / 64], khi_sh[DEM_BLOCK / 64];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * DEM_BLOCK + t;
+  const bool active = i < s.np;
+  double z = 0.0, r = 0.0, vz = 0.0, rho = 0.0;
+  int klo = 0x7fffffff, khi = -1;
+  if (active) {
+    const double4 P = pos[i];
+    const int c = (int)P.w;
+    z = P.z;
+    r = s.comp[c].radius;
+    rho = dens[c];
+    vz = kin[i].v1.x;
+    // layers the particle may cut, one more each side for the rounding
+    const double lo = (z - r - f.mh_2) / f.delta_z, hi = (z + r - f.mh_2) / f.delta_z;
+    klo = lo < 0.0 ? 0 : ((int)lo > f.nl ? f.nl + 1 : (int)lo);
+    khi = hi < 0.0 ? -1 : ((int)hi + 1 > f.nl ? f.nl : (int)hi + 1);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    klo = min(klo, __shfl_xor(klo, o, 64));
+    khi = max(khi, __shfl_xor(khi, o, 64));
+  }
+  if ((t & 63) == 0) {
+    klo_sh[t >> 6] = klo;
+    khi_sh[t >> 6] = khi;
+  }
+  __syncthreads();
+  int bl = klo_sh[0], bh = khi_sh[0];
+  for (int w = 1; w < DEM_BLOCK / 64; ++w) {
+    bl = min(bl, klo_sh[w]);
+    bh = max(bh, khi_sh[w]);
+  }
+  const size_t row = (size_t)blockIdx.x * (size_t)(f.nl + 1);
+  for (int k = bl; k <= bh; ++k) {
+    const double zk = f.mh_2 + (double)k * f.delta_z;
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (active) {
+      const double d = zk - z;
+      const double diff2 = d * d, r2 = r * r;
+      if (diff2 < r2) {
+        a = DEM_PI_REF * (r2 - diff2);
+        const double dai = a * rho;
+        b = dai * vz;
+        c = dai;
+      }
+    }
+    a = block_sum_det(a, sh);
+    b = block_sum_det(b, sh);
+    c = block_sum_det(c, sh);
+    if (t == 0) {
+      double* q = part + 3 * (row + (size_t)k);
+      q[0] = a;
+      q[1] = b;
+      q[2] = c;
+    }
+  }
+  if (t == 0) range[blockIdx.x] = make_int2(bl, bh);
+}
+
+// Vf(k), Ef(k) (src/mFluid.f90:128-136), one thread per layer.
+__global__ void k_fluid_field(FluidConst f, int nblk, const double* part, const int2* range,
+                              double* vf, double* ef) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > f.nl) return;
+  double sa = 0.0, sb = 0.0, sc = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    const int2 rg = range[b];
+    if (k < rg.x || k > rg.y) continue;
+    const double* q = part + 3 * ((size_t)b * (size_t)(f.nl + 1) + (size_t)k);
+    sa = sa + q[0];
+    sb = sb + q[1];
+    sc = sc + q[2];
+  }
+  if (sa == 0.0) {
+    vf[k] = f.vf0;
+    ef[k] = 1.0;
+  } else {
+    const double es = sa / f.area;
+    const double e = 1.0 - es;
+    ef[k] = e;
+    const double vs = sb / sc;
+    vf[k] = (f.vf0 - es * vs) / e;
+  }
+}
+
+// DragForce (src/mFluid.f90:144-197) of every evaluated particle.
+__global__ void k_fluid_drag(SysConst s, FluidConst f, const double4* pos, const Kin* kin,
+                             const double* vf, const double* ef, double* drag, Ctl* ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.np) return;
+  const double4 P = pos[i];
+  const Kin K = kin[i];
+  double fd[3] = {0.0, 0.0, 0.0};
+  const double jr = (P.z + f.h_2) / f.delta_z;
+  const int j = (int)jr;
+  if (!(jr >= 0.0) || j + 1 > f.nl) {
+    atomicOr(&ctl->error, DEM_KERR_FLUID);
+  } else {
+    double vfz = vf[j];
+    vfz = vfz + (jr - (double)j) * (vf[j + 1] - vfz);
+    double efz = ef[j];
+    efz = efz + (jr - (double)j) * (ef[j + 1] - efz);
+    const d3 v = kin_v(K);
+    d3 vr = neg3(v);
+    vr.z = vfz + vr.z;
+    const double vra = sqrt(dot3(vr, vr));
+    if (vra > 0.0) {
+      const double ri = s.comp[(int)P.w].radius;
+      const double re = 2.0 * efz * ri * vra * f.rho_f / f.mu;
+      double n = 3.6, n0 = n + 1.0, x = 0.0;
+      // the fixed point converges in a few iterations; the bound keeps the
+      // kernel finite for inputs where it would not
+      for (int it = 0; it < 200 && fabs(n - n0) > 1.e-8; ++it) {
+        n0 = n;
+        x = re / pow(efz, n);
+        const double sx = sqrt(x);
+        const double x075 = sx * sqrt(sx);  // x**0.75 as the reference's compiler forms it
+        n = (4.8 + 0.42 * x075) / (1.0 + 0.175 * x075);
+      }
+      double cd1 = 0.63 + 4.8 / sqrt(x);
+      cd1 = cd1 * cd1;
+      double cd = 0.63 + 4.8 / sqrt(re);
+      cd = cd * cd;
+      const double fi = (cd1 / cd) * pow(efz, 2.0 * (1.0 - n));
+      const double c = cd * (DEM_PI_REF / 2.0) * ri * ri * f.rho_f * vra;
+      fd[0] = c * vr.x * fi;
+      fd[1] = c * vr.y * fi;
+      fd[2] = c * vr.z * fi;
+    }
+  }
+  drag[3 * (size_t)i] = fd[0];
+  drag[3 * (size_t)i + 1] = fd[1];
+  drag[3 * (size_t)i + 2] = fd[2];
 }
 
 // Reference cell and list rank of every particle, given by reference id: the

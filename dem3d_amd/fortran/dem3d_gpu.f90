@@ -111,6 +111,13 @@ module dem3d_gpu
     type(c_ptr) :: wkeys = c_null_ptr, wft = c_null_ptr, wvt = c_null_ptr
   end type dem_state_sections_t
 
+  ! dem_fluid_t: FluidFlow of mFluid, reduced.
+  type, bind(c), public :: dem_fluid_t
+    real(c_double) :: fluid_density = 0, viscosity = 0, flow_rate = 0
+    integer(c_int32_t) :: nl = 0, pad_ = 0
+    type(c_ptr) :: density = c_null_ptr
+  end type dem_fluid_t
+
   ! dem_rng_t: the generator state of MUtil (aam, Dummy, iix, iiy, iik).
   type, bind(c), public :: dem_rng_t
     real(c_double) :: aam = 0
@@ -126,7 +133,7 @@ module dem3d_gpu
   end type dem_pour_t
 
   public :: dem_create, dem_destroy, dem_last_error, dem_set_walls
-  public :: dem_ranf, dem_pour, dem_remove_escaped
+  public :: dem_ranf, dem_pour, dem_remove_escaped, dem_set_fluid, dem_download_fluid_field
   public :: dem_config_write, dem_config_read, dem_state_file_write, dem_state_file_read
   public :: dem_save_state_file, dem_load_state_file, dem_download_cell_rank
   public :: dem_set_wall_motion, dem_download_walls
@@ -257,6 +264,20 @@ module dem3d_gpu
       type(c_ptr), value :: h
       character(kind=c_char), intent(in) :: name(*)
       real(c_double), value :: value
+    end function
+    ! FluidFlow with the specification's fluid (the C entry also takes a
+    ! null pointer to turn it off).
+    integer(c_int) function dem_set_fluid(h, f) bind(c, name="dem_set_fluid")
+      import :: c_int, c_ptr, dem_fluid_t
+      type(c_ptr), value :: h
+      type(dem_fluid_t), intent(in) :: f
+    end function
+    integer(c_int) function dem_download_fluid_field(h, nl, vf, ef) &
+        bind(c, name="dem_download_fluid_field")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nl
+      real(c_double), intent(out) :: vf(0:*), ef(0:*)
     end function
     real(c_double) function dem_ranf(rng) bind(c, name="dem_ranf")
       import :: c_double, dem_rng_t
