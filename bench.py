@@ -98,6 +98,13 @@ def timed_run(wl, args, precision, hmode):
     g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode, precision=precision)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
+    tune = None
+    if args.tune_steps > 0:
+        # dem_autotune: every launch shape of the passes timed on the live bed
+        # (these are ordinary steps: the trajectory does not change)
+        tune = g.autotune(steps_per_shape=args.tune_steps)
+        print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}, "
+              f"ms {[round(m, 4) for m in tune['ms']]}", file=sys.stderr, flush=True)
     if args.warmup:
         g.step(args.warmup)
     g.lib.set_option(g.h, b"timing", 1.0)
@@ -109,7 +116,8 @@ def timed_run(wl, args, precision, hmode):
     st = g.stats()
     sec = t1 - t0
     out = {"ms": 1e3 * sec / args.steps, "value": wl.np * args.steps / sec, "stats": st,
-           "x": g.download()["x"] if args.precision == "both" else None}
+           "x": g.download()["x"] if args.precision == "both" else None, "tune": tune,
+           "steps_before": g.next_step - args.steps - wl.red.method_order}
     g.close()
     return out
 
@@ -135,28 +143,39 @@ def ensemble_run(args):
             s.synchronize()
         return out
 
-    def batched(systems, steps):
+    def batched(systems, steps, graph=True):
+        """Autotuned launch shape (graph replays) or the default shape
+        launched kernel by kernel; throughput over `steps` steps, then the
+        per-step device times of args.latency_steps more steps."""
         b = Batch(lib)
-        b.set_timing(True)
+        b.set_option("graph", 1 if graph else 0)
         for s in systems:
             b.add(s)
-        b.step(20)  # warm-up
+        b.step(20)  # warm-up (and the graph capture)
+        tune = b.autotune(steps_per_shape=args.tune_steps) if graph and args.tune_steps else None
+        b.step(8)
         t0 = time.perf_counter()
         b.step(steps)
         sec = time.perf_counter() - t0
+        b.set_timing(True)
+        b.step(args.latency_steps)
         st = b.step_times()
         b.close()
-        return ntot * steps / sec, st
+        return ntot * steps / sec, st, tune
 
     res = {"systems": len(members), "spheres": ntot,
            "sizes": [int(min(w.np for w in members)), int(max(w.np for w in members))]}
     sysm = build(DEM_MIXED)
-    v_mx, st_mx = batched(sysm, args.ens_steps)
+    v_mx, st_mx, tune_mx = batched(sysm, args.ens_steps)
     for s in sysm:
         s.close()
-    print(f"ensemble mixed batched: {v_mx:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    print(f"ensemble mixed batched: {v_mx:.4g} particle-steps/s, tune {tune_mx}",
+          file=sys.stderr, flush=True)
     sys64 = build(DEM_FP64)
-    v_64b, st_64 = batched(sys64, args.ens_steps)
+    v_64e, st_64e, _ = batched(sys64, args.ens_steps, graph=False)
+    print(f"ensemble fp64 batched, launched one by one: {v_64e:.4g}", file=sys.stderr,
+          flush=True)
+    v_64b, st_64, tune_64 = batched(sys64, args.ens_steps)
     t_alone = 0.0
     ps_alone = 0.0
     for s, w in zip(sys64, members):
@@ -176,8 +195,13 @@ def ensemble_run(args):
         "mixed_batched_steps": args.ens_steps,
         "step_ms_p50": float(np.percentile(st_mx, 50)),
         "step_ms_p99": float(np.percentile(st_mx, 99)),
+        "mixed_pass_shape": tune_mx,
         "fp64_batched_particle_steps_per_s": v_64b,
         "fp64_batched_step_ms_p50": float(np.percentile(st_64, 50)),
+        "fp64_pass_shape": tune_64,
+        "fp64_batched_no_graph_particle_steps_per_s": v_64e,
+        "fp64_batched_no_graph_step_ms_p50": float(np.percentile(st_64e, 50)),
+        "graph_speedup_fp64": v_64b / v_64e,
         "fp64_alone_particle_steps_per_s": v_alone,
         "fp64_alone_sample_steps": args.alone_steps,
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
@@ -216,6 +240,10 @@ def main(argv=None):
                          "compare its energy and packing fraction with the FP64 run")
     ap.add_argument("--ens-steps", type=int, default=10000,
                     help="steps of the batched 256-system ensemble (0 = skip the ensemble)")
+    ap.add_argument("--latency-steps", type=int, default=2000,
+                    help="batched steps timed one by one for the p50/p99 step latency")
+    ap.add_argument("--tune-steps", type=int, default=20,
+                    help="timed steps per launch shape of the autotuner (0 = default shape)")
     ap.add_argument("--alone-steps", type=int, default=100,
                     help="steps per system of the one-at-a-time FP64 sample")
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -271,7 +299,8 @@ def main(argv=None):
             "ms_pass1": rmx["stats"]["ms_pass1"],
             "ms_pass2": rmx["stats"]["ms_pass2"],
             "pair_contacts": rmx["stats"]["pair_contacts"],
-            "after_steps": args.warmup + args.steps,
+            "autotune": rmx["tune"],
+            "after_steps": rmx["steps_before"] + args.steps,
             "kinetic_energy_rel_diff_vs_fp64": (kemx - ke64) / ke64 if ke64 else None,
             "packing_fraction_fp64": phi64,
             "packing_fraction_mixed": phimx,
@@ -315,6 +344,8 @@ def main(argv=None):
             "ms_pass1": st["ms_pass1"],
             "ms_pass2": st["ms_pass2"],
             "ms_rebuild_per_step": st["ms_rebuild"],
+            "steps_before_timing": r64["steps_before"],
+            "autotune": r64["tune"],
             "parallelism": "single GPU",
         },
         "roofline": {

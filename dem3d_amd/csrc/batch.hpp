@@ -36,13 +36,13 @@ struct BatchRec {
   SysConst s;
   PassArgs p[2];  // pass 1 (forces at x_n + predictor), pass 2 (forces at the prediction + corrector)
   RebuildArgs r;
-  int nblk;       // particle blocks of the system
+  int nblk;       // particle blocks of the system (DEM_BLOCK lanes)
+  int pass_nblk;  // blocks of its force passes (the batch's launch shape)
   // step bookkeeping, advanced by k_batch_prepare
   int nprev;
   int first;      // 1 until the first step after the system joined the batch
   int step;       // steps of the current dem_batch_step call so far
   int ipc[3];     // ring slot order, 1-based (ipc(1) = most recent)
-  int pad;
   long long eval, stamp_base;
   double* ring;
   int* rb;
@@ -101,18 +101,21 @@ __global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
 }
 
 // Block maps: (system, block within the system) per block of a launch.
-enum { MAP_PART = 0, MAP_CELL = 1, MAP_CTILE = 2, MAP_STILE = 3, MAP_KINDS = 4 };
+// MAP_PASS: blocks of the force passes, of the batch's launch shape.
+enum { MAP_PART = 0, MAP_CELL = 1, MAP_CTILE = 2, MAP_STILE = 3, MAP_PASS = 4, MAP_KINDS = 5 };
 
 __device__ __forceinline__ int cell_blocks(const RebuildArgs& r) {
   return (r.gncell + 1 + DEM_BLOCK - 1) / DEM_BLOCK;
 }
 
-template <int MODE, bool UPDATE, bool MIXED, int PI>
-__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass_batch(
+// map: the pass-block map of the launch shape (BS lanes per block).
+template <int MODE, bool UPDATE, bool MIXED, int PI, int BS = DEM_BLOCK,
+          int WAVES = DEM_PASS_WAVES>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass_batch(
     const BatchRec* __restrict__ rec, const int2* __restrict__ map) {
   const int2 m = map[xcd_block()];
   const BatchRec& r = rec[m.x];
-  pass_body<MODE, UPDATE, MIXED>(r.s, r.p[PI], m.y);
+  pass_body<MODE, UPDATE, MIXED, BS>(r.s, r.p[PI], m.y);
 }
 
 // One block per system.
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __re
                                                             int pi, int update) {
   const BatchRec& r = rec[blockIdx.x];
   const PassArgs& a = r.p[pi];
-  pass_reduce_body(a.ctl, a.part, r.nblk, a.par, update);
+  pass_reduce_body(a.ctl, a.part, r.pass_nblk, a.par, update);
 }
 
 __global__ void k_fill_cells_batch(const BatchRec* __restrict__ rec,

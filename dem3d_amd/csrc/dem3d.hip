@@ -131,6 +131,7 @@ struct dem_system {
   int64_t uploaded_unlisted = 0;  // uploaded pairs with no edge in the list
   bool timing = false;
   bool mixed = false;        // DEM_MIXED force arithmetic
+  int shape = 0;             // launch shape of the force passes (DEM_SHAPE_*)
   bool in_batch = false;     // stepped by a dem_batch (its own stream must stay idle)
   dem_batch_t batch = nullptr;
   bool batch_first = false;  // next batched step is the first since joining
@@ -389,7 +390,11 @@ int dem_set_option(dem_handle_t h, const char* name, double value) {
   else if (!strcmp(name, "keep_predicted_acc")) h->keep_pred_acc = value != 0.0;
   else if (!strcmp(name, "time")) h->time = value;
   else if (!strcmp(name, "next_step")) h->next_step = (int64_t)value;
-  else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
+  else if (!strcmp(name, "pass_shape")) {
+    if (!(value >= 0.0 && value < (double)DEM_NSHAPES) || value != (double)(int)value)
+      return fail(h, DEM_ERR_ARG, "pass_shape must be an integer in [0, %d)", DEM_NSHAPES);
+    h->shape = (int)value;
+  } else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
   return DEM_OK;
 }
 
@@ -586,7 +591,7 @@ static int alloc_state(dem_handle_t h, int np) {
   if (e == hipSuccess) e = dalloc(&h->old_to_new, N);
   if (e == hipSuccess) e = dalloc(&h->slice_w, (size_t)h->nslices);
   if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
-  if (e == hipSuccess) e = dalloc(&h->part, 4 * (size_t)nblocks(N));
+  if (e == hipSuccess) e = dalloc(&h->part, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK));
   if (e != hipSuccess) {
     free_state(h);
     return fail(h, DEM_ERR_DEVICE, "allocation for %d particles failed: %s", np,
@@ -1524,6 +1529,43 @@ static void enqueue_fluid(dem_handle_t h, const double4* pos, const Kin* kin) {
                      (const double*)h->d_vf, (const double*)h->d_ef, h->d_drag, h->ctl);
 }
 
+static const int kShapeBlock[] = DEM_SHAPE_BLOCKS;
+static const int kShapeWaves[] = DEM_SHAPE_WAVES;
+static_assert(sizeof(kShapeBlock) == DEM_NSHAPES * sizeof(int) &&
+                  sizeof(kShapeWaves) == DEM_NSHAPES * sizeof(int),
+              "one block size and one occupancy per launch shape");
+
+// The pass kernel in launch shape `shape` (nb blocks).  Only the steps'
+// passes (predictor, corrector) come in every shape; the others run in the
+// default one.
+template <int MODE, bool UPDATE, bool MIXED>
+static void launch_shape(int shape, int nb, hipStream_t st, const SysConst& s, const PassArgs& a) {
+  if constexpr (MODE == MODE_PRED || MODE == MODE_CORR) {
+    switch (shape) {
+      case 1:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 128>), dim3(nb), dim3(128), 0, st, s, a);
+        return;
+      case 2:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 64>), dim3(nb), dim3(64), 0, st, s, a);
+        return;
+      case 3:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 512>), dim3(nb), dim3(512), 0, st, s, a);
+        return;
+      case 4:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 4>), dim3(nb), dim3(256), 0, st, s,
+                           a);
+        return;
+      case 5:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 2>), dim3(nb), dim3(256), 0, st, s,
+                           a);
+        return;
+      default:
+        break;
+    }
+  }
+  hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED>), dim3(nb), dim3(DEM_BLOCK), 0, st, s, a);
+}
+
 // One force evaluation: the pass kernel, then the sum of its counters.
 template <int MODE, bool UPDATE>
 static void launch_pass(dem_handle_t h, const PassArgs& a0) {
@@ -1532,13 +1574,12 @@ static void launch_pass(dem_handle_t h, const PassArgs& a0) {
     enqueue_fluid(h, a.pos_in, a.kin_in);
     a.drag = h->drag_cap ? h->d_drag : nullptr;
   }
-  const int nb = nblocks(h->np);
+  const int nb = (MODE == MODE_PRED || MODE == MODE_CORR) ? nblocks(h->np, kShapeBlock[h->shape])
+                                                          : nblocks(h->np);
   if (h->mixed)
-    hipLaunchKernelGGL((k_pass<MODE, UPDATE, true>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream,
-                       h->sc, a);
+    launch_shape<MODE, UPDATE, true>(h->shape, nb, h->stream, h->sc, a);
   else
-    hipLaunchKernelGGL((k_pass<MODE, UPDATE, false>), dim3(nb), dim3(DEM_BLOCK), 0, h->stream,
-                       h->sc, a);
+    launch_shape<MODE, UPDATE, false>(h->shape, nb, h->stream, h->sc, a);
   hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl,
                      (const unsigned*)a.part, nb, a.par, UPDATE ? 1 : 0);
 }
@@ -1846,6 +1887,45 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
   if (c.escaped > 0)
     return fail(h, DEM_ERR_ESCAPE, "%d particles left the box in step %lld of the call",
                 c.escaped, (long long)done);
+  return DEM_OK;
+}
+
+int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, dem_tune_t* out) {
+  if (!h || !out) return DEM_ERR_ARG;
+  memset(out, 0, sizeof *out);
+  if (steps_per_shape < 1) return fail(h, DEM_ERR_ARG, "steps_per_shape must be at least 1");
+  if (!h->has_state || h->np == 0) return fail(h, DEM_ERR_STATE, "upload a non-empty state first");
+  const bool timing = h->timing;
+  const int keep = h->shape;
+  int64_t next = first_step;
+  int best = 0;
+  for (int c = 0; c < DEM_NSHAPES; ++c) {
+    h->shape = c;
+    int64_t done = 0;
+    h->timing = false;  // the first launch of a shape loads its code
+    int rc = dem_step(h, 1, next, &done, nullptr);
+    out->steps += done;
+    next += done;
+    if (rc == DEM_OK) {
+      h->timing = true;
+      rc = dem_step(h, steps_per_shape, next, &done, nullptr);
+      out->steps += done;
+      next += done;
+    }
+    if (rc != DEM_OK) {
+      h->timing = timing;
+      h->shape = keep;
+      return rc;
+    }
+    out->block[c] = kShapeBlock[c];
+    out->waves[c] = kShapeWaves[c];
+    out->ms[c] = h->ms_pass1 + h->ms_pass2;
+    if (out->ms[c] < out->ms[best]) best = c;
+  }
+  h->timing = timing;
+  h->shape = best;
+  out->nshapes = DEM_NSHAPES;
+  out->chosen = best;
   return DEM_OK;
 }
 
@@ -2424,9 +2504,9 @@ struct dem_batch {
   bool ev_used[K] = {false, false, false, false};
   size_t rec_cap = 0;
   BatchRec* d_rec = nullptr;
-  int2* d_map[MAP_KINDS] = {nullptr, nullptr, nullptr, nullptr};
-  size_t map_cap[MAP_KINDS] = {0, 0, 0, 0};
-  int map_len[MAP_KINDS] = {0, 0, 0, 0};
+  int2* d_map[MAP_KINDS] = {};
+  size_t map_cap[MAP_KINDS] = {};
+  int map_len[MAP_KINDS] = {};
   bool dirty = true;
   Ctl* d_ctl = nullptr;
   Ctl* h_ctl = nullptr;
@@ -2434,6 +2514,14 @@ struct dem_batch {
   bool timing = false;
   std::vector<hipEvent_t> tev;
   std::vector<double> step_ms;
+  // Captured step sequences: gexec[0] replays one step, gexec[1]
+  // graph_steps steps.  Valid while the table, the block maps and the
+  // launch shapes they were captured with stay the same (gkey).
+  bool use_graph = true;
+  int graph_steps = 8;
+  int shape = 0;  // launch shape of the force passes (DEM_SHAPE_*)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<long long> gkey;
 };
 
 static int bfail(dem_batch_t b, int code, const char* fmt, ...) {
@@ -2508,6 +2596,7 @@ static void batch_rec_init(dem_handle_t h, BatchRec& r) {
   r.p[1] = b;
   r.r = rebuild_args(h, h->cur, h->rb);
   r.nblk = nblocks(h->np);
+  r.pass_nblk = nblocks(h->np, kShapeBlock[h->batch ? h->batch->shape : 0]);
   r.nprev = h->nprev;
   r.first = h->batch_first ? 1 : 0;
   r.step = 0;
@@ -2546,7 +2635,7 @@ static int batch_maps(dem_batch_t b) {
   for (size_t i = 0; i < b->sys.size(); ++i) {
     const dem_handle_t h = b->sys[i];
     const int cnt[MAP_KINDS] = {nblocks(h->np), nblocks((long long)h->gncell + 1), h->ntiles,
-                                h->ntiles_np};
+                                h->ntiles_np, nblocks(h->np, kShapeBlock[b->shape])};
     for (int k = 0; k < MAP_KINDS; ++k)
       for (int j = 0; j < cnt[k]; ++j) m[k].push_back(make_int2((int)i, j));
   }
@@ -2589,6 +2678,137 @@ static int batch_capacity(dem_batch_t b, size_t n) {
   return DEM_OK;
 }
 
+// The batched pass kernel in launch shape `shape` over the pass-block map.
+template <int MODE, bool UPDATE, bool MIXED, int PI>
+static void launch_batch_shape(int shape, int nb, hipStream_t st, const BatchRec* rec,
+                               const int2* map) {
+  switch (shape) {
+    case 1:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128>), dim3(nb), dim3(128), 0, st,
+                         rec, map);
+      return;
+    case 2:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 64>), dim3(nb), dim3(64), 0, st,
+                         rec, map);
+      return;
+    case 3:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 512>), dim3(nb), dim3(512), 0, st,
+                         rec, map);
+      return;
+    case 4:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 4>), dim3(nb), dim3(256), 0,
+                         st, rec, map);
+      return;
+    case 5:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2>), dim3(nb), dim3(256), 0,
+                         st, rec, map);
+      return;
+    default:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI>), dim3(nb), dim3(DEM_BLOCK), 0, st,
+                         rec, map);
+  }
+}
+
+// The launches of one batched step (every phase is one launch for all the
+// systems; the arguments live in the device table, so the launches are the
+// same from step to step and can be replayed from a captured graph).
+static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
+  hipStream_t st = b->stream;
+  BatchRec* rec = b->d_rec;
+  const int gp = b->map_len[MAP_PART], gc = b->map_len[MAP_CELL];
+  const int gt = b->map_len[MAP_CTILE], gs = b->map_len[MAP_STILE];
+  const int2* mp = b->d_map[MAP_PART];
+  hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
+  // pass 1, pass 2 (one launch each for all systems) and their counters
+  const int gq = b->map_len[MAP_PASS];
+  const int2* mq = b->d_map[MAP_PASS];
+  if (mixed)
+    launch_batch_shape<MODE_PRED, true, true, 0>(b->shape, gq, st, rec, mq);
+  else
+    launch_batch_shape<MODE_PRED, true, false, 0>(b->shape, gq, st, rec, mq);
+  hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 0, 1);
+  if (moving) hipLaunchKernelGGL(k_move_walls_batch, dim3(ns), dim3(64), 0, st, rec);
+  if (mixed)
+    launch_batch_shape<MODE_CORR, false, true, 1>(b->shape, gq, st, rec, mq);
+  else
+    launch_batch_shape<MODE_CORR, false, false, 1>(b->shape, gq, st, rec, mq);
+  hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 1, 0);
+  // neighbour lists of the systems whose pass 2 asked for one
+  hipLaunchKernelGGL(k_fill_cells_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_CELL]);
+  hipLaunchKernelGGL(k_bin_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+  hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_CTILE], 0);
+  hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 0);
+  hipLaunchKernelGGL(k_scan_apply_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_CTILE], 0);
+  hipLaunchKernelGGL(k_bin_scatter_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+  hipLaunchKernelGGL(k_cell_sort_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_CELL]);
+  hipLaunchKernelGGL(k_nbr_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+  hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_STILE], 1);
+  hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 1);
+  hipLaunchKernelGGL(k_scan_apply_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
+                     (const int2*)b->d_map[MAP_STILE], 1);
+  hipLaunchKernelGGL(k_nbr_fill_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
+  hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(64), 0, st, rec);
+}
+
+static void drop_graphs(dem_batch_t b) {
+  for (int k = 0; k < 2; ++k)
+    if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+  b->gexec[0] = b->gexec[1] = nullptr;
+  b->gkey.clear();
+}
+
+// Captures nsteps batched steps from b->stream into an executable graph.
+// The stream is captured in relaxed mode, so allocations other threads make
+// meanwhile (systems being built for admission) are not disturbed.
+static int capture_steps(dem_batch_t b, int nsteps, int ns, bool mixed, bool moving,
+                         hipGraphExec_t* out) {
+  hipStream_t st = b->stream;
+  BHIPCHK(b, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  for (int k = 0; k < nsteps; ++k) enqueue_batch_step(b, ns, mixed, moving);
+  hipGraph_t g = nullptr;
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = hipStreamEndCapture(st, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(b, DEM_ERR_DEVICE, "capturing the batched step failed: %s",
+                 hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess)
+    return bfail(b, DEM_ERR_DEVICE, "instantiating the batched step graph failed: %s",
+                 hipGetErrorString(ie));
+  return DEM_OK;
+}
+
+// The graphs of the current batch, captured again when what their launches
+// depend on changed (the table and the maps, their sizes, the precision and
+// the moving walls).
+static int batch_graphs(dem_batch_t b, int ns, bool mixed, bool moving) {
+  std::vector<long long> key = {(long long)(uintptr_t)b->d_rec, ns, mixed, moving,
+                                b->graph_steps, b->shape};
+  for (int k = 0; k < MAP_KINDS; ++k) {
+    key.push_back((long long)(uintptr_t)b->d_map[k]);
+    key.push_back(b->map_len[k]);
+  }
+  if (b->gexec[0] && key == b->gkey) return DEM_OK;
+  drop_graphs(b);
+  int rc = capture_steps(b, 1, ns, mixed, moving, &b->gexec[0]);
+  if (rc == DEM_OK && b->graph_steps > 1)
+    rc = capture_steps(b, b->graph_steps, ns, mixed, moving, &b->gexec[1]);
+  if (rc) {
+    drop_graphs(b);
+    return rc;
+  }
+  b->gkey = key;
+  return DEM_OK;
+}
+
 extern "C" {
 
 int dem_batch_create(dem_batch_t* out) {
@@ -2626,6 +2846,7 @@ void dem_batch_destroy(dem_batch_t b) {
   if (b->d_ctl) (void)hipFree(b->d_ctl);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   for (auto e : b->tev) (void)hipEventDestroy(e);
+  drop_graphs(b);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -2672,8 +2893,22 @@ int dem_batch_size(dem_batch_t b) { return b ? (int)b->sys.size() : 0; }
 
 int dem_batch_set_option(dem_batch_t b, const char* name, double value) {
   if (!b || !name) return bfail(b, DEM_ERR_ARG, "null argument");
-  if (!strcmp(name, "timing")) b->timing = value != 0.0;
-  else return bfail(b, DEM_ERR_ARG, "unknown option '%s'", name);
+  if (!strcmp(name, "timing")) {
+    b->timing = value != 0.0;
+  } else if (!strcmp(name, "graph")) {
+    b->use_graph = value != 0.0;
+  } else if (!strcmp(name, "pass_shape")) {
+    if (!(value >= 0.0 && value < (double)DEM_NSHAPES) || value != (double)(int)value)
+      return bfail(b, DEM_ERR_ARG, "pass_shape must be an integer in [0, %d)", DEM_NSHAPES);
+    if ((int)value != b->shape) b->dirty = true;  // the pass-block map changes
+    b->shape = (int)value;
+  } else if (!strcmp(name, "graph_steps")) {
+    if (!(value >= 1.0 && value <= 256.0))
+      return bfail(b, DEM_ERR_ARG, "graph_steps must lie in [1, 256]");
+    b->graph_steps = (int)value;
+  } else {
+    return bfail(b, DEM_ERR_ARG, "unknown option '%s'", name);
+  }
   return DEM_OK;
 }
 
@@ -2700,8 +2935,6 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
       b->tev.push_back(e);
     }
   }
-  const int gp = b->map_len[MAP_PART], gc = b->map_len[MAP_CELL];
-  const int gt = b->map_len[MAP_CTILE], gs = b->map_len[MAP_STILE];
   const int ns = (int)n;
   // the table of this call, once; every step then advances it on the device
   BatchRec* R = b->h_rec[0];
@@ -2715,46 +2948,20 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   for (size_t i = 0; i < n; ++i) moving = moving || b->sys[i]->nw_move > 0;
   hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
                      ns);
-  for (int64_t s = 0; s < nsteps; ++s) {
+  if (b->use_graph) {
+    rc = batch_graphs(b, ns, mixed, moving);
+    if (rc) return rc;
+  }
+  for (int64_t s = 0; s < nsteps;) {
+    if (b->use_graph && !b->timing && b->graph_steps > 1 && nsteps - s >= b->graph_steps) {
+      BHIPCHK(b, hipGraphLaunch(b->gexec[1], st));
+      s += b->graph_steps;
+      continue;
+    }
     if (b->timing) BHIPCHK(b, hipEventRecord(b->tev[s], st));
-    hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
-    // pass 1, pass 2 (one launch each for all systems) and their counters
-    if (mixed)
-      hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, true, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
-                         st, rec, (const int2*)b->d_map[MAP_PART]);
-    else
-      hipLaunchKernelGGL((k_pass_batch<MODE_PRED, true, false, 0>), dim3(gp), dim3(DEM_BLOCK), 0,
-                         st, rec, (const int2*)b->d_map[MAP_PART]);
-    hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 0, 1);
-    if (moving) hipLaunchKernelGGL(k_move_walls_batch, dim3(ns), dim3(64), 0, st, rec);
-    if (mixed)
-      hipLaunchKernelGGL((k_pass_batch<MODE_CORR, false, true, 1>), dim3(gp), dim3(DEM_BLOCK), 0,
-                         st, rec, (const int2*)b->d_map[MAP_PART]);
-    else
-      hipLaunchKernelGGL((k_pass_batch<MODE_CORR, false, false, 1>), dim3(gp), dim3(DEM_BLOCK),
-                         0, st, rec, (const int2*)b->d_map[MAP_PART]);
-    hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 1, 0);
-    // neighbour lists of the systems whose pass 2 asked for one
-    const int2* mp = b->d_map[MAP_PART];
-    hipLaunchKernelGGL(k_fill_cells_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_CELL]);
-    hipLaunchKernelGGL(k_bin_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
-    hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_CTILE], 0);
-    hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 0);
-    hipLaunchKernelGGL(k_scan_apply_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_CTILE], 0);
-    hipLaunchKernelGGL(k_bin_scatter_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
-    hipLaunchKernelGGL(k_cell_sort_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_CELL]);
-    hipLaunchKernelGGL(k_nbr_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
-    hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_STILE], 1);
-    hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 1);
-    hipLaunchKernelGGL(k_scan_apply_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
-                       (const int2*)b->d_map[MAP_STILE], 1);
-    hipLaunchKernelGGL(k_nbr_fill_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
-    hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(64), 0, st, rec);
+    if (b->use_graph) BHIPCHK(b, hipGraphLaunch(b->gexec[0], st));
+    else enqueue_batch_step(b, ns, mixed, moving);
+    s += 1;
   }
   if (b->timing) BHIPCHK(b, hipEventRecord(b->tev[nsteps], st));
   BHIPCHK(b, hipGetLastError());
@@ -2797,6 +3004,48 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
     }
   }
   return result;
+}
+
+int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) {
+  if (!b || !out) return bfail(b, DEM_ERR_ARG, "null argument");
+  memset(out, 0, sizeof *out);
+  if (steps_per_shape < 1) return bfail(b, DEM_ERR_ARG, "steps_per_shape must be at least 1");
+  const size_t n = b->sys.size();
+  if (n == 0) return bfail(b, DEM_ERR_STATE, "the batch has no systems");
+  const bool timing = b->timing;
+  const int keep = b->shape;
+  std::vector<int64_t> done(n);
+  int best = 0;
+  for (int c = 0; c < DEM_NSHAPES; ++c) {
+    if (c != b->shape) b->dirty = true;
+    b->shape = c;
+    b->timing = false;  // the first launch of a shape loads its code
+    int rc = dem_batch_step(b, 1, done.data(), nullptr);
+    out->steps += done[0];
+    if (rc == DEM_OK) {
+      b->timing = true;
+      rc = dem_batch_step(b, steps_per_shape, done.data(), nullptr);
+      out->steps += done[0];
+    }
+    if (rc != DEM_OK) {
+      b->timing = timing;
+      if (b->shape != keep) b->dirty = true;
+      b->shape = keep;
+      return rc;
+    }
+    double sum = 0.0;
+    for (double t : b->step_ms) sum += t;
+    out->block[c] = kShapeBlock[c];
+    out->waves[c] = kShapeWaves[c];
+    out->ms[c] = sum / (double)steps_per_shape;
+    if (out->ms[c] < out->ms[best]) best = c;
+  }
+  b->timing = timing;
+  if (b->shape != best) b->dirty = true;
+  b->shape = best;
+  out->nshapes = DEM_NSHAPES;
+  out->chosen = best;
+  return DEM_OK;
 }
 
 // Device time of each step of the last dem_batch_step (option "timing").

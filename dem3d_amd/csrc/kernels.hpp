@@ -37,9 +37,8 @@
 #ifndef DEM_PASS_WAVES
 #define DEM_PASS_WAVES 3
 #endif
-// Occupancy target of the force passes (wavefronts per SIMD): the register
-// budget is 512 / DEM_PASS_WAVES VGPRs.
-#define DEM_PASS_ATTR __attribute__((amdgpu_waves_per_eu(DEM_PASS_WAVES, 8)))
+// Default occupancy target of the force passes (wavefronts per SIMD): the
+// register budget is 512 / DEM_PASS_WAVES VGPRs.
 
 enum { MODE_ACC = 0, MODE_EULER = 1, MODE_PRED = 2, MODE_CORR = 3 };
 
@@ -287,8 +286,11 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
   }
 }
 
-// One force pass over the particles [blk*256, blk*256+256) of one system.
-template <int MODE, bool UPDATE, bool MIXED>
+// One force pass over the particles [blk*BS, blk*BS+BS) of one system,
+// BS lanes per block.  The block size changes neither the arithmetic of a
+// lane nor the integer counter sums, so every block shape gives the same
+// results bit for bit.
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK>
 __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
@@ -300,13 +302,13 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   if (blk == 0 && t == 0) *a.rb_clear = 0;
   const bool lds_tab = s.nc <= DEM_LDS_NC;
   if (lds_tab) {
-    for (int q = t; q < s.nc * s.nc; q += DEM_BLOCK) sh_pair[q] = s.pair[q];
+    for (int q = t; q < s.nc * s.nc; q += BS) sh_pair[q] = s.pair[q];
     if (t < s.nc) sh_comp[t] = s.comp[t];
   }
   const size_t N = (size_t)s.np;
   const int ep = *a.E.epar;
 
-  const int i = blk * DEM_BLOCK + t;
+  const int i = blk * BS + t;
   const bool active = i < s.np;
   unsigned long long n_wall = 0;
   int need_rb = 0, n_esc = 0;
@@ -500,7 +502,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   }
   // Counters: block sums to a per-block slot (k_pass_reduce adds them up);
   // flags, which are rare, by atomics.
-  __shared__ unsigned sh_cnt[DEM_BLOCK / 64][4];
+  __shared__ unsigned sh_cnt[BS / 64][4];
   const unsigned c0 = (unsigned)wave_sum_u64(n_col), c1 = (unsigned)wave_sum_u64(n_pair);
   const unsigned c2 = (unsigned)wave_sum_u64(n_wall), c3 = (unsigned)wave_sum_u64(n_stored);
   err = wave_or(err);
@@ -521,14 +523,27 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   __syncthreads();
   if (t < 4) {
     unsigned v = 0;
-    for (int w = 0; w < DEM_BLOCK / 64; ++w) v += sh_cnt[w][t];
+    for (int w = 0; w < BS / 64; ++w) v += sh_cnt[w][t];
     a.part[4 * (size_t)blk + t] = v;
   }
 }
 
-template <int MODE, bool UPDATE, bool MIXED>
-__global__ __launch_bounds__(DEM_BLOCK) DEM_PASS_ATTR void k_pass(SysConst s, PassArgs a) {
-  pass_body<MODE, UPDATE, MIXED>(s, a, xcd_block());
+// Launch shapes of the force passes: lanes per block and the occupancy the
+// register allocation targets (wavefronts per SIMD, 512/WAVES VGPRs each;
+// above 3 the pass spills to scratch).  Shape 0 is the default; the others
+// are candidates of the autotuner (dem_autotune).  Every shape gives the
+// same results bit for bit.
+#ifndef DEM_NSHAPES
+#define DEM_NSHAPES 6
+#endif
+#define DEM_SHAPE_BLOCKS {256, 128, 64, 512, 256, 256}
+#define DEM_SHAPE_WAVES {DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, 4, 2}
+#define DEM_MIN_PASS_BLOCK 64
+
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int WAVES = DEM_PASS_WAVES>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass(
+    SysConst s, PassArgs a) {
+  pass_body<MODE, UPDATE, MIXED, BS>(s, a, xcd_block());
 }
 
 // Sums the per-block counters of the pass just run into the control block

@@ -17,6 +17,7 @@ program check_bindings
   type(dem_rng_t) :: rg
   type(dem_pour_t) :: po
   type(dem_fluid_t) :: fl
+  type(dem_tune_t) :: tu
   type(c_ptr) :: h
   real(c_double), target :: radius(1), tab(1)
   integer(c_int) :: rc
@@ -32,6 +33,7 @@ program check_bindings
   write(*, '(a, i0)') 'sizeof dem_rng_t ', c_sizeof(rg)
   write(*, '(a, i0)') 'sizeof dem_pour_t ', c_sizeof(po)
   write(*, '(a, i0)') 'sizeof dem_fluid_t ', c_sizeof(fl)
+  write(*, '(a, i0)') 'sizeof dem_tune_t ', c_sizeof(tu)
 
   radius = 0.5d0
   tab = 1.0d0

@@ -75,6 +75,15 @@ module dem3d_gpu
     integer(c_int64_t) :: next_step
   end type dem_stats_t
 
+  ! dem_tune_t: the autotuner's report (launch shapes of the force passes).
+  integer(c_int), parameter, public :: DEM_NSHAPES = 6
+  type, bind(c), public :: dem_tune_t
+    integer(c_int32_t) :: nshapes = 0, chosen = 0
+    integer(c_int64_t) :: steps = 0
+    integer(c_int32_t) :: block(DEM_NSHAPES) = 0, waves(DEM_NSHAPES) = 0
+    real(c_double) :: ms(DEM_NSHAPES) = 0
+  end type dem_tune_t
+
   ! dem_config_t: a configuration file (SaveConfiguration, src/MConfig.f90:109-236).
   type, bind(c), public :: dem_config_t
     integer(c_int32_t) :: form = DEM_CFG_COMPLETE, version = 2
@@ -141,6 +150,7 @@ module dem3d_gpu
   public :: dem_upload_contacts, dem_download_contacts, dem_download_cells
   public :: dem_compute_accelerations_aos, dem_compute_accelerations
   public :: dem_euler_startup, dem_step, dem_synchronize, dem_stats, dem_set_option
+  public :: dem_autotune
   public :: dem_error_message
 
   interface
@@ -264,6 +274,16 @@ module dem3d_gpu
       type(c_ptr), value :: h
       character(kind=c_char), intent(in) :: name(*)
       real(c_double), value :: value
+    end function
+    ! Times every launch shape of the force passes while stepping
+    ! (steps_per_shape + 1 MoveParticles steps per shape, numbered from
+    ! first_step) and keeps the fastest; report%steps = steps advanced.
+    integer(c_int) function dem_autotune(h, steps_per_shape, first_step, report) &
+        bind(c, name="dem_autotune")
+      import :: c_int, c_int64_t, c_ptr, dem_tune_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: steps_per_shape, first_step
+      type(dem_tune_t), intent(inout) :: report
     end function
     ! FluidFlow with the specification's fluid (the C entry also takes a
     ! null pointer to turn it off).

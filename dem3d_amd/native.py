@@ -69,6 +69,21 @@ class ParticleT(C.Structure):
 assert C.sizeof(ParticleT) == 128
 
 
+NSHAPES = 6
+
+
+class TuneT(C.Structure):
+    """dem_tune_t: the autotuner's report."""
+    _fields_ = [("nshapes", C.c_int32), ("chosen", C.c_int32), ("steps", C.c_int64),
+                ("block", C.c_int32 * NSHAPES), ("waves", C.c_int32 * NSHAPES),
+                ("ms", C.c_double * NSHAPES)]
+
+    def asdict(self):
+        n = self.nshapes
+        return dict(chosen=self.chosen, steps=self.steps, block=list(self.block[:n]),
+                    waves=list(self.waves[:n]), ms=list(self.ms[:n]))
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("np", C.c_int64), ("pair_contacts", C.c_int64), ("wall_contacts", C.c_int64),
@@ -211,6 +226,8 @@ class Library:
         "batch_step": (C.c_int, [C.c_void_p, C.c_int64, _lp, _ip]),
         "batch_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
         "batch_step_times": (C.c_int64, [C.c_void_p, C.c_int64, _dp]),
+        "autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(TuneT)]),
+        "batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TuneT)]),
     }
 
     def __init__(self, path: str, prefix: str):
@@ -379,6 +396,20 @@ class System:
 
     def set_time(self, t):
         self._check(self.lib.set_option(self.h, b"time", float(t)))
+
+    def set_option(self, name: str, value: float):
+        """dem_set_option ("timing", "resort_interval", "pass_shape", ...)."""
+        self._check(self.lib.set_option(self.h, name.encode(), float(value)))
+
+    def autotune(self, steps_per_shape=20):
+        """dem_autotune: times every launch shape of the force passes while
+        stepping (the steps count, the trajectory is unchanged) and keeps the
+        fastest.  Returns the report as a dict."""
+        t = TuneT()
+        rc = self.lib.autotune(self.h, int(steps_per_shape), int(self.next_step), C.byref(t))
+        self.next_step += int(t.steps)
+        self._check(rc)
+        return t.asdict()
 
     def _check(self, rc):
         if rc != DEM_OK:
@@ -558,6 +589,20 @@ class Batch:
 
     def set_timing(self, on=True):
         self._check(self.lib.batch_set_option(self.b, b"timing", 1.0 if on else 0.0))
+
+    def set_option(self, name: str, value: float):
+        """dem_batch_set_option: "timing", "graph", "graph_steps", "pass_shape"."""
+        self._check(self.lib.batch_set_option(self.b, name.encode(), float(value)))
+
+    def autotune(self, steps_per_shape=20):
+        """dem_batch_autotune (see System.autotune); every system advances by
+        the report's "steps"."""
+        t = TuneT()
+        rc = self.lib.batch_autotune(self.b, int(steps_per_shape), C.byref(t))
+        for sysm in self.systems:
+            sysm.next_step += int(t.steps)
+        self._check(rc)
+        return t.asdict()
 
     def step(self, nsteps, allow_escape=False):
         n = len(self.systems)

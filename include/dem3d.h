@@ -258,7 +258,8 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *                       particle and step); default 0
  *   "time"              sets Time (src/MConfig.f90:61), e.g. from a saved
  *                       configuration before an Euler start-up
- *   "next_step"         sets the step number of the next batched step */
+ *   "next_step"         sets the step number of the next batched step
+ *   "pass_shape"        launch shape of the force passes (see dem_tune_t) */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
 /* ---- fluid drag (mFluid) --------------------------------------------------- */
@@ -446,9 +447,43 @@ int dem_batch_size(dem_batch_t b);
 int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
                    int32_t* escaped);
 /* "timing": 1 = record the device time of every step of a dem_batch_step
- * call, read back with dem_batch_step_times (returns the count). */
+ * call, read back with dem_batch_step_times (returns the count).
+ * "graph": 1 (default) = replay the step's launches from HIP graphs captured
+ * once per batch layout (recaptured when systems join or leave); 0 = launch
+ * them one by one.  "graph_steps": steps per replayed graph (1..256,
+ * default 8); with "timing" every step is its own replay.
+ * "pass_shape": launch shape of the force passes (see dem_tune_t). */
 int dem_batch_set_option(dem_batch_t b, const char* name, double value);
 int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
+
+/* ---- launch shapes and the autotuner ------------------------------------ */
+/* No counterpart in the reference.  The force passes come in DEM_NSHAPES
+ * launch shapes: lanes per block and the occupancy (wavefronts per SIMD)
+ * their register allocation targets.  Every shape gives bitwise the same
+ * results, so a shape changes speed only; shape 0 (256 lanes, 3 waves) is
+ * the default.  The autotuner times every shape on the live system while
+ * stepping it: each shape runs 1 untimed step (its first launch) and then
+ * steps_per_shape steps timed with HIP events, and the fastest becomes the
+ * system's (batch's) shape.  The steps count as ordinary MoveParticles steps
+ * (numbered from first_step, resp. each system's next_step), and the
+ * trajectory is bitwise the one dem_step (dem_batch_step) gives for the same
+ * number of steps.  Escapes and errors stop the tuning with dem_step's
+ * return code, the shape unchanged.  A shape can also be set directly with
+ * the option "pass_shape". */
+#define DEM_NSHAPES 6
+typedef struct {
+  int32_t nshapes;             /* shapes timed                                 */
+  int32_t chosen;              /* the fastest, now in use                      */
+  int64_t steps;               /* MoveParticles steps the tuning advanced      */
+  int32_t block[DEM_NSHAPES];  /* lanes per block of each shape                */
+  int32_t waves[DEM_NSHAPES];  /* wavefronts per SIMD it targets               */
+  double ms[DEM_NSHAPES];      /* mean device ms per step: the two passes
+                                  (dem_autotune), the whole step
+                                  (dem_batch_autotune)                         */
+} dem_tune_t;
+int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step,
+                 dem_tune_t* out);
+int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out);
 
 #ifdef __cplusplus
 }

@@ -172,3 +172,44 @@ def test_batch_with_moving_walls_equals_alone(hip):
         _assert_same(snap(s), ref)
         assert s.stats()["next_step"] == s.red.method_order + 200
     b.close()
+
+
+@pytest.mark.parametrize("precision", [DEM_FP64, DEM_MIXED])
+def test_graph_replays_equal_launches(hip, precision):
+    """The batched step replayed from captured HIP graphs (one step per
+    graph, and 8 steps per graph with the remainder one by one) gives after
+    every call the state the same launches give one by one.  The calls
+    interleave the batch's own eager work (table upload, escape reset,
+    control read-back, synchronisation) with the replays, a system leaves
+    between calls (the graphs are captured again) and one timed call replays
+    the one-step graph between events."""
+    def run(graph, graph_steps):
+        systems = [_make(hip, kw, precision) for kw in SYSTEMS]
+        b = Batch(hip)
+        b.set_option("graph", graph)
+        b.set_option("graph_steps", graph_steps)
+        for s in systems:
+            b.add(s)
+        snaps = []
+        for n in (21, 8, 3):
+            b.step(n)
+            snaps.append([_state(s) for s in systems])
+        b.remove(systems[3])
+        b.set_timing(True)
+        b.step(11)
+        b.set_timing(False)
+        snaps.append([_state(s) for s in systems])
+        b.step(17)
+        snaps.append([_state(s) for s in systems])
+        b.close()
+        for s in systems:
+            s.close()
+        return snaps
+
+    ref = run(0, 1)
+    for graph_steps in (1, 8):
+        got = run(1, graph_steps)
+        for call, (sa, sb) in enumerate(zip(got, ref)):
+            for k, (a, b_) in enumerate(zip(sa, sb)):
+                for key in a:
+                    assert np.array_equal(a[key], b_[key]), (graph_steps, call, k, key)

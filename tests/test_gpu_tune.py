@@ -1,0 +1,108 @@
+"""Launch shapes of the force passes and the autotuner (dem_autotune,
+dem_batch_autotune).
+
+Every shape (lanes per block, occupancy target) runs the same per-lane
+arithmetic and integer counter sums, so a system stepped in any shape must
+reach bitwise the state of the default shape, alone and batched, and a run
+that autotunes while stepping must follow bitwise the trajectory of plain
+stepping."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from dem3d_amd import workloads
+from dem3d_amd.native import DEM_FP64, DEM_MIXED, NSHAPES, Batch, System
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(n_side=(9, 8, 7), nc=2, seed=31, vel=0.4, mu=0.4, e=0.85),
+    dict(n_side=(7, 7, 5), nc=3, ratio=1.4, seed=37, walls=False, pbc=(True, True, True),
+         vel=0.5, spacing=1.0, mu=0.2, e=0.7),
+]
+
+
+def _make(hip, kw, precision=DEM_FP64):
+    wl = workloads.small_box(**kw)
+    s = System(hip, wl.red, precision=precision)
+    s.upload(wl.comp, wl.x, wl.v, wl.w)
+    s.startup()
+    return s
+
+
+def _state(s):
+    st = s.download()
+    con = s.download_contacts()
+    return {k: np.array(st[k]) for k in ("x", "v", "w", "a", "alpha", "ring")} | {
+        "keys": con["keys"].copy(), "ft": con["ft"].copy(), "vt": con["vt"].copy(),
+        "cells": s.download_cells(), "ncol": np.array(s.stats()["ncol"]),
+        "pairs": np.array(s.stats()["pair_contacts"])}
+
+
+def _same(a, b, tag):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (tag, k)
+
+
+@pytest.mark.parametrize("precision", [DEM_FP64, DEM_MIXED])
+def test_every_shape_gives_the_same_trajectory(hip, precision):
+    for kw in CASES:
+        ref = None
+        for shape in range(NSHAPES):
+            s = _make(hip, kw, precision)
+            s.set_option("pass_shape", shape)
+            s.step(37)
+            st = _state(s)
+            s.close()
+            if ref is None:
+                ref = st
+            else:
+                _same(st, ref, shape)
+
+
+def test_autotune_keeps_the_trajectory(hip):
+    kw = CASES[0]
+    a = _make(hip, kw)
+    rep = a.autotune(steps_per_shape=3)
+    assert rep["steps"] == NSHAPES * 4
+    assert len(rep["ms"]) == NSHAPES and all(m > 0 for m in rep["ms"])
+    assert rep["chosen"] == int(np.argmin(rep["ms"]))
+    assert rep["block"][0] == 256 and sorted(set(rep["block"])) == [64, 128, 256, 512]
+    a.step(11)
+    b = _make(hip, kw)
+    b.step(rep["steps"] + 11)
+    _same(_state(a), _state(b), "autotuned")
+    assert a.stats()["next_step"] == b.stats()["next_step"]
+
+
+def test_batch_shapes_and_autotune_equal_alone(hip):
+    systems_kw = CASES + [dict(n_side=(3, 3, 3), nc=1, seed=13, vel=0.1, mu=0.1, e=0.6)]
+    alone = []
+    for kw in systems_kw:
+        s = _make(hip, kw)
+        s.step(NSHAPES * 3 + 9 + 9)
+        alone.append(_state(s))
+        s.close()
+    systems = [_make(hip, kw) for kw in systems_kw]
+    b = Batch(hip)
+    for s in systems:
+        b.add(s)
+    b.set_option("pass_shape", 2)
+    b.step(9)
+    rep = b.autotune(steps_per_shape=2)
+    assert rep["steps"] == NSHAPES * 3
+    assert rep["chosen"] == int(np.argmin(rep["ms"]))
+    b.step(9)
+    for s, ref in zip(systems, alone):
+        _same(_state(s), ref, "batch")
+    b.close()
+
+
+def test_bad_shape_is_refused(hip):
+    s = _make(hip, CASES[1])
+    with pytest.raises(Exception, match="pass_shape"):
+        s.set_option("pass_shape", NSHAPES)
+    with pytest.raises(Exception, match="pass_shape"):
+        s.set_option("pass_shape", 1.5)
+    s.close()
