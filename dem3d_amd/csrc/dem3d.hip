@@ -2517,7 +2517,7 @@ struct dem_batch {
   // Captured step sequences: gexec[0] replays one step, gexec[1]
   // graph_steps steps.  Valid while the table, the block maps and the
   // launch shapes they were captured with stay the same (gkey).
-  bool use_graph = true;
+  bool use_graph = false;
   int graph_steps = 8;
   int shape = 0;  // launch shape of the force passes (DEM_SHAPE_*)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};

@@ -448,9 +448,10 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
                    int32_t* escaped);
 /* "timing": 1 = record the device time of every step of a dem_batch_step
  * call, read back with dem_batch_step_times (returns the count).
- * "graph": 1 (default) = replay the step's launches from HIP graphs captured
- * once per batch layout (recaptured when systems join or leave); 0 = launch
- * them one by one.  "graph_steps": steps per replayed graph (1..256,
+ * "graph": 1 = replay the step's launches from HIP graphs captured once per
+ * batch layout (recaptured when systems join or leave); 0 (default) = launch
+ * them one by one (as fast on this hardware, and no recapture when the
+ * membership changes).  "graph_steps": steps per replayed graph (1..256,
  * default 8); with "timing" every step is its own replay.
  * "pass_shape": launch shape of the force passes (see dem_tune_t). */
 int dem_batch_set_option(dem_batch_t b, const char* name, double value);
