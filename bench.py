@@ -127,7 +127,7 @@ def ensemble_run(args):
     mixed precision for --ens-steps steps, batched FP64, and the plain FP64
     path (each system stepped alone, on a --alone-steps sample)."""
     from dem3d_amd import workloads
-    from dem3d_amd.native import DEM_FP64, DEM_MIXED, Batch, System, load_hip
+    from dem3d_amd.native import DEM_FP64, DEM_MIXED, NSHAPES, Batch, System, load_hip
     lib = load_hip()
     members = workloads.ensemble_256()
     ntot = sum(w.np for w in members)
@@ -143,16 +143,23 @@ def ensemble_run(args):
             s.synchronize()
         return out
 
-    def batched(systems, steps, graph=True):
-        """Autotuned launch shape (graph replays) or the default shape
-        launched kernel by kernel; throughput over `steps` steps, then the
-        per-step device times of args.latency_steps more steps."""
+    def batched(systems, steps, graph=False, shape=None, tune_steps=0):
+        """Launches one by one (the default) or replayed from captured HIP
+        graphs; autotuned launch shape (tune_steps > 0) or a given one.
+        Throughput over `steps` steps, then the per-step device times of
+        args.latency_steps more steps."""
         b = Batch(lib)
         b.set_option("graph", 1 if graph else 0)
+        if shape is not None:
+            b.set_option("pass_shape", shape)
         for s in systems:
             b.add(s)
-        b.step(20)  # warm-up (and the graph capture)
-        tune = b.autotune(steps_per_shape=args.tune_steps) if graph and args.tune_steps else None
+        b.step(20)  # warm-up
+        tune = None
+        if tune_steps:
+            tune = b.autotune(steps_per_shape=tune_steps)
+        else:
+            b.step(NSHAPES * (args.tune_steps + 1))  # the same trajectory as a tuned run
         b.step(8)
         t0 = time.perf_counter()
         b.step(steps)
@@ -166,16 +173,21 @@ def ensemble_run(args):
     res = {"systems": len(members), "spheres": ntot,
            "sizes": [int(min(w.np for w in members)), int(max(w.np for w in members))]}
     sysm = build(DEM_MIXED)
-    v_mx, st_mx, tune_mx = batched(sysm, args.ens_steps)
+    v_mx, st_mx, tune_mx = batched(sysm, args.ens_steps, tune_steps=args.tune_steps)
     for s in sysm:
         s.close()
     print(f"ensemble mixed batched: {v_mx:.4g} particle-steps/s, tune {tune_mx}",
           file=sys.stderr, flush=True)
     sys64 = build(DEM_FP64)
-    v_64e, st_64e, _ = batched(sys64, args.ens_steps, graph=False)
-    print(f"ensemble fp64 batched, launched one by one: {v_64e:.4g}", file=sys.stderr,
-          flush=True)
-    v_64b, st_64, tune_64 = batched(sys64, args.ens_steps)
+    v_64b, st_64, tune_64 = batched(sys64, args.ens_steps, tune_steps=args.tune_steps)
+    for s in sys64:
+        s.close()
+    # the same steps from the same start, replayed from captured graphs
+    sys64 = build(DEM_FP64)
+    v_64g, st_64g, _ = batched(sys64, args.ens_steps, graph=True,
+                               shape=tune_64["chosen"] if tune_64 else 0)
+    print(f"ensemble fp64 batched: {v_64b:.4g} launched one by one, {v_64g:.4g} from graphs",
+          file=sys.stderr, flush=True)
     t_alone = 0.0
     ps_alone = 0.0
     for s, w in zip(sys64, members):
@@ -199,9 +211,9 @@ def ensemble_run(args):
         "fp64_batched_particle_steps_per_s": v_64b,
         "fp64_batched_step_ms_p50": float(np.percentile(st_64, 50)),
         "fp64_pass_shape": tune_64,
-        "fp64_batched_no_graph_particle_steps_per_s": v_64e,
-        "fp64_batched_no_graph_step_ms_p50": float(np.percentile(st_64e, 50)),
-        "graph_speedup_fp64": v_64b / v_64e,
+        "fp64_batched_graph_particle_steps_per_s": v_64g,
+        "fp64_batched_graph_step_ms_p50": float(np.percentile(st_64g, 50)),
+        "graph_speedup_fp64": v_64g / v_64b,
         "fp64_alone_particle_steps_per_s": v_alone,
         "fp64_alone_sample_steps": args.alone_steps,
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,

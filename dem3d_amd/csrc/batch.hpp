@@ -45,6 +45,7 @@ struct BatchRec {
   int ipc[3];     // ring slot order, 1-based (ipc(1) = most recent)
   long long eval, stamp_base;
   double* ring;
+  double* psum;  // the next predictor's sums (PassArgs::psum_rd / psum_wr)
   int* rb;
   // MoveWalls of systems with WallsInMovement (nw_move > 0), at
   // Time = (next_step + step)*TimeStep before pass 2 (src/MDEM.f90:333-337)
@@ -77,6 +78,8 @@ __global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
   for (int q = 0; q < np; ++q) a.slot_rd[q] = r.ring + (size_t)(ipc[q] - 1) * 9 * N;
   a.slot_wr = np > 0 ? r.ring + (size_t)(ipc[np - 1] - 1) * 9 * N : nullptr;
   a.check_pred = r.first;
+  // the corrector of the previous step of this call formed the sums
+  a.psum_rd = (np > 0 && r.step > 0) ? r.psum : nullptr;
   r.first = 0;
   ev += 1;
   if (np > 0) {
@@ -92,6 +95,7 @@ __global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
   b.rb_set = r.rb + (par ^ 1);
   for (int q = 0; q < np; ++q) b.slot_rd[q] = r.ring + (size_t)(ipc[q] - 1) * 9 * N;
   b.step_idx = r.step;
+  b.psum_wr = np > 0 ? r.psum : nullptr;
   r.time = (double)(r.next_step + r.step) * r.dt;
   r.step += 1;
   ev += 1;

@@ -90,6 +90,7 @@ struct dem_system {
   double2* acc_p = nullptr;  // Predicted%Acceleration/Alpha (optional)
   double* ring = nullptr;
   double* ring2 = nullptr;
+  double* psum = nullptr;  // the next predictor's ring sums (9 planes of np)
   int ipc[3] = {1, 2, 3};
   bool acc_in_ring = false;
   int* id = nullptr;
@@ -181,7 +182,7 @@ static void free_state(dem_handle_t h) {
   }
   dfree(h->es.epar);
   dfree(h->d_es);
-  dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->id); dfree(h->id2);
+  dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->psum); dfree(h->id); dfree(h->id2);
   dfree(h->xref); dfree(h->key); dfree(h->cell_count);
   dfree(h->cell_start); dfree(h->cursor); dfree(h->cell_idx); dfree(h->tile_sum);
   dfree(h->perm_src); dfree(h->old_to_new); dfree(h->slice_w);
@@ -576,6 +577,7 @@ static int alloc_state(dem_handle_t h, int np) {
   if (e == hipSuccess) e = dalloc(&h->acc_p, 3 * N);
   if (e == hipSuccess) e = dalloc(&h->ring, nring * N);
   if (e == hipSuccess) e = dalloc(&h->ring2, nring * N);
+  if (e == hipSuccess) e = dalloc(&h->psum, (size_t)9 * N);
   if (e == hipSuccess) e = dalloc(&h->id, N);
   if (e == hipSuccess) e = dalloc(&h->id2, N);
   if (e == hipSuccess) e = dalloc(&h->xref, N);
@@ -1823,6 +1825,7 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
   size_t ev = 0;
   double t1 = 0, t2 = 0, tr = 0;
   bool first = true;
+  bool have_psum = false;  // the previous step's corrector formed this predictor's sums
   int64_t enqueued = 0;
   for (int64_t s = 0; s < nsteps; ++s) {
     if (h->resort_interval > 0 && h->steps_since_resort >= h->resort_interval) {
@@ -1833,9 +1836,15 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
       if (c0.escaped > 0) break;
       rc = resort(h);
       if (rc) return rc;
+      have_psum = false;  // the sums were not permuted with the particles
     }
     PassArgs a, b;
     step_args(h, s, first, a, b);
+    if (h->nprev > 0) {
+      a.psum_rd = have_psum ? h->psum : nullptr;
+      b.psum_wr = h->psum;
+      have_psum = true;
+    }
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
     launch_pass<MODE_PRED, true>(h, a);
     if (h->timing) (void)hipEventRecord(h->timer.ev[ev++], st);
@@ -2604,6 +2613,7 @@ static void batch_rec_init(dem_handle_t h, BatchRec& r) {
   r.eval = h->eval;
   r.stamp_base = h->stamp_base;
   r.ring = h->ring;
+  r.psum = h->psum;
   r.rb = h->rb;
   r.walls = h->d_walls;
   r.wm = h->d_wm;

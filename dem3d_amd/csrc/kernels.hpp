@@ -144,6 +144,14 @@ struct PassArgs {
   const double* pred_w;
   // fluid drag of every particle of this evaluation (k_fluid_drag), or null
   const double* drag;
+  // Sums of the AB predictor over the ring (9 planes of np: velocity,
+  // acceleration, angular acceleration terms).  The corrector of a step reads
+  // the ring slots the next predictor reads, so it forms the next predictor's
+  // sums in the same order and stores them (psum_wr); that predictor then
+  // reads 72 bytes per particle instead of the three 72-byte slots
+  // (psum_rd; null = read the slots).
+  const double* psum_rd;
+  double* psum_wr;
 };
 
 __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
@@ -422,14 +430,21 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
     } else if (MODE == MODE_PRED) {
       // AB predictor (src/MDEM.f90:297-331)
       d3 sv = mk3(0, 0, 0), sa = sv, sl = sv;
+      if (a.psum_rd) {
+        const double* r = a.psum_rd + i;
+        sv = mk3(r[0], r[N], r[2 * N]);
+        sa = mk3(r[3 * N], r[4 * N], r[5 * N]);
+        sl = mk3(r[6 * N], r[7 * N], r[8 * N]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (k >= s.nprev) break;
-        const double* r = a.slot_rd[k] + i;
-        const double c = a.coef[k];
-        sv = add3(sv, scl3(c, mk3(r[0], r[N], r[2 * N])));
-        sa = add3(sa, scl3(c, mk3(r[3 * N], r[4 * N], r[5 * N])));
-        sl = add3(sl, scl3(c, mk3(r[6 * N], r[7 * N], r[8 * N])));
+        for (int k = 0; k < 3; ++k) {
+          if (k >= s.nprev) break;
+          const double* r = a.slot_rd[k] + i;
+          const double c = a.coef[k];
+          sv = add3(sv, scl3(c, mk3(r[0], r[N], r[2 * N])));
+          sa = add3(sa, scl3(c, mk3(r[3 * N], r[4 * N], r[5 * N])));
+          sl = add3(sl, scl3(c, mk3(r[6 * N], r[7 * N], r[8 * N])));
+        }
       }
       d3 xp = add3(add3(xi, scl3(a.coef0, vi)), sv);
       const d3 vp = add3(add3(vi, scl3(a.coef0, acc)), sa);
@@ -461,17 +476,28 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       const double4 X = a.pos[i];
       const Kin K0 = load_kin(a.kin + i);
       const d3 v0 = kin_v(K0), w0 = kin_w(K0);
-      d3 sv = mk3(0, 0, 0), sa = sv, sl = sv, sb = sv;
+      // sb, sba, sbl: the next predictor's sums (same slots, same order)
+      d3 sv = mk3(0, 0, 0), sa = sv, sl = sv, sb = sv, sba = sv, sbl = sv;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (k >= s.nprev) break;
         const double* r = a.slot_rd[k] + i;
         const double c = a.coef[k];
         const d3 rv = mk3(r[0], r[N], r[2 * N]);
+        const d3 ra = mk3(r[3 * N], r[4 * N], r[5 * N]);
+        const d3 rl = mk3(r[6 * N], r[7 * N], r[8 * N]);
         sv = add3(sv, scl3(c, rv));
-        sa = add3(sa, scl3(c, mk3(r[3 * N], r[4 * N], r[5 * N])));
-        sl = add3(sl, scl3(c, mk3(r[6 * N], r[7 * N], r[8 * N])));
+        sa = add3(sa, scl3(c, ra));
+        sl = add3(sl, scl3(c, rl));
         sb = add3(sb, scl3(a.ab[k], rv));
+        sba = add3(sba, scl3(a.ab[k], ra));
+        sbl = add3(sbl, scl3(a.ab[k], rl));
+      }
+      if (a.psum_wr) {
+        double* r = a.psum_wr + i;
+        r[0] = sb.x; r[N] = sb.y; r[2 * N] = sb.z;
+        r[3 * N] = sba.x; r[4 * N] = sba.y; r[5 * N] = sba.z;
+        r[6 * N] = sbl.x; r[7 * N] = sbl.y; r[8 * N] = sbl.z;
       }
       d3 x = add3(add3(mk3(X.x, X.y, X.z), scl3(a.coef0, vi)), sv);
       const d3 v = add3(add3(v0, scl3(a.coef0, acc)), sa);
