@@ -99,7 +99,9 @@ def timed_run(wl, args, precision, hmode):
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
     tune = None
-    if args.tune_steps > 0:
+    if args.pass_shape >= 0:
+        g.set_option("pass_shape", args.pass_shape)
+    elif args.tune_steps > 0:
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
         tune = g.autotune(steps_per_shape=args.tune_steps)
@@ -256,6 +258,9 @@ def main(argv=None):
                     help="batched steps timed one by one for the p50/p99 step latency")
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="timed steps per launch shape of the autotuner (0 = default shape)")
+    ap.add_argument("--pass-shape", type=int, default=-1,
+                    help="launch shape of the 4M runs' force passes instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--alone-steps", type=int, default=100,
                     help="steps per system of the one-at-a-time FP64 sample")
     ap.add_argument("--serve-requests", type=int, default=24,

@@ -136,13 +136,6 @@ __global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __re
   pass_reduce_body(a.ctl, a.part, r.pass_nblk, a.par, update);
 }
 
-__global__ void k_fill_cells_batch(const BatchRec* __restrict__ rec,
-                                   const int2* __restrict__ map) {
-  const int2 m = map[blockIdx.x];
-  const RebuildArgs& r = rec[m.x].r;
-  fill_i32_body(r.cell_count, r.gncell + 1, 0, r.gate, 0, r.ctl, m.y, cell_blocks(r));
-}
-
 __global__ void k_bin_count_batch(const BatchRec* __restrict__ rec,
                                   const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
@@ -163,23 +156,16 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_tiles_batch(const BatchRec* 
     scan_tiles_body(r.slice_w, r.nslices, r.tile_sum, r.gate, 0, r.ctl, m.y);
 }
 
-// One block per system.
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine_batch(const BatchRec* __restrict__ rec,
-                                                                int which) {
-  const RebuildArgs& r = rec[blockIdx.x].r;
-  scan_spine_body(r.tile_sum, which == 0 ? r.ntiles : r.ntiles_np, r.gate, 0, r.ctl);
-}
-
 __global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply_batch(const BatchRec* __restrict__ rec,
                                                                 const int2* __restrict__ map,
                                                                 int which) {
   const int2 m = map[blockIdx.x];
   const RebuildArgs& r = rec[m.x].r;
   if (which == 0)
-    scan_apply_body(r.cell_count, r.gncell, r.tile_sum, r.cell_start, r.cursor, nullptr, r.gate,
-                    0, r.ctl, m.y);
+    scan_apply_body(r.cell_count, r.gncell, r.tile_sum, r.cell_start, r.cursor, nullptr, 1,
+                    r.gate, 0, r.ctl, m.y);
   else
-    scan_apply_body(r.slice_w, r.nslices, r.tile_sum, nullptr, nullptr, r.d_es, r.gate, 0,
+    scan_apply_body(r.slice_w, r.nslices, r.tile_sum, nullptr, nullptr, r.d_es, 0, r.gate, 0,
                     r.ctl, m.y);
 }
 

@@ -601,6 +601,7 @@ static int alloc_state(dem_handle_t h, int np) {
   }
   HIPCHK(h, hipMemcpyAsync(h->d_es, &h->es, sizeof(EdgeSets), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemsetAsync(h->acc_p, 0, 3 * N * sizeof(double2), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->cell_count, 0, ((size_t)h->gncell + 1) * sizeof(int), h->stream));
   h->np = np;
   h->sc.np = np;
   return DEM_OK;
@@ -628,17 +629,14 @@ static void enqueue_binning(dem_handle_t h, const double4* pos, const int* gate,
   hipStream_t st = h->stream;
   const int ng = gs_blocks(h->gncell + 1);
   const int npb = gs_blocks(s.np);
-  hipLaunchKernelGGL(k_fill_i32, dim3(ng), dim3(DEM_BLOCK), 0, st, h->cell_count, h->gncell + 1,
-                     0, gate, force, h->ctl);
+  // the cell counts are zero: allocated so, and cleared by every scan of them
   hipLaunchKernelGGL(k_bin_count, dim3(npb), dim3(DEM_BLOCK), 0, st, s, pos, h->key,
                      h->cell_count, gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles), dim3(DEM_BLOCK), 0, st, h->cell_count,
                      h->gncell, h->tile_sum, gate, force, h->ctl);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles, gate,
-                     force, h->ctl);
   hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles), dim3(DEM_BLOCK), 0, st, h->cell_count,
                      h->gncell, h->tile_sum, h->cell_start, h->cursor, (const EdgeSets*)nullptr,
-                     gate, force, h->ctl);
+                     1, gate, force, h->ctl);
   hipLaunchKernelGGL(k_bin_scatter, dim3(npb), dim3(DEM_BLOCK), 0, st, s.np, h->key, h->cursor,
                      h->cell_idx, gate, force, h->ctl);
   hipLaunchKernelGGL(k_cell_sort, dim3(ng), dim3(DEM_BLOCK), 0, st, h->gncell, h->cell_start,
@@ -661,11 +659,9 @@ static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
                      h->cell_idx, h->es, h->slice_w, gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_tiles, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
                      h->nslices, h->tile_sum, gate, force, h->ctl);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(DEM_BLOCK), 0, st, h->tile_sum, h->ntiles_np,
-                     gate, force, h->ctl);
   hipLaunchKernelGGL(k_scan_apply, dim3(h->ntiles_np), dim3(DEM_BLOCK), 0, st, h->slice_w,
-                     h->nslices, h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, gate, force,
-                     h->ctl);
+                     h->nslices, h->tile_sum, (int*)nullptr, (int*)nullptr, h->d_es, 0, gate,
+                     force, h->ctl);
   hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->cell_start,
                      h->cell_idx, h->es, src, o2n, (const Kin*)h->kin[b], h->xref, gate, force,
                      h->ctl);
@@ -1582,7 +1578,8 @@ static void launch_pass(dem_handle_t h, const PassArgs& a0) {
     launch_shape<MODE, UPDATE, true>(h->shape, nb, h->stream, h->sc, a);
   else
     launch_shape<MODE, UPDATE, false>(h->shape, nb, h->stream, h->sc, a);
-  hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl,
+  const int rb = std::min(32, nblocks(nb, 1024));
+  hipLaunchKernelGGL(k_pass_reduce_wide, dim3(rb), dim3(1024), 0, h->stream, h->ctl,
                      (const unsigned*)a.part, nb, a.par, UPDATE ? 1 : 0);
 }
 
@@ -2744,12 +2741,9 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
     launch_batch_shape<MODE_CORR, false, false, 1>(b->shape, gq, st, rec, mq);
   hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 1, 0);
   // neighbour lists of the systems whose pass 2 asked for one
-  hipLaunchKernelGGL(k_fill_cells_batch, dim3(gc), dim3(DEM_BLOCK), 0, st, rec,
-                     (const int2*)b->d_map[MAP_CELL]);
   hipLaunchKernelGGL(k_bin_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
   hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
                      (const int2*)b->d_map[MAP_CTILE], 0);
-  hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 0);
   hipLaunchKernelGGL(k_scan_apply_batch, dim3(gt), dim3(DEM_BLOCK), 0, st, rec,
                      (const int2*)b->d_map[MAP_CTILE], 0);
   hipLaunchKernelGGL(k_bin_scatter_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
@@ -2758,7 +2752,6 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
   hipLaunchKernelGGL(k_nbr_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
   hipLaunchKernelGGL(k_scan_tiles_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
                      (const int2*)b->d_map[MAP_STILE], 1);
-  hipLaunchKernelGGL(k_scan_spine_batch, dim3(ns), dim3(DEM_BLOCK), 0, st, rec, 1);
   hipLaunchKernelGGL(k_scan_apply_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
                      (const int2*)b->d_map[MAP_STILE], 1);
   hipLaunchKernelGGL(k_nbr_fill_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);

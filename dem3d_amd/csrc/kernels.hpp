@@ -574,39 +574,87 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) 
 
 // Sums the per-block counters of the pass just run into the control block
 // and publishes the escapes its corrector found.  Runs as one block.
-__device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part, int nblk,
-                                                 int par, int update) {
-  if (ctl->escaped) return;  // the pass did not run
+// Sums of the four counters over the per-block slots [b0, b1) by one block
+// of 1024 threads; the totals are valid in thread 0.
+__device__ __forceinline__ void sum_slots(const unsigned* part, int b0, int b1,
+                                          unsigned long long s[4]) {
   __shared__ unsigned long long sh[1024 / 64][4];
   const int t = threadIdx.x;
   unsigned long long v[4] = {0, 0, 0, 0};
-  for (int b = t; b < nblk; b += 1024)
+  for (int b = b0 + t; b < b1; b += 1024)
     for (int c = 0; c < 4; ++c) v[c] += part[4 * (size_t)b + c];
   for (int c = 0; c < 4; ++c) v[c] = wave_sum_u64(v[c]);
   if ((t & 63) == 0)
     for (int c = 0; c < 4; ++c) sh[t >> 6][c] = v[c];
   __syncthreads();
-  if (t == 0) {
-    unsigned long long s[4] = {0, 0, 0, 0};
+  for (int c = 0; c < 4; ++c) s[c] = 0;
+  if (t == 0)
     for (int w = 0; w < 1024 / 64; ++w)
       for (int c = 0; c < 4; ++c) s[c] += sh[w][c];
-    ctl->ncol += s[0];
-    ctl->pairs[par] = s[1];
-    ctl->wallc[par] = s[2];
-    if (update) {
-      ctl->stored[par] = s[3];
-      ctl->pairs_upd_sum += s[1];
-      ctl->upd_evals += 1;
-    }
-    if (ctl->esc_pending) {
-      ctl->escaped = ctl->esc_pending;
-      ctl->esc_pending = 0;
-    }
+}
+
+// Counter totals s of a pass into the control block, and the escapes its
+// corrector found published (thread 0 only).
+__device__ __forceinline__ void publish_counts(Ctl* ctl, const unsigned long long s[4], int par,
+                                               int update) {
+  ctl->ncol += s[0];
+  ctl->pairs[par] = s[1];
+  ctl->wallc[par] = s[2];
+  if (update) {
+    ctl->stored[par] = s[3];
+    ctl->pairs_upd_sum += s[1];
+    ctl->upd_evals += 1;
   }
+  if (ctl->esc_pending) {
+    ctl->escaped = ctl->esc_pending;
+    ctl->esc_pending = 0;
+  }
+}
+
+__device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part, int nblk,
+                                                 int par, int update) {
+  if (ctl->escaped) return;  // the pass did not run
+  const int t = threadIdx.x;
+  unsigned long long s[4];
+  sum_slots(part, 0, nblk, s);
+  if (t == 0) publish_counts(ctl, s, par, update);
 }
 __global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* part, int nblk,
                                                      int par, int update) {
   pass_reduce_body(ctl, part, nblk, par, update);
+}
+
+// The same sums for a large pass, spread over gridDim.x <= 32 blocks: each
+// block sums a contiguous range of slots into ctl->red_part, and the block
+// that finishes last (counted on ctl->red_done; stores made visible with
+// device-scope fences before the count and read after it) adds the partial
+// sums in block order and publishes them.  The integer sums are the same as
+// k_pass_reduce's.
+__global__ __launch_bounds__(1024) void k_pass_reduce_wide(Ctl* ctl, const unsigned* part,
+                                                          int nblk, int par, int update) {
+  if (ctl->escaped) return;  // the pass did not run (the same in every block)
+  __shared__ int last;
+  const int t = threadIdx.x, nb = gridDim.x;
+  const int per = (nblk + nb - 1) / nb;
+  const int b0 = min(nblk, (int)blockIdx.x * per), b1 = min(nblk, b0 + per);
+  unsigned long long s[4];
+  sum_slots(part, b0, b1, s);
+  if (t == 0) {
+    for (int c = 0; c < 4; ++c) ctl->red_part[blockIdx.x][c] = s[c];
+    __threadfence();
+    last = atomicAdd(&ctl->red_done, 1u) == (unsigned)(nb - 1);
+  }
+  __syncthreads();
+  if (last && t == 0) {
+    __threadfence();
+    unsigned long long tot[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nb; ++k)
+      for (int c = 0; c < 4; ++c)
+        tot[c] += __hip_atomic_load(&ctl->red_part[k][c], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    publish_counts(ctl, tot, par, update);
+    ctl->red_done = 0;
+  }
 }
 
 // MoveWalls(Time) (src/MDEM.f90:392-408): every axis of every wall, one
@@ -685,15 +733,6 @@ __device__ __forceinline__ bool gate_closed(const int* gate, int force, const Ct
 // the single-system kernels pass blockIdx.x / gridDim.x, the batched ones
 // (batch.hpp) the block's position within its own system.
 
-__device__ __forceinline__ void fill_i32_body(int* p, int n, int v, const int* gate, int force,
-                                              const Ctl* ctl, int blk, int nblk) {
-  if (gate_closed(gate, force, ctl)) return;
-  for (int i = blk * DEM_BLOCK + threadIdx.x; i < n; i += nblk * DEM_BLOCK) p[i] = v;
-}
-__global__ void k_fill_i32(int* p, int n, int v, const int* gate, int force, const Ctl* ctl) {
-  fill_i32_body(p, n, v, gate, force, ctl, blockIdx.x, gridDim.x);
-}
-
 __device__ __forceinline__ void bin_count_body(const SysConst& s, const double4* pos, int* key,
                                                int* count, const int* gate, int force,
                                                const Ctl* ctl, int blk, int nblk) {
@@ -758,34 +797,22 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_scan_tiles(const int* in, int n, 
   scan_tiles_body(in, n, tile_sum, gate, force, ctl, blockIdx.x);
 }
 
-__device__ __forceinline__ void scan_spine_body(int* tile_sum, int ntiles, const int* gate,
-                                                int force, const Ctl* ctl) {
-  if (gate_closed(gate, force, ctl)) return;
-  int carry = 0;
-  for (int b = 0; b < ntiles; b += DEM_BLOCK) {
-    const int t = b + threadIdx.x;
-    const int v = t < ntiles ? tile_sum[t] : 0;
-    int tot;
-    const int ex = block_scan_excl(v, &tot);
-    if (t < ntiles) tile_sum[t] = carry + ex;
-    carry += tot;
-  }
-}
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_spine(int* tile_sum, int ntiles,
-                                                          const int* gate, int force,
-                                                          const Ctl* ctl) {
-  scan_spine_body(tile_sum, ntiles, gate, force, ctl);
-}
-
 // Exclusive scan of in[0..n) into start[0..n] (start[n] = total); cursor (may
 // be null) receives a copy of start[0..n).  With out_sel != null the output
-// is the slice offsets of the edge set being built (1 - *epar).
-__device__ __forceinline__ void scan_apply_body(const int* in, int n, const int* tile_off,
-                                                int* start, int* cursor,
-                                                const EdgeSets* out_sel, const int* gate,
-                                                int force, const Ctl* ctl, int blk) {
+// is the slice offsets of the edge set being built (1 - *epar).  The offset
+// of tile blk is the sum of the tile totals before it (scan_tiles_body),
+// which the block adds up itself.  clear != 0 sets in[0..n) back to zero
+// once read (the cell counts, so the next binning starts from zero).
+__device__ __forceinline__ void scan_apply_body(int* in, int n, const int* tile_sum, int* start,
+                                                int* cursor, const EdgeSets* out_sel,
+                                                int clear, const int* gate, int force,
+                                                const Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
   if (out_sel) start = out_sel->soff[1 - *out_sel->epar];
+  int pre = 0;
+  for (int q = threadIdx.x; q < blk; q += DEM_BLOCK) pre += tile_sum[q];
+  int tile_off;
+  block_scan_excl(pre, &tile_off);
   const int base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
@@ -793,8 +820,11 @@ __device__ __forceinline__ void scan_apply_body(const int* in, int n, const int*
     v[k] = (base + k < n) ? in[base + k] : 0;
     s += v[k];
   }
+  if (clear)
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k < n) in[base + k] = 0;
   int tot;
-  int ex = block_scan_excl(s, &tot) + tile_off[blk];
+  int ex = block_scan_excl(s, &tot) + tile_off;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) {
       start[base + k] = ex;
@@ -804,12 +834,12 @@ __device__ __forceinline__ void scan_apply_body(const int* in, int n, const int*
   }
   if (base <= n - 1 && n - 1 < base + SCAN_ITEMS) start[n] = ex;
 }
-__global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(const int* in, int n,
-                                                          const int* tile_off, int* start,
-                                                          int* cursor, const EdgeSets* out_sel,
+__global__ __launch_bounds__(DEM_BLOCK) void k_scan_apply(int* in, int n, const int* tile_sum,
+                                                          int* start, int* cursor,
+                                                          const EdgeSets* out_sel, int clear,
                                                           const int* gate, int force,
                                                           const Ctl* ctl) {
-  scan_apply_body(in, n, tile_off, start, cursor, out_sel, gate, force, ctl, blockIdx.x);
+  scan_apply_body(in, n, tile_sum, start, cursor, out_sel, clear, gate, force, ctl, blockIdx.x);
 }
 
 __device__ __forceinline__ void bin_scatter_body(int np, const int* key, int* cursor,
