@@ -186,10 +186,12 @@ def ensemble_run(args):
         s.close()
     # the same steps from the same start, replayed from captured graphs
     sys64 = build(DEM_FP64)
-    v_64g, st_64g, _ = batched(sys64, args.ens_steps, graph=True,
-                               shape=tune_64["chosen"] if tune_64 else 0)
-    print(f"ensemble fp64 batched: {v_64b:.4g} launched one by one, {v_64g:.4g} from graphs",
-          file=sys.stderr, flush=True)
+    v_64g, st_64g = None, None
+    if not args.no_ens_graph:
+        v_64g, st_64g, _ = batched(sys64, args.ens_steps, graph=True,
+                                   shape=tune_64["chosen"] if tune_64 else 0)
+        print(f"ensemble fp64 batched: {v_64b:.4g} launched one by one, {v_64g:.4g} from graphs",
+              file=sys.stderr, flush=True)
     t_alone = 0.0
     ps_alone = 0.0
     for s, w in zip(sys64, members):
@@ -214,8 +216,9 @@ def ensemble_run(args):
         "fp64_batched_step_ms_p50": float(np.percentile(st_64, 50)),
         "fp64_pass_shape": tune_64,
         "fp64_batched_graph_particle_steps_per_s": v_64g,
-        "fp64_batched_graph_step_ms_p50": float(np.percentile(st_64g, 50)),
-        "graph_speedup_fp64": v_64g / v_64b,
+        "fp64_batched_graph_step_ms_p50": (float(np.percentile(st_64g, 50))
+                                           if st_64g is not None else None),
+        "graph_speedup_fp64": v_64g / v_64b if v_64g else None,
         "fp64_alone_particle_steps_per_s": v_alone,
         "fp64_alone_sample_steps": args.alone_steps,
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
@@ -258,6 +261,8 @@ def main(argv=None):
                     help="batched steps timed one by one for the p50/p99 step latency")
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="timed steps per launch shape of the autotuner (0 = default shape)")
+    ap.add_argument("--no-ens-graph", action="store_true",
+                    help="skip the graph-replay run of the ensemble (profiler runs)")
     ap.add_argument("--pass-shape", type=int, default=-1,
                     help="launch shape of the 4M runs' force passes instead of autotuning "
                          "(profiling runs)")
