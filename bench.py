@@ -105,7 +105,8 @@ def timed_run(wl, args, precision, hmode):
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
         tune = g.autotune(steps_per_shape=args.tune_steps)
-        print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}, "
+        print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}/"
+              f"{tune['depth']}, "
               f"ms {[round(m, 4) for m in tune['ms']]}", file=sys.stderr, flush=True)
     if args.warmup:
         g.step(args.warmup)

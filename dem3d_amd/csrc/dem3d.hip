@@ -1529,9 +1529,11 @@ static void enqueue_fluid(dem_handle_t h, const double4* pos, const Kin* kin) {
 
 static const int kShapeBlock[] = DEM_SHAPE_BLOCKS;
 static const int kShapeWaves[] = DEM_SHAPE_WAVES;
+static const int kShapeDepth[] = DEM_SHAPE_DEPTHS;
 static_assert(sizeof(kShapeBlock) == DEM_NSHAPES * sizeof(int) &&
-                  sizeof(kShapeWaves) == DEM_NSHAPES * sizeof(int),
-              "one block size and one occupancy per launch shape");
+                  sizeof(kShapeWaves) == DEM_NSHAPES * sizeof(int) &&
+                  sizeof(kShapeDepth) == DEM_NSHAPES * sizeof(int),
+              "one block size, occupancy and pipeline depth per launch shape");
 
 // The pass kernel in launch shape `shape` (nb blocks).  Only the steps'
 // passes (predictor, corrector) come in every shape; the others run in the
@@ -1556,6 +1558,14 @@ static void launch_shape(int shape, int nb, hipStream_t st, const SysConst& s, c
       case 5:
         hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 2>), dim3(nb), dim3(256), 0, st, s,
                            a);
+        return;
+      case 6:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 128, 2, 2>), dim3(nb), dim3(128), 0, st,
+                           s, a);
+        return;
+      case 7:
+        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 2, 2>), dim3(nb), dim3(256), 0, st,
+                           s, a);
         return;
       default:
         break;
@@ -1925,6 +1935,7 @@ int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, de
     }
     out->block[c] = kShapeBlock[c];
     out->waves[c] = kShapeWaves[c];
+    out->depth[c] = kShapeDepth[c];
     out->ms[c] = h->ms_pass1 + h->ms_pass2;
     if (out->ms[c] < out->ms[best]) best = c;
   }
@@ -2710,6 +2721,14 @@ static void launch_batch_shape(int shape, int nb, hipStream_t st, const BatchRec
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2>), dim3(nb), dim3(256), 0,
                          st, rec, map);
       return;
+    case 6:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128, 2, 2>), dim3(nb), dim3(128),
+                         0, st, rec, map);
+      return;
+    case 7:
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2, 2>), dim3(nb), dim3(256),
+                         0, st, rec, map);
+      return;
     default:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI>), dim3(nb), dim3(DEM_BLOCK), 0, st,
                          rec, map);
@@ -3040,6 +3059,7 @@ int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) 
     for (double t : b->step_ms) sum += t;
     out->block[c] = kShapeBlock[c];
     out->waves[c] = kShapeWaves[c];
+    out->depth[c] = kShapeDepth[c];
     out->ms[c] = sum / (double)steps_per_shape;
     if (out->ms[c] < out->ms[best]) best = c;
   }

@@ -198,7 +198,12 @@ __device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
   }
 }
 
-template <bool UPDATE, bool MIXED>
+// DEPTH = 1: an edge waits for one memory round trip (its index and flag were
+// requested during the previous edge).  DEPTH = 2: the partner records and
+// history of edge k+1 are requested before edge k is computed, and the
+// index of edge k+2 with them; ~40 more VGPRs, so it pairs with a 2-wave
+// occupancy target.  The arithmetic and its order are the same.
+template <bool UPDATE, bool MIXED, int DEPTH = 1>
 __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
                                           const PairConst* ptab, const CompConst* ctab,
                                           const int* __restrict__ nbr,
@@ -213,15 +218,39 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     j_n = nbr[e];
     fl_n = flag[e];
   }
-  for (int k = 0; k < deg; ++k, e += 64) {
-    const int j = j_n;
-    const unsigned char fl = fl_n;
-    if (k + 1 < deg) {
+  EdgeData nxt;
+  unsigned char fl_nx = 0;
+  if (DEPTH == 2 && deg > 0) {
+    load_edge(nxt, a, hist, ecap, e, j_n, fl_n);
+    fl_nx = fl_n;
+    if (deg > 1) {
       j_n = nbr[e + 64];
       fl_n = flag[e + 64];
     }
+  }
+  for (int k = 0; k < deg; ++k, e += 64) {
     EdgeData cur;
-    load_edge(cur, a, hist, ecap, e, j, fl);
+    unsigned char fl;
+    if constexpr (DEPTH == 2) {
+      cur = nxt;
+      fl = fl_nx;
+      if (k + 1 < deg) {
+        load_edge(nxt, a, hist, ecap, e + 64, j_n, fl_n);
+        fl_nx = fl_n;
+        if (k + 2 < deg) {
+          j_n = nbr[e + 128];
+          fl_n = flag[e + 128];
+        }
+      }
+    } else {
+      const int j = j_n;
+      fl = fl_n;
+      if (k + 1 < deg) {
+        j_n = nbr[e + 64];
+        fl_n = flag[e + 64];
+      }
+      load_edge(cur, a, hist, ecap, e, j, fl);
+    }
     const double4 Q = cur.Q;
     const Kin& KJ = cur.K;
     const double2 h0 = cur.h0, h1 = cur.h1, h2 = cur.h2;
@@ -298,7 +327,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
 // BS lanes per block.  The block size changes neither the arithmetic of a
 // lane nor the integer counter sums, so every block shape gives the same
 // results bit for bit.
-template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK>
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int DEPTH = 1>
 __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
@@ -349,10 +378,10 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   acc.err = 0;
   if (lds_tab) {
     __syncthreads();
-    pair_loop<UPDATE, MIXED>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+    pair_loop<UPDATE, MIXED, DEPTH>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
                       e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   } else {
-    pair_loop<UPDATE, MIXED>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
+    pair_loop<UPDATE, MIXED, DEPTH>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
                       e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   }
   d3 F = acc.F, T = acc.T;
@@ -554,22 +583,24 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   }
 }
 
-// Launch shapes of the force passes: lanes per block and the occupancy the
+// Launch shapes of the force passes: lanes per block, the occupancy the
 // register allocation targets (wavefronts per SIMD, 512/WAVES VGPRs each;
-// above 3 the pass spills to scratch).  Shape 0 is the default; the others
-// are candidates of the autotuner (dem_autotune).  Every shape gives the
-// same results bit for bit.
+// above 3 the pass spills to scratch) and the edge pipeline depth
+// (pair_loop).  Shape 0 is the default; the others are candidates of the
+// autotuner (dem_autotune).  Every shape gives the same results bit for bit.
 #ifndef DEM_NSHAPES
-#define DEM_NSHAPES 6
+#define DEM_NSHAPES 8
 #endif
-#define DEM_SHAPE_BLOCKS {256, 128, 64, 512, 256, 256}
-#define DEM_SHAPE_WAVES {DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, 4, 2}
+#define DEM_SHAPE_BLOCKS {256, 128, 64, 512, 256, 256, 128, 256}
+#define DEM_SHAPE_WAVES {DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, 4, 2, 2, 2}
+#define DEM_SHAPE_DEPTHS {1, 1, 1, 1, 1, 1, 2, 2}
 #define DEM_MIN_PASS_BLOCK 64
 
-template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int WAVES = DEM_PASS_WAVES>
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int WAVES = DEM_PASS_WAVES,
+          int DEPTH = 1>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass(
     SysConst s, PassArgs a) {
-  pass_body<MODE, UPDATE, MIXED, BS>(s, a, xcd_block());
+  pass_body<MODE, UPDATE, MIXED, BS, DEPTH>(s, a, xcd_block());
 }
 
 // Sums the per-block counters of the pass just run into the control block

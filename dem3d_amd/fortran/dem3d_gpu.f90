@@ -76,11 +76,12 @@ module dem3d_gpu
   end type dem_stats_t
 
   ! dem_tune_t: the autotuner's report (launch shapes of the force passes).
-  integer(c_int), parameter, public :: DEM_NSHAPES = 6
+  integer(c_int), parameter, public :: DEM_NSHAPES = 8
   type, bind(c), public :: dem_tune_t
     integer(c_int32_t) :: nshapes = 0, chosen = 0
     integer(c_int64_t) :: steps = 0
     integer(c_int32_t) :: block(DEM_NSHAPES) = 0, waves(DEM_NSHAPES) = 0
+    integer(c_int32_t) :: depth(DEM_NSHAPES) = 0
     real(c_double) :: ms(DEM_NSHAPES) = 0
   end type dem_tune_t
 

@@ -69,19 +69,20 @@ class ParticleT(C.Structure):
 assert C.sizeof(ParticleT) == 128
 
 
-NSHAPES = 6
+NSHAPES = 8
 
 
 class TuneT(C.Structure):
     """dem_tune_t: the autotuner's report."""
     _fields_ = [("nshapes", C.c_int32), ("chosen", C.c_int32), ("steps", C.c_int64),
                 ("block", C.c_int32 * NSHAPES), ("waves", C.c_int32 * NSHAPES),
-                ("ms", C.c_double * NSHAPES)]
+                ("depth", C.c_int32 * NSHAPES), ("ms", C.c_double * NSHAPES)]
 
     def asdict(self):
         n = self.nshapes
         return dict(chosen=self.chosen, steps=self.steps, block=list(self.block[:n]),
-                    waves=list(self.waves[:n]), ms=list(self.ms[:n]))
+                    waves=list(self.waves[:n]), depth=list(self.depth[:n]),
+                    ms=list(self.ms[:n]))
 
 
 class Stats(C.Structure):

@@ -459,10 +459,12 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
 
 /* ---- launch shapes and the autotuner ------------------------------------ */
 /* No counterpart in the reference.  The force passes come in DEM_NSHAPES
- * launch shapes: lanes per block and the occupancy (wavefronts per SIMD)
- * their register allocation targets.  Every shape gives bitwise the same
- * results, so a shape changes speed only; shape 0 (256 lanes, 3 waves) is
- * the default.  The autotuner times every shape on the live system while
+ * launch shapes: lanes per block, the occupancy (wavefronts per SIMD) their
+ * register allocation targets, and the depth of the edge pipeline (1: an
+ * edge waits for one memory round trip; 2: the next edge's partner records
+ * are in flight while an edge is computed).  Every shape gives bitwise the
+ * same results, so a shape changes speed only; shape 0 (256 lanes, 3 waves,
+ * depth 1) is the default.  The autotuner times every shape on the live system while
  * stepping it: each shape runs 1 untimed step (its first launch) and then
  * steps_per_shape steps timed with HIP events, and the fastest becomes the
  * system's (batch's) shape.  The steps count as ordinary MoveParticles steps
@@ -471,13 +473,14 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
  * number of steps.  Escapes and errors stop the tuning with dem_step's
  * return code, the shape unchanged.  A shape can also be set directly with
  * the option "pass_shape". */
-#define DEM_NSHAPES 6
+#define DEM_NSHAPES 8
 typedef struct {
   int32_t nshapes;             /* shapes timed                                 */
   int32_t chosen;              /* the fastest, now in use                      */
   int64_t steps;               /* MoveParticles steps the tuning advanced      */
   int32_t block[DEM_NSHAPES];  /* lanes per block of each shape                */
   int32_t waves[DEM_NSHAPES];  /* wavefronts per SIMD it targets               */
+  int32_t depth[DEM_NSHAPES];  /* edge pipeline depth                          */
   double ms[DEM_NSHAPES];      /* mean device ms per step: the two passes
                                   (dem_autotune), the whole step
                                   (dem_batch_autotune)                         */
