@@ -49,5 +49,33 @@ def build(verbose: bool = True) -> str:
     return OUT
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN_DIR, "drive_steps")
+
+
+def build_fortran_driver(verbose: bool = True):
+    """The example Fortran driver (fortran/drive_steps.f90 over the bindings
+    of fortran/dem3d_gpu.f90), linked against the built library with an
+    rpath relative to itself.  Skipped (returns None) without amdflang."""
+    fc = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod:
+        cmd = [fc, "-O2", "-module-dir", mod,
+               os.path.join(FORTRAN_DIR, "dem3d_gpu.f90"),
+               os.path.join(FORTRAN_DIR, "drive_steps.f90"),
+               "-o", DRIVER + ".tmp", OUT, "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build()
+    build_fortran_driver()
