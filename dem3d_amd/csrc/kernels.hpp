@@ -332,7 +332,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
   Ctl* ctl = a.ctl;
-  // Escapes are published between passes (k_pass_reduce), so every block of
+  // Escapes are published between passes (k_pass_reduce_*), so every block of
   // a pass reads the same value here.
   if (ctl->escaped) return;
   const int t = threadIdx.x;
@@ -555,7 +555,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       if (!(x.x == x.x && x.y == x.y && x.z == x.z)) err |= DEM_KERR_NAN;
     }
   }
-  // Counters: block sums to a per-block slot (k_pass_reduce adds them up);
+  // Counters: block sums to a per-block slot (k_pass_reduce_* add them up);
   // flags, which are rare, by atomics.
   __shared__ unsigned sh_cnt[BS / 64][4];
   const unsigned c0 = (unsigned)wave_sum_u64(n_col), c1 = (unsigned)wave_sum_u64(n_pair);
@@ -650,17 +650,12 @@ __device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part,
   sum_slots(part, 0, nblk, s);
   if (t == 0) publish_counts(ctl, s, par, update);
 }
-__global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, const unsigned* part, int nblk,
-                                                     int par, int update) {
-  pass_reduce_body(ctl, part, nblk, par, update);
-}
-
 // The same sums for a large pass, spread over gridDim.x <= 32 blocks: each
 // block sums a contiguous range of slots into ctl->red_part, and the block
 // that finishes last (counted on ctl->red_done; stores made visible with
 // device-scope fences before the count and read after it) adds the partial
 // sums in block order and publishes them.  The integer sums are the same as
-// k_pass_reduce's.
+// one block's (pass_reduce_body).
 __global__ __launch_bounds__(1024) void k_pass_reduce_wide(Ctl* ctl, const unsigned* part,
                                                           int nblk, int par, int update) {
   if (ctl->escaped) return;  // the pass did not run (the same in every block)
