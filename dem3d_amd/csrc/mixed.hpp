@@ -6,7 +6,8 @@
 // numbers); and the stored tangential spring FT -- its rotation into the new
 // tangent plane (ChangePlane, src/MForces.f90:391-406), its accumulation
 // (src/MForces.f90:356-358) and its Coulomb rescale (src/MForces.f90:376-379)
-// -- because it integrates over the whole life of a contact.  Everything
+// -- because it integrates over the whole life of a contact; and the time
+// since the first touch of a new contact (a cancelling root).  Everything
 // else of CommonCalculations (src/MForces.f90:339-387) -- normal, relative
 // velocities, normal force, the increment of the spring, the stored tangential
 // velocity VT, torques -- is FP32, and the per-particle sums of the FP32 pair
@@ -95,15 +96,19 @@ __device__ __forceinline__ MixedOut contact_mixed(bool contacting, d3 rij, doubl
     const f3 dvt = subf(o.vt, vt0r);
     ft = add3(ft0r, tod(addf(sclf(kt, ddt), sclf(yt, dvt))));
   } else {
-    const float vsq = dotf(vij, vij);
-    float pdt = 0.0f;
-    if (vsq > 0.0f) {
-      const float rv = dotf(tof(rij), vij);
-      const float t = (rv + sqrtf(rv * rv + vsq * (float)delta_dsq)) / vsq;
-      const float two_half = 2.0f * half_dt;
+    // Time since the first touch (src/MForces.f90:360-370) in FP64: for an
+    // approaching pair rv < 0 and vsq*delta << rv^2, so rv + sqrt(rv^2 +
+    // vsq*delta) cancels and FP32 would keep few digits of it.  Only new
+    // contacts take this branch.
+    const double vsq = dot3(dv, dv);
+    double pdt = 0.0;
+    if (vsq > 0.0) {
+      const double rv = dot3(rij, dv);
+      const double t = (rv + sqrt(rv * rv + vsq * delta_dsq)) / vsq;
+      const double two_half = 2.0 * (double)half_dt;
       pdt = t < two_half ? t : two_half;
     }
-    ft = tod(sclf(kt * pdt + yt, o.vt));
+    ft = tod(sclf(kt * (float)pdt + yt, o.vt));
   }
   const double mu_fn_sq = musq * (double)dotf(o.fn, o.fn);
   const double ft_sq = dot3(ft, ft);

@@ -95,8 +95,11 @@ def mono_settling_2k(seed: int = 20240001, steps: int = 20000) -> Workload:
 
 def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000, nbins: int = 16) -> Workload:
     """200 000 polydisperse spheres, radii U[0.4, 0.6] mm discretised into 16
-    equal-width components, at volume fraction ~0.45 (jittered FCC lattice),
-    base 40 x 40 mm, periodic x,y, floor, settling under gravity."""
+    equal-width components, on a jittered FCC lattice, base 40 x 40 mm,
+    periodic x,y, floor, settling under gravity.  The lattice aims at volume
+    fraction 0.45, but its nearest-neighbour distance a/sqrt(2) must exceed
+    the largest diameter (1.2 mm) so that no two spheres start in contact;
+    that bound sets a = 1.74 mm and phi ~ 0.41."""
     kn = 1e3
     rng = np.random.default_rng(seed)
     edges = np.linspace(0.4e-3, 0.6e-3, nbins + 1)
@@ -106,10 +109,11 @@ def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000
     rr = centres[comp]
     mean_vol = float(np.mean(4.0 / 3.0 * math.pi * rr ** 3))
     lx = ly = 40e-3
-    # FCC lattice constant for phi = 0.45
+    # FCC lattice constant for phi = 0.45, widened so the largest spheres do
+    # not touch (nearest neighbours at a/sqrt(2) > 1.2 mm, 2% margin)
     dens = 0.45 / mean_vol
-    a = (4.0 / dens) ** (1.0 / 3.0)
-    nxy = int(round(lx / a))
+    a = max((4.0 / dens) ** (1.0 / 3.0), math.sqrt(2.0) * 1.2e-3 * 1.02)
+    nxy = int(lx // a)
     a = lx / nxy
     nz = int(math.ceil(n / (4 * nxy * nxy)))
     lz = 2.0 * nz * a + 10e-3
@@ -126,7 +130,8 @@ def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000
     vel = rng.uniform(-0.05, 0.05, pos.shape)
     return Workload("poly-settling-200k", spec, red, (comp + 1).astype(np.int32), pos / L,
                     vel * T / L, np.zeros_like(pos), steps,
-                    note=f"FCC a={a*1e3:.4f} mm, phi~0.45, Lz={lz*1e3:.2f} mm")
+                    note=f"FCC a={a*1e3:.4f} mm, phi~{4.0 * mean_vol / a ** 3:.3f}, "
+                         f"Lz={lz*1e3:.2f} mm")
 
 
 def _fcc(nx, ny, nz, a):
