@@ -18,7 +18,7 @@ its last step), and the aggregate particle-steps/s over the run.
 from __future__ import annotations
 
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Callable, List, Optional
 
 import numpy as np

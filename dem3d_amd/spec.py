@@ -20,7 +20,7 @@ are the same bits (x**2 -> x*x, x**3 -> (x*x)*x, left-to-right products).
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 import numpy as np

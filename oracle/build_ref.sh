@@ -1,5 +1,5 @@
 #!/usr/bin/env bash
-# Builds the checkers of the DEM hot path.  TEST INFRASTRUCTURE ONLY.
+# Builds the oracles of the DEM hot path.  TEST INFRASTRUCTURE ONLY.
 #
 #  oracle/liboracle.so            C restatement (dem3d_oracle.c), gcc, no FMA
 #                                 contraction.  Always built.

@@ -1,7 +1,7 @@
 /*
  * dem3d_oracle.h -- CPU restatement of the DEM3D per-timestep path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the GPU path is
  * compared with; only tests/, __graft_entry__.smoke() and the cpu_baseline leg
  * of bench.py may load it.  The product (libdem3d_hip.so) never links or calls
  * it.

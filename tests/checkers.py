@@ -1,4 +1,4 @@
-"""Test-side access to the checkers: the C restatement (oracle/liboracle.so)
+"""Test-side access to the oracles: the C restatement (oracle/liboracle.so)
 and the reference Fortran modules (oracle/_ref/libdem3d_ref*.so).
 
 Only tests, smoke() and bench.py's cpu_baseline leg use this module.

@@ -22,8 +22,7 @@ import pytest
 
 from dem3d_amd import files, workloads
 from dem3d_amd.native import load_hip
-from checkers import (cells_to_rank, contacts_to_keys, oracle_system, ref_available,
-                      run_reference)
+from checkers import contacts_to_keys, oracle_system, ref_available, run_reference
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 live = pytest.mark.skipif(not ref_available(), reason="reference build not present")
