@@ -184,6 +184,10 @@ struct EdgeData {
   double2 h0, h1, h2;
 };
 
+// The partner's records and, for a stored pair, its history as the three
+// (FTx,FTy) (FTz,VTx) (VTy,VTz) pairs; the mixed mode reads its 36-byte
+// layout (FTz double, VT float3: hist_ftz, hist_vt32).
+template <bool MIXED>
 __device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
                                           const double2* __restrict__ hist, long long ecap,
                                           long long e, int j, unsigned char fl) {
@@ -191,8 +195,14 @@ __device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
   d.K = load_kin(a.kin_in + j);
   if (fl & 1) {
     d.h0 = hist[e];
-    d.h1 = hist[ecap + e];
-    d.h2 = hist[2 * ecap + e];
+    if (MIXED) {
+      const float* vt = hist_vt32(hist, ecap) + 3 * e;
+      d.h1 = make_double2(hist_ftz(hist, ecap)[e], (double)vt[0]);
+      d.h2 = make_double2((double)vt[1], (double)vt[2]);
+    } else {
+      d.h1 = hist[ecap + e];
+      d.h2 = hist[2 * ecap + e];
+    }
   } else {
     d.h0 = d.h1 = d.h2 = make_double2(0.0, 0.0);
   }
@@ -221,7 +231,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
   EdgeData nxt;
   unsigned char fl_nx = 0;
   if (DEPTH == 2 && deg > 0) {
-    load_edge(nxt, a, hist, ecap, e, j_n, fl_n);
+    load_edge<MIXED>(nxt, a, hist, ecap, e, j_n, fl_n);
     fl_nx = fl_n;
     if (deg > 1) {
       j_n = nbr[e + 64];
@@ -235,7 +245,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
       cur = nxt;
       fl = fl_nx;
       if (k + 1 < deg) {
-        load_edge(nxt, a, hist, ecap, e + 64, j_n, fl_n);
+        load_edge<MIXED>(nxt, a, hist, ecap, e + 64, j_n, fl_n);
         fl_nx = fl_n;
         if (k + 2 < deg) {
           j_n = nbr[e + 128];
@@ -249,7 +259,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
         j_n = nbr[e + 64];
         fl_n = flag[e + 64];
       }
-      load_edge(cur, a, hist, ecap, e, j, fl);
+      load_edge<MIXED>(cur, a, hist, ecap, e, j, fl);
     }
     const double4 Q = cur.Q;
     const Kin& KJ = cur.K;
@@ -284,10 +294,14 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
         kahan_add3(acc.Ff, o.ft);
         kahan_add3(acc.Tf, sclf((float)ri, o.tq_dir));
         if (UPDATE) {
-          const d3 f = scl3(sg, o.ft_store), g = scl3(sg, tod(o.vt));
+          const d3 f = scl3(sg, o.ft_store);
+          const float fs = (float)sg;
           hist[e] = make_double2(f.x, f.y);
-          hist[ecap + e] = make_double2(f.z, g.x);
-          hist[2 * ecap + e] = make_double2(g.y, g.z);
+          hist_ftz(hist, ecap)[e] = f.z;
+          float* vt = hist_vt32(hist, ecap) + 3 * e;
+          vt[0] = fs * o.vt.x;
+          vt[1] = fs * o.vt.y;
+          vt[2] = fs * o.vt.z;
           if (!(fl & 1)) flag[e] = 1;
           ++acc.n_stored;
         }
@@ -1028,8 +1042,17 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
           if (fl & 1) {
             double2* g = E.hist[q];
             g[e] = ohist[om];
-            g[ecap + e] = ohist[ecap + om];
-            g[2 * ecap + e] = ohist[2 * ecap + om];
+            if (s.vt32) {
+              hist_ftz(g, ecap)[e] = hist_ftz(ohist, ecap)[om];
+              const float* ov = hist_vt32(ohist, ecap) + 3 * om;
+              float* nv = hist_vt32(g, ecap) + 3 * e;
+              nv[0] = ov[0];
+              nv[1] = ov[1];
+              nv[2] = ov[2];
+            } else {
+              g[ecap + e] = ohist[ecap + om];
+              g[2 * ecap + e] = ohist[2 * ecap + om];
+            }
           }
           found |= 1ull << m;
           break;
