@@ -95,3 +95,34 @@ def test_mixed_run_to_run_bitwise_deterministic(hip):
         out.append(g.download()["x"].tobytes())
         g.close()
     assert out[0] == out[1]
+
+
+def test_mixed_state_round_trip_is_exact(hip):
+    """The mixed mode keeps VT in FP32 on the device: a contact list
+    downloaded (converted to FP64) and uploaded again (converted back) into a
+    fresh mixed system reads back bit for bit, and the two systems continue
+    together (the new one re-sorts its particles and rebuilds its lists, so
+    its sums run in another order: within round-off, not bitwise)."""
+    from helpers import full_state
+    wl = workloads.small_box(**CASES["bidisperse-floor"])
+    a = System(hip, wl.red, precision=DEM_MIXED)
+    a.upload(wl.comp, wl.x, wl.v, wl.w)
+    a.startup()
+    a.step(30)
+    st = full_state(a, with_rank=True)
+    con = a.download_contacts()
+    assert len(con["keys"]) > 20
+    b = System(hip, wl.red, precision=DEM_MIXED)
+    b.upload(st["comp"], st["x"], st["v"], st["w"], st["a"], st["alpha"], st["ring"], st["ipc"],
+             st["cell_rank"])
+    b.upload_contacts(con["keys"], con["ft"], con["vt"], con["wkeys"], con["wft"], con["wvt"])
+    cb = b.download_contacts()
+    for k in ("keys", "ft", "vt", "wkeys", "wft", "wvt"):
+        assert np.array_equal(cb[k], con[k]), k
+    a.step(20, first_step=a.next_step)
+    b.step(20, first_step=a.next_step - 20)
+    sa, sb = a.download(), b.download()
+    for k, rel in (("x", 1e-10), ("v", 1e-5), ("w", 1e-5)):
+        scale = np.abs(sa[k]).max()
+        assert np.abs(sa[k] - sb[k]).max() <= rel * scale, k
+    assert np.array_equal(a.download_contacts()["keys"], b.download_contacts()["keys"])
