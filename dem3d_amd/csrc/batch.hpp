@@ -57,14 +57,11 @@ struct BatchRec {
   double dt, time;
 };
 
-// Arguments of the next MoveParticles step of every system, from its
+// Arguments of the next MoveParticles step of a system, from its
 // bookkeeping; the same as the host's step_args would give (evaluation
 // parity, entry stamps, ring slots rotated as ipc = cshift(ipc, -1) between
-// the passes, src/MDEM.f90:329-334).  One thread per system.
-__global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  BatchRec& r = rec[k];
+// the passes, src/MDEM.f90:329-334).
+__device__ __forceinline__ void prepare_step(BatchRec& r) {
   const size_t N = (size_t)r.s.np;
   const int np = r.nprev;
   int ipc[3] = {r.ipc[0], r.ipc[1], r.ipc[2]};
@@ -104,6 +101,13 @@ __global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
   for (int q = 0; q < 3; ++q) r.ipc[q] = ipc[q];
 }
 
+// The first step of a dem_batch_step call; later steps are prepared by the
+// last kernel of the step before (k_nbr_flip_batch).  One thread per system.
+__global__ void k_batch_prepare(BatchRec* __restrict__ rec, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) prepare_step(rec[k]);
+}
+
 // Block maps: (system, block within the system) per block of a launch.
 // MAP_PASS: blocks of the force passes, of the batch's launch shape.
 enum { MAP_PART = 0, MAP_CELL = 1, MAP_CTILE = 2, MAP_STILE = 3, MAP_PASS = 4, MAP_KINDS = 5 };
@@ -123,17 +127,17 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) 
 }
 
 // One block per system.
-__global__ void k_move_walls_batch(const BatchRec* __restrict__ rec) {
-  const BatchRec& r = rec[blockIdx.x];
-  if (r.nw_move > 0) move_walls_body(r.walls, r.wm, r.nw_move, r.time, r.r.ctl);
-}
-
-// One block per system.
 __global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __restrict__ rec,
                                                             int pi, int update) {
   const BatchRec& r = rec[blockIdx.x];
   const PassArgs& a = r.p[pi];
   pass_reduce_body(a.ctl, a.part, r.pass_nblk, a.par, update);
+}
+
+// One block per system.
+__global__ void k_move_walls_batch(const BatchRec* __restrict__ rec) {
+  const BatchRec& r = rec[blockIdx.x];
+  if (r.nw_move > 0) move_walls_body(r.walls, r.wm, r.nw_move, r.time, r.r.ctl);
 }
 
 __global__ void k_bin_count_batch(const BatchRec* __restrict__ rec,
@@ -199,10 +203,13 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill_batch(const BatchRec* __
                 b.r.xref, b.r.gate, 0, b.r.ctl, m.y);
 }
 
-// One block per system.
-__global__ void k_nbr_flip_batch(const BatchRec* __restrict__ rec) {
-  const RebuildArgs& r = rec[blockIdx.x].r;
-  nbr_flip_body(r.es, r.gate, 0, r.ctl);
+// One block per system: the flip of the rebuild, then the arguments of the
+// system's next step.
+__global__ void k_nbr_flip_batch(BatchRec* __restrict__ rec) {
+  BatchRec& b = rec[blockIdx.x];
+  nbr_flip_body(b.r.es, b.r.gate, 0, b.r.ctl);
+  __syncthreads();
+  if (threadIdx.x == 0) prepare_step(b);
 }
 
 // Control blocks of all systems into one array (one copy back per call).
@@ -221,5 +228,6 @@ __global__ void k_reset_escape(const BatchRec* __restrict__ rec, int n) {
     c->esc_pending = 0;
     c->pairs_upd_sum = 0;
     c->upd_evals = 0;
+    c->red_done = 0;
   }
 }

@@ -2785,8 +2785,9 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
   const int gp = b->map_len[MAP_PART], gc = b->map_len[MAP_CELL];
   const int gt = b->map_len[MAP_CTILE], gs = b->map_len[MAP_STILE];
   const int2* mp = b->d_map[MAP_PART];
-  hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
-  // pass 1, pass 2 (one launch each for all systems) and their counters
+  // pass 1, pass 2 (one launch each for all systems) and their counters;
+  // the step's arguments were prepared by k_batch_prepare (first step of the
+  // call) or by the previous step's flip
   const int gq = b->map_len[MAP_PASS];
   const int2* mq = b->d_map[MAP_PASS];
   if (mixed)
@@ -3011,6 +3012,7 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   for (size_t i = 0; i < n; ++i) moving = moving || b->sys[i]->nw_move > 0;
   hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
                      ns);
+  hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
   if (b->use_graph) {
     rc = batch_graphs(b, ns, mixed, moving);
     if (rc) return rc;
