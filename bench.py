@@ -101,12 +101,14 @@ def timed_run(wl, args, precision, hmode):
     tune = None
     if args.pass_shape >= 0:
         g.set_option("pass_shape", args.pass_shape)
+        if args.block_map >= 0:
+            g.set_option("block_map", args.block_map)
     elif args.tune_steps > 0:
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
         tune = g.autotune(steps_per_shape=args.tune_steps)
         print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}/"
-              f"{tune['depth']}, "
+              f"{tune['depth']}, block order {tune['block_map']} {tune['ms_map']}, "
               f"ms {[round(m, 4) for m in tune['ms']]}", file=sys.stderr, flush=True)
     if args.warmup:
         g.step(args.warmup)
@@ -262,6 +264,9 @@ def main(argv=None):
                     help="batched steps timed one by one for the p50/p99 step latency")
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="timed steps per launch shape of the autotuner (0 = default shape)")
+    ap.add_argument("--block-map", type=int, default=-1,
+                    help="with --pass-shape: block order of the force passes (0 = particle "
+                         "ranges per XCD, 1 = dispatch order)")
     ap.add_argument("--no-ens-graph", action="store_true",
                     help="skip the graph-replay run of the ensemble (profiler runs)")
     ap.add_argument("--pass-shape", type=int, default=-1,

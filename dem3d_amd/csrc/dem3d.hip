@@ -133,6 +133,7 @@ struct dem_system {
   bool timing = false;
   bool mixed = false;        // DEM_MIXED force arithmetic
   int shape = 0;             // launch shape of the force passes (DEM_SHAPE_*)
+  int block_map = 0;         // 0: particle ranges per XCD, 1: dispatch order
   bool in_batch = false;     // stepped by a dem_batch (its own stream must stay idle)
   dem_batch_t batch = nullptr;
   bool batch_first = false;  // next batched step is the first since joining
@@ -396,6 +397,9 @@ int dem_set_option(dem_handle_t h, const char* name, double value) {
     if (!(value >= 0.0 && value < (double)DEM_NSHAPES) || value != (double)(int)value)
       return fail(h, DEM_ERR_ARG, "pass_shape must be an integer in [0, %d)", DEM_NSHAPES);
     h->shape = (int)value;
+  } else if (!strcmp(name, "block_map")) {
+    if (value != 0.0 && value != 1.0) return fail(h, DEM_ERR_ARG, "block_map must be 0 or 1");
+    h->block_map = (int)value;
   } else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
   return DEM_OK;
 }
@@ -1522,6 +1526,7 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
 
 static PassArgs base_args(dem_handle_t h) {
   PassArgs a{};
+  a.xcd_map = h->block_map == 0 ? 1 : 0;
   const int p = (int)(h->eval & 1);
   a.stamp_next = (int)(h->stamp_base + h->eval + 1);
   a.par = p;
@@ -1980,10 +1985,36 @@ int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, de
     out->ms[c] = h->ms_pass1 + h->ms_pass2;
     if (out->ms[c] < out->ms[best]) best = c;
   }
-  h->timing = timing;
+  // the other block order, in the best shape
+  const int map0 = h->block_map;
+  out->ms_map[map0] = out->ms[best];
   h->shape = best;
+  h->block_map = 1 - map0;
+  {
+    int64_t done = 0;
+    h->timing = false;
+    int rc = dem_step(h, 1, next, &done, nullptr);
+    out->steps += done;
+    next += done;
+    if (rc == DEM_OK) {
+      h->timing = true;
+      rc = dem_step(h, steps_per_shape, next, &done, nullptr);
+      out->steps += done;
+      next += done;
+    }
+    if (rc != DEM_OK) {
+      h->timing = timing;
+      h->shape = keep;
+      h->block_map = map0;
+      return rc;
+    }
+    out->ms_map[1 - map0] = h->ms_pass1 + h->ms_pass2;
+  }
+  if (!(out->ms_map[1 - map0] < out->ms_map[map0])) h->block_map = map0;
+  h->timing = timing;
   out->nshapes = DEM_NSHAPES;
   out->chosen = best;
+  out->block_map = h->block_map;
   return DEM_OK;
 }
 
@@ -2578,6 +2609,7 @@ struct dem_batch {
   bool use_graph = false;
   int graph_steps = 8;
   int shape = 0;  // launch shape of the force passes (DEM_SHAPE_*)
+  int block_map = 0;  // 0: particle ranges per XCD, 1: dispatch order
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<long long> gkey;
 };
@@ -2740,39 +2772,39 @@ static int batch_capacity(dem_batch_t b, size_t n) {
 // The batched pass kernel in launch shape `shape` over the pass-block map.
 template <int MODE, bool UPDATE, bool MIXED, int PI>
 static void launch_batch_shape(int shape, int nb, hipStream_t st, const BatchRec* rec,
-                               const int2* map) {
+                               const int2* map, int xcd) {
   switch (shape) {
     case 1:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128>), dim3(nb), dim3(128), 0, st,
-                         rec, map);
+                         rec, map, xcd);
       return;
     case 2:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 64>), dim3(nb), dim3(64), 0, st,
-                         rec, map);
+                         rec, map, xcd);
       return;
     case 3:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 512>), dim3(nb), dim3(512), 0, st,
-                         rec, map);
+                         rec, map, xcd);
       return;
     case 4:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 4>), dim3(nb), dim3(256), 0,
-                         st, rec, map);
+                         st, rec, map, xcd);
       return;
     case 5:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2>), dim3(nb), dim3(256), 0,
-                         st, rec, map);
+                         st, rec, map, xcd);
       return;
     case 6:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128, 2, 2>), dim3(nb), dim3(128),
-                         0, st, rec, map);
+                         0, st, rec, map, xcd);
       return;
     case 7:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2, 2>), dim3(nb), dim3(256),
-                         0, st, rec, map);
+                         0, st, rec, map, xcd);
       return;
     default:
       hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI>), dim3(nb), dim3(DEM_BLOCK), 0, st,
-                         rec, map);
+                         rec, map, xcd);
   }
 }
 
@@ -2791,15 +2823,19 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
   const int gq = b->map_len[MAP_PASS];
   const int2* mq = b->d_map[MAP_PASS];
   if (mixed)
-    launch_batch_shape<MODE_PRED, true, true, 0>(b->shape, gq, st, rec, mq);
+    launch_batch_shape<MODE_PRED, true, true, 0>(b->shape, gq, st, rec, mq,
+                                                    b->block_map == 0);
   else
-    launch_batch_shape<MODE_PRED, true, false, 0>(b->shape, gq, st, rec, mq);
+    launch_batch_shape<MODE_PRED, true, false, 0>(b->shape, gq, st, rec, mq,
+                                                    b->block_map == 0);
   hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 0, 1);
   if (moving) hipLaunchKernelGGL(k_move_walls_batch, dim3(ns), dim3(64), 0, st, rec);
   if (mixed)
-    launch_batch_shape<MODE_CORR, false, true, 1>(b->shape, gq, st, rec, mq);
+    launch_batch_shape<MODE_CORR, false, true, 1>(b->shape, gq, st, rec, mq,
+                                                    b->block_map == 0);
   else
-    launch_batch_shape<MODE_CORR, false, false, 1>(b->shape, gq, st, rec, mq);
+    launch_batch_shape<MODE_CORR, false, false, 1>(b->shape, gq, st, rec, mq,
+                                                    b->block_map == 0);
   hipLaunchKernelGGL(k_pass_reduce_batch, dim3(ns), dim3(1024), 0, st, rec, 1, 0);
   // neighbour lists of the systems whose pass 2 asked for one
   hipLaunchKernelGGL(k_bin_count_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
@@ -2855,7 +2891,7 @@ static int capture_steps(dem_batch_t b, int nsteps, int ns, bool mixed, bool mov
 // the moving walls).
 static int batch_graphs(dem_batch_t b, int ns, bool mixed, bool moving) {
   std::vector<long long> key = {(long long)(uintptr_t)b->d_rec, ns, mixed, moving,
-                                b->graph_steps, b->shape};
+                                b->graph_steps, b->shape, b->block_map};
   for (int k = 0; k < MAP_KINDS; ++k) {
     key.push_back((long long)(uintptr_t)b->d_map[k]);
     key.push_back(b->map_len[k]);
@@ -2966,6 +3002,9 @@ int dem_batch_set_option(dem_batch_t b, const char* name, double value) {
       return bfail(b, DEM_ERR_ARG, "pass_shape must be an integer in [0, %d)", DEM_NSHAPES);
     if ((int)value != b->shape) b->dirty = true;  // the pass-block map changes
     b->shape = (int)value;
+  } else if (!strcmp(name, "block_map")) {
+    if (value != 0.0 && value != 1.0) return bfail(b, DEM_ERR_ARG, "block_map must be 0 or 1");
+    b->block_map = (int)value;
   } else if (!strcmp(name, "graph_steps")) {
     if (!(value >= 1.0 && value <= 256.0))
       return bfail(b, DEM_ERR_ARG, "graph_steps must lie in [1, 256]");
@@ -3106,11 +3145,37 @@ int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) 
     out->ms[c] = sum / (double)steps_per_shape;
     if (out->ms[c] < out->ms[best]) best = c;
   }
-  b->timing = timing;
+  // the other block order, in the best shape
   if (b->shape != best) b->dirty = true;
   b->shape = best;
+  const int map0 = b->block_map;
+  out->ms_map[map0] = out->ms[best];
+  b->block_map = 1 - map0;
+  {
+    b->timing = false;
+    int rc = dem_batch_step(b, 1, done.data(), nullptr);
+    out->steps += done[0];
+    if (rc == DEM_OK) {
+      b->timing = true;
+      rc = dem_batch_step(b, steps_per_shape, done.data(), nullptr);
+      out->steps += done[0];
+    }
+    if (rc != DEM_OK) {
+      b->timing = timing;
+      if (b->shape != keep) b->dirty = true;
+      b->shape = keep;
+      b->block_map = map0;
+      return rc;
+    }
+    double sum = 0.0;
+    for (double t : b->step_ms) sum += t;
+    out->ms_map[1 - map0] = sum / (double)steps_per_shape;
+  }
+  if (!(out->ms_map[1 - map0] < out->ms_map[map0])) b->block_map = map0;
+  b->timing = timing;
   out->nshapes = DEM_NSHAPES;
   out->chosen = best;
+  out->block_map = b->block_map;
   return DEM_OK;
 }
 

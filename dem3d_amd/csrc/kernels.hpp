@@ -76,7 +76,8 @@ __device__ __forceinline__ d3 kin_v(const Kin& k) { return mk3(k.v0.x, k.v0.y, k
 __device__ __forceinline__ d3 kin_w(const Kin& k) { return mk3(k.v1.y, k.v2.x, k.v2.y); }
 
 // Blocks are dispatched round-robin over the 8 XCDs; map consecutive block
-// ranges to one XCD so a spatial slab of particles shares one L2.
+// ranges to one XCD so a spatial slab of particles shares one L2.  (The
+// force passes choose between this and the dispatch order: PassArgs::xcd_map.)
 __device__ __forceinline__ int xcd_block() {
   const int nb = gridDim.x, b = blockIdx.x;
   const int q = nb >> 3, r = nb & 7, x = b & 7, l = b >> 3;
@@ -152,6 +153,11 @@ struct PassArgs {
   // (psum_rd; null = read the slots).
   const double* psum_rd;
   double* psum_wr;
+  // block order: 1 = consecutive particle ranges per XCD (xcd_block), 0 =
+  // the dispatch order (consecutive blocks on different XCDs, all XCDs in
+  // one region of the particles at a time)
+  int xcd_map;
+  int pad_;
 };
 
 __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
@@ -614,7 +620,7 @@ template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int WAVES = DEM
           int DEPTH = 1>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass(
     SysConst s, PassArgs a) {
-  pass_body<MODE, UPDATE, MIXED, BS, DEPTH>(s, a, xcd_block());
+  pass_body<MODE, UPDATE, MIXED, BS, DEPTH>(s, a, a.xcd_map ? xcd_block() : (int)blockIdx.x);
 }
 
 // Sums the per-block counters of the pass just run into the control block

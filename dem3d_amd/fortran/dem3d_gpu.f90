@@ -83,6 +83,8 @@ module dem3d_gpu
     integer(c_int32_t) :: block(DEM_NSHAPES) = 0, waves(DEM_NSHAPES) = 0
     integer(c_int32_t) :: depth(DEM_NSHAPES) = 0
     real(c_double) :: ms(DEM_NSHAPES) = 0
+    integer(c_int32_t) :: block_map = 0, pad_ = 0
+    real(c_double) :: ms_map(2) = 0
   end type dem_tune_t
 
   ! dem_config_t: a configuration file (SaveConfiguration, src/MConfig.f90:109-236).

@@ -76,13 +76,15 @@ class TuneT(C.Structure):
     """dem_tune_t: the autotuner's report."""
     _fields_ = [("nshapes", C.c_int32), ("chosen", C.c_int32), ("steps", C.c_int64),
                 ("block", C.c_int32 * NSHAPES), ("waves", C.c_int32 * NSHAPES),
-                ("depth", C.c_int32 * NSHAPES), ("ms", C.c_double * NSHAPES)]
+                ("depth", C.c_int32 * NSHAPES), ("ms", C.c_double * NSHAPES),
+                ("block_map", C.c_int32), ("pad_", C.c_int32), ("ms_map", C.c_double * 2)]
 
     def asdict(self):
         n = self.nshapes
         return dict(chosen=self.chosen, steps=self.steps, block=list(self.block[:n]),
                     waves=list(self.waves[:n]), depth=list(self.depth[:n]),
-                    ms=list(self.ms[:n]))
+                    ms=list(self.ms[:n]), block_map=self.block_map,
+                    ms_map=list(self.ms_map))
 
 
 class Stats(C.Structure):
@@ -399,7 +401,7 @@ class System:
         self._check(self.lib.set_option(self.h, b"time", float(t)))
 
     def set_option(self, name: str, value: float):
-        """dem_set_option ("timing", "resort_interval", "pass_shape", ...)."""
+        """dem_set_option ("timing", "resort_interval", "pass_shape", "block_map", ...)."""
         self._check(self.lib.set_option(self.h, name.encode(), float(value)))
 
     def autotune(self, steps_per_shape=20):
@@ -592,7 +594,8 @@ class Batch:
         self._check(self.lib.batch_set_option(self.b, b"timing", 1.0 if on else 0.0))
 
     def set_option(self, name: str, value: float):
-        """dem_batch_set_option: "timing", "graph", "graph_steps", "pass_shape"."""
+        """dem_batch_set_option: "timing", "graph", "graph_steps", "pass_shape",
+        "block_map"."""
         self._check(self.lib.batch_set_option(self.b, name.encode(), float(value)))
 
     def autotune(self, steps_per_shape=20):

@@ -259,7 +259,8 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *   "time"              sets Time (src/MConfig.f90:61), e.g. from a saved
  *                       configuration before an Euler start-up
  *   "next_step"         sets the step number of the next batched step
- *   "pass_shape"        launch shape of the force passes (see dem_tune_t) */
+ *   "pass_shape"        launch shape of the force passes (see dem_tune_t)
+ *   "block_map"         block order of the force passes (see dem_tune_t) */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
 /* ---- fluid drag (mFluid) --------------------------------------------------- */
@@ -453,7 +454,8 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
  * them one by one (as fast on this hardware, and no recapture when the
  * membership changes).  "graph_steps": steps per replayed graph (1..256,
  * default 8); with "timing" every step is its own replay.
- * "pass_shape": launch shape of the force passes (see dem_tune_t). */
+ * "pass_shape", "block_map": launch shape and block order of the force
+ * passes (see dem_tune_t). */
 int dem_batch_set_option(dem_batch_t b, const char* name, double value);
 int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
 
@@ -470,9 +472,12 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
  * system's (batch's) shape.  The steps count as ordinary MoveParticles steps
  * (numbered from first_step, resp. each system's next_step), and the
  * trajectory is bitwise the one dem_step (dem_batch_step) gives for the same
- * number of steps.  Escapes and errors stop the tuning with dem_step's
- * return code, the shape unchanged.  A shape can also be set directly with
- * the option "pass_shape". */
+ * number of steps.  Then the best shape runs steps_per_shape + 1 more steps
+ * in the other block order and the faster order is kept: "block_map" 0 gives
+ * each XCD consecutive particle ranges, 1 keeps the dispatch order (all XCDs
+ * in one region of the particles at a time).  Escapes and errors stop the
+ * tuning with dem_step's return code, the shape and order unchanged.  Both
+ * can also be set directly with the options "pass_shape" and "block_map". */
 #define DEM_NSHAPES 8
 typedef struct {
   int32_t nshapes;             /* shapes timed                                 */
@@ -484,6 +489,9 @@ typedef struct {
   double ms[DEM_NSHAPES];      /* mean device ms per step: the two passes
                                   (dem_autotune), the whole step
                                   (dem_batch_autotune)                         */
+  int32_t block_map;           /* block order chosen (option "block_map")      */
+  int32_t pad_;
+  double ms_map[2];            /* the chosen shape in block order 0 and 1      */
 } dem_tune_t;
 int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step,
                  dem_tune_t* out);

@@ -50,22 +50,26 @@ def test_every_shape_gives_the_same_trajectory(hip, precision):
     for kw in CASES:
         ref = None
         for shape in range(NSHAPES):
-            s = _make(hip, kw, precision)
-            s.set_option("pass_shape", shape)
-            s.step(37)
-            st = _state(s)
-            s.close()
-            if ref is None:
-                ref = st
-            else:
-                _same(st, ref, shape)
+            for block_map in ((0, 1) if shape in (0, 1, 6) else (0,)):
+                s = _make(hip, kw, precision)
+                s.set_option("pass_shape", shape)
+                s.set_option("block_map", block_map)
+                s.step(37)
+                st = _state(s)
+                s.close()
+                if ref is None:
+                    ref = st
+                else:
+                    _same(st, ref, (shape, block_map))
 
 
 def test_autotune_keeps_the_trajectory(hip):
     kw = CASES[0]
     a = _make(hip, kw)
     rep = a.autotune(steps_per_shape=3)
-    assert rep["steps"] == NSHAPES * 4
+    assert rep["steps"] == NSHAPES * 4 + 4   # every shape, then the other block order
+    assert rep["block_map"] in (0, 1) and all(m > 0 for m in rep["ms_map"])
+    assert rep["ms_map"][rep["block_map"]] <= rep["ms_map"][1 - rep["block_map"]]
     assert len(rep["ms"]) == NSHAPES and all(m > 0 for m in rep["ms"])
     assert rep["chosen"] == int(np.argmin(rep["ms"]))
     assert rep["block"][0] == 256 and sorted(set(rep["block"])) == [64, 128, 256, 512]
@@ -82,7 +86,7 @@ def test_batch_shapes_and_autotune_equal_alone(hip):
     alone = []
     for kw in systems_kw:
         s = _make(hip, kw)
-        s.step(NSHAPES * 3 + 9 + 9)
+        s.step(NSHAPES * 3 + 3 + 9 + 9)
         alone.append(_state(s))
         s.close()
     systems = [_make(hip, kw) for kw in systems_kw]
@@ -90,9 +94,10 @@ def test_batch_shapes_and_autotune_equal_alone(hip):
     for s in systems:
         b.add(s)
     b.set_option("pass_shape", 2)
+    b.set_option("block_map", 1)
     b.step(9)
     rep = b.autotune(steps_per_shape=2)
-    assert rep["steps"] == NSHAPES * 3
+    assert rep["steps"] == NSHAPES * 3 + 3
     assert rep["chosen"] == int(np.argmin(rep["ms"]))
     b.step(9)
     for s, ref in zip(systems, alone):
@@ -106,4 +111,6 @@ def test_bad_shape_is_refused(hip):
         s.set_option("pass_shape", NSHAPES)
     with pytest.raises(Exception, match="pass_shape"):
         s.set_option("pass_shape", 1.5)
+    with pytest.raises(Exception, match="block_map"):
+        s.set_option("block_map", 2)
     s.close()
