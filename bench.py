@@ -252,7 +252,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cells", default="100,100,50", help="rock-salt cells (8 spheres each)")
     ap.add_argument("--cpu-cells", default="100,100,12")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--precision", choices=("fp64", "both"), default="both",
