@@ -234,3 +234,40 @@ def test_moving_walls_match_oracle(hip, case):
     assert np.array_equal(cg["keys"], co["keys"])
     assert np.array_equal(cg["wkeys"], co["wkeys"])
     assert len(co["wkeys"]) > 3
+
+
+@pytest.mark.parametrize("case", ["bidisperse-floor", "tri-periodic", "mono-dense",
+                                  "settling-2k"])
+def test_1000_steps_trajectory_and_observables(hip, case):
+    """Start-up plus 1 000 ABM steps from the same state on the GPU and in
+    the oracle.  The two sum each particle's contacts in different orders,
+    so they differ by round-off that the dynamics amplify.  The stated bars:
+    positions within 1e-10 and velocities within 1e-8 of their largest
+    values, kinetic energy within 1e-10 relative, the same pair-contact count
+    (so the same coordination number 2P/N), packing fraction within 1e-12,
+    and identical contact sets."""
+    from dem3d_amd.observables import coordination_number, packing_fraction
+    wl = (workloads.mono_settling_2k() if case == "settling-2k"
+          else workloads.small_box(**CASES[case]))
+    g, o = _pair(hip, wl)
+    upload_same([g, o], wl)
+    for s in (g, o):
+        s.startup()
+        s.step(1000)
+    sg, so = g.download(), o.download()
+    dx = np.abs(sg["x"] - so["x"]).max() / (np.abs(so["x"]).max() + 1e-300)
+    dv = np.abs(sg["v"] - so["v"]).max() / (np.abs(so["v"]).max() + 1e-300)
+    kg, ko = g.stats()["kinetic_energy"], o.stats()["kinetic_energy"]
+    pc_g, pc_o = g.stats()["pair_contacts"], o.stats()["pair_contacts"]
+    zg, zo = coordination_number(pc_g, wl.np), coordination_number(pc_o, wl.np)
+    pg = packing_fraction(wl.red, sg["x"], wl.comp)
+    po = packing_fraction(wl.red, so["x"], wl.comp)
+    keys_g, keys_o = g.download_contacts()["keys"], o.download_contacts()["keys"]
+    print(f"{case}: dx {dx:.3g} dv {dv:.3g} dKE {abs(kg - ko) / ko:.3g} Z {zg:.6f}/{zo:.6f} "
+          f"phi {pg:.12f}/{po:.12f} stored pairs {len(keys_g)}/{len(keys_o)}", flush=True)
+    assert dx <= 1e-10, dx
+    assert dv <= 1e-8, dv
+    assert abs(kg - ko) <= 1e-10 * ko, (kg, ko)
+    assert pc_g == pc_o and zg == zo, (zg, zo)
+    assert abs(pg - po) <= 1e-12, (pg, po)
+    assert np.array_equal(keys_g, keys_o)
