@@ -236,8 +236,11 @@ def test_moving_walls_match_oracle(hip, case):
     assert len(co["wkeys"]) > 3
 
 
+SETTLED_BED = dict(n_side=(7, 7, 6), nc=2, seed=21, vel=0.05, gravity=-5000.0, e=0.3, mu=0.4)
+
+
 @pytest.mark.parametrize("case", ["bidisperse-floor", "tri-periodic", "mono-dense",
-                                  "settling-2k"])
+                                  "settling-2k", "settled-bed"])
 def test_1000_steps_trajectory_and_observables(hip, case):
     """Start-up plus 1 000 ABM steps from the same state on the GPU and in
     the oracle.  The two sum each particle's contacts in different orders,
@@ -247,8 +250,12 @@ def test_1000_steps_trajectory_and_observables(hip, case):
     (so the same coordination number 2P/N), packing fraction within 1e-12,
     and identical contact sets."""
     from dem3d_amd.observables import coordination_number, packing_fraction
-    wl = (workloads.mono_settling_2k() if case == "settling-2k"
-          else workloads.small_box(**CASES[case]))
+    if case == "settling-2k":
+        wl = workloads.mono_settling_2k()
+    elif case == "settled-bed":   # packs onto the floor: ~550 lasting pair contacts
+        wl = workloads.small_box(**SETTLED_BED)
+    else:
+        wl = workloads.small_box(**CASES[case])
     g, o = _pair(hip, wl)
     upload_same([g, o], wl)
     for s in (g, o):
