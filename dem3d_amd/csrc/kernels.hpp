@@ -160,6 +160,43 @@ struct PassArgs {
   int pad_;
 };
 
+// The pair history and the ABM ring slots and sums are streamed: read or
+// written once per pass, by one lane.  Their loads and stores carry the
+// non-temporal hint, so the caches favour the partner records, which every
+// neighbour of a particle reads again.  (The same hint on the neighbour
+// indices and flags, or on the corrector's own Particle record, measured
+// slower: the next pass reads those again.)
+typedef double dem_dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+  const dem_dv2 v = __builtin_nontemporal_load(reinterpret_cast<const dem_dv2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  dem_dv2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<dem_dv2*>(p));
+}
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+// Nine planar components r[0], r[N], ..., r[8N] as three vectors.
+__device__ __forceinline__ void ld_ring(const double* r, size_t N, d3& v, d3& a, d3& l) {
+  v = mk3(ld_nt(r), ld_nt(r + N), ld_nt(r + 2 * N));
+  a = mk3(ld_nt(r + 3 * N), ld_nt(r + 4 * N), ld_nt(r + 5 * N));
+  l = mk3(ld_nt(r + 6 * N), ld_nt(r + 7 * N), ld_nt(r + 8 * N));
+}
+__device__ __forceinline__ void st_ring(double* r, size_t N, d3 v, d3 a, d3 l) {
+  st_nt(r, v.x); st_nt(r + N, v.y); st_nt(r + 2 * N, v.z);
+  st_nt(r + 3 * N, a.x); st_nt(r + 4 * N, a.y); st_nt(r + 5 * N, a.z);
+  st_nt(r + 6 * N, l.x); st_nt(r + 7 * N, l.y); st_nt(r + 8 * N, l.z);
+}
+
 __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
   o[0] = make_double2(a.x, a.y);
   o[1] = make_double2(a.z, b.x);
@@ -200,14 +237,14 @@ __device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
   d.Q = a.pos_in[j];
   d.K = load_kin(a.kin_in + j);
   if (fl & 1) {
-    d.h0 = hist[e];
+    d.h0 = ld_nt(hist + e);
     if (MIXED) {
       const float* vt = hist_vt32(hist, ecap) + 3 * e;
-      d.h1 = make_double2(hist_ftz(hist, ecap)[e], (double)vt[0]);
-      d.h2 = make_double2((double)vt[1], (double)vt[2]);
+      d.h1 = make_double2(ld_nt(hist_ftz(hist, ecap) + e), (double)ld_nt(vt));
+      d.h2 = make_double2((double)ld_nt(vt + 1), (double)ld_nt(vt + 2));
     } else {
-      d.h1 = hist[ecap + e];
-      d.h2 = hist[2 * ecap + e];
+      d.h1 = ld_nt(hist + ecap + e);
+      d.h2 = ld_nt(hist + 2 * ecap + e);
     }
   } else {
     d.h0 = d.h1 = d.h2 = make_double2(0.0, 0.0);
@@ -302,12 +339,12 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
         if (UPDATE) {
           const d3 f = scl3(sg, o.ft_store);
           const float fs = (float)sg;
-          hist[e] = make_double2(f.x, f.y);
-          hist_ftz(hist, ecap)[e] = f.z;
+          st_nt(hist + e, make_double2(f.x, f.y));
+          st_nt(hist_ftz(hist, ecap) + e, f.z);
           float* vt = hist_vt32(hist, ecap) + 3 * e;
-          vt[0] = fs * o.vt.x;
-          vt[1] = fs * o.vt.y;
-          vt[2] = fs * o.vt.z;
+          st_nt(vt, fs * o.vt.x);
+          st_nt(vt + 1, fs * o.vt.y);
+          st_nt(vt + 2, fs * o.vt.z);
           if (!(fl & 1)) flag[e] = 1;
           ++acc.n_stored;
         }
@@ -327,9 +364,9 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
       acc.T = add3(acc.T, role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft)));
       if (UPDATE) {
         const d3 f = scl3(sg, o.ft), g = scl3(sg, o.vt);
-        hist[e] = make_double2(f.x, f.y);
-        hist[ecap + e] = make_double2(f.z, g.x);
-        hist[2 * ecap + e] = make_double2(g.y, g.z);
+        st_nt(hist + e, make_double2(f.x, f.y));
+        st_nt(hist + ecap + e, make_double2(f.z, g.x));
+        st_nt(hist + 2 * ecap + e, make_double2(g.y, g.z));
         if (!(fl & 1)) flag[e] = 1;
         ++acc.n_stored;
       }
@@ -480,19 +517,17 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       // AB predictor (src/MDEM.f90:297-331)
       d3 sv = mk3(0, 0, 0), sa = sv, sl = sv;
       if (a.psum_rd) {
-        const double* r = a.psum_rd + i;
-        sv = mk3(r[0], r[N], r[2 * N]);
-        sa = mk3(r[3 * N], r[4 * N], r[5 * N]);
-        sl = mk3(r[6 * N], r[7 * N], r[8 * N]);
+        ld_ring(a.psum_rd + i, N, sv, sa, sl);
       } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           if (k >= s.nprev) break;
-          const double* r = a.slot_rd[k] + i;
           const double c = a.coef[k];
-          sv = add3(sv, scl3(c, mk3(r[0], r[N], r[2 * N])));
-          sa = add3(sa, scl3(c, mk3(r[3 * N], r[4 * N], r[5 * N])));
-          sl = add3(sl, scl3(c, mk3(r[6 * N], r[7 * N], r[8 * N])));
+          d3 rv, ra, rl;
+          ld_ring(a.slot_rd[k] + i, N, rv, ra, rl);
+          sv = add3(sv, scl3(c, rv));
+          sa = add3(sa, scl3(c, ra));
+          sl = add3(sl, scl3(c, rl));
         }
       }
       d3 xp = add3(add3(xi, scl3(a.coef0, vi)), sv);
@@ -505,10 +540,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       const RefCell rc = ref_cell(s, xp.x, xp.y, xp.z);
       store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
       if (s.nprev > 0) {
-        double* r = a.slot_wr + i;
-        r[0] = vi.x; r[N] = vi.y; r[2 * N] = vi.z;
-        r[3 * N] = acc.x; r[4 * N] = acc.y; r[5 * N] = acc.z;
-        r[6 * N] = alp.x; r[7 * N] = alp.y; r[8 * N] = alp.z;
+        st_ring(a.slot_wr + i, N, vi, acc, alp);
       } else {
         store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       }
@@ -530,11 +562,9 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (k >= s.nprev) break;
-        const double* r = a.slot_rd[k] + i;
         const double c = a.coef[k];
-        const d3 rv = mk3(r[0], r[N], r[2 * N]);
-        const d3 ra = mk3(r[3 * N], r[4 * N], r[5 * N]);
-        const d3 rl = mk3(r[6 * N], r[7 * N], r[8 * N]);
+        d3 rv, ra, rl;
+        ld_ring(a.slot_rd[k] + i, N, rv, ra, rl);
         sv = add3(sv, scl3(c, rv));
         sa = add3(sa, scl3(c, ra));
         sl = add3(sl, scl3(c, rl));
@@ -543,10 +573,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
         sbl = add3(sbl, scl3(a.ab[k], rl));
       }
       if (a.psum_wr) {
-        double* r = a.psum_wr + i;
-        r[0] = sb.x; r[N] = sb.y; r[2 * N] = sb.z;
-        r[3 * N] = sba.x; r[4 * N] = sba.y; r[5 * N] = sba.z;
-        r[6 * N] = sbl.x; r[7 * N] = sbl.y; r[8 * N] = sbl.z;
+        st_ring(a.psum_wr + i, N, sb, sba, sbl);
       }
       d3 x = add3(add3(mk3(X.x, X.y, X.z), scl3(a.coef0, vi)), sv);
       const d3 v = add3(add3(v0, scl3(a.coef0, acc)), sa);
