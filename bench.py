@@ -79,14 +79,15 @@ def cpu_model():
     return "unknown CPU"
 
 
-def traffic_from_profiles(workload):
-    """HBM bytes per pass-1 launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), committed in profiles/; None when absent."""
+def traffic_from_profiles(workload, kernel):
+    """HBM bytes per launch of kernel ("pass1" or "pass2") measured with
+    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) for this workload, committed in
+    profiles/; None when absent."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("pass1_bytes_per_launch")
+        return json.load(f).get(f"{kernel}_bytes_per_launch")
 
 
 def timed_run(wl, args, precision, hmode):
@@ -345,7 +346,7 @@ def main(argv=None):
     if st["ms_pass2"] > st["ms_pass1"]:
         dom_ms, dom_b, dom = (st["ms_pass2"], b2, "pass2")
     achieved = dom_b / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = traffic_from_profiles(wl.name)
+    traffic = traffic_from_profiles(wl.name, dom)
     out = {
         "metric": "particle-steps/sec at 4M spheres (FP64 vs mixed) + batched p50/p99 step latency",
         "value": value,

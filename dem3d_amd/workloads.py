@@ -182,7 +182,10 @@ def bidisperse_sheared_4m(seed: int = 20240003, steps: int = 2000, cells=(100, 1
     comp = np.where(large, 2, 1).astype(np.int32)
     L = red.length_scale
     n = comp.shape[0]
-    return Workload("bidisperse-sheared-4M", spec, red, comp, pos / L,
+    # the 4 M bed under its own name; a cut-down bed is named by its cells
+    name = ("bidisperse-sheared-4M" if (nx, ny, nz) == (100, 100, 50)
+            else f"bidisperse-sheared-{nx}x{ny}x{nz}")
+    return Workload(name, spec, red, comp, pos / L,
                     np.zeros((3, n)), np.zeros((3, n)), steps,
                     note=f"rock-salt a={a*1e3:.5f} mm, {nx}x{ny}x{nz} cells, phi=0.567, "
                          f"top wall v_x={vtop:.4f} m/s")
