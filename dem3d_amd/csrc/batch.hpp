@@ -131,7 +131,7 @@ __global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __re
                                                             int pi, int update) {
   const BatchRec& r = rec[blockIdx.x];
   const PassArgs& a = r.p[pi];
-  pass_reduce_body(a.ctl, a.part, r.pass_nblk, a.par, update);
+  pass_reduce_body(a.ctl, a.part, min(r.pass_nblk, DEM_RED_SLOTS), a.par, update);
 }
 
 // One block per system.
@@ -228,6 +228,5 @@ __global__ void k_reset_escape(const BatchRec* __restrict__ rec, int n) {
     c->esc_pending = 0;
     c->pairs_upd_sum = 0;
     c->upd_evals = 0;
-    c->red_done = 0;
   }
 }

@@ -63,11 +63,6 @@ struct Ctl {
   unsigned long long stale_dropped; // stored pairs lost from the list at a rebuild
   unsigned long long pairs_upd_sum; // pair contacts summed over updating passes
   unsigned long long upd_evals;     // updating passes summed
-  // k_pass_reduce_wide: partial counter sums of its blocks, and the count of
-  // blocks done (back to 0 when the last block has summed them)
-  unsigned long long red_part[32][4];
-  unsigned red_done;
-  int pad_;
 };
 
 enum {

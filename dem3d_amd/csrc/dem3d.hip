@@ -599,6 +599,10 @@ static int alloc_state(dem_handle_t h, int np) {
   if (e == hipSuccess) e = dalloc(&h->slice_w, (size_t)h->nslices);
   if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
   if (e == hipSuccess) e = dalloc(&h->part, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK));
+  // the counter slots start at zero; every reduction clears what it read
+  if (e == hipSuccess)
+    e = hipMemsetAsync(h->part, 0, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK) * sizeof(unsigned),
+                       h->stream);
   if (e != hipSuccess) {
     free_state(h);
     return fail(h, DEM_ERR_DEVICE, "allocation for %d particles failed: %s", np,
@@ -1634,9 +1638,8 @@ static void launch_pass(dem_handle_t h, const PassArgs& a0) {
     launch_shape<MODE, UPDATE, true>(h->shape, nb, h->stream, h->sc, a);
   else
     launch_shape<MODE, UPDATE, false>(h->shape, nb, h->stream, h->sc, a);
-  const int rb = std::min(32, nblocks(nb, 1024));
-  hipLaunchKernelGGL(k_pass_reduce_wide, dim3(rb), dim3(1024), 0, h->stream, h->ctl,
-                     (const unsigned*)a.part, nb, a.par, UPDATE ? 1 : 0);
+  hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl, a.part, nb, a.par,
+                     UPDATE ? 1 : 0);
 }
 
 static void after_eval(dem_handle_t h, bool update, int b) {

@@ -37,6 +37,10 @@
 #ifndef DEM_PASS_WAVES
 #define DEM_PASS_WAVES 3
 #endif
+// Counter slots of a pass: block b adds its counts into slot b % DEM_RED_SLOTS,
+// and the reduction after the pass reads and clears min(blocks, DEM_RED_SLOTS)
+// slots, so a pass leaves them zero for the next one.
+#define DEM_RED_SLOTS 256
 // Default occupancy target of the force passes (wavefronts per SIMD): the
 // register budget is 512 / DEM_PASS_WAVES VGPRs.
 
@@ -138,7 +142,7 @@ struct PassArgs {
   double* whist;
   int* rb_clear;             // rebuild flag consumed before this evaluation
   int* rb_set;               // rebuild flag for the next evaluation
-  unsigned* part;            // per-block counters: collisions, pairs, walls, stored
+  unsigned* part;            // counter slots (DEM_RED_SLOTS): collisions, pairs, walls, stored
   Ctl* ctl;
   // PRED: Predicted%Index by reference id when it differs from Particle%Index
   // (after removals, until the next resize, src/MDEM.f90:636-640); else null
@@ -626,7 +630,8 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   if (t < 4) {
     unsigned v = 0;
     for (int w = 0; w < BS / 64; ++w) v += sh_cnt[w][t];
-    a.part[4 * (size_t)blk + t] = v;
+    // integer adds: the totals do not depend on the order of the blocks
+    if (v) atomicAdd(&a.part[4 * (size_t)(blk % DEM_RED_SLOTS) + t], v);
   }
 }
 
@@ -654,13 +659,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) 
 // and publishes the escapes its corrector found.  Runs as one block.
 // Sums of the four counters over the per-block slots [b0, b1) by one block
 // of 1024 threads; the totals are valid in thread 0.
-__device__ __forceinline__ void sum_slots(const unsigned* part, int b0, int b1,
+__device__ __forceinline__ void sum_slots(unsigned* part, int b0, int b1,
                                           unsigned long long s[4]) {
   __shared__ unsigned long long sh[1024 / 64][4];
   const int t = threadIdx.x;
   unsigned long long v[4] = {0, 0, 0, 0};
   for (int b = b0 + t; b < b1; b += 1024)
-    for (int c = 0; c < 4; ++c) v[c] += part[4 * (size_t)b + c];
+    for (int c = 0; c < 4; ++c) {
+      v[c] += part[4 * (size_t)b + c];
+      part[4 * (size_t)b + c] = 0u;
+    }
   for (int c = 0; c < 4; ++c) v[c] = wave_sum_u64(v[c]);
   if ((t & 63) == 0)
     for (int c = 0; c < 4; ++c) sh[t >> 6][c] = v[c];
@@ -689,45 +697,19 @@ __device__ __forceinline__ void publish_counts(Ctl* ctl, const unsigned long lon
   }
 }
 
-__device__ __forceinline__ void pass_reduce_body(Ctl* ctl, const unsigned* part, int nblk,
-                                                 int par, int update) {
+// nslots <= DEM_RED_SLOTS counter slots of a pass, summed by one block,
+// cleared and published.
+__device__ __forceinline__ void pass_reduce_body(Ctl* ctl, unsigned* part, int nslots, int par,
+                                                 int update) {
   if (ctl->escaped) return;  // the pass did not run
   const int t = threadIdx.x;
   unsigned long long s[4];
-  sum_slots(part, 0, nblk, s);
+  sum_slots(part, 0, nslots, s);
   if (t == 0) publish_counts(ctl, s, par, update);
 }
-// The same sums for a large pass, spread over gridDim.x <= 32 blocks: each
-// block sums a contiguous range of slots into ctl->red_part, and the block
-// that finishes last (counted on ctl->red_done; stores made visible with
-// device-scope fences before the count and read after it) adds the partial
-// sums in block order and publishes them.  The integer sums are the same as
-// one block's (pass_reduce_body).
-__global__ __launch_bounds__(1024) void k_pass_reduce_wide(Ctl* ctl, const unsigned* part,
-                                                          int nblk, int par, int update) {
-  if (ctl->escaped) return;  // the pass did not run (the same in every block)
-  __shared__ int last;
-  const int t = threadIdx.x, nb = gridDim.x;
-  const int per = (nblk + nb - 1) / nb;
-  const int b0 = min(nblk, (int)blockIdx.x * per), b1 = min(nblk, b0 + per);
-  unsigned long long s[4];
-  sum_slots(part, b0, b1, s);
-  if (t == 0) {
-    for (int c = 0; c < 4; ++c) ctl->red_part[blockIdx.x][c] = s[c];
-    __threadfence();
-    last = atomicAdd(&ctl->red_done, 1u) == (unsigned)(nb - 1);
-  }
-  __syncthreads();
-  if (last && t == 0) {
-    __threadfence();
-    unsigned long long tot[4] = {0, 0, 0, 0};
-    for (int k = 0; k < nb; ++k)
-      for (int c = 0; c < 4; ++c)
-        tot[c] += __hip_atomic_load(&ctl->red_part[k][c], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    publish_counts(ctl, tot, par, update);
-    ctl->red_done = 0;
-  }
+__global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, unsigned* part, int nblk,
+                                                      int par, int update) {
+  pass_reduce_body(ctl, part, min(nblk, DEM_RED_SLOTS), par, update);
 }
 
 // MoveWalls(Time) (src/MDEM.f90:392-408): every axis of every wall, one
