@@ -191,6 +191,72 @@ def bidisperse_sheared_4m(seed: int = 20240003, steps: int = 2000, cells=(100, 1
                          f"top wall v_x={vtop:.4f} m/s")
 
 
+BED_ASPECT = 0.4726  # bed height / side of the 4 M bed (0.1134 m over 0.24 m)
+
+
+def bidisperse_sheared_bed(n: int = 4_000_000, seed: int = 20240003, phi: float = 0.60,
+                           steps: int = 2000, shear_rate: float = 10.0,
+                           lxy: Optional[float] = None):
+    """The dense sheared bed of bidisperse spheres: n spheres of d = 1.0 and
+    1.4 mm, half of each (species assigned by a seeded shuffle), 2500 kg/m^3,
+    kn = 1e3 N/m, e = 0.8, mu = 0.3 (pairs and walls).  The box is periodic in
+    x and y (side lxy, 0.24 m by default from 1 M spheres up, else chosen so
+    that the bed is 0.47 times as high as wide), with a floor at z = -Lz/2 and
+    a rough top wall (mu = 0.3) at the height that holds the spheres at
+    volume fraction phi; the top wall moves in +x at shear_rate * H.  Gravity
+    -9.81 z.
+
+    Returns (spec, comp, x, growth): the specification, the components, the
+    seeded sparse-lattice start (metres, radii to be grown) and the growth
+    protocol that packs it (dem3d_amd.packing) -- the packed bed is the
+    product of running that protocol on the GPU, see packed_bed()."""
+    from .packing import Growth, lattice_start
+
+    ds, dl, kn = 1.0e-3, 1.4e-3, 1e3
+    vmean = 0.5 * (math.pi / 6.0) * (ds ** 3 + dl ** 3)
+    if lxy is None:
+        lxy = 0.24 if n >= 1_000_000 else (n * vmean / (phi * BED_ASPECT)) ** (1.0 / 3.0)
+    nl = n // 2
+    rng = np.random.default_rng(seed)
+    comp = np.where(rng.permutation(n) < nl, 1, 2).astype(np.int32)
+    vol = nl * (math.pi / 6.0) * ds ** 3 + (n - nl) * (math.pi / 6.0) * dl ** 3
+    h = vol / (phi * lxy * lxy)
+    lz = h + 2.0e-3
+    floor_z = -0.5 * lz
+    ztop = floor_z + h
+    vtop = shear_rate * h
+    walls = _floor(lxy, lxy, lz, kn, top=ztop, top_velocity=(vtop, 0.0, 0.0))
+    # walls[1] normal must point into the bed: W x U flips N = U x W
+    walls[1].u, walls[1].w = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
+    spec = _spec((lxy, lxy, lz), [ds, dl], 2500.0, kn, 0.8, 0.3, walls=walls, total_steps=steps)
+    g = Growth(phi=phi, seed=seed)
+    x = lattice_start(spec, n, floor_z, h, g, comp)
+    return spec, comp, x, g
+
+
+def packed_bed(lib, n: int = 4_000_000, seed: int = 20240003, phi: float = 0.60,
+               steps: int = 2000, shear_rate: float = 10.0, lxy: Optional[float] = None,
+               log=None, **growth) -> Workload:
+    """bidisperse_sheared_bed grown on the GPU (library `lib`) into the dense
+    disordered bed: the Workload of the sheared bed at the packed state."""
+    from .packing import grow
+
+    spec, comp, x0, g = bidisperse_sheared_bed(n, seed, phi, steps, shear_rate, lxy)
+    for k, v in growth.items():
+        setattr(g, k, v)
+    g.log = log
+    x, v, w = grow(lib, spec, comp, x0, g, e=0.8, mu=0.3)
+    red = reduce_spec(spec)
+    L, T = red.length_scale, red.time_scale
+    name = (f"bidisperse-sheared-{n // 1_000_000}M-phi{phi:.2f}" if n % 1_000_000 == 0
+            else f"bidisperse-sheared-{n}-phi{phi:.2f}")
+    return Workload(name, spec, red, comp, x / L, v * (T / L), w * T, steps,
+                    note=(f"disordered bed grown to phi={phi} from a seeded sparse lattice "
+                          f"(seed {seed}, dem3d_amd/packing.py), side {spec.lbox[0] * 1e3:.1f} mm, "
+                          f"bed {(spec.walls[1].position[2] - spec.walls[0].position[2]) * 1e3:.2f}"
+                          f" mm, top wall v_x={spec.walls[1].velocity[0]:.4f} m/s"))
+
+
 def small_box(n_side=(6, 6, 6), seed=1, jitter=0.05, spacing=1.02, nc=2, ratio=1.3,
               walls=True, pbc=(True, True, False), vel=0.5, steps=1000, gravity=-9.81,
               mu=0.5, e=0.9) -> Workload:

@@ -72,7 +72,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
 def test_fortran_bindings_compile_link_and_match_layout(tmp_path):
     exe = tmp_path / "check_bindings"
     subprocess.run(["amdflang", "-O1", "-J", str(tmp_path),
-                    os.path.join(FDIR, "dem3d_gpu.f90"), os.path.join(FDIR, "check_bindings.f90"),
+                    os.path.join(FDIR, "dem3d_gpu.f90"), os.path.join(os.path.dirname(os.path.abspath(__file__)), "check_bindings.f90"),
                     "-o", str(exe), LIB, f"-Wl,-rpath,{os.path.dirname(LIB)}"],
                    check=True, cwd=tmp_path)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True,
