@@ -1,29 +1,34 @@
 """Benchmark of the DEM per-timestep path on one MI355X.
 
-Workload: the 4 000 000-sphere bidisperse sheared bed (rock-salt packing of
-1.0/1.4 mm spheres, phi = 0.567, periodic x,y, floor, top wall moving at
-10 s^-1 * H; mu = 0.3, e = 0.8), FP64.  One step = MoveParticles
-(src/MDEM.f90:285-369: two force evaluations, AB predictor, AM corrector) +
-the escape scan, with the state resident in HBM.
+Workload: the 4 000 000-sphere dense sheared bed -- bidisperse spheres
+(d = 1.0 / 1.4 mm, half of each, species by a seeded shuffle) packed
+disordered at volume fraction 0.60 by radius growth on the GPU
+(dem3d_amd/packing.py, seed 20240003), periodic x,y, floor and a rough top
+wall moving at 10 s^-1 * H; mu = 0.3, e = 0.8; FP64.  The bed is packed and
+then sheared under gravity for 1 000 steps before the timing.  One step =
+MoveParticles (src/MDEM.f90:285-369: two force evaluations, AB predictor, AM
+corrector) + the escape scan, with the state resident in HBM.
 
 Prints one JSON line:
   value        particle-steps/s over exactly --steps timed steps
-  roofline     pass-1 force kernel (the dominant one): algorithmic bytes per
-               launch / mean launch time from HIP events on the library's
-               stream, against 8 TB/s
-  cpu_baseline the reference Fortran modules (oracle/_ref, capacity
-               constants raised) timed on one host core on a bounded sample
-               of the same bed
+  roofline     the dominant force pass: algorithmic bytes per launch / mean
+               launch time from HIP events on the library's stream, against
+               8 TB/s
+  cpu_baseline the reference Fortran modules (oracle/_ref, capacity constants
+               raised, built -O3 with AVX2/FMA) timed on one host core on a
+               bounded sample: --cpu-steps steps of the same packed bed
   mixed        the same run in the mixed-precision mode (FP32 force
                arithmetic, FP64 state and spring history): rate, speed-up over
                FP64, and its kinetic energy and packing fraction against the
                FP64 run's after the same steps
+  crystal      the ordered bed of the same spheres (rock-salt packing,
+               phi = 0.567) as a second, labelled leg
   ensemble     256 independent systems (500..20 000 spheres each) stepped as
                one batch (one launch per phase) in mixed and in FP64, against
                the plain FP64 path that steps each system alone
-  serving      a prefix of the Poisson request trace served with continuous
-               batching: p50/p99 step time, waiting and completion latency,
-               aggregate particle-steps/s
+  serving      the Poisson request trace served with continuous batching:
+               p50/p99 step time, waiting and completion latency, aggregate
+               particle-steps/s
 """
 from __future__ import annotations
 
@@ -46,25 +51,24 @@ PASS1_B_PER_PARTICLE, PASS1_B_PER_CONTACT = 436.0, 112.0
 PASS2_B_PER_PARTICLE, PASS2_B_PER_CONTACT = 436.0, 56.0
 
 
-def cpu_baseline(cells, steps):
+def cpu_baseline(wl, steps):
+    """The reference on `steps` MoveParticles steps of the benchmark's own bed
+    (zero ABM ring), in a spawned process (the GPU is initialised here)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from dem3d_amd import workloads
     from checkers import ref_available, run_reference
-    if not ref_available(big=True):
+    if not ref_available(fast=True):
         return None
-    wl = workloads.bidisperse_sheared_4m(cells=cells)
-    res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, big=True,
-                             actions=[("time", 1, steps)]), timeout=900)
+    res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w, fast=True,
+                             actions=[("time", 1, steps)]), timeout=900, method="spawn")
     sec = res["seconds"]
     return {
         "value": wl.np * steps / sec,
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "reference",
-        "sample": (f"reference Fortran MoveParticles (MaxNeighbours/MaxNCell raised), "
-                   f"{steps} steps of the same sheared bed cut to {cells[0]}x{cells[1]}x{cells[2]} "
-                   f"rock-salt cells ({wl.np} spheres), zero ABM ring, {sec:.2f} s, 1 core, "
-                   f"{cpu_model()}"),
+        "sample": (f"reference Fortran MoveParticles (MaxNeighbours/MaxNCell raised; amdflang "
+                   f"-O3 -march=x86-64-v3), {steps} steps of the benchmark's own packed bed "
+                   f"({wl.np} spheres) from a zero ABM ring, {sec:.2f} s, 1 core, {cpu_model()}"),
     }
 
 
@@ -109,7 +113,7 @@ def timed_run(wl, args, precision, hmode):
         # (these are ordinary steps: the trajectory does not change)
         tune = g.autotune(steps_per_shape=args.tune_steps)
         print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}/"
-              f"{tune['depth']}, block order {tune['block_map']} {tune['ms_map']}, "
+              f"{tune['front']}, block order {tune['block_map']} {tune['ms_map']}, "
               f"ms {[round(m, 4) for m in tune['ms']]}", file=sys.stderr, flush=True)
     if args.warmup:
         g.step(args.warmup)
@@ -251,9 +255,19 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cells", default="100,100,50", help="rock-salt cells (8 spheres each)")
-    ap.add_argument("--cpu-cells", default="100,100,12")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--n", type=int, default=4_000_000, help="spheres of the packed bed")
+    ap.add_argument("--settle-steps", type=int, default=1000,
+                    help="steps of the packed bed sheared under gravity before the timing")
+    ap.add_argument("--cells", default="100,100,50",
+                    help="rock-salt cells (8 spheres each) of the crystal leg")
+    ap.add_argument("--bed-cache", default="",
+                    help="npz file holding the packed bed: loaded when it exists, else written "
+                         "after packing (profiler runs pack once, outside the profiler)")
+    ap.add_argument("--pack-only", action="store_true",
+                    help="pack the bed (and write --bed-cache), then exit")
+    ap.add_argument("--crystal-steps", type=int, default=500,
+                    help="timed steps of the crystal leg (0 = skip it)")
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--precision", choices=("fp64", "both"), default="both",
@@ -298,22 +312,33 @@ def main(argv=None):
               f"(a multi-GPU run would decompose the box into slabs, one per GPU)",
               file=sys.stderr)
         return 2
-    cells = tuple(int(x) for x in args.cells.split(","))
+    from dem3d_amd import workloads
+    from dem3d_amd.native import DEM_FP64, DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, DEM_MIXED, load_hip
+    from dem3d_amd.observables import packing_fraction
+    lib = load_hip()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    state = None
+    if args.bed_cache and os.path.exists(args.bed_cache):
+        with np.load(args.bed_cache) as z:
+            state = (z["x"], z["v"], z["w"])
+    wl = workloads.packed_bed(lib, n=args.n, settle_steps=args.settle_steps, log=log,
+                              state=state)
+    if args.bed_cache and state is None:
+        np.savez(args.bed_cache, x=wl.x, v=wl.v, w=wl.w)
+    if args.pack_only:
+        return 0
+    print(f"workload: {wl.name} np={wl.np} {wl.note} ({time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        # before the GPU is initialised: the reference runs in a forked process
-        cpu = cpu_baseline(tuple(int(x) for x in args.cpu_cells.split(",")), args.cpu_steps)
+        cpu = cpu_baseline(wl, args.cpu_steps)
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
-
-    from dem3d_amd import workloads
-    from dem3d_amd.native import DEM_FP64, DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, DEM_MIXED
-    from dem3d_amd.observables import packing_fraction
-    wl = workloads.bidisperse_sheared_4m(cells=cells)
-    print(f"workload: {wl.name} np={wl.np} {wl.note}", file=sys.stderr, flush=True)
     hmode = DEM_HIST_REFERENCE if args.history == "reference" else DEM_HIST_CANONICAL
     r64 = timed_run(wl, args, DEM_FP64, hmode)
     print(f"fp64: {r64['ms']:.4f} ms/step", file=sys.stderr, flush=True)
     st, ms, value = r64["stats"], r64["ms"], r64["value"]
+    phi = packing_fraction(wl.red, wl.x, wl.comp)
     mixed = None
     if args.precision == "both":
         rmx = timed_run(wl, args, DEM_MIXED, hmode)
@@ -335,8 +360,25 @@ def main(argv=None):
             "packing_fraction_mixed": phimx,
             "packing_fraction_diff": phimx - phi64,
         }
+    crystal = None
+    if args.crystal_steps > 0:
+        cwl = workloads.bidisperse_sheared_4m(cells=tuple(int(x) for x in args.cells.split(",")))
+        cargs = argparse.Namespace(**vars(args))
+        cargs.steps = args.crystal_steps
+        cargs.precision = "fp64"
+        rc4 = timed_run(cwl, cargs, DEM_FP64, hmode)
+        cst = rc4["stats"]
+        crystal = {"workload": cwl.name, "packing": cwl.note, "spheres": cwl.np,
+                   "value": rc4["value"], "ms_per_step": rc4["ms"], "steps": args.crystal_steps,
+                   "ms_pass1": cst["ms_pass1"], "ms_pass2": cst["ms_pass2"],
+                   "ms_rebuild_per_step": cst["ms_rebuild"],
+                   "pair_contacts_mean": cst["mean_pair_contacts"],
+                   "neighbour_entries_per_sphere": cst["neighbour_pairs"] / cwl.np,
+                   "autotune": rc4["tune"]}
+        print(f"crystal: {rc4['ms']:.4f} ms/step", file=sys.stderr, flush=True)
     ensemble = ensemble_run(args) if args.ens_steps > 0 else None
     serving = serving_run(args) if args.serve_requests > 0 else None
+    nbr = st["neighbour_pairs"] / wl.np
     # contacts averaged over the timed steps (the bed relaxes during the run)
     P = st["mean_pair_contacts"] or st["pair_contacts"]
     W = st["wall_contacts"]
@@ -364,6 +406,9 @@ def main(argv=None):
             "workload": wl.name,
             "spheres": wl.np,
             "packing": wl.note,
+            "packing_fraction_bulk": phi,
+            "coordination_number": 2.0 * st["pair_contacts"] / wl.np,
+            "neighbour_entries_per_sphere": nbr,
             "precision": "fp64",
             "history_mode": args.history,
             "pair_contacts_mean": P,
@@ -390,6 +435,7 @@ def main(argv=None):
         },
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "crystal": crystal,
         "ensemble": ensemble,
         "serving": serving,
     }

@@ -118,12 +118,12 @@ __device__ __forceinline__ int cell_blocks(const RebuildArgs& r) {
 
 // map: the pass-block map of the launch shape (BS lanes per block).
 template <int MODE, bool UPDATE, bool MIXED, int PI, int BS = DEM_BLOCK,
-          int WAVES = DEM_PASS_WAVES, int DEPTH = 1>
+          int WAVES = DEM_PASS_WAVES, int FRONT = 1>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass_batch(
     const BatchRec* __restrict__ rec, const int2* __restrict__ map, int xcd_map) {
   const int2 m = map[xcd_map ? xcd_block() : (int)blockIdx.x];
   const BatchRec& r = rec[m.x];
-  pass_body<MODE, UPDATE, MIXED, BS, DEPTH>(r.s, r.p[PI], m.y);
+  pass_body<MODE, UPDATE, MIXED, BS, FRONT>(r.s, r.p[PI], m.y);
 }
 
 // One block per system.

@@ -1373,7 +1373,7 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
   std::sort(kv.begin(), kv.end());
   size_t used = 0;
   for (size_t i = 0; i < N; ++i) {
-    for (int m = 0; m < he.cnt[i]; ++m) {
+    for (int m = 0; m < cnt_deg(he.cnt[i]); ++m) {
       const size_t e = he.edge(i, m);
       const int a = idh[i], b = idh[he.nbr[e]];
       const uint64_t key = ((uint64_t)(std::min(a, b) + 1) << 32) | (uint64_t)(std::max(a, b) + 1);
@@ -1474,7 +1474,7 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
   std::vector<Ent> pe, we;
   for (size_t i = 0; i < N; ++i) {
     const int a = idh[i];
-    for (int m = 0; m < he.cnt[i]; ++m) {
+    for (int m = 0; m < cnt_deg(he.cnt[i]); ++m) {
       const size_t e = he.edge(i, m);
       if (!(he.flag[e] & 1)) continue;
       const int b = idh[he.nbr[e]];
@@ -1577,51 +1577,37 @@ static void enqueue_fluid(dem_handle_t h, const double4* pos, const Kin* kin) {
                      (const double*)h->d_vf, (const double*)h->d_ef, h->d_drag, h->ctl);
 }
 
-static const int kShapeBlock[] = DEM_SHAPE_BLOCKS;
-static const int kShapeWaves[] = DEM_SHAPE_WAVES;
-static const int kShapeDepth[] = DEM_SHAPE_DEPTHS;
-static_assert(sizeof(kShapeBlock) == DEM_NSHAPES * sizeof(int) &&
-                  sizeof(kShapeWaves) == DEM_NSHAPES * sizeof(int) &&
-                  sizeof(kShapeDepth) == DEM_NSHAPES * sizeof(int),
-              "one block size, occupancy and pipeline depth per launch shape");
+template <int S = 0>
+static void shape_table(int* block, int* waves, int* front) {
+  if constexpr (S < DEM_NSHAPES) {
+    block[S] = PassShape<S>::block;
+    waves[S] = PassShape<S>::waves;
+    front[S] = PassShape<S>::front;
+    shape_table<S + 1>(block, waves, front);
+  }
+}
+static int shape_block(int shape) {
+  int b[DEM_NSHAPES], w[DEM_NSHAPES], d[DEM_NSHAPES];
+  shape_table(b, w, d);
+  return b[shape];
+}
 
 // The pass kernel in launch shape `shape` (nb blocks).  Only the steps'
 // passes (predictor, corrector) come in every shape; the others run in the
 // default one.
-template <int MODE, bool UPDATE, bool MIXED>
+template <int MODE, bool UPDATE, bool MIXED, int S = 0>
 static void launch_shape(int shape, int nb, hipStream_t st, const SysConst& s, const PassArgs& a) {
-  if constexpr (MODE == MODE_PRED || MODE == MODE_CORR) {
-    switch (shape) {
-      case 1:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 128>), dim3(nb), dim3(128), 0, st, s, a);
-        return;
-      case 2:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 64>), dim3(nb), dim3(64), 0, st, s, a);
-        return;
-      case 3:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 512>), dim3(nb), dim3(512), 0, st, s, a);
-        return;
-      case 4:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 4>), dim3(nb), dim3(256), 0, st, s,
-                           a);
-        return;
-      case 5:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 2>), dim3(nb), dim3(256), 0, st, s,
-                           a);
-        return;
-      case 6:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 128, 2, 2>), dim3(nb), dim3(128), 0, st,
-                           s, a);
-        return;
-      case 7:
-        hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, 256, 2, 2>), dim3(nb), dim3(256), 0, st,
-                           s, a);
-        return;
-      default:
-        break;
+  if constexpr ((MODE == MODE_PRED || MODE == MODE_CORR) && S < DEM_NSHAPES) {
+    using Sh = PassShape<S>;
+    if (shape == S) {
+      hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED, Sh::block, Sh::waves, Sh::front>), dim3(nb),
+                         dim3(Sh::block), 0, st, s, a);
+      return;
     }
+    launch_shape<MODE, UPDATE, MIXED, S + 1>(shape, nb, st, s, a);
+  } else {
+    hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED>), dim3(nb), dim3(DEM_BLOCK), 0, st, s, a);
   }
-  hipLaunchKernelGGL((k_pass<MODE, UPDATE, MIXED>), dim3(nb), dim3(DEM_BLOCK), 0, st, s, a);
 }
 
 // One force evaluation: the pass kernel, then the sum of its counters.
@@ -1632,7 +1618,7 @@ static void launch_pass(dem_handle_t h, const PassArgs& a0) {
     enqueue_fluid(h, a.pos_in, a.kin_in);
     a.drag = h->drag_cap ? h->d_drag : nullptr;
   }
-  const int nb = (MODE == MODE_PRED || MODE == MODE_CORR) ? nblocks(h->np, kShapeBlock[h->shape])
+  const int nb = (MODE == MODE_PRED || MODE == MODE_CORR) ? nblocks(h->np, shape_block(h->shape))
                                                           : nblocks(h->np);
   if (h->mixed)
     launch_shape<MODE, UPDATE, true>(h->shape, nb, h->stream, h->sc, a);
@@ -1982,9 +1968,7 @@ int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, de
       h->shape = keep;
       return rc;
     }
-    out->block[c] = kShapeBlock[c];
-    out->waves[c] = kShapeWaves[c];
-    out->depth[c] = kShapeDepth[c];
+    shape_table(out->block, out->waves, out->front);
     out->ms[c] = h->ms_pass1 + h->ms_pass2;
     if (out->ms[c] < out->ms[best]) best = c;
   }
@@ -2689,7 +2673,7 @@ static void batch_rec_init(dem_handle_t h, BatchRec& r) {
   r.p[1] = b;
   r.r = rebuild_args(h, h->cur, h->rb);
   r.nblk = nblocks(h->np);
-  r.pass_nblk = nblocks(h->np, kShapeBlock[h->batch ? h->batch->shape : 0]);
+  r.pass_nblk = nblocks(h->np, shape_block(h->batch ? h->batch->shape : 0));
   r.nprev = h->nprev;
   r.first = h->batch_first ? 1 : 0;
   r.step = 0;
@@ -2729,7 +2713,7 @@ static int batch_maps(dem_batch_t b) {
   for (size_t i = 0; i < b->sys.size(); ++i) {
     const dem_handle_t h = b->sys[i];
     const int cnt[MAP_KINDS] = {nblocks(h->np), nblocks((long long)h->gncell + 1), h->ntiles,
-                                h->ntiles_np, nblocks(h->np, kShapeBlock[b->shape])};
+                                h->ntiles_np, nblocks(h->np, shape_block(b->shape))};
     for (int k = 0; k < MAP_KINDS; ++k)
       for (int j = 0; j < cnt[k]; ++j) m[k].push_back(make_int2((int)i, j));
   }
@@ -2773,41 +2757,17 @@ static int batch_capacity(dem_batch_t b, size_t n) {
 }
 
 // The batched pass kernel in launch shape `shape` over the pass-block map.
-template <int MODE, bool UPDATE, bool MIXED, int PI>
+template <int MODE, bool UPDATE, bool MIXED, int PI, int S = 0>
 static void launch_batch_shape(int shape, int nb, hipStream_t st, const BatchRec* rec,
                                const int2* map, int xcd) {
-  switch (shape) {
-    case 1:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128>), dim3(nb), dim3(128), 0, st,
-                         rec, map, xcd);
+  if constexpr (S < DEM_NSHAPES) {
+    using Sh = PassShape<S>;
+    if (shape == S) {
+      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, Sh::block, Sh::waves, Sh::front>),
+                         dim3(nb), dim3(Sh::block), 0, st, rec, map, xcd);
       return;
-    case 2:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 64>), dim3(nb), dim3(64), 0, st,
-                         rec, map, xcd);
-      return;
-    case 3:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 512>), dim3(nb), dim3(512), 0, st,
-                         rec, map, xcd);
-      return;
-    case 4:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 4>), dim3(nb), dim3(256), 0,
-                         st, rec, map, xcd);
-      return;
-    case 5:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2>), dim3(nb), dim3(256), 0,
-                         st, rec, map, xcd);
-      return;
-    case 6:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 128, 2, 2>), dim3(nb), dim3(128),
-                         0, st, rec, map, xcd);
-      return;
-    case 7:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI, 256, 2, 2>), dim3(nb), dim3(256),
-                         0, st, rec, map, xcd);
-      return;
-    default:
-      hipLaunchKernelGGL((k_pass_batch<MODE, UPDATE, MIXED, PI>), dim3(nb), dim3(DEM_BLOCK), 0, st,
-                         rec, map, xcd);
+    }
+    launch_batch_shape<MODE, UPDATE, MIXED, PI, S + 1>(shape, nb, st, rec, map, xcd);
   }
 }
 
@@ -3142,9 +3102,7 @@ int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) 
     }
     double sum = 0.0;
     for (double t : b->step_ms) sum += t;
-    out->block[c] = kShapeBlock[c];
-    out->waves[c] = kShapeWaves[c];
-    out->depth[c] = kShapeDepth[c];
+    shape_table(out->block, out->waves, out->front);
     out->ms[c] = sum / (double)steps_per_shape;
     if (out->ms[c] < out->ms[best]) best = c;
   }

@@ -13,7 +13,8 @@
 //         slice per wavefront): the k-th neighbour of particle i is edge
 //         e = soff[i/64] + 64*k + i%64, so a wavefront reading the k-th
 //         edge of its 64 particles touches one contiguous run.
-//         cnt  int[N]          neighbours of each particle
+//         cnt  int[N]          neighbours of each particle (low 16 bits) and
+//                              how many of them come first as contacts
 //         soff int[N/64+1]     first edge of each slice (64 * slice width)
 //         nbr  int[E]          neighbour j of edge e
 //         flag u8[E]           bit 0: the pair has stored history
@@ -100,6 +101,15 @@ __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
+// Inclusive scan of one value per lane across the 64 lanes of a wavefront.
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
 
 struct EdgeSets {
   int* cnt[2];
@@ -110,6 +120,12 @@ struct EdgeSets {
   long long ecap;
   int* epar;  // device word: index of the current set
 };
+
+// cnt[i] holds the list length of particle i in its low 16 bits and, once
+// the list is filled, the number of front (contact-first) entries above them
+// (nbr_fill_body).
+__host__ __device__ __forceinline__ int cnt_deg(int c) { return c & 0xffff; }
+__host__ __device__ __forceinline__ int cnt_front(int c) { return c >> 16; }
 
 __device__ __forceinline__ long long sell_edge(const int* soff, int i, int k) {
   return (long long)soff[i >> 6] + 64LL * k + (i & 63);
@@ -213,11 +229,20 @@ __device__ __forceinline__ void store_d3x2(double2* o, d3 a, d3 b) {
 #define DEM_LDS_NC 16
 
 // Pair contacts of one particle, in list order (src/MForces.f90:203-271).
-// The index and flag of edge k+1 are requested before the partner records
-// and history of edge k, so an edge waits for one memory round trip.  (A
-// deeper pipeline, with the partner records of edge k+1 in flight during
-// edge k, needs ~40 more VGPRs, drops to 2 waves per SIMD and measured
-// slower.)
+//
+// One lane walks its particle's list.  The list rebuild puts the pairs that
+// touch (or carry history) first, nfront of them, and the rest after them
+// (nbr_fill_body), so the partner records an edge needs are known from its
+// place in the list:
+//  - a front edge is a contact almost always: the partner's position, its
+//    64-byte Kin line (velocities, cell state) and the stored history are
+//    requested together, and the edge waits for one memory round trip;
+//  - a back edge is a contact rarely (a pair that came into contact since
+//    the rebuild): only the partner's position (and history, if any) is
+//    requested, and the Kin line follows only when the pair overlaps.
+// With FRONT = 0 every edge is treated as a front edge (the autotuner tries
+// both).  The index and flag of edge k+1 are requested before the records
+// of edge k.  The arithmetic and its order do not depend on FRONT.
 struct EdgeAcc {
   d3 F, T;          // FP64 sums (and gravity)
   KahanF3 Ff, Tf;   // compensated FP32 sums of the mixed mode
@@ -225,98 +250,59 @@ struct EdgeAcc {
   int err;
 };
 
-struct EdgeData {
-  double4 Q;
-  Kin K;
-  double2 h0, h1, h2;
-};
-
-// The partner's records and, for a stored pair, its history as the three
-// (FTx,FTy) (FTz,VTx) (VTy,VTz) pairs; the mixed mode reads its 36-byte
-// layout (FTz double, VT float3: hist_ftz, hist_vt32).
+// The stored history of edge e as the three (FTx,FTy) (FTz,VTx) (VTy,VTz)
+// pairs; the mixed mode reads its 36-byte layout (FTz double, VT float3:
+// hist_ftz, hist_vt32).
 template <bool MIXED>
-__device__ __forceinline__ void load_edge(EdgeData& d, const PassArgs& a,
-                                          const double2* __restrict__ hist, long long ecap,
-                                          long long e, int j, unsigned char fl) {
-  d.Q = a.pos_in[j];
-  d.K = load_kin(a.kin_in + j);
-  if (fl & 1) {
-    d.h0 = ld_nt(hist + e);
-    if (MIXED) {
-      const float* vt = hist_vt32(hist, ecap) + 3 * e;
-      d.h1 = make_double2(ld_nt(hist_ftz(hist, ecap) + e), (double)ld_nt(vt));
-      d.h2 = make_double2((double)ld_nt(vt + 1), (double)ld_nt(vt + 2));
-    } else {
-      d.h1 = ld_nt(hist + ecap + e);
-      d.h2 = ld_nt(hist + 2 * ecap + e);
-    }
+__device__ __forceinline__ void load_hist(const double2* __restrict__ hist, long long ecap,
+                                          long long e, double2& h0, double2& h1, double2& h2) {
+  h0 = ld_nt(hist + e);
+  if (MIXED) {
+    const float* vt = hist_vt32(hist, ecap) + 3 * e;
+    h1 = make_double2(ld_nt(hist_ftz(hist, ecap) + e), (double)ld_nt(vt));
+    h2 = make_double2((double)ld_nt(vt + 1), (double)ld_nt(vt + 2));
   } else {
-    d.h0 = d.h1 = d.h2 = make_double2(0.0, 0.0);
+    h1 = ld_nt(hist + ecap + e);
+    h2 = ld_nt(hist + 2 * ecap + e);
   }
 }
 
-// DEPTH = 1: an edge waits for one memory round trip (its index and flag were
-// requested during the previous edge).  DEPTH = 2: the partner records and
-// history of edge k+1 are requested before edge k is computed, and the
-// index of edge k+2 with them; ~40 more VGPRs, so it pairs with a 2-wave
-// occupancy target.  The arithmetic and its order are the same.
-template <bool UPDATE, bool MIXED, int DEPTH = 1>
+template <bool UPDATE, bool MIXED, int FRONT>
 __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
                                           const PairConst* ptab, const CompConst* ctab,
                                           const int* __restrict__ nbr,
                                           unsigned char* __restrict__ flag,
                                           double2* __restrict__ hist, long long ecap,
-                                          long long e, int deg, const double4& P, d3 vi, d3 wi,
-                                          int ci, double ri, int cell_p, int xyz_p, int st_p,
-                                          int id_p, EdgeAcc& acc) {
+                                          long long e, int deg, int nfront, const double4& P,
+                                          d3 vi, d3 wi, int ci, double ri, int cell_p, int xyz_p,
+                                          int st_p, int id_p, EdgeAcc& acc) {
+  if (!FRONT) nfront = deg;
   int j_n = 0;
   unsigned char fl_n = 0;
   if (deg > 0) {
     j_n = nbr[e];
     fl_n = flag[e];
   }
-  EdgeData nxt;
-  unsigned char fl_nx = 0;
-  if (DEPTH == 2 && deg > 0) {
-    load_edge<MIXED>(nxt, a, hist, ecap, e, j_n, fl_n);
-    fl_nx = fl_n;
-    if (deg > 1) {
+  for (int k = 0; k < deg; ++k, e += 64) {
+    const int j = j_n;
+    const unsigned char fl = fl_n;
+    if (k + 1 < deg) {
       j_n = nbr[e + 64];
       fl_n = flag[e + 64];
     }
-  }
-  for (int k = 0; k < deg; ++k, e += 64) {
-    EdgeData cur;
-    unsigned char fl;
-    if constexpr (DEPTH == 2) {
-      cur = nxt;
-      fl = fl_nx;
-      if (k + 1 < deg) {
-        load_edge<MIXED>(nxt, a, hist, ecap, e + 64, j_n, fl_n);
-        fl_nx = fl_n;
-        if (k + 2 < deg) {
-          j_n = nbr[e + 128];
-          fl_n = flag[e + 128];
-        }
-      }
-    } else {
-      const int j = j_n;
-      fl = fl_n;
-      if (k + 1 < deg) {
-        j_n = nbr[e + 64];
-        fl_n = flag[e + 64];
-      }
-      load_edge<MIXED>(cur, a, hist, ecap, e, j, fl);
-    }
-    const double4 Q = cur.Q;
-    const Kin& KJ = cur.K;
-    const double2 h0 = cur.h0, h1 = cur.h1, h2 = cur.h2;
+    const bool front = k < nfront;
+    const double4 Q = a.pos_in[j];
+    Kin KJ;
+    if (front) KJ = load_kin(a.kin_in + j);
+    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
+    if (fl & 1) load_hist<MIXED>(hist, ecap, e, h0, h1, h2);
     const int cj = (int)Q.w;
     const d3 rij = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
     const double dsq = dot3(rij, rij);
     const PairConst& pc = ptab[ci + cj * s.nc];
     const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
     if (delta > 0.0) {
+      if (!front) KJ = load_kin(a.kin_in + j);
       int role;
       if (s.hist_mode == 0) {
         role = ref_role(s, cell_p, xyz_p, KJ.rc, KJ.rxyz, st_p, KJ.st, id_p, KJ.id);
@@ -377,7 +363,16 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     } else if (UPDATE && (fl & 1)) {
       // The reference removes a stored pair only when its cell loop visits
       // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
-      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, xyz_p, KJ.rc, KJ.rxyz);
+      int rc, rxyz;
+      if (front) {
+        rc = KJ.rc;
+        rxyz = KJ.rxyz;
+      } else {
+        const int4 c = reinterpret_cast<const int4*>(a.kin_in + j)[3];
+        rc = c.y;
+        rxyz = c.w;
+      }
+      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, xyz_p, rc, rxyz);
       if (keep) ++acc.n_stored;
       else flag[e] = 0;
     }
@@ -385,10 +380,10 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
 }
 
 // One force pass over the particles [blk*BS, blk*BS+BS) of one system,
-// BS lanes per block.  The block size changes neither the arithmetic of a
-// lane nor the integer counter sums, so every block shape gives the same
-// results bit for bit.
-template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int DEPTH = 1>
+// BS lanes per block.  Neither the block size nor FRONT changes the
+// arithmetic of a contact, the order of a particle's sums or the integer
+// counter sums, so every launch shape gives the same results bit for bit.
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int FRONT = 1>
 __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
@@ -413,7 +408,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
 
   double4 P = make_double4(0, 0, 0, 0);
   d3 vi = mk3(0, 0, 0), wi = vi;
-  int cell_p = 0, xyz_p = 0, st_p = 0, id_p = 0, deg = 0;
+  int cell_p = 0, xyz_p = 0, st_p = 0, id_p = 0, deg = 0, nfront = 0;
   long long e = 0;
   if (active) {
     P = a.pos_in[i];
@@ -424,7 +419,9 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
     xyz_p = K.rxyz;
     st_p = K.st;
     id_p = K.id;
-    deg = a.E.cnt[ep][i];
+    const int cn = a.E.cnt[ep][i];
+    deg = cnt_deg(cn);
+    nfront = cnt_front(cn);
     e = sell_edge(a.E.soff[ep], i, 0);
   }
   const int ci = (int)P.w;
@@ -437,14 +434,10 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   acc.Ff.s = acc.Ff.c = acc.Tf.s = acc.Tf.c = mkf(0.0f, 0.0f, 0.0f);
   acc.n_col = acc.n_pair = acc.n_stored = 0;
   acc.err = 0;
-  if (lds_tab) {
-    __syncthreads();
-    pair_loop<UPDATE, MIXED, DEPTH>(s, a, sh_pair, sh_comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
-                      e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
-  } else {
-    pair_loop<UPDATE, MIXED, DEPTH>(s, a, s.pair, s.comp, a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap,
-                      e, deg, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
-  }
+  if (lds_tab) __syncthreads();
+  pair_loop<UPDATE, MIXED, FRONT>(s, a, lds_tab ? sh_pair : s.pair, lds_tab ? sh_comp : s.comp,
+                                  a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap, e, deg,
+                                  nfront, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
   d3 F = acc.F, T = acc.T;
   if (MIXED) {
     F = add3(F, kahan_value(acc.Ff));
@@ -637,22 +630,38 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
 
 // Launch shapes of the force passes: lanes per block, the occupancy the
 // register allocation targets (wavefronts per SIMD, 512/WAVES VGPRs each;
-// above 3 the pass spills to scratch) and the edge pipeline depth
-// (pair_loop).  Shape 0 is the default; the others are candidates of the
+// above 3 the pass spills to scratch) and FRONT (pair_loop: 1 = the partner's
+// Kin line is gathered with its position only for the front edges, 0 = for
+// every edge).  Shape 0 is the default; the others are candidates of the
 // autotuner (dem_autotune).  Every shape gives the same results bit for bit.
 #ifndef DEM_NSHAPES
 #define DEM_NSHAPES 8
 #endif
-#define DEM_SHAPE_BLOCKS {256, 128, 64, 512, 256, 256, 128, 256}
-#define DEM_SHAPE_WAVES {DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, DEM_PASS_WAVES, 4, 2, 2, 2}
-#define DEM_SHAPE_DEPTHS {1, 1, 1, 1, 1, 1, 2, 2}
+template <int S>
+struct PassShape;
+#define DEM_PASS_SHAPE(S, B, W, F)           \
+  template <>                                \
+  struct PassShape<S> {                      \
+    static constexpr int block = B;          \
+    static constexpr int waves = W;          \
+    static constexpr int front = F;          \
+  };
+DEM_PASS_SHAPE(0, 256, DEM_PASS_WAVES, 1)
+DEM_PASS_SHAPE(1, 128, DEM_PASS_WAVES, 1)
+DEM_PASS_SHAPE(2, 64, DEM_PASS_WAVES, 1)
+DEM_PASS_SHAPE(3, 512, DEM_PASS_WAVES, 1)
+DEM_PASS_SHAPE(4, 256, 4, 1)
+DEM_PASS_SHAPE(5, 256, 2, 1)
+DEM_PASS_SHAPE(6, 128, DEM_PASS_WAVES, 0)
+DEM_PASS_SHAPE(7, 256, DEM_PASS_WAVES, 0)
+#undef DEM_PASS_SHAPE
 #define DEM_MIN_PASS_BLOCK 64
 
 template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int WAVES = DEM_PASS_WAVES,
-          int DEPTH = 1>
+          int FRONT = 1>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_pass(
     SysConst s, PassArgs a) {
-  pass_body<MODE, UPDATE, MIXED, BS, DEPTH>(s, a, a.xcd_map ? xcd_block() : (int)blockIdx.x);
+  pass_body<MODE, UPDATE, MIXED, BS, FRONT>(s, a, a.xcd_map ? xcd_block() : (int)blockIdx.x);
 }
 
 // Sums the per-block counters of the pass just run into the control block
@@ -807,15 +816,6 @@ __global__ void k_bin_count(SysConst s, const double4* pos, int* key, int* count
 #define SCAN_ITEMS 8
 #define SCAN_TILE (DEM_BLOCK * SCAN_ITEMS)
 
-// Inclusive scan of one value per lane across the 64 lanes of a wavefront.
-__device__ __forceinline__ int wave_scan_incl(int v) {
-  const int lane = threadIdx.x & 63;
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
 
 // Exclusive scan of one value per thread across the block; returns the block
 // total in *total.
@@ -936,7 +936,8 @@ __global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const in
 }
 
 // Visits every j != i with |r_ij|^2 < (r_i + r_j + skin)^2 (minimum image),
-// scanning the 27 cells around i's cell in a fixed order.
+// scanning the 27 cells around i's cell in a fixed order: fn(j, |r_ij|^2,
+// component of j).
 template <typename F>
 __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const double4* pos,
                                                    int i, const double4& P,
@@ -973,7 +974,9 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
           if (j == i) continue;
           const double4 Q = pos[j];
           const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
-          if (dot3(r, r) < (s.cut_all ? s.rc2_all : s.pair[ci + (int)Q.w * s.nc].rc2)) fn(j);
+          const double dsq = dot3(r, r);
+          const int cj = (int)Q.w;
+          if (dsq < (s.cut_all ? s.rc2_all : s.pair[ci + cj * s.nc].rc2)) fn(j, dsq, cj);
         }
       }
     }
@@ -993,7 +996,7 @@ __device__ __forceinline__ void nbr_count_body(const SysConst& s, const double4*
   int n = 0, err = 0;
   if (i < s.np) {
     const double4 P = pos[i];
-    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int) { ++n; });
+    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int, double, int) { ++n; });
     if (n > s.max_nb) {
       err = DEM_KERR_NBR_OVERFLOW;
       n = s.max_nb;
@@ -1018,6 +1021,8 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
   nbr_count_body(s, pos, start, cell_idx, E, slice_w, gate, force, ctl, blockIdx.x);
 }
 
+#define DEM_FILL_STORED 24
+
 // Pass 2 of the list build: fill the new edge set and carry the history of
 // every pair that is still listed.  src maps new particle indices to the ones
 // the current edge set uses (null = identity; a resort passes its
@@ -1036,47 +1041,81 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
   if (i < s.np) {
     const double4 P = pos[i];
     const int o = src ? src[i] : i;
-    const int odeg = E.cnt[p][o];
+    const int odeg = cnt_deg(E.cnt[p][o]);
     const long long ob = sell_edge(E.soff[p], o, 0);
     const int* onbr = E.nbr[p];
     const unsigned char* oflag = E.flag[p];
     const double2* ohist = E.hist[p];
     const int ndeg = E.cnt[q][i];
-    long long e = sell_edge(E.soff[q], i, 0);
-    int k = 0;
-    unsigned long long found = 0;
-    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int j) {
-      if (k >= ndeg) return;
-      E.nbr[q][e] = j;
+    const long long e0 = sell_edge(E.soff[q], i, 0);
+    const int ci = (int)P.w;
+    // Contacts first: pairs that carry history or overlap now take the first
+    // slots (in scan order), the others the last ones (in reverse scan
+    // order).  The force passes then find the stored history of a wavefront
+    // in the first few 64-slot runs of each history plane instead of spread
+    // over all of them.
+    int kf = 0, kb = ndeg - 1;
+    // The pairs of the old list that store history, held in registers: each
+    // new neighbour is looked up there instead of in the old list in memory.
+    // More than DEM_FILL_STORED of them (a dense cluster around a large
+    // sphere) falls back to the search in memory.
+    int sj[DEM_FILL_STORED], sm[DEM_FILL_STORED];
+    int ns = 0;
+    bool many = false;
+#pragma unroll
+    for (int t = 0; t < DEM_FILL_STORED; ++t) sj[t] = sm[t] = -1;
+    for (int m = 0; m < odeg; ++m) {
+      if (!(oflag[ob + 64LL * m] & 1)) continue;
+      const int jo = onbr[ob + 64LL * m];
+#pragma unroll
+      for (int t = 0; t < DEM_FILL_STORED; ++t)
+        if (t == ns) {
+          sj[t] = jo;
+          sm[t] = m;
+        }
+      if (ns == DEM_FILL_STORED) many = true;
+      else ++ns;
+    }
+    unsigned long long found = 0;  // bit m: stored entry m of the old list was carried
+    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int j, double dsq, int cj) {
+      if (kf > kb) return;
       const int jo = src ? src[j] : j;
-      unsigned char fl = 0;
-      for (int m = 0; m < odeg; ++m) {
-        const long long om = ob + 64LL * m;
-        if (onbr[om] == jo) {
-          fl = oflag[om];
-          if (fl & 1) {
-            double2* g = E.hist[q];
-            g[e] = ohist[om];
-            if (s.vt32) {
-              hist_ftz(g, ecap)[e] = hist_ftz(ohist, ecap)[om];
-              const float* ov = hist_vt32(ohist, ecap) + 3 * om;
-              float* nv = hist_vt32(g, ecap) + 3 * e;
-              nv[0] = ov[0];
-              nv[1] = ov[1];
-              nv[2] = ov[2];
-            } else {
-              g[ecap + e] = ohist[ecap + om];
-              g[2 * ecap + e] = ohist[2 * ecap + om];
-            }
+      int mo = -1;
+      if (!many) {
+#pragma unroll
+        for (int t = 0; t < DEM_FILL_STORED; ++t)
+          if (sj[t] == jo) mo = sm[t];
+      } else {
+        for (int m = 0; m < odeg; ++m)
+          if ((oflag[ob + 64LL * m] & 1) && onbr[ob + 64LL * m] == jo) {
+            mo = m;
+            break;
           }
-          found |= 1ull << m;
-          break;
+      }
+      const unsigned char fl = mo >= 0 ? 1 : 0;
+      if (mo >= 0) found |= 1ull << mo;
+      const bool front = fl || s.pair[ci + cj * s.nc].mindijsq - dsq > 0.0;
+      const long long e = e0 + 64LL * (front ? kf++ : kb--);
+      E.nbr[q][e] = j;
+      E.flag[q][e] = fl;
+      if (fl) {
+        const long long om = ob + 64LL * mo;
+        double2* g = E.hist[q];
+        g[e] = ohist[om];
+        if (s.vt32) {
+          hist_ftz(g, ecap)[e] = hist_ftz(ohist, ecap)[om];
+          const float* ov = hist_vt32(ohist, ecap) + 3 * om;
+          float* nv = hist_vt32(g, ecap) + 3 * e;
+          nv[0] = ov[0];
+          nv[1] = ov[1];
+          nv[2] = ov[2];
+        } else {
+          g[ecap + e] = ohist[ecap + om];
+          g[2 * ecap + e] = ohist[2 * ecap + om];
         }
       }
-      E.flag[q][e] = fl;
-      ++k;
-      e += 64;
     });
+    E.cnt[q][i] = ndeg | (kf << 16);
     nedges = (unsigned long long)ndeg;
     for (int m = 0; m < odeg; ++m) {
       const long long om = ob + 64LL * m;

@@ -76,13 +76,13 @@ class TuneT(C.Structure):
     """dem_tune_t: the autotuner's report."""
     _fields_ = [("nshapes", C.c_int32), ("chosen", C.c_int32), ("steps", C.c_int64),
                 ("block", C.c_int32 * NSHAPES), ("waves", C.c_int32 * NSHAPES),
-                ("depth", C.c_int32 * NSHAPES), ("ms", C.c_double * NSHAPES),
+                ("front", C.c_int32 * NSHAPES), ("ms", C.c_double * NSHAPES),
                 ("block_map", C.c_int32), ("pad_", C.c_int32), ("ms_map", C.c_double * 2)]
 
     def asdict(self):
         n = self.nshapes
         return dict(chosen=self.chosen, steps=self.steps, block=list(self.block[:n]),
-                    waves=list(self.waves[:n]), depth=list(self.depth[:n]),
+                    waves=list(self.waves[:n]), front=list(self.front[:n]),
                     ms=list(self.ms[:n]), block_map=self.block_map,
                     ms_map=list(self.ms_map))
 

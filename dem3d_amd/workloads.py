@@ -236,25 +236,43 @@ def bidisperse_sheared_bed(n: int = 4_000_000, seed: int = 20240003, phi: float 
 
 def packed_bed(lib, n: int = 4_000_000, seed: int = 20240003, phi: float = 0.60,
                steps: int = 2000, shear_rate: float = 10.0, lxy: Optional[float] = None,
-               log=None, **growth) -> Workload:
+               settle_steps: int = 1000, log=None, state=None, **growth) -> Workload:
     """bidisperse_sheared_bed grown on the GPU (library `lib`) into the dense
-    disordered bed: the Workload of the sheared bed at the packed state."""
+    disordered bed, then run settle_steps steps with its own contact law,
+    gravity and the top wall shearing: the Workload of the sheared bed at
+    that state.  state = (x, v, w) in reduced units, from an earlier call
+    with the same arguments, skips the growth and the settling."""
+    from .native import System
     from .packing import grow
 
     spec, comp, x0, g = bidisperse_sheared_bed(n, seed, phi, steps, shear_rate, lxy)
-    for k, v in growth.items():
-        setattr(g, k, v)
-    g.log = log
-    x, v, w = grow(lib, spec, comp, x0, g, e=0.8, mu=0.3)
     red = reduce_spec(spec)
     L, T = red.length_scale, red.time_scale
+    if state is None:
+        for k, v in growth.items():
+            setattr(g, k, v)
+        g.log = log
+        x, v, w = grow(lib, spec, comp, x0, g, e=0.8, mu=0.3)
+    else:
+        x, v, w = state[0] * L, state[1] * (L / T), state[2] / T
+    if settle_steps > 0 and state is None:
+        sysm = System(lib, red)
+        sysm.upload(comp, x / L, v * (T / L), w * T)
+        sysm.startup()
+        sysm.step(settle_steps)
+        st = sysm.download()
+        sysm.close()
+        x, v, w = st["x"] * L, st["v"] * (L / T), st["w"] / T
+        if log:
+            log(f"packing settle: {settle_steps} steps with gravity and the shearing top wall")
     name = (f"bidisperse-sheared-{n // 1_000_000}M-phi{phi:.2f}" if n % 1_000_000 == 0
             else f"bidisperse-sheared-{n}-phi{phi:.2f}")
     return Workload(name, spec, red, comp, x / L, v * (T / L), w * T, steps,
                     note=(f"disordered bed grown to phi={phi} from a seeded sparse lattice "
                           f"(seed {seed}, dem3d_amd/packing.py), side {spec.lbox[0] * 1e3:.1f} mm, "
                           f"bed {(spec.walls[1].position[2] - spec.walls[0].position[2]) * 1e3:.2f}"
-                          f" mm, top wall v_x={spec.walls[1].velocity[0]:.4f} m/s"))
+                          f" mm, top wall v_x={spec.walls[1].velocity[0]:.4f} m/s, then "
+                          f"{settle_steps} steps sheared under gravity"))
 
 
 def small_box(n_side=(6, 6, 6), seed=1, jitter=0.05, spacing=1.02, nc=2, ratio=1.3,

@@ -462,11 +462,12 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
 /* ---- launch shapes and the autotuner ------------------------------------ */
 /* No counterpart in the reference.  The force passes come in DEM_NSHAPES
  * launch shapes: lanes per block, the occupancy (wavefronts per SIMD) their
- * register allocation targets, and the depth of the edge pipeline (1: an
- * edge waits for one memory round trip; 2: the next edge's partner records
- * are in flight while an edge is computed).  Every shape gives bitwise the
- * same results, so a shape changes speed only; shape 0 (256 lanes, 3 waves,
- * depth 1) is the default.  The autotuner times every shape on the live system while
+ * register allocation targets, and whether the partner's 64-byte velocity
+ * record is gathered with its position for every list entry or only for the
+ * entries the list rebuild put first as contacts (the others fetch it when
+ * they overlap).  Every shape gives bitwise the same results, so a shape
+ * changes speed only; shape 0 (256 lanes, 3 waves, contacts only) is the
+ * default.  The autotuner times every shape on the live system while
  * stepping it: each shape runs 1 untimed step (its first launch) and then
  * steps_per_shape steps timed with HIP events, and the fastest becomes the
  * system's (batch's) shape.  The steps count as ordinary MoveParticles steps
@@ -485,7 +486,9 @@ typedef struct {
   int64_t steps;               /* MoveParticles steps the tuning advanced      */
   int32_t block[DEM_NSHAPES];  /* lanes per block of each shape                */
   int32_t waves[DEM_NSHAPES];  /* wavefronts per SIMD it targets               */
-  int32_t depth[DEM_NSHAPES];  /* edge pipeline depth                          */
+  int32_t front[DEM_NSHAPES];  /* 1: partner velocities gathered with the
+                                  position only for the contact-first entries
+                                  of a list; 0: for every entry               */
   double ms[DEM_NSHAPES];      /* mean device ms per step: the two passes
                                   (dem_autotune), the whole step
                                   (dem_batch_autotune)                         */

@@ -13,6 +13,11 @@
 #                                 unchecked Neighbour(50) buffer.  The sources
 #                                 are streamed through sed into the compiler;
 #                                 no copy of them is written anywhere.
+#  oracle/_ref/libdem3d_ref_fast.so  capacity-raised as _big, built for speed
+#                                 (-O3, contraction on, AVX2/FMA): the CPU
+#                                 baseline of the benchmark.  Not bit-exact.
+#  oracle/_ref/libdem3d_ref_native.so  the same with -march=native, only when
+#                                 DEM3D_REF_NATIVE=1 (timings on this host).
 #
 # The reference part is skipped when $REF/src is absent (e.g. on a machine that
 # only runs the prebuilt libraries).
@@ -34,7 +39,7 @@ FFLAGS="-O2 -ffp-contract=off -fPIC -cpp"
 MODS="MTypes MTime BinFiles MDimension3D MUtil MCells MConfig MContacts mFluid MForces MDEM"
 
 build_variant() {
-  local name="$1" patch="$2"
+  local name="$1" patch="$2" flags="${3:-$FFLAGS}"
   local obj="$HERE/_ref/obj_$name"
   rm -rf "$obj"
   mkdir -p "$obj"
@@ -43,15 +48,15 @@ build_variant() {
     if [ "$m" = "MForces" ] && [ -n "$patch" ]; then
       sed -e 's/MaxNeighbours = 50/MaxNeighbours = 65536/' \
           -e 's/MaxNCell = 30000_ib/MaxNCell = 2147483647_ib/' "$REF/src/$m.f90" |
-        (cd "$obj" && "$FC" $FFLAGS -x f95-cpp-input -c - -o "$obj/$m.o")
+        (cd "$obj" && "$FC" $flags -x f95-cpp-input -c - -o "$obj/$m.o")
       grep -q 'MaxNeighbours = 50' "$REF/src/$m.f90"
       grep -q 'MaxNCell = 30000_ib' "$REF/src/$m.f90"
     else
-      (cd "$obj" && "$FC" $FFLAGS -c "$REF/src/$m.f90" -o "$obj/$m.o")
+      (cd "$obj" && "$FC" $flags -c "$REF/src/$m.f90" -o "$obj/$m.o")
     fi
     objs+=("$obj/$m.o")
   done
-  (cd "$obj" && "$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o "$obj/ref_harness.o")
+  (cd "$obj" && "$FC" $flags -c "$HERE/ref_harness.f90" -o "$obj/ref_harness.o")
   "$FC" -shared -fPIC -o "$HERE/_ref/libdem3d_ref$name.so" "${objs[@]}" "$obj/ref_harness.o"
   rm -rf "$obj"
 }
@@ -59,4 +64,11 @@ build_variant() {
 mkdir -p "$HERE/_ref"
 build_variant "" ""
 build_variant "_big" "yes"
-echo "build_ref.sh: built $HERE/_ref/libdem3d_ref.so and libdem3d_ref_big.so"
+# the reference's own optimisation (Makefile: -O3 -march=native, contraction
+# left on) for timing only; -march=x86-64-v3 (AVX2 + FMA) instead of native so
+# that the library also runs on the GPU machine's host CPU
+build_variant "_fast" "yes" "-O3 -march=x86-64-v3 -fPIC -cpp"
+if [ "${DEM3D_REF_NATIVE:-0}" = "1" ]; then
+  build_variant "_native" "yes" "-O3 -march=native -fPIC -cpp"
+fi
+echo "build_ref.sh: built $HERE/_ref/libdem3d_ref.so, libdem3d_ref_big.so and libdem3d_ref_fast.so"

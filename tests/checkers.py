@@ -21,6 +21,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref.so")
 REF_LIB_BIG = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_big.so")
+# capacity-raised and built for speed (-O3, AVX2/FMA): timing only
+REF_LIB_FAST = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_fast.so")
 
 _oracle = None
 
@@ -36,8 +38,8 @@ def oracle_system(red, **kw) -> System:
     return System(oracle_lib(), red, **kw)
 
 
-def ref_available(big=False) -> bool:
-    return os.path.exists(REF_LIB_BIG if big else REF_LIB)
+def ref_available(big=False, fast=False) -> bool:
+    return os.path.exists(REF_LIB_FAST if fast else (REF_LIB_BIG if big else REF_LIB))
 
 
 # ---------------------------------------------------------------------------
@@ -165,7 +167,7 @@ def _d(a):
 
 def _ref_job_body(job):
     red = job["red"]
-    lib = C.CDLL(REF_LIB_BIG if job.get("big") else REF_LIB)
+    lib = C.CDLL(REF_LIB_FAST if job.get("fast") else (REF_LIB_BIG if job.get("big") else REF_LIB))
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
     P = lambda a: a.ctypes.data_as(dp)

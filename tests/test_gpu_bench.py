@@ -1,8 +1,8 @@
-"""bench.py's one JSON line, on the sheared bed cut to 12 x 12 x 6 rock-salt
-cells (6 912 spheres): the keys of the benchmark line, value and
-ms_per_step consistent with the sphere count, the roofline fraction equal to
-achieved / peak, and the CPU baseline leg on a bounded sample.  A request for
-several GPUs is refused with exit status 2."""
+"""bench.py's one JSON line, on a packed bed of 6 000 spheres (and the
+crystal leg on 12 x 12 x 6 rock-salt cells): the keys of the benchmark line,
+value and ms_per_step consistent with the sphere count, the roofline fraction
+equal to achieved / peak, and the CPU baseline leg on a bounded sample.  A
+request for several GPUs is refused with exit status 2."""
 from __future__ import annotations
 
 import json
@@ -24,9 +24,9 @@ def _bench(*args, timeout=240):
 
 
 def test_bench_line_on_a_small_bed():
-    p = _bench("--cells", "12,12,6", "--steps", "20", "--warmup", "5", "--tune-steps", "2",
-               "--ens-steps", "0", "--serve-requests", "0", "--cpu-cells", "6,6,4",
-               "--cpu-steps", "1")
+    p = _bench("--n", "6000", "--settle-steps", "50", "--cells", "12,12,6", "--crystal-steps",
+               "20", "--steps", "20", "--warmup", "5", "--tune-steps", "2", "--ens-steps", "0",
+               "--serve-requests", "0", "--cpu-steps", "1")
     assert p.returncode == 0, p.stderr[-3000:]
     d = json.loads(p.stdout.strip().splitlines()[-1])
     for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
@@ -37,8 +37,10 @@ def test_bench_line_on_a_small_bed():
     assert d["unit"] == "particle-steps/s" and d["higher_is_better"] is True
     assert d["dtype"] == "f64" and d["data"] == "synthetic"
     c = d["config"]
-    n = 8 * 12 * 12 * 6
-    assert c["spheres"] == n and c["workload"] == "bidisperse-sheared-12x12x6"
+    n = 6000
+    assert c["spheres"] == n and c["workload"] == "bidisperse-sheared-6000-phi0.60"
+    assert 0.55 < c["packing_fraction_bulk"] < 0.66
+    assert d["crystal"]["spheres"] == 8 * 12 * 12 * 6 and d["crystal"]["value"] > 0.0
     # value = spheres x steps / seconds of the timed steps
     assert d["value"] > 0.0
     assert abs(d["value"] * d["ms_per_step"] * 1e-3 - n) <= 1e-9 * n

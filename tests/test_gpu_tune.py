@@ -73,7 +73,7 @@ def test_autotune_keeps_the_trajectory(hip):
     assert len(rep["ms"]) == NSHAPES and all(m > 0 for m in rep["ms"])
     assert rep["chosen"] == int(np.argmin(rep["ms"]))
     assert rep["block"][0] == 256 and sorted(set(rep["block"])) == [64, 128, 256, 512]
-    assert rep["depth"][0] == 1 and sorted(set(rep["depth"])) == [1, 2]
+    assert rep["front"][0] == 1 and sorted(set(rep["front"])) == [0, 1]
     a.step(11)
     b = _make(hip, kw)
     b.step(rep["steps"] + 11)
