@@ -8,6 +8,7 @@
 #include "../../include/dem3d.h"
 #include "batch.hpp"
 #include "files.hpp"
+#include "observe.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -139,6 +140,9 @@ struct dem_system {
   bool batch_first = false;  // next batched step is the first since joining
   Timer timer;
   double ms_pass1 = 0, ms_pass2 = 0, ms_rebuild = 0, ms_step = 0;
+  int device = 0;             // the GPU of the handle's stream
+  double* d_obs = nullptr;    // dem_observables partial sums
+  size_t obs_cap = 0;         // doubles in d_obs
 };
 
 static int fail(dem_handle_t h, int code, const char* fmt, ...) {
@@ -342,7 +346,8 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     q.muwsq = p->muwsq[t];
   }
   h->walls_h.assign(std::max(1, nw), WallGeom{});
-  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&h->d_pair, pair.size());
   if (e == hipSuccess) e = dalloc(&h->d_comp, comp.size());
   if (e == hipSuccess) e = dalloc(&h->d_wallc, h->wallc_h.size());
@@ -381,6 +386,7 @@ void dem_destroy(dem_handle_t h) {
   dfree(h->d_pred_w);
   dfree(h->d_dens); dfree(h->d_vf); dfree(h->d_ef); dfree(h->d_fpart); dfree(h->d_frange);
   dfree(h->d_drag);
+  dfree(h->d_obs);
   dfree(h->ctl); dfree(h->rb);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -2053,6 +2059,68 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->ms_step = h->ms_step;
   out->mean_pair_contacts = c.upd_evals ? (double)c.pairs_upd_sum / (double)c.upd_evals : 0.0;
   return DEM_OK;
+}
+
+}  // extern "C"
+
+// ---- observables -------------------------------------------------------------------
+
+// The four observable kernels on the handle's stream with the partial sums in
+// `scr` (obs_scratch_doubles(nl) doubles), then the results into out/profile.
+static int run_observables(dem_handle_t h, int nl, double* scr, dem_observables_t* out,
+                           double* profile) {
+  memset(out, 0, sizeof *out);
+  if (h->in_batch) HIPCHK(h, hipDeviceSynchronize());
+  hipStream_t st = h->stream;
+  ObsArgs o;
+  o.pos = h->pos[h->cur];
+  o.kin = h->kin[h->cur];
+  o.E = h->es;
+  o.nl = nl;
+  o.nblk = std::max(1, std::min(DEM_OBS_BLOCKS, nblocks(h->np)));
+  o.part = scr;
+  o.lpart = scr + (size_t)DEM_OBS_BLOCKS * DEM_OBS_NPART;
+  o.out = o.lpart + (size_t)DEM_OBS_BLOCKS * (size_t)nl * 4;
+  hipLaunchKernelGGL(k_obs_partial, dim3(o.nblk), dim3(DEM_BLOCK), 0, st, h->sc, o);
+  hipLaunchKernelGGL(k_obs_finish, dim3(1), dim3(DEM_BLOCK), 0, st, h->sc, o);
+  hipLaunchKernelGGL(k_obs_bulk, dim3(o.nblk), dim3(DEM_BLOCK), 0, st, h->sc, o);
+  hipLaunchKernelGGL(k_obs_phi, dim3(1), dim3(DEM_BLOCK), 0, st, h->sc, o);
+  HIPCHK(h, hipGetLastError());
+  double r[DEM_OBS_NOUT];
+  HIPCHK(h, hipMemcpyAsync(r, o.out, sizeof r, hipMemcpyDeviceToHost, st));
+  if (nl > 0 && profile)
+    HIPCHK(h, hipMemcpyAsync(profile, o.out + DEM_OBS_NOUT, 4 * (size_t)nl * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  out->kinetic_energy = r[OBS_KE];
+  out->potential_energy = r[OBS_PE];
+  out->pair_contacts = (int64_t)r[OBS_PAIRS];
+  out->wall_contacts = (int64_t)r[OBS_WALLS];
+  out->coordination = h->np > 0 ? 2.0 * r[OBS_PAIRS] / (double)h->np : 0.0;
+  out->packing_fraction = r[OBS_PHI];
+  out->region[0] = r[OBS_ZLO];
+  out->region[1] = r[OBS_ZHI];
+  out->solid_volume = r[OBS_VOL];
+  out->top = r[OBS_TOP];
+  return DEM_OK;
+}
+
+extern "C" {
+
+int dem_observables(dem_handle_t h, int32_t nlayers, dem_observables_t* out, double* profile) {
+  if (!h || !out) return DEM_ERR_ARG;
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (nlayers < 0 || nlayers > 65536)
+    return fail(h, DEM_ERR_ARG, "nlayers must be 0..65536");
+  if (nlayers > 0 && !profile) return fail(h, DEM_ERR_ARG, "null profile");
+  const size_t need = obs_scratch_doubles(nlayers);
+  if (h->obs_cap < need) {
+    dfree(h->d_obs);
+    h->obs_cap = 0;
+    HIPCHK(h, dalloc(&h->d_obs, need));
+    h->obs_cap = need;
+  }
+  return run_observables(h, nlayers, h->d_obs, out, profile);
 }
 
 }  // extern "C"

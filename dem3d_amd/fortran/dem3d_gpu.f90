@@ -81,11 +81,22 @@ module dem3d_gpu
     integer(c_int32_t) :: nshapes = 0, chosen = 0
     integer(c_int64_t) :: steps = 0
     integer(c_int32_t) :: block(DEM_NSHAPES) = 0, waves(DEM_NSHAPES) = 0
-    integer(c_int32_t) :: depth(DEM_NSHAPES) = 0
+    integer(c_int32_t) :: front(DEM_NSHAPES) = 0
     real(c_double) :: ms(DEM_NSHAPES) = 0
     integer(c_int32_t) :: block_map = 0, pad_ = 0
     real(c_double) :: ms_map(2) = 0
   end type dem_tune_t
+
+  ! dem_observables_t: energies (KineticEnergy, PotentialEnergy,
+  ! src/MDEM.f90:183-213), contact counts, coordination number and bulk
+  ! packing fraction of the resident state.
+  type, bind(c), public :: dem_observables_t
+    real(c_double) :: kinetic_energy = 0, potential_energy = 0
+    integer(c_int64_t) :: pair_contacts = 0, wall_contacts = 0
+    real(c_double) :: coordination = 0, packing_fraction = 0
+    real(c_double) :: region(2) = 0
+    real(c_double) :: solid_volume = 0, top = 0
+  end type dem_observables_t
 
   ! dem_config_t: a configuration file (SaveConfiguration, src/MConfig.f90:109-236).
   type, bind(c), public :: dem_config_t
@@ -153,7 +164,7 @@ module dem3d_gpu
   public :: dem_upload_contacts, dem_download_contacts, dem_download_cells
   public :: dem_compute_accelerations_aos, dem_compute_accelerations
   public :: dem_euler_startup, dem_step, dem_synchronize, dem_stats, dem_set_option
-  public :: dem_autotune
+  public :: dem_autotune, dem_observables
   public :: dem_error_message
 
   interface
@@ -287,6 +298,16 @@ module dem3d_gpu
       type(c_ptr), value :: h
       integer(c_int64_t), value :: steps_per_shape, first_step
       type(dem_tune_t), intent(inout) :: report
+    end function
+    ! Observables of the resident state; profile(4, nlayers) when nlayers > 0
+    ! (pass a one-element array when nlayers = 0).
+    integer(c_int) function dem_observables(h, nlayers, out, profile) &
+        bind(c, name="dem_observables")
+      import :: c_int, c_int32_t, c_double, c_ptr, dem_observables_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nlayers
+      type(dem_observables_t), intent(out) :: out
+      real(c_double), intent(out) :: profile(*)
     end function
     ! FluidFlow with the specification's fluid (the C entry also takes a
     ! null pointer to turn it off).

@@ -106,6 +106,21 @@ class Stats(C.Structure):
         return out
 
 
+class ObservablesT(C.Structure):
+    """dem_observables_t."""
+    _fields_ = [("kinetic_energy", C.c_double), ("potential_energy", C.c_double),
+                ("pair_contacts", C.c_int64), ("wall_contacts", C.c_int64),
+                ("coordination", C.c_double), ("packing_fraction", C.c_double),
+                ("region", C.c_double * 2), ("solid_volume", C.c_double), ("top", C.c_double)]
+
+    def asdict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
 class FluidT(C.Structure):
     _fields_ = [("fluid_density", C.c_double), ("viscosity", C.c_double),
                 ("flow_rate", C.c_double), ("nl", C.c_int32), ("pad_", C.c_int32),
@@ -199,6 +214,8 @@ class Library:
         "stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     }
     OPTIONAL = {
+        "observables": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ObservablesT), _dp]),
+        "potential_energy": (C.c_double, [C.c_void_p]),
         "set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
         "set_wall_motion": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallMotionT)]),
         "download_walls": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallT)]),
@@ -551,6 +568,19 @@ class System:
         self._check(self.lib.remove_escaped(self.h, cap, out.ctypes.data_as(_ip), C.byref(n)))
         self.np -= n.value
         return out[: n.value].copy()
+
+    def observables(self, nlayers: int = 0):
+        """dem_observables: energies, contact
+        counts, coordination number, bulk packing fraction and, with
+        nlayers > 0, the vertical profile as an array [nlayers, 4] (solid
+        fraction, mean vx, vy, vz)."""
+        o = ObservablesT()
+        prof = np.zeros((max(nlayers, 0), 4))
+        self._check(self.lib.observables(self.h, int(nlayers), C.byref(o), _ptr(prof) if nlayers > 0 else None))
+        d = o.asdict()
+        if nlayers > 0:
+            d["profile"] = prof
+        return d
 
     def synchronize(self):
         if hasattr(self.lib, "synchronize"):

@@ -263,6 +263,35 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *   "block_map"         block order of the force passes (see dem_tune_t) */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
+/* ---- observables ---------------------------------------------------------- */
+/* Bulk observables of the resident Particle state, computed on the GPU with
+ * fixed-order reductions (bitwise reproducible).  The energies are the
+ * reference's; the rest is defined here (the reference driver has no
+ * coordination number or packing fraction). */
+typedef struct {
+  double kinetic_energy;    /* KineticEnergy() (src/MDEM.f90:199-213)          */
+  double potential_energy;  /* PotentialEnergy() (src/MDEM.f90:183-195):
+                               -sum Mass*(Position . Gravity)                   */
+  int64_t pair_contacts;    /* overlapping pairs at the current positions
+                               (MinDijSq - DijSq > 0, each pair once)           */
+  int64_t wall_contacts;    /* overlapping particle-wall pairs (Overlap)       */
+  double coordination;      /* 2 * pair_contacts / NP                          */
+  double packing_fraction;  /* solid_volume / (Lx * Ly * (region[1] - region[0])) */
+  double region[2];         /* [zlo, zhi): the whole box when z is periodic,
+                               else from one largest diameter above -Lz/2 to
+                               one largest diameter below `top`               */
+  double solid_volume;      /* volume of the spheres whose centre lies in the
+                               region                                          */
+  double top;               /* max (z + r) over the particles                  */
+} dem_observables_t;
+/* Observables of the Particle state.  With nlayers > 0, profile[4*nlayers]
+ * receives, for nlayers equal layers of the box height from -Lz/2, the
+ * volume of sphere material inside each layer (exact sphere-slab
+ * intersection) over the layer volume, then the volume-weighted mean
+ * velocity (x, y, z) of that material (zero in an empty layer).  Device
+ * scratch for the partial sums is allocated on first use and kept by the
+ * handle; nlayers <= 65536.  Waits for the handle's stream. */
+int dem_observables(dem_handle_t h, int32_t nlayers, dem_observables_t* out, double* profile);
 /* ---- fluid drag (mFluid) --------------------------------------------------- */
 /* An upward fluid flow through the box (src/mFluid.f90), reduced units.  At
  * every force evaluation the porosity Ef and the fluid velocity Vf of NL+1

@@ -1600,6 +1600,20 @@ int orc_stats(orc_handle_t h, dem_stats_t* s) {
   return DEM_OK;
 }
 
+/* PotentialEnergy (src/MDEM.f90:183-195): Ep = Ep - Mass*(Position .dot.
+ * Gravity) over i = 1..NP, the dot product accumulated from zero in index
+ * order (dot_product, src/MDimension3D.f90:75-80). */
+double orc_potential_energy(orc_handle_t h) {
+  double ep = 0.0;
+  for (int i = 0; i < h->np; i++) {
+    const particle_t* q = &h->part[i];
+    double d = 0.0;
+    for (int k = 0; k < 3; k++) d = d + q->position[k] * h->g[k];
+    ep = ep - h->mass[q->index - 1] * d;
+  }
+  return ep;
+}
+
 int orc_download_force_scale(orc_handle_t h, int32_t np, double* scale) {
   if (np != h->np) return fail(h, DEM_ERR_ARG, "np mismatch");
   for (int i = 0; i < np; i++) {

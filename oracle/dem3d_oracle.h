@@ -67,6 +67,8 @@ int orc_euler_startup(orc_handle_t h);
 int orc_step(orc_handle_t h, int64_t nsteps, int64_t first_step,
              int64_t* steps_done, int32_t* escaped);
 int orc_stats(orc_handle_t h, dem_stats_t* out);
+/* PotentialEnergy() (src/MDEM.f90:183-195) of the Particle state. */
+double orc_potential_energy(orc_handle_t h);
 
 /* Per-particle |contributions| accumulator used by the force-parity tests
  * (scale_i = |m_eff g| + sum_k (|F_N,ik| + |F_T,ik|), same for torques) of the

@@ -547,4 +547,9 @@ contains
     e = KineticEnergy()
   end function ref_kinetic_energy
 
+  function ref_potential_energy() result( e ) bind(C, name="ref_potential_energy")
+    real(c_double) :: e
+    e = PotentialEnergy()
+  end function ref_potential_energy
+
 end module ref_harness

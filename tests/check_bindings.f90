@@ -18,6 +18,7 @@ program check_bindings
   type(dem_pour_t) :: po
   type(dem_fluid_t) :: fl
   type(dem_tune_t) :: tu
+  type(dem_observables_t) :: ob
   type(c_ptr) :: h
   real(c_double), target :: radius(1), tab(1)
   integer(c_int) :: rc
@@ -34,6 +35,7 @@ program check_bindings
   write(*, '(a, i0)') 'sizeof dem_pour_t ', c_sizeof(po)
   write(*, '(a, i0)') 'sizeof dem_fluid_t ', c_sizeof(fl)
   write(*, '(a, i0)') 'sizeof dem_tune_t ', c_sizeof(tu)
+  write(*, '(a, i0)') 'sizeof dem_observables_t ', c_sizeof(ob)
 
   radius = 0.5d0
   tab = 1.0d0

@@ -274,6 +274,7 @@ def _ref_job_body(job):
     else:
         lib.ref_fresh_state()
     lib.ref_kinetic_energy.restype = C.c_double
+    lib.ref_potential_energy.restype = C.c_double
     lib.ref_ncol.restype = C.c_int32
 
     lib.ref_get_np.restype = C.c_int32
@@ -295,7 +296,8 @@ def _ref_job_body(job):
             lib.ref_get_walls(C.c_int32(nw), P(wpos), P(wvel))
         return dict(x=xs.T.copy(), v=vs.T.copy(), w=ws.T.copy(), a=as_.T.copy(),
                     alpha=als.T.copy(), cell_index=ci, state=st, ncol=int(lib.ref_ncol()),
-                    ke=float(lib.ref_kinetic_energy()), wall_pos=wpos[:, :nw].T.copy(),
+                    ke=float(lib.ref_kinetic_energy()), pe=float(lib.ref_potential_energy()),
+                    wall_pos=wpos[:, :nw].T.copy(),
                     wall_vel=wvel[:, :nw].T.copy(), time=float(lib.ref_get_time()))
 
     out["snapshots"] = []

@@ -32,7 +32,7 @@ def c_sizes(tmp_path):
                              for t in ("dem_params_t", "dem_wall_t", "dem_particle_t",
                                        "dem_stats_t", "dem_wall_motion_t", "dem_config_t",
                                        "dem_run_t", "dem_state_sections_t", "dem_rng_t",
-                                       "dem_pour_t", "dem_fluid_t", "dem_tune_t"))
+                                       "dem_pour_t", "dem_fluid_t", "dem_tune_t", "dem_observables_t"))
                    + "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
@@ -65,6 +65,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert C.sizeof(native.PourT) == cs["dem_pour_t"]
     assert C.sizeof(native.FluidT) == cs["dem_fluid_t"]
     assert C.sizeof(native.TuneT) == cs["dem_tune_t"]
+    assert C.sizeof(native.ObservablesT) == cs["dem_observables_t"]
 
 
 @pytest.mark.skipif(shutil.which("amdflang") is None or not os.path.exists(LIB),

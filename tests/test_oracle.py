@@ -197,6 +197,9 @@ def test_oracle_matches_live_reference(seed, nc, pbc, walls):
         assert np.array_equal(con["ft"].reshape(ft.shape), ft)
         assert np.array_equal(o.download_cells(), snap["cell_index"])
         assert o.stats()["ncol"] == snap["ncol"]
+        # KineticEnergy and PotentialEnergy (src/MDEM.f90:183-213), bit for bit
+        assert o.stats()["kinetic_energy"] == snap["ke"]
+        assert o.lib.potential_energy(o.h) == snap["pe"]
 
 
 @live

@@ -13,7 +13,8 @@ happens not to exercise it):
   overlap follows m (g + a_floor(t)) / Knw less the dashpot's share, with the
   floor where A cos(omega t) puts it at t = step * TimeStep;
 * the contact set equals an O(N^2) all-pairs search with minimum image;
-* Newton's third law: internal pair forces sum to zero.
+* Newton's third law: internal pair forces sum to zero;
+* PotentialEnergy: m g h summed over spheres at rest at known heights.
 """
 from __future__ import annotations
 
@@ -193,3 +194,19 @@ def test_internal_forces_sum_to_zero():
     f = st["a"] * m[None, :]
     assert o.stats()["pair_contacts"] > 20
     assert np.abs(f.sum(axis=1)).max() <= 1e-12 * np.abs(f).sum(axis=1).max()
+
+
+def test_potential_energy_is_m_g_h():
+    """PotentialEnergy = -sum m (x . g): with g = -g0 z, the sum of m g0 z."""
+    red = _system((8.0, 8.0, 16.0), 1.0, gravity=(0.0, 0.0, -9.81), floor=False)
+    g0 = -float(red.gravity[2])
+    z = np.array([-4.0, -1.5, 0.0, 2.25, 6.5])
+    x = np.stack([np.array([-2.0, -1.0, 0.0, 1.0, 2.0]), np.zeros(5), z])
+    o = oracle_system(red)
+    o.upload(np.ones(5, np.int32), x, np.zeros_like(x), np.zeros_like(x))
+    m = float(red.mass[0])
+    ep = o.lib.potential_energy(o.h)
+    assert abs(ep - m * g0 * float(z.sum())) <= 1e-14 * m * g0 * float(np.abs(z).sum())
+    # a transposed gravity (x instead of z) would give m g0 sum(x) = 0, a
+    # flipped sign -m g0 sum(z)
+    assert ep > 0.5 * m * g0 * float(z.sum())
