@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,23 @@ struct Timer {
 };
 
 }  // namespace
+
+// Device memory of dem_observables_noalloc: one fixed array per GPU (each GPU
+// holds its own copy of a __device__ variable), big enough for the partial
+// sums of up to DEM_OBS_MAX_LAYERS profile layers.  Host code uses the
+// device address of each GPU's copy, taken once per device in dem_create;
+// the name `scratch` itself is only a host-side stand-in and is never
+// passed to a kernel.
+#define DEM_OBS_SCRATCH_DOUBLES                                                         \
+  ((size_t)DEM_OBS_BLOCKS * DEM_OBS_NPART + (size_t)DEM_OBS_BLOCKS * DEM_OBS_MAX_LAYERS * 4 + \
+   DEM_OBS_NOUT + 4 * (size_t)DEM_OBS_MAX_LAYERS)
+static_assert(DEM_OBS_SCRATCH_DOUBLES * sizeof(double) == DEM_OBS_NOALLOC_BYTES,
+              "DEM_OBS_NOALLOC_BYTES in dem3d.h matches the array");
+static __device__ __align__(256) double scratch[DEM_OBS_SCRATCH_DOUBLES];
+#define DEM_MAX_DEVICES 64
+static std::mutex g_scratch_mu;               // guards g_scratch_ptr
+static double* g_scratch_ptr[DEM_MAX_DEVICES];  // device address of `scratch`, per GPU
+static std::mutex g_obs_noalloc_mu;           // one dem_observables_noalloc at a time
 
 struct dem_system {
   char err[512];
@@ -347,6 +365,19 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   }
   h->walls_h.assign(std::max(1, nw), WallGeom{});
   hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess && (h->device < 0 || h->device >= DEM_MAX_DEVICES)) {
+    delete h;
+    return fail(nullptr, DEM_ERR_UNSUPPORTED, "device index beyond %d", DEM_MAX_DEVICES - 1);
+  }
+  if (e == hipSuccess) {
+    // the device address of this GPU's copy of `scratch`, once per device
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+      void* ptr = nullptr;
+      e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) g_scratch_ptr[h->device] = static_cast<double*>(ptr);
+    }
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&h->d_pair, pair.size());
   if (e == hipSuccess) e = dalloc(&h->d_comp, comp.size());
@@ -2121,6 +2152,21 @@ int dem_observables(dem_handle_t h, int32_t nlayers, dem_observables_t* out, dou
     h->obs_cap = need;
   }
   return run_observables(h, nlayers, h->d_obs, out, profile);
+}
+
+int dem_observables_noalloc(dem_handle_t h, int32_t nlayers, dem_observables_t* out,
+                            double* profile) {
+  if (!h || !out) return DEM_ERR_ARG;
+  if (!h->has_state) return fail(h, DEM_ERR_STATE, "upload the state first");
+  if (nlayers < 0 || nlayers > DEM_OBS_MAX_LAYERS)
+    return fail(h, DEM_ERR_CAPACITY,
+                "nlayers=%d: the allocation-free observables hold at most %d layers "
+                "(dem_observables takes more)", nlayers, DEM_OBS_MAX_LAYERS);
+  if (nlayers > 0 && !profile) return fail(h, DEM_ERR_ARG, "null profile");
+  double* scr = g_scratch_ptr[h->device];
+  if (!scr) return fail(h, DEM_ERR_STATE, "no scratch address for device %d", h->device);
+  std::lock_guard<std::mutex> lk(g_obs_noalloc_mu);
+  return run_observables(h, nlayers, scr, out, profile);
 }
 
 }  // extern "C"

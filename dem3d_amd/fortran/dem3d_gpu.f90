@@ -90,6 +90,7 @@ module dem3d_gpu
   ! dem_observables_t: energies (KineticEnergy, PotentialEnergy,
   ! src/MDEM.f90:183-213), contact counts, coordination number and bulk
   ! packing fraction of the resident state.
+  integer(c_int), parameter, public :: DEM_OBS_MAX_LAYERS = 256
   type, bind(c), public :: dem_observables_t
     real(c_double) :: kinetic_energy = 0, potential_energy = 0
     integer(c_int64_t) :: pair_contacts = 0, wall_contacts = 0
@@ -164,7 +165,7 @@ module dem3d_gpu
   public :: dem_upload_contacts, dem_download_contacts, dem_download_cells
   public :: dem_compute_accelerations_aos, dem_compute_accelerations
   public :: dem_euler_startup, dem_step, dem_synchronize, dem_stats, dem_set_option
-  public :: dem_autotune, dem_observables
+  public :: dem_autotune, dem_observables, dem_observables_noalloc
   public :: dem_error_message
 
   interface
@@ -303,6 +304,15 @@ module dem3d_gpu
     ! (pass a one-element array when nlayers = 0).
     integer(c_int) function dem_observables(h, nlayers, out, profile) &
         bind(c, name="dem_observables")
+      import :: c_int, c_int32_t, c_double, c_ptr, dem_observables_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nlayers
+      type(dem_observables_t), intent(out) :: out
+      real(c_double), intent(out) :: profile(*)
+    end function
+    ! The same with no device allocation (nlayers <= DEM_OBS_MAX_LAYERS).
+    integer(c_int) function dem_observables_noalloc(h, nlayers, out, profile) &
+        bind(c, name="dem_observables_noalloc")
       import :: c_int, c_int32_t, c_double, c_ptr, dem_observables_t
       type(c_ptr), value :: h
       integer(c_int32_t), value :: nlayers

@@ -216,6 +216,7 @@ class Library:
     OPTIONAL = {
         "observables": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ObservablesT), _dp]),
         "potential_energy": (C.c_double, [C.c_void_p]),
+        "observables_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ObservablesT), _dp]),
         "set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
         "set_wall_motion": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallMotionT)]),
         "download_walls": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(WallT)]),
@@ -569,14 +570,15 @@ class System:
         self.np -= n.value
         return out[: n.value].copy()
 
-    def observables(self, nlayers: int = 0):
-        """dem_observables: energies, contact
+    def observables(self, nlayers: int = 0, noalloc: bool = False):
+        """dem_observables (or dem_observables_noalloc): energies, contact
         counts, coordination number, bulk packing fraction and, with
         nlayers > 0, the vertical profile as an array [nlayers, 4] (solid
         fraction, mean vx, vy, vz)."""
         o = ObservablesT()
         prof = np.zeros((max(nlayers, 0), 4))
-        self._check(self.lib.observables(self.h, int(nlayers), C.byref(o), _ptr(prof) if nlayers > 0 else None))
+        f = self.lib.observables_noalloc if noalloc else self.lib.observables
+        self._check(f(self.h, int(nlayers), C.byref(o), _ptr(prof) if nlayers > 0 else None))
         d = o.asdict()
         if nlayers > 0:
             d["profile"] = prof

@@ -292,6 +292,20 @@ typedef struct {
  * scratch for the partial sums is allocated on first use and kept by the
  * handle; nlayers <= 65536.  Waits for the handle's stream. */
 int dem_observables(dem_handle_t h, int32_t nlayers, dem_observables_t* out, double* profile);
+/* dem_observables without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, code that must not call hipMalloc).  The partial sums live in one
+ * fixed array of the library in each GPU's memory (DEM_OBS_NOALLOC_BYTES,
+ * about 2.1 MB, held until the process ends), so nlayers <= DEM_OBS_MAX_LAYERS;
+ * larger values return DEM_ERR_CAPACITY.  The array is shared by every handle
+ * on that GPU: calls from one process are serialised by the library, calls
+ * that overlap on one GPU from other code paths would race.  Results are
+ * bitwise those of dem_observables. */
+#define DEM_OBS_MAX_LAYERS 256
+#define DEM_OBS_NOALLOC_BYTES ((256 * 8 + 256 * DEM_OBS_MAX_LAYERS * 4 + 16 + 4 * DEM_OBS_MAX_LAYERS) * 8)
+int dem_observables_noalloc(dem_handle_t h, int32_t nlayers, dem_observables_t* out,
+                            double* profile);
+
 /* ---- fluid drag (mFluid) --------------------------------------------------- */
 /* An upward fluid flow through the box (src/mFluid.f90), reduced units.  At
  * every force evaluation the porosity Ef and the fluid velocity Vf of NL+1
