@@ -126,7 +126,7 @@ def timed_run(wl, args, precision, hmode):
     st = g.stats()
     sec = t1 - t0
     out = {"ms": 1e3 * sec / args.steps, "value": wl.np * args.steps / sec, "stats": st,
-           "x": g.download()["x"] if args.precision == "both" else None, "tune": tune,
+           "obs": g.observables(), "tune": tune,
            "steps_before": g.next_step - args.steps - wl.red.method_order}
     g.close()
     return out
@@ -236,17 +236,25 @@ def ensemble_run(args):
 
 
 def serving_run(args):
-    """The first --serve-requests requests of the Poisson serving trace
-    (50 req/s, sizes log-uniform in [1 000, 500 000], steps U[1 000, 20 000]),
+    """The Poisson serving trace (200 requests at 50 req/s, sizes log-uniform
+    in [1 000, 500 000], steps U[1 000, 20 000]; --serve-requests cuts it),
     served in real time with continuous batching in mixed precision."""
     from dem3d_amd import serve, workloads
     from dem3d_amd.native import DEM_MIXED, load_hip
     trace = workloads.serving_trace()[: args.serve_requests]
-    rep = serve.serve(load_hip(), trace, precision=DEM_MIXED, chunk=100,
-                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    last = [0.0]
+
+    def log(m):  # a progress line every few seconds
+        t = time.perf_counter()
+        if t - last[0] >= 5.0:
+            last[0] = t
+            print(m, file=sys.stderr, flush=True)
+
+    rep = serve.serve(load_hip(), trace, precision=DEM_MIXED, chunk=100, log=log)
     out = rep.summary()
-    out["trace"] = (f"first {len(trace)} of 200 requests, Poisson 50/s, n log-uniform "
-                    f"[1000, 500000], steps U[1000, 20000], mixed precision")
+    out["trace"] = (f"{'all' if len(trace) == 200 else f'first {len(trace)} of'} 200 requests, "
+                    f"Poisson 50/s, n log-uniform [1000, 500000], steps U[1000, 20000], "
+                    f"mixed precision; built on a worker thread while the batch steps")
     return out
 
 
@@ -289,7 +297,7 @@ def main(argv=None):
                          "(profiling runs)")
     ap.add_argument("--alone-steps", type=int, default=100,
                     help="steps per system of the one-at-a-time FP64 sample")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving trace to run (0 = skip serving)")
     ap.add_argument("--history", choices=("reference", "canonical"), default="reference",
                     help="orientation of stored tangential history (see include/dem3d.h)")
@@ -314,7 +322,6 @@ def main(argv=None):
         return 2
     from dem3d_amd import workloads
     from dem3d_amd.native import DEM_FP64, DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, DEM_MIXED, load_hip
-    from dem3d_amd.observables import packing_fraction
     lib = load_hip()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -338,13 +345,13 @@ def main(argv=None):
     r64 = timed_run(wl, args, DEM_FP64, hmode)
     print(f"fp64: {r64['ms']:.4f} ms/step", file=sys.stderr, flush=True)
     st, ms, value = r64["stats"], r64["ms"], r64["value"]
-    phi = packing_fraction(wl.red, wl.x, wl.comp)
+    obs = r64["obs"]
     mixed = None
     if args.precision == "both":
         rmx = timed_run(wl, args, DEM_MIXED, hmode)
         print(f"mixed: {rmx['ms']:.4f} ms/step", file=sys.stderr, flush=True)
-        phi64 = packing_fraction(wl.red, r64["x"], wl.comp)
-        phimx = packing_fraction(wl.red, rmx["x"], wl.comp)
+        phi64 = r64["obs"]["packing_fraction"]
+        phimx = rmx["obs"]["packing_fraction"]
         ke64, kemx = r64["stats"]["kinetic_energy"], rmx["stats"]["kinetic_energy"]
         mixed = {
             "value": rmx["value"],
@@ -406,8 +413,9 @@ def main(argv=None):
             "workload": wl.name,
             "spheres": wl.np,
             "packing": wl.note,
-            "packing_fraction_bulk": phi,
-            "coordination_number": 2.0 * st["pair_contacts"] / wl.np,
+            "packing_fraction_bulk": obs["packing_fraction"],
+            "coordination_number": obs["coordination"],
+            "wall_contacts_final": obs["wall_contacts"],
             "neighbour_entries_per_sphere": nbr,
             "precision": "fp64",
             "history_mode": args.history,

@@ -106,24 +106,4 @@ struct SysConst {
   // predictor pass then evaluates it with another radius)
   double rc2_all;
   int cut_all;
-  // edge history layout: 0 = three double2 planes (FTx,FTy) (FTz,VTx)
-  // (VTy,VTz); 1 (mixed mode) = (FTx,FTy) double2, FTz double, VT float3
-  // (hist_ftz, hist_vt32)
-  int vt32;
 };
-
-// The second and third arrays of the mixed-mode edge history, inside the
-// same allocation of three double2 planes of ecap entries: FTz doubles from
-// byte 16*ecap, VT float triples from byte 24*ecap (36 bytes per edge).
-__host__ __device__ inline double* hist_ftz(double2* base, long long ecap) {
-  return reinterpret_cast<double*>(base + ecap);
-}
-__host__ __device__ inline float* hist_vt32(double2* base, long long ecap) {
-  return reinterpret_cast<float*>(reinterpret_cast<double*>(base + ecap) + ecap);
-}
-__host__ __device__ inline const double* hist_ftz(const double2* base, long long ecap) {
-  return reinterpret_cast<const double*>(base + ecap);
-}
-__host__ __device__ inline const float* hist_vt32(const double2* base, long long ecap) {
-  return reinterpret_cast<const float*>(reinterpret_cast<const double*>(base + ecap) + ecap);
-}

@@ -242,7 +242,6 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   s.nw = nw;
   s.hist_mode = h->hist_mode = p->history_mode;
   h->mixed = p->precision == DEM_MIXED;
-  s.vt32 = h->mixed ? 1 : 0;
   h->order = p->method_order;
   h->nprev = s.nprev = p->method_order - 1;
   h->n_euler = p->n_euler;
@@ -946,28 +945,9 @@ static int download_edges(dem_handle_t h, HostEdges& he, bool with_hist) {
   }
   if (with_hist) {
     he.hist.resize(3 * E);
-    if (!h->sc.vt32) {
-      for (int c = 0; c < 3 && E; ++c)
-        HIPCHK(h, hipMemcpyAsync(he.hist.data() + c * E, h->es.hist[he.par] + c * h->ecap,
-                                 E * sizeof(double2), hipMemcpyDeviceToHost, st));
-    } else if (E) {
-      // the mixed layout (FTx,FTy) / FTz / VT float3, back to the three pairs
-      std::vector<double> ftz(E);
-      std::vector<float> vt(3 * E);
-      double2* base = h->es.hist[he.par];
-      const long long ec = (long long)h->ecap;
-      HIPCHK(h, hipMemcpyAsync(he.hist.data(), base, E * sizeof(double2), hipMemcpyDeviceToHost,
-                               st));
-      HIPCHK(h, hipMemcpyAsync(ftz.data(), hist_ftz(base, ec), E * sizeof(double),
-                               hipMemcpyDeviceToHost, st));
-      HIPCHK(h, hipMemcpyAsync(vt.data(), hist_vt32(base, ec), 3 * E * sizeof(float),
-                               hipMemcpyDeviceToHost, st));
-      HIPCHK(h, hipStreamSynchronize(st));
-      for (size_t e = 0; e < E; ++e) {
-        he.hist[E + e] = make_double2(ftz[e], (double)vt[3 * e]);
-        he.hist[2 * E + e] = make_double2((double)vt[3 * e + 1], (double)vt[3 * e + 2]);
-      }
-    }
+    for (int c = 0; c < 3 && E; ++c)
+      HIPCHK(h, hipMemcpyAsync(he.hist.data() + c * E, h->es.hist[he.par] + c * h->ecap,
+                               E * sizeof(double2), hipMemcpyDeviceToHost, st));
   }
   HIPCHK(h, hipStreamSynchronize(st));
   return DEM_OK;
@@ -1442,32 +1422,11 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
     }
   }
   hipStream_t st = h->stream;
-  std::vector<double> ftz;
-  std::vector<float> vt32;
   if (E) {
     HIPCHK(h, hipMemcpyAsync(h->es.flag[he.par], flag.data(), E, hipMemcpyHostToDevice, st));
-    if (!s.vt32) {
-      for (int c = 0; c < 3; ++c)
-        HIPCHK(h, hipMemcpyAsync(h->es.hist[he.par] + c * h->ecap, hist.data() + c * E,
-                                 E * sizeof(double2), hipMemcpyHostToDevice, st));
-    } else {
-      // the mixed layout: VT is kept in FP32 (the mixed arithmetic reads it so)
-      ftz.resize(E);
-      vt32.resize(3 * E);
-      for (size_t e = 0; e < E; ++e) {
-        ftz[e] = hist[E + e].x;
-        vt32[3 * e] = (float)hist[E + e].y;
-        vt32[3 * e + 1] = (float)hist[2 * E + e].x;
-        vt32[3 * e + 2] = (float)hist[2 * E + e].y;
-      }
-      double2* base = h->es.hist[he.par];
-      const long long ec = (long long)h->ecap;
-      HIPCHK(h, hipMemcpyAsync(base, hist.data(), E * sizeof(double2), hipMemcpyHostToDevice, st));
-      HIPCHK(h, hipMemcpyAsync(hist_ftz(base, ec), ftz.data(), E * sizeof(double),
-                               hipMemcpyHostToDevice, st));
-      HIPCHK(h, hipMemcpyAsync(hist_vt32(base, ec), vt32.data(), 3 * E * sizeof(float),
-                               hipMemcpyHostToDevice, st));
-    }
+    for (int c = 0; c < 3; ++c)
+      HIPCHK(h, hipMemcpyAsync(h->es.hist[he.par] + c * h->ecap, hist.data() + c * E,
+                               E * sizeof(double2), hipMemcpyHostToDevice, st));
   }
   if (N) {
     HIPCHK(h, hipMemcpyAsync(h->wflag[0], wf.data(), N * sizeof(unsigned), hipMemcpyHostToDevice,

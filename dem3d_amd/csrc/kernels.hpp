@@ -20,7 +20,8 @@
 //         flag u8[E]           bit 0: the pair has stored history
 //         hist double2[3][E]   planes (FTx,FTy) (FTz,VTx) (VTy,VTz): tangential
 //                              force FT and velocity VT of the pair as the
-//                              reference stores them (TPartner%Vector1/Vector2)
+//                              reference stores them (TPartner%Vector1/Vector2),
+//                              FP64 in both precision modes
 //   wall history: wflag uint[N] (bit w: contact with wall w), whist [nw][6][N]
 //
 // A force pass runs one lane per particle.  Each lane walks its own edges in
@@ -251,20 +252,12 @@ struct EdgeAcc {
 };
 
 // The stored history of edge e as the three (FTx,FTy) (FTz,VTx) (VTy,VTz)
-// pairs; the mixed mode reads its 36-byte layout (FTz double, VT float3:
-// hist_ftz, hist_vt32).
-template <bool MIXED>
+// pairs.
 __device__ __forceinline__ void load_hist(const double2* __restrict__ hist, long long ecap,
                                           long long e, double2& h0, double2& h1, double2& h2) {
   h0 = ld_nt(hist + e);
-  if (MIXED) {
-    const float* vt = hist_vt32(hist, ecap) + 3 * e;
-    h1 = make_double2(ld_nt(hist_ftz(hist, ecap) + e), (double)ld_nt(vt));
-    h2 = make_double2((double)ld_nt(vt + 1), (double)ld_nt(vt + 2));
-  } else {
-    h1 = ld_nt(hist + ecap + e);
-    h2 = ld_nt(hist + 2 * ecap + e);
-  }
+  h1 = ld_nt(hist + ecap + e);
+  h2 = ld_nt(hist + 2 * ecap + e);
 }
 
 template <bool UPDATE, bool MIXED, int FRONT>
@@ -295,7 +288,7 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     Kin KJ;
     if (front) KJ = load_kin(a.kin_in + j);
     double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
-    if (fl & 1) load_hist<MIXED>(hist, ecap, e, h0, h1, h2);
+    if (fl & 1) load_hist(hist, ecap, e, h0, h1, h2);
     const int cj = (int)Q.w;
     const d3 rij = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
     const double dsq = dot3(rij, rij);
@@ -318,28 +311,6 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
       if (!in.contacting && role) ++acc.n_col;
       if (role) ++acc.n_pair;
       const double rj = ctab[cj].radius;
-      if (MIXED) {
-        const MixedOut o = contact_mixed(
-            in.contacting, rij, dsq, delta, pc.mindij, sub3(kin_v(KJ), vi), wi, kin_w(KJ), ri, rj,
-            in.ft0, in.vt0, (float)a.half_dt, (float)pc.kn, (float)pc.yn, (float)pc.kt,
-            (float)pc.yt, pc.musq);
-        kahan_add3(acc.Ff, o.fn);
-        kahan_add3(acc.Ff, o.ft);
-        kahan_add3(acc.Tf, sclf((float)ri, o.tq_dir));
-        if (UPDATE) {
-          const d3 f = scl3(sg, o.ft_store);
-          const float fs = (float)sg;
-          st_nt(hist + e, make_double2(f.x, f.y));
-          st_nt(hist_ftz(hist, ecap) + e, f.z);
-          float* vt = hist_vt32(hist, ecap) + 3 * e;
-          st_nt(vt, fs * o.vt.x);
-          st_nt(vt + 1, fs * o.vt.y);
-          st_nt(vt + 2, fs * o.vt.z);
-          if (!(fl & 1)) flag[e] = 1;
-          ++acc.n_stored;
-        }
-        continue;
-      }
       const double dij = sqrt(dsq);
       in.nij = div3(rij, dij);
       in.disp_n = scl3(dij - pc.mindij, in.nij);
@@ -348,10 +319,19 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
       in.delta_dsq = delta;
       in.half_dt = a.half_dt;
       const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
-      // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381); torque of
-      // "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
-      acc.F = add3(add3(acc.F, o.fn), o.ft);
-      acc.T = add3(acc.T, role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft)));
+      if (MIXED) {
+        // the contact in FP64, its contributions summed in FP32 (mixed.hpp)
+        const f3 ft = tof(o.ft);
+        kahan_add3(acc.Ff, tof(o.fn));
+        kahan_add3(acc.Ff, ft);
+        kahan_add3(acc.Tf, sclf((float)ri, crossf(tof(in.nij), ft)));
+      } else {
+        // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381); torque
+        // of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
+        acc.F = add3(add3(acc.F, o.fn), o.ft);
+        acc.T = add3(acc.T,
+                     role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft)));
+      }
       if (UPDATE) {
         const d3 f = scl3(sg, o.ft), g = scl3(sg, o.vt);
         st_nt(hist + e, make_double2(f.x, f.y));
@@ -1102,17 +1082,8 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
         const long long om = ob + 64LL * mo;
         double2* g = E.hist[q];
         g[e] = ohist[om];
-        if (s.vt32) {
-          hist_ftz(g, ecap)[e] = hist_ftz(ohist, ecap)[om];
-          const float* ov = hist_vt32(ohist, ecap) + 3 * om;
-          float* nv = hist_vt32(g, ecap) + 3 * e;
-          nv[0] = ov[0];
-          nv[1] = ov[1];
-          nv[2] = ov[2];
-        } else {
-          g[ecap + e] = ohist[ecap + om];
-          g[2 * ecap + e] = ohist[2 * ecap + om];
-        }
+        g[ecap + e] = ohist[ecap + om];
+        g[2 * ecap + e] = ohist[2 * ecap + om];
       }
     });
     E.cnt[q][i] = ndeg | (kf << 16);

@@ -1,39 +1,30 @@
-"""Bulk observables of a granular state that the reference does not compute
-itself, defined here (host side, numpy) for the accuracy statements of the
-mixed-precision mode.
+"""Bulk observables of a resident system, computed on the GPU
+(dem_observables, dem3d_amd/csrc/observe.hpp):
 
-* Coordination number Z = 2 P / N, with P the overlapping pairs
-  (dem_stats().pair_contacts); wall contacts are reported separately.
-* Packing fraction phi = sum of sphere volumes / region volume, counting a
-  sphere when its centre lies in the region.  The region is the whole box
-  when z is periodic; otherwise the slab that starts one largest diameter
-  above the floor (z = -Lz/2) and ends one largest diameter below the top of
-  the packing (max z_i + r_i), so the layers ordered by a flat wall or
-  thinned at the free surface are left out.
+* KineticEnergy and PotentialEnergy (src/MDEM.f90:183-213);
+* coordination number Z = 2 P / N, with P the overlapping pairs at the
+  current positions; wall contacts are reported separately;
+* packing fraction phi = volume of the spheres whose centre lies in a bulk
+  region / region volume.  The region is the whole box when z is periodic;
+  otherwise the slab that starts one largest diameter above the bottom of the
+  box (z = -Lz/2) and ends one largest diameter below the top of the packing
+  (max z_i + r_i), so the layers ordered by a flat wall or thinned at the
+  free surface are left out;
+* optionally a vertical profile of solid fraction and mean velocity.
 """
 from __future__ import annotations
 
-import numpy as np
-
-from .spec import Reduced
+from .native import System
 
 
 def coordination_number(pair_contacts: int, np_: int) -> float:
     return 2.0 * pair_contacts / max(np_, 1)
 
 
-def packing_fraction(red: Reduced, x: np.ndarray, comp: np.ndarray) -> float:
-    """x: positions [3, np] in reduced units; comp: 1-based components."""
-    r = np.asarray(red.radius, np.float64)[np.asarray(comp) - 1]
-    vol = 4.0 / 3.0 * np.pi * r ** 3
-    lx, ly, lz = (float(v) for v in red.lbox)
-    if red.pbc[2]:
-        return float(vol.sum() / (lx * ly * lz))
-    dmax = 2.0 * float(np.max(red.radius))
-    z = x[2]
-    zlo = -0.5 * lz + dmax
-    zhi = float(np.max(z + r)) - dmax
-    if zhi <= zlo:
-        zlo, zhi = -0.5 * lz, 0.5 * lz
-    sel = (z >= zlo) & (z < zhi)
-    return float(vol[sel].sum() / (lx * ly * (zhi - zlo)))
+def observables(system: System, nlayers: int = 0) -> dict:
+    """dem_observables of the system's current state."""
+    return system.observables(nlayers)
+
+
+def packing_fraction(system: System) -> float:
+    return system.observables()["packing_fraction"]

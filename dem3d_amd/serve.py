@@ -3,13 +3,17 @@
 Requests (workloads.Request: arrival time, size, steps, contact parameters)
 arrive on their own clock.  The server admits a waiting request as soon as
 its estimated device footprint fits the budget next to the systems already
-resident, builds and starts it on its own stream (upload + Euler start-up)
-while the batch keeps running, and then adds it to the batch.  The batch
-advances all resident systems together in chunks of at most `chunk` steps
-(bounded by the fewest remaining steps, so a finished system leaves at once
-and its memory is freed for the next admission).  Every system's state
-evolves exactly as it would alone (the batched kernels run the same bodies
-with the same arguments), which tests/test_gpu_serve.py checks.
+resident.  A worker thread builds and starts each admitted request -- the
+packing on the host, then upload and Euler start-up on the system's own
+stream -- while the main thread keeps stepping the batch, so the device
+runs both at once; a built system joins the batch between two chunks.  The
+batch advances all resident systems together in chunks of at most `chunk`
+steps (bounded by the fewest remaining steps, so a finished system leaves at
+once and its memory is freed for the next admission).  A system in which
+particles escape stops there and leaves with the steps it did.  Every
+system's state evolves exactly as it would alone (the batched kernels run
+the same bodies with the same arguments), which tests/test_gpu_serve.py and
+tests/test_gpu_ensemble_serving.py check.
 
 Reported: per-step device time of the batch (p50/p99), per-request waiting
 time (arrival to admission) and completion latency (arrival to the end of
@@ -18,6 +22,7 @@ its last step), and the aggregate particle-steps/s over the run.
 from __future__ import annotations
 
 import time
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Callable, List, Optional
 
@@ -45,8 +50,11 @@ class Record:
     started: float = float("nan")
     finished: float = float("nan")
     remaining: int = 0
+    steps_done: int = 0
+    escaped: int = 0
     system: Optional[System] = None
     result: Optional[dict] = None
+    build: Optional[Future] = None
 
 
 @dataclass
@@ -98,9 +106,11 @@ def serve(lib: Library, trace: List[workloads.Request], precision: int = DEM_MIX
     is destroyed."""
     recs = [Record(q, k) for k, q in enumerate(sorted(trace, key=lambda q: q.arrival))]
     pending = list(recs)
+    building: List[Record] = []
     active: List[Record] = []
     batch = Batch(lib)
     batch.set_timing(True)
+    builder = ThreadPoolExecutor(max_workers=1)
     step_ms: List[np.ndarray] = []
     used = 0
     peak = 0
@@ -109,51 +119,76 @@ def serve(lib: Library, trace: List[workloads.Request], precision: int = DEM_MIX
     psteps = 0.0
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0
+
+    def retire(r: Record, t: float):
+        nonlocal used
+        r.finished = t
+        if keep is not None:
+            r.result = keep(r)
+        batch.remove(r.system)
+        r.system.close()
+        r.system = None
+        used -= estimate_device_bytes(r.request.n)
+
     try:
-        while pending or active:
-            # admit everything that has arrived and fits
+        while pending or building or active:
+            # admit everything that has arrived and fits: its build starts on
+            # the worker thread
             while pending and (not realtime or pending[0].request.arrival <= now()):
                 need = estimate_device_bytes(pending[0].request.n)
-                if active and used + need > budget_bytes:
+                if (active or building) and used + need > budget_bytes:
                     break
                 r = pending.pop(0)
                 r.admitted = now()
-                r.system = build_system(lib, r.request, precision)
                 r.remaining = r.request.steps
-                batch.add(r.system)
-                r.started = now()
-                active.append(r)
+                r.build = builder.submit(build_system, lib, r.request, precision)
+                building.append(r)
                 used += need
                 peak = max(peak, used)
+            # built systems join the batch between chunks; with nothing to
+            # step, wait for the next build to finish
+            if not active and building:
+                building[0].build.result()
+            for r in [q for q in building if q.build.done()]:
+                r.system = r.build.result()
+                r.build = None
+                batch.add(r.system)
+                r.started = now()
+                building.remove(r)
+                active.append(r)
             if not active:
-                if pending:  # idle until the next arrival
+                if pending and not building:  # idle until the next arrival
                     time.sleep(max(0.0, min(0.05, pending[0].request.arrival - now())))
                 continue
             max_res = max(max_res, len(active))
             k = min(chunk, min(r.remaining for r in active))
-            n_part = sum(r.request.n for r in active)
-            done, esc = batch.step(k)
+            members = list(batch.systems)
+            done, esc = batch.step(k, allow_escape=True)
             chunks += 1
             step_ms.append(batch.step_times())
-            psteps += float(n_part) * k
+            by_sys = {id(sy): (int(d), int(e)) for sy, d, e in zip(members, done, esc)}
             t = now()
             still = []
             for r in active:
-                r.remaining -= k
-                if r.remaining <= 0:
-                    r.finished = t
-                    if keep is not None:
-                        r.result = keep(r)
-                    batch.remove(r.system)
-                    r.system.close()
-                    r.system = None
-                    used -= estimate_device_bytes(r.request.n)
+                d, e = by_sys[id(r.system)]
+                psteps += float(r.request.n) * d
+                r.remaining -= d
+                r.steps_done += d
+                if e > 0:
+                    r.escaped = e
+                if r.remaining <= 0 or e > 0:
+                    retire(r, t)
                 else:
                     still.append(r)
             active = still
             if log is not None:
-                log(f"t={t:.2f}s chunk={k} resident={len(active)} pending={len(pending)}")
+                log(f"t={t:.2f}s chunk={k} resident={len(active)} building={len(building)} "
+                    f"pending={len(pending)}")
     finally:
+        builder.shutdown(wait=True)
+        for r in building:
+            if r.build is not None and r.build.done() and r.build.exception() is None:
+                r.build.result().close()
         batch.close()
     wall = now()
     sm = np.concatenate(step_ms) if step_ms else np.zeros(0)

@@ -134,6 +134,46 @@ def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000
                          f"Lz={lz*1e3:.2f} mm")
 
 
+def poly_loose_packed(lib, seed: int = 20240002, steps: int = 5000, n: int = 200_000,
+                      nbins: int = 16, phi: float = 0.45, lxy: float = 40e-3,
+                      log=None) -> Workload:
+    """The settling bed: n polydisperse spheres, radii U[0.4, 0.6] mm (seeded)
+    discretised into nbins equal-width components, 2500 kg/m^3, kn = 1e3 N/m,
+    e = 0.9, mu = 0.5, in a box periodic in x and y (side lxy) with a floor.
+    The spheres are first grown on the GPU (dem3d_amd/packing.py) into a
+    random loose packing at volume fraction phi, held between the floor and
+    a lid at the height that gives phi and without gravity; the returned
+    workload has no lid and gravity on, so the packing settles."""
+    from .packing import Growth, grow, lattice_start
+
+    kn = 1e3
+    rng = np.random.default_rng(seed)
+    edges = np.linspace(0.4e-3, 0.6e-3, nbins + 1)
+    centres = 0.5 * (edges[1:] + edges[:-1])
+    comp = np.clip(np.searchsorted(edges, rng.uniform(0.4e-3, 0.6e-3, n)) - 1, 0, nbins - 1)
+    vol = float(np.sum(4.0 / 3.0 * math.pi * centres[comp] ** 3))
+    h = vol / (phi * lxy * lxy)
+    lz = 2.0 * h + 4.0e-3
+    floor_z = -0.5 * lz
+    # growth: the bed region between the floor and a lid
+    walls = _floor(lxy, lxy, lz, kn, top=floor_z + h)
+    walls[1].u, walls[1].w = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
+    gspec = _spec((lxy, lxy, lz), list(2.0 * centres), 2500.0, kn, 0.9, 0.5, walls=walls,
+                  total_steps=steps)
+    g = Growth(phi=phi, seed=seed, log=log)
+    c1 = (comp + 1).astype(np.int32)
+    x0 = lattice_start(gspec, n, floor_z, h, g, c1)
+    x, v, w = grow(lib, gspec, c1, x0, g, e=0.9, mu=0.5)
+    spec = _spec((lxy, lxy, lz), list(2.0 * centres), 2500.0, kn, 0.9, 0.5,
+                 walls=_floor(lxy, lxy, lz, kn), total_steps=steps)
+    red = reduce_spec(spec)
+    L, T = red.length_scale, red.time_scale
+    return Workload(f"poly-loose-{n}-phi{phi:.2f}", spec, red, c1, x / L, v * (T / L), w * T,
+                    steps, note=(f"random loose packing grown to phi={phi} (seed {seed}, "
+                                 f"dem3d_amd/packing.py) in a {lxy * 1e3:.0f} mm box, bed "
+                                 f"{h * 1e3:.2f} mm, lid removed, settling under gravity"))
+
+
 def _fcc(nx, ny, nz, a):
     base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
     ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")

@@ -3,16 +3,23 @@
 * 2 000 monodisperse spheres settling (mono_settling_2k): the GPU against
   the reference's own output for the start-up and 60 steps
   (tests/golden/ref_mono_settling_2k.npz, written by the reference).
-* 200 000 polydisperse spheres in 16 components (poly_settling_200k): from a
-  state with contacts and history that the GPU reached, one evaluation and
-  three ABM steps against the oracle (FP64 bar), and the mixed mode by
-  shadow evaluation -- at each of three successive FP64 states, the mixed
-  forces of the same state against the oracle's (mixed bar).
-* 4 000 000 spheres of the sheared bed (bidisperse_sheared_4m), too large
-  for the oracle: the pair contacts the force pass counts at the current
-  positions equal the overlapping pairs an independent k-d tree search finds
-  on the downloaded positions with the same overlap test
-  (MinDijSq - DijSq > 0, minimum image in x and y).
+* 200 000 polydisperse spheres in 16 components, grown into a random loose
+  packing at volume fraction 0.45 and settling under gravity
+  (poly_loose_packed): from a state with contacts and history that the GPU
+  reached, one evaluation and three ABM steps against the oracle (FP64 bar);
+  300 steps compared with the oracle after every step (accelerations and
+  kinetic energy), the oracle running six 50-step windows from the GPU's
+  checkpoints in parallel threads; and the mixed mode by shadow evaluation --
+  at each of three successive FP64 states, the mixed forces of the same state
+  against the oracle's (mixed bar).
+* the dense sheared bed, packed disordered at volume fraction 0.60
+  (packed_bed): at 960 000 spheres, one updating evaluation and three ABM
+  steps against the oracle from the same state -- reference cells, contact
+  keys, Ncol and history bit for bit, accelerations within 1e-12 of each
+  particle's force scale; at the 4 000 000 spheres of the benchmark, too
+  large for the oracle, the pair contacts of the force pass and of
+  dem_observables equal the overlapping pairs an independent k-d tree search
+  finds on the downloaded positions with the same overlap test.
 """
 from __future__ import annotations
 
@@ -66,16 +73,16 @@ def test_config0_mono_2k_against_reference_fixture(hip):
 
 @pytest.fixture(scope="module")
 def poly200k(hip):
-    wl = workloads.poly_settling_200k()
+    wl = workloads.poly_loose_packed(hip)
     g = System(hip, wl.red)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
-    # settle until the bed has a few thousand contacts with history (within
-    # the configuration's 5 000 steps)
-    for _ in range(5):
-        g.step(1000)
-        if g.stats()["pair_contacts"] > 2000:
-            break
+    # some settling, so that contacts carry history (within the
+    # configuration's 5 000 steps)
+    g.step(200)
+    ob = g.observables()
+    print(f"poly 200k: phi {ob['packing_fraction']:.4f}, {ob['pair_contacts']} pairs", flush=True)
+    assert 0.40 < ob["packing_fraction"] < 0.50
     return wl, g
 
 
@@ -112,6 +119,75 @@ def test_config1_poly_200k_fp64_against_oracle(hip, poly200k):
     g.close()
 
 
+def test_config1_every_step_for_300_steps(hip, poly200k):
+    """Six windows of 50 steps.  The GPU steps one step at a time, keeping the
+    accelerations of every step's first evaluation and the kinetic energy;
+    from each window's starting state the oracle takes the same 50 steps in
+    a thread of its own.  After every step: on a window's first step
+    (identical input) accelerations within 1e-12 of each particle's force
+    scale; on the later ones, where the two trajectories separate by
+    rounding, within 1e-10 of the largest force scale of the system; the
+    kinetic energy within 1e-12 and 1e-10 relative; at a window's end the
+    contact sets are identical."""
+    import threading
+    wl, g0 = poly200k
+    im, ii = _inv(wl.comp, wl.red)
+    nwin, nstep = 6, 50
+    out = [None] * nwin
+
+    def oracle_window(w, st, con, a, al, ke, keys):
+        try:
+            o = oracle_system(wl.red)
+            upload_same([o], wl, state=st, contacts=con)
+            worst = [0.0, 0.0, 0.0, 0.0]  # first-step a, later a, first-step ke, later ke
+            for k in range(nstep):
+                o.step(1)
+                so = o.download()
+                fs, ts = force_scale(o)
+                if k == 0:
+                    _, ra = accel_tolerance_ok(a[k], so["a"], fs, im, 1.0)
+                    _, rb = accel_tolerance_ok(al[k], so["alpha"], ts + 1e-300, ii, 1.0)
+                else:
+                    ra = np.abs(a[k] - so["a"]).max() / (fs * im).max()
+                    rb = np.abs(al[k] - so["alpha"]).max() / (ts * ii).max()
+                ko = o.stats()["kinetic_energy"]
+                q = 0 if k == 0 else 1
+                worst[q] = max(worst[q], ra, rb)
+                worst[2 + q] = max(worst[2 + q], abs(ke[k] - ko) / ko)
+            same = bool(np.array_equal(o.download_contacts()["keys"], keys))
+            o.close()
+            out[w] = (worst, same)
+        except BaseException as exc:  # reported by the main thread
+            out[w] = exc
+
+    threads = []
+    for w in range(nwin):
+        st = full_state(g0, with_rank=True)
+        con = g0.download_contacts()
+        a, al, ke = [], [], []
+        for _ in range(nstep):
+            g0.step(1)
+            s = g0.download()
+            a.append(s["a"])
+            al.append(s["alpha"])
+            ke.append(g0.stats()["kinetic_energy"])
+        keys = g0.download_contacts()["keys"]
+        t = threading.Thread(target=oracle_window, args=(w, st, con, a, al, ke, keys))
+        t.start()
+        threads.append(t)
+        print(f"window {w}: GPU done, oracle started", flush=True)
+    for t in threads:
+        t.join()
+    for w, r in enumerate(out):
+        assert not isinstance(r, BaseException), r
+        worst, same = r
+        print(f"window {w}: accelerations {worst[0]:.3g} / {worst[1]:.3g}, kinetic energy "
+              f"{worst[2]:.3g} / {worst[3]:.3g} (first step / later steps)", flush=True)
+        assert worst[0] <= 1e-12 and worst[1] <= 1e-10, (w, worst)
+        assert worst[2] <= 1e-12 and worst[3] <= 1e-10, (w, worst)
+        assert same, w
+
+
 def test_config1_poly_200k_mixed_shadow_evaluation(hip, poly200k):
     wl, g0 = poly200k
     im, ii = _inv(wl.comp, wl.red)
@@ -127,14 +203,12 @@ def test_config1_poly_200k_mixed_shadow_evaluation(hip, poly200k):
         sm, so = m.download(), o.download()
         fs, ts = force_scale(o)
         ok_a, ra = accel_tolerance_ok(sm["a"], so["a"], fs, im, 1e-5)
-        # torques against the particle's radius times its force scale (a
-        # contact's torque is r x F_T): the mixed bar of forces and torques.
-        # Against sum r |F_T| alone a nearly head-on new contact, whose tiny
-        # tangential velocity is a difference of FP32 vectors, sets the scale.
-        ok_b, rb = accel_tolerance_ok(sm["alpha"], so["alpha"], rad * fs, ii, 1e-5)
-        _, rb_t = accel_tolerance_ok(sm["alpha"], so["alpha"], ts + 1e-300, ii, 1e-5)
+        # torques against sum r |F_T| of the particle's contacts (the oracle's
+        # torque scale), and for information against r times its force scale
+        ok_b, rb = accel_tolerance_ok(sm["alpha"], so["alpha"], ts + 1e-300, ii, 1e-5)
+        _, rb_f = accel_tolerance_ok(sm["alpha"], so["alpha"], rad * fs, ii, 1e-5)
         print(f"state {k}: force error {ra:.3g}, torque error {rb:.3g} of the bar "
-              f"({rb_t:.3g} against sum r|F_T|)", flush=True)
+              f"({rb_f:.3g} against r times the force scale)", flush=True)
         assert ok_a, f"state {k}: acceleration error {ra:.3g} x tolerance"
         assert ok_b, f"state {k}: angular acceleration error {rb:.3g} x tolerance"
         assert m.stats()["pair_contacts"] == o.stats()["pair_contacts"], k
@@ -142,34 +216,64 @@ def test_config1_poly_200k_mixed_shadow_evaluation(hip, poly200k):
     m.close()
 
 
-def test_config2_sheared_4m_pair_count_against_kdtree(hip):
-    from scipy.spatial import cKDTree
-    wl = workloads.bidisperse_sheared_4m()
-    red = wl.red
-    g = System(hip, red)
+def test_config2_disordered_960k_against_oracle(hip):
+    wl = workloads.packed_bed(hip, n=960_000, settle_steps=200)
+    g0 = System(hip, wl.red)
+    g0.upload(wl.comp, wl.x, wl.v, wl.w)
+    g0.startup()
+    g0.step(10)
+    st = full_state(g0, with_rank=True)
+    con = g0.download_contacts()
+    g0.close()
+    print(f"packed 960k: {len(con['keys'])} stored pairs, {len(con['wkeys'])} wall contacts",
+          flush=True)
+    assert len(con["keys"]) > wl.np
+    g, o = System(hip, wl.red), oracle_system(wl.red)
+    upload_same([g, o], wl, state=st, contacts=con)
+    for s in (g, o):
+        s.compute_accelerations(0, True)
+    sg, so = g.download(), o.download()
+    fs, ts = force_scale(o)
+    im, ii = _inv(wl.comp, wl.red)
+    ok_a, ra = accel_tolerance_ok(sg["a"], so["a"], fs, im, 1e-12)
+    ok_b, rb = accel_tolerance_ok(sg["alpha"], so["alpha"], ts + 1e-300, ii, 1e-12)
+    print(f"960k evaluation: acceleration error {ra:.3g}, angular {rb:.3g} of the bar", flush=True)
+    assert ok_a and ok_b, (ra, rb)
+    assert np.array_equal(g.download_cells(), o.download_cells())
+    compare_contacts(g.download_contacts(), o.download_contacts(), exact=True)
+    assert g.stats()["ncol"] == o.stats()["ncol"]
+    # three ABM steps from the same state
+    upload_same([g, o], wl, state=st, contacts=con)
+    ncol0 = [s.stats()["ncol"] for s in (g, o)]
+    for s in (g, o):
+        s.step(3)
+    sg, so = g.download(), o.download()
+    for k, rel in (("x", 1e-12), ("v", 1e-10), ("w", 1e-10)):
+        assert _close(sg[k], so[k], rel), k
+    assert np.array_equal(g.download_cells(), o.download_cells())
+    cg, co = g.download_contacts(), o.download_contacts()
+    assert np.array_equal(cg["keys"], co["keys"])
+    assert np.array_equal(cg["wkeys"], co["wkeys"])
+    for k in ("ft", "vt"):
+        assert _close(cg[k], co[k], 1e-9), k
+    assert g.stats()["ncol"] - ncol0[0] == o.stats()["ncol"] - ncol0[1]
+    g.close()
+    o.close()
+
+
+def test_config2_packed_4m_pair_count_against_kdtree(hip):
+    from obs_ref import overlapping_pairs
+    wl = workloads.packed_bed(hip, n=4_000_000, settle_steps=200)
+    g = System(hip, wl.red)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
     g.step(20)
     g.compute_accelerations(0, False)
     p_gpu = g.stats()["pair_contacts"]
-    x = g.download()["x"]
-    print(f"sheared 4M: {p_gpu} pairs on the GPU", flush=True)
-    L = np.asarray(red.lbox, np.float64)
-    box = np.array([L[0], L[1], 4.0 * L[2]])
-    pts = np.stack([np.mod(x[0] + 0.5 * L[0], L[0]), np.mod(x[1] + 0.5 * L[1], L[1]),
-                    x[2] + 1.5 * L[2]], axis=1)
-    pts = np.minimum(pts, np.nextafter(box, 0))
-    rmax = float(np.max(red.radius))
-    pairs = cKDTree(pts, boxsize=box).query_pairs(2.0 * rmax * 1.001, output_type="ndarray")
-    i, j = pairs[:, 0], pairs[:, 1]
-    r = x[:, j] - x[:, i]
-    for d in range(2):  # minimum image, periodic x and y (invl = 1/L as the kernels use)
-        if red.pbc[d]:
-            r[d] = r[d] - L[d] * np.trunc(r[d] * (1.0 / L[d]) + np.copysign(0.5, r[d]))
-    dsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2]
-    c = wl.comp - 1
-    mindijsq = np.asarray(red.min_dij_sq).reshape(red.nc, red.nc)[c[i], c[j]]
-    p_tree = int(np.count_nonzero(mindijsq - dsq > 0.0))
-    assert p_gpu == p_tree, (p_gpu, p_tree)
-    assert p_gpu > 3 * wl.np // 2
+    p_obs = g.observables()["pair_contacts"]
+    st = g.download()
     g.close()
+    print(f"packed 4M: {p_gpu} pairs on the GPU", flush=True)
+    p_tree = overlapping_pairs(wl.red, st["x"], st["comp"])
+    assert p_gpu == p_tree == p_obs, (p_gpu, p_tree, p_obs)
+    assert p_gpu > wl.np // 2

@@ -1,9 +1,10 @@
 """The mixed-precision mode (DEM_MIXED) against the FP64 oracle.
 
 Bar: contact sets identical to the FP64 ones (the overlap test stays FP64);
-per-particle forces and torques within 1e-5 of the particle's force scale
-(|m_eff g| + sum of |F_N| + |F_T| over its contacts, from the oracle); after
-a start-up and 200 ABM steps, kinetic energy within 1e-4 relative and the
+per-particle forces within 1e-5 of the particle's force scale
+(|m_eff g| + sum of |F_N| + |F_T| over its contacts, from the oracle) and
+torques within 1e-5 of the particle's torque scale (sum of r |F_T|); after a
+start-up and 200 ABM steps, kinetic energy within 1e-4 relative and the
 packing fraction within 1e-4 absolute of the FP64 oracle's."""
 from __future__ import annotations
 
@@ -12,7 +13,7 @@ import pytest
 
 from dem3d_amd import workloads
 from dem3d_amd.native import DEM_MIXED, System
-from dem3d_amd.observables import packing_fraction
+from obs_ref import packing_fraction
 from checkers import oracle_system
 from helpers import accel_tolerance_ok, force_scale, full_state, upload_same
 
@@ -76,8 +77,8 @@ def test_mixed_trajectory_energy_and_packing(hip, case):
     kg, ko = g.stats()["kinetic_energy"], o.stats()["kinetic_energy"]
     assert abs(kg - ko) <= KE_TOL * abs(ko), (kg, ko)
     sg, so = g.download(), o.download()
-    pg = packing_fraction(wl.red, sg["x"], wl.comp)
-    po = packing_fraction(wl.red, so["x"], wl.comp)
+    pg = g.observables()["packing_fraction"]   # on the GPU
+    po = packing_fraction(wl.red, so["x"], wl.comp)[0]
     assert abs(pg - po) <= PHI_TOL, (pg, po)
     # the bulk state stays close; contacts may differ only by pairs at the edge
     pc_g, pc_o = g.stats()["pair_contacts"], o.stats()["pair_contacts"]
@@ -98,8 +99,7 @@ def test_mixed_run_to_run_bitwise_deterministic(hip):
 
 
 def test_mixed_state_round_trip_is_exact(hip):
-    """The mixed mode keeps VT in FP32 on the device: a contact list
-    downloaded (converted to FP64) and uploaded again (converted back) into a
+    """A contact list of the mixed mode downloaded and uploaded again into a
     fresh mixed system reads back bit for bit, and the two systems continue
     together (the new one re-sorts its particles and rebuilds its lists, so
     its sums run in another order: within round-off, not bitwise)."""
@@ -143,8 +143,8 @@ def test_mixed_1000_steps_packed_bed(hip):
         s.startup()
         s.step(1000)
     kg, ko = g.stats()["kinetic_energy"], o.stats()["kinetic_energy"]
-    pg = packing_fraction(wl.red, g.download()["x"], wl.comp)
-    po = packing_fraction(wl.red, o.download()["x"], wl.comp)
+    pg = g.observables()["packing_fraction"]
+    po = packing_fraction(wl.red, o.download()["x"], wl.comp)[0]
     zg = coordination_number(g.stats()["pair_contacts"], wl.np)
     zo = coordination_number(o.stats()["pair_contacts"], wl.np)
     print(f"mixed packed bed: dKE {abs(kg - ko) / ko:.3g} dphi {abs(pg - po):.3g} Z {zg:.4f}/{zo:.4f}",

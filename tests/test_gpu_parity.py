@@ -249,7 +249,8 @@ def test_1000_steps_trajectory_and_observables(hip, case):
     values, kinetic energy within 1e-10 relative, the same pair-contact count
     (so the same coordination number 2P/N), packing fraction within 1e-12,
     and identical contact sets."""
-    from dem3d_amd.observables import coordination_number, packing_fraction
+    from dem3d_amd.observables import coordination_number
+    from obs_ref import packing_fraction
     if case == "settling-2k":
         wl = workloads.mono_settling_2k()
     elif case == "settled-bed":   # packs onto the floor: ~550 lasting pair contacts
@@ -267,8 +268,8 @@ def test_1000_steps_trajectory_and_observables(hip, case):
     kg, ko = g.stats()["kinetic_energy"], o.stats()["kinetic_energy"]
     pc_g, pc_o = g.stats()["pair_contacts"], o.stats()["pair_contacts"]
     zg, zo = coordination_number(pc_g, wl.np), coordination_number(pc_o, wl.np)
-    pg = packing_fraction(wl.red, sg["x"], wl.comp)
-    po = packing_fraction(wl.red, so["x"], wl.comp)
+    pg = g.observables()["packing_fraction"]   # on the GPU
+    po = packing_fraction(wl.red, so["x"], wl.comp)[0]
     keys_g, keys_o = g.download_contacts()["keys"], o.download_contacts()["keys"]
     print(f"{case}: dx {dx:.3g} dv {dv:.3g} dKE {abs(kg - ko) / ko:.3g} Z {zg:.6f}/{zo:.6f} "
           f"phi {pg:.12f}/{po:.12f} stored pairs {len(keys_g)}/{len(keys_o)}", flush=True)
