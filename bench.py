@@ -72,6 +72,49 @@ def cpu_baseline(wl, steps):
     }
 
 
+def cpu_configs(args):
+    """The reference timed on this host's cores alone (no GPU needed when the
+    200 k packing comes from --poly-cache): the 2 000-sphere settling case over
+    its full 20 000 steps and the 200 000-sphere loose packing over
+    --cpu-poly-steps steps, both from a zero ABM ring, one core each, with
+    the -march=native build when it exists (oracle/build_ref.sh with
+    DEM3D_REF_NATIVE=1), else the -O3 AVX2/FMA one.  One JSON line each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checkers import ref_available, run_reference
+    from dem3d_amd import workloads
+    native = ref_available(native=True)
+    if not native and not ref_available(fast=True):
+        print("bench.py: the reference timing build (oracle/_ref) is absent", file=sys.stderr)
+        return 2
+    build = ("amdflang -O3 -march=native" if native else "amdflang -O3 -march=x86-64-v3")
+
+    def time_case(wl, steps):
+        res = run_reference(dict(red=wl.red, comp=wl.comp, x=wl.x, v=wl.v, w=wl.w,
+                                 fast=not native, native=native,
+                                 actions=[("time", 1, steps)]), timeout=3600, method="spawn")
+        sec = res["seconds"]
+        print(json.dumps({"workload": wl.name, "spheres": wl.np, "steps": steps,
+                          "seconds": sec, "value": wl.np * steps / sec,
+                          "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+                          "build": build + " (MaxNeighbours/MaxNCell raised)",
+                          "cpu": cpu_model()}), flush=True)
+
+    time_case(workloads.mono_settling_2k(), 20000)
+    state = None
+    if args.poly_cache and os.path.exists(args.poly_cache):
+        with np.load(args.poly_cache) as z:
+            state = (z["x"], z["v"], z["w"])
+    lib = None
+    if state is None:
+        from dem3d_amd.native import load_hip
+        lib = load_hip()
+    poly = workloads.poly_loose_packed(lib, state=state)
+    if args.poly_cache and state is None:
+        np.savez(args.poly_cache, x=poly.x, v=poly.v, w=poly.w)
+    time_case(poly, args.cpu_poly_steps)
+    return 0
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -276,6 +319,14 @@ def main(argv=None):
     ap.add_argument("--crystal-steps", type=int, default=500,
                     help="timed steps of the crystal leg (0 = skip it)")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="only time the reference on the 2 000-sphere settling case (20 000 "
+                         "steps) and the 200 000-sphere loose packing (--cpu-poly-steps) on "
+                         "this host, one JSON line each")
+    ap.add_argument("--cpu-poly-steps", type=int, default=100)
+    ap.add_argument("--poly-cache", default="",
+                    help="npz file holding the 200 000-sphere loose packing: loaded when it "
+                         "exists, else grown on the GPU and written")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0)
     ap.add_argument("--precision", choices=("fp64", "both"), default="both",
@@ -315,6 +366,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return 0
+    if args.cpu_configs:
+        return cpu_configs(args)
     if args.gpus != 1:
         print(f"bench.py: the DEM path runs on one GPU; --gpus {args.gpus} is not supported "
               f"(a multi-GPU run would decompose the box into slabs, one per GPU)",

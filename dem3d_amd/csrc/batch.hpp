@@ -131,7 +131,8 @@ __global__ __launch_bounds__(1024) void k_pass_reduce_batch(const BatchRec* __re
                                                             int pi, int update) {
   const BatchRec& r = rec[blockIdx.x];
   const PassArgs& a = r.p[pi];
-  pass_reduce_body(a.ctl, a.part, min(r.pass_nblk, DEM_RED_SLOTS), a.par, update);
+  pass_reduce_body(a.ctl, a.part, min(r.pass_nblk, DEM_RED_SLOTS), a.par, update, r.s, a.kin_in,
+                   r.r.es.st);
 }
 
 // One block per system.
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill_batch(const BatchRec* __
 
 // One block per system: the flip of the rebuild, then the arguments of the
 // system's next step.
-__global__ void k_nbr_flip_batch(BatchRec* __restrict__ rec) {
+__global__ __launch_bounds__(DEM_MERGE_THREADS) void k_nbr_flip_batch(BatchRec* __restrict__ rec) {
   BatchRec& b = rec[blockIdx.x];
   nbr_flip_body(b.r.es, b.r.gate, 0, b.r.ctl);
   __syncthreads();

@@ -204,6 +204,8 @@ static void free_state(dem_handle_t h) {
     dfree(h->wflag[k]); dfree(h->whist[k]);
   }
   dfree(h->es.epar);
+  dfree(h->es.st.key); dfree(h->es.st.slot); dfree(h->es.st.hist); dfree(h->es.st.state);
+  dfree(h->es.st.cnt);
   dfree(h->d_es);
   dfree(h->acc); dfree(h->acc_p); dfree(h->ring); dfree(h->ring2); dfree(h->psum); dfree(h->id); dfree(h->id2);
   dfree(h->xref); dfree(h->key); dfree(h->cell_count);
@@ -613,6 +615,11 @@ static int alloc_state(dem_handle_t h, int np) {
     if (e == hipSuccess) e = dalloc(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
   }
   if (e == hipSuccess) e = dalloc(&h->es.epar, 1);
+  if (e == hipSuccess) e = dalloc(&h->es.st.key, DEM_STALE_ROWS);
+  if (e == hipSuccess) e = dalloc(&h->es.st.slot, DEM_STALE_ROWS);
+  if (e == hipSuccess) e = dalloc(&h->es.st.hist, 3 * DEM_STALE_ROWS);
+  if (e == hipSuccess) e = dalloc(&h->es.st.state, DEM_STALE_ROWS);
+  if (e == hipSuccess) e = dalloc(&h->es.st.cnt, 3);
   if (e == hipSuccess) e = dalloc(&h->d_es, 1);
   if (e == hipSuccess) e = dalloc(&h->acc, 3 * N);
   if (e == hipSuccess) e = dalloc(&h->acc_p, 3 * N);
@@ -645,6 +652,9 @@ static int alloc_state(dem_handle_t h, int np) {
                 hipGetErrorString(e));
   }
   HIPCHK(h, hipMemcpyAsync(h->d_es, &h->es, sizeof(EdgeSets), hipMemcpyHostToDevice, h->stream));
+  // an empty stale-pair table (also for a state of no particles, whose
+  // upload clears nothing else)
+  HIPCHK(h, hipMemsetAsync(h->es.st.cnt, 0, 3 * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(h->acc_p, 0, 3 * N * sizeof(double2), h->stream));
   HIPCHK(h, hipMemsetAsync(h->cell_count, 0, ((size_t)h->gncell + 1) * sizeof(int), h->stream));
   h->np = np;
@@ -662,6 +672,7 @@ static int clear_contacts(dem_handle_t h) {
     HIPCHK(h, hipMemsetAsync(h->wflag[k], 0, N * sizeof(unsigned), h->stream));
   }
   HIPCHK(h, hipMemsetAsync(h->es.epar, 0, sizeof(int), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->es.st.cnt, 0, 3 * sizeof(int), h->stream));
   return DEM_OK;
 }
 
@@ -710,7 +721,8 @@ static void enqueue_rebuild(dem_handle_t h, int b, const int* gate, int force,
   hipLaunchKernelGGL(k_nbr_fill, dim3(nb), dim3(DEM_BLOCK), 0, st, s, pos, h->cell_start,
                      h->cell_idx, h->es, src, o2n, (const Kin*)h->kin[b], h->xref, gate, force,
                      h->ctl);
-  hipLaunchKernelGGL(k_nbr_flip, dim3(1), dim3(64), 0, st, h->es, gate, force, h->ctl);
+  hipLaunchKernelGGL(k_nbr_flip, dim3(1), dim3(DEM_MERGE_THREADS), 0, st, h->es, gate, force,
+                     h->ctl, o2n);
 }
 
 // Re-sort the particle arrays of Particle (cur) into neighbour-grid cell
@@ -1388,12 +1400,12 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
     kv[k] = {keys[k], k};
   }
   std::sort(kv.begin(), kv.end());
-  size_t used = 0;
+  std::vector<unsigned char> listed(kv.size(), 0);
   for (size_t i = 0; i < N; ++i) {
     for (int m = 0; m < cnt_deg(he.cnt[i]); ++m) {
       const size_t e = he.edge(i, m);
       const int a = idh[i], b = idh[he.nbr[e]];
-      const uint64_t key = ((uint64_t)(std::min(a, b) + 1) << 32) | (uint64_t)(std::max(a, b) + 1);
+      const uint64_t key = pair_key(a, b);
       auto it = std::lower_bound(kv.begin(), kv.end(), std::make_pair(key, (int64_t)-1));
       if (it == kv.end() || it->first != key) continue;
       const int64_t k = it->second;
@@ -1401,15 +1413,33 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
       hist[e] = make_double2(ft[3 * k], ft[3 * k + 1]);
       hist[E + e] = make_double2(ft[3 * k + 2], vt[3 * k]);
       hist[2 * E + e] = make_double2(vt[3 * k + 1], vt[3 * k + 2]);
-      ++used;
+      listed[it - kv.begin()] = 1;
     }
   }
-  // A stored pair farther apart than r_i + r_j + skin has no edge; it is
-  // dropped (counted in uploaded_unlisted).
-  std::vector<unsigned> wf(N, 0u);
-  std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N, 0.0);
   std::vector<int> inv(N);
   for (size_t i = 0; i < N; ++i) inv[idh[i]] = (int)i;
+  // A stored pair farther apart than r_i + r_j + skin has no edge: a row of
+  // the stale-pair table (rows in key order), or, in the canonical history
+  // mode or beyond the table's rows, dropped (counted in uploaded_unlisted).
+  std::vector<unsigned long long> tkey;
+  std::vector<int2> tslot;
+  std::vector<double2> thist;
+  int64_t unlisted = 0;
+  for (size_t r = 0; r < kv.size(); ++r) {
+    if (listed[r]) continue;
+    if (h->hist_mode != 0 || tkey.size() == DEM_STALE_ROWS) {
+      ++unlisted;
+      continue;
+    }
+    const int64_t k = kv[r].second;
+    tkey.push_back(kv[r].first);
+    tslot.push_back(make_int2(inv[(kv[r].first >> 32) - 1], inv[(kv[r].first & 0xffffffffu) - 1]));
+    thist.push_back(make_double2(ft[3 * k], ft[3 * k + 1]));
+    thist.push_back(make_double2(ft[3 * k + 2], vt[3 * k]));
+    thist.push_back(make_double2(vt[3 * k + 1], vt[3 * k + 2]));
+  }
+  std::vector<unsigned> wf(N, 0u);
+  std::vector<double> wh((size_t)std::max(s.nw, 1) * 6 * N, 0.0);
   for (int64_t k = 0; k < nwall; ++k) {
     const long long a = (long long)(wkeys[k] >> 32) - 1, jw = (long long)(wkeys[k] & 0xffffffffu) - 1;
     if (a < 0 || a >= (long long)N || jw < 0 || jw >= s.nw)
@@ -1435,8 +1465,23 @@ int dem_upload_contacts(dem_handle_t h, int64_t npairs, const uint64_t* keys, co
       HIPCHK(h, hipMemcpyAsync(h->whist[0], wh.data(), (size_t)s.nw * 6 * N * sizeof(double),
                                hipMemcpyHostToDevice, st));
   }
+  const int nt = (int)tkey.size();
+  if (nt) {
+    std::vector<double2> planes(3 * (size_t)DEM_STALE_ROWS);
+    for (int r = 0; r < nt; ++r)
+      for (int c = 0; c < 3; ++c) planes[(size_t)c * DEM_STALE_ROWS + r] = thist[3 * (size_t)r + c];
+    HIPCHK(h, hipMemcpyAsync(h->es.st.key, tkey.data(), nt * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->es.st.slot, tslot.data(), nt * sizeof(int2),
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->es.st.hist, planes.data(), planes.size() * sizeof(double2),
+                             hipMemcpyHostToDevice, st));
+    HIPCHK(h, hipMemsetAsync(h->es.st.state, 0, nt, st));
+  }
+  const int tcnt[3] = {nt, 0, 0};
+  HIPCHK(h, hipMemcpyAsync(h->es.st.cnt, tcnt, sizeof tcnt, hipMemcpyHostToDevice, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  h->uploaded_unlisted = (int64_t)npairs - (int64_t)(used / 2);
+  h->uploaded_unlisted = unlisted;
   return DEM_OK;
 }
 
@@ -1499,6 +1544,32 @@ int dem_download_contacts(dem_handle_t h, int64_t cap, int64_t* npairs, uint64_t
     memcpy(x.f, o.ft, sizeof x.f);
     memcpy(x.g, o.vt, sizeof x.g);
     pe.push_back(x);
+  }
+  // the stale pairs the table keeps
+  int tcnt[3] = {0, 0, 0};
+  HIPCHK(h, hipMemcpyAsync(tcnt, h->es.st.cnt, sizeof tcnt, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  if (tcnt[0] > DEM_STALE_ROWS) return fail(h, DEM_ERR_STATE, "stale-pair table corrupt");
+  if (tcnt[0] > 0) {
+    const size_t nt = (size_t)tcnt[0];
+    std::vector<unsigned long long> tk(nt);
+    std::vector<unsigned char> ts(nt);
+    std::vector<double2> th(3 * (size_t)DEM_STALE_ROWS);
+    HIPCHK(h, hipMemcpyAsync(tk.data(), h->es.st.key, nt * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(ts.data(), h->es.st.state, nt, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(th.data(), h->es.st.hist, th.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    for (size_t r = 0; r < nt; ++r) {
+      if (ts[r] != 0) continue;
+      const double2 p0 = th[r], p1 = th[DEM_STALE_ROWS + r], p2 = th[2 * DEM_STALE_ROWS + r];
+      Ent x;
+      x.key = tk[r];
+      x.f[0] = p0.x; x.f[1] = p0.y; x.f[2] = p1.x;
+      x.g[0] = p1.y; x.g[1] = p2.x; x.g[2] = p2.y;
+      pe.push_back(x);
+    }
   }
   auto by_key = [](const Ent& x, const Ent& y) { return x.key < y.key; };
   std::sort(pe.begin(), pe.end(), by_key);
@@ -1621,7 +1692,7 @@ static void launch_pass(dem_handle_t h, const PassArgs& a0) {
   else
     launch_shape<MODE, UPDATE, false>(h->shape, nb, h->stream, h->sc, a);
   hipLaunchKernelGGL(k_pass_reduce, dim3(1), dim3(1024), 0, h->stream, h->ctl, a.part, nb, a.par,
-                     UPDATE ? 1 : 0);
+                     UPDATE ? 1 : 0, h->sc, a.kin_in, h->es.st);
 }
 
 static void after_eval(dem_handle_t h, bool update, int b) {
@@ -2888,7 +2959,7 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
   hipLaunchKernelGGL(k_scan_apply_batch, dim3(gs), dim3(DEM_BLOCK), 0, st, rec,
                      (const int2*)b->d_map[MAP_STILE], 1);
   hipLaunchKernelGGL(k_nbr_fill_batch, dim3(gp), dim3(DEM_BLOCK), 0, st, rec, mp);
-  hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(64), 0, st, rec);
+  hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(DEM_MERGE_THREADS), 0, st, rec);
 }
 
 static void drop_graphs(dem_batch_t b) {

@@ -23,6 +23,9 @@
 //                              reference stores them (TPartner%Vector1/Vector2),
 //                              FP64 in both precision modes
 //   wall history: wflag uint[N] (bit w: contact with wall w), whist [nw][6][N]
+//   stale pairs (StaleTab): stored pairs that have left the neighbour list
+//         while their reference cells are not adjacent, by sorted 64-bit
+//         pair key, with their history (reference semantics, below)
 //
 // A force pass runs one lane per particle.  Each lane walks its own edges in
 // list order (all lanes of a wavefront read their k-th edge together),
@@ -112,6 +115,37 @@ __device__ __forceinline__ int wave_scan_incl(int v) {
   return v;
 }
 
+// Stale pairs.  The reference forgets a stored pair only when the cell loop
+// of an updating ComputeAccelerations visits the pair (reference cells the
+// same or neighbours) and finds no overlap (RemovePairContact,
+// src/MForces.f90:262-270, src/MContacts.f90:427-437).  A pair that leaves
+// the neighbour list while its reference cells are not adjacent therefore
+// keeps its history there, and a later contact starts from it.  The table
+// holds such pairs between rebuilds:
+//   key   u64[DEM_STALE_ROWS]   (id_a + 1) << 32 | (id_b + 1), id_a < id_b;
+//                               rows [0, cnt[0]) sorted ascending
+//   slot  int2[DEM_STALE_ROWS]  the two particles' indices (current order)
+//   hist  double2[3][DEM_STALE_ROWS]  the history as the edges hold it
+//   state u8[DEM_STALE_ROWS]    0 kept; 1 restored into the list by the last
+//                               rebuild; 2 forgotten (visited, no overlap)
+//   cnt   int[3]                rows; rows appended by the running rebuild;
+//                               pairs lost because the table was full
+// A rebuild looks every new edge without carried history up in the table
+// (binary search) and restores a kept row; it appends the stored pairs its
+// old list loses while their reference cells are not adjacent (rows
+// cnt[0] + k); then the flip kernel merges: kept rows and new rows, sorted
+// by key, the others dropped.  The reduction after an updating pass marks
+// the rows whose reference cells are adjacent at that pass as forgotten (the
+// pair is no neighbour, so it does not overlap).
+#define DEM_STALE_ROWS 4096
+struct StaleTab {
+  unsigned long long* key;
+  int2* slot;
+  double2* hist;
+  unsigned char* state;
+  int* cnt;
+};
+
 struct EdgeSets {
   int* cnt[2];
   int* soff[2];
@@ -120,7 +154,25 @@ struct EdgeSets {
   double2* hist[2];  // three planes of ecap entries each
   long long ecap;
   int* epar;  // device word: index of the current set
+  StaleTab st;  // null pointers in the canonical history mode
 };
+
+__host__ __device__ __forceinline__ unsigned long long pair_key(int ida, int idb) {
+  const int lo = ida < idb ? ida : idb, hi = ida < idb ? idb : ida;
+  return ((unsigned long long)(lo + 1) << 32) | (unsigned long long)(hi + 1);
+}
+
+// Row of key in the sorted rows [0, n) of the table, or -1.
+__device__ __forceinline__ int stale_find(const unsigned long long* key, int n,
+                                          unsigned long long k) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && key[lo] == k) ? lo : -1;
+}
 
 // cnt[i] holds the list length of particle i in its low 16 bits and, once
 // the list is filled, the number of front (contact-first) entries above them
@@ -686,19 +738,50 @@ __device__ __forceinline__ void publish_counts(Ctl* ctl, const unsigned long lon
   }
 }
 
+// After an updating pass: the stale pairs whose reference cells are now
+// adjacent were visited by the reference's cell loop without overlapping
+// (they are no neighbours) and are forgotten; returns, on thread 0 of the
+// block (1024 threads), how many rows are kept.
+__device__ __forceinline__ unsigned long long stale_visit(const SysConst& s, const Kin* kin,
+                                                          const StaleTab& T) {
+  __shared__ unsigned long long sh_kept[1024 / 64];
+  const int t = threadIdx.x;
+  const int n = min(T.cnt[0], DEM_STALE_ROWS);
+  unsigned long long kept = 0;
+  for (int k = t; k < n; k += 1024) {
+    if (T.state[k] != 0) continue;
+    const int2 q = T.slot[k];
+    const int4 a = reinterpret_cast<const int4*>(kin + q.x)[3];
+    const int4 b = reinterpret_cast<const int4*>(kin + q.y)[3];
+    if (ref_adjacent(s, a.y, a.w, b.y, b.w)) T.state[k] = 2;
+    else ++kept;
+  }
+  kept = wave_sum_u64(kept);
+  if ((t & 63) == 0) sh_kept[t >> 6] = kept;
+  __syncthreads();
+  kept = 0;
+  if (t == 0)
+    for (int w = 0; w < 1024 / 64; ++w) kept += sh_kept[w];
+  return kept;
+}
+
 // nslots <= DEM_RED_SLOTS counter slots of a pass, summed by one block,
-// cleared and published.
+// cleared and published; after an updating pass, the visit of the stale
+// pairs, whose kept rows count as stored pairs (both directions).
 __device__ __forceinline__ void pass_reduce_body(Ctl* ctl, unsigned* part, int nslots, int par,
-                                                 int update) {
+                                                 int update, const SysConst& sc, const Kin* kin,
+                                                 const StaleTab& T) {
   if (ctl->escaped) return;  // the pass did not run
   const int t = threadIdx.x;
   unsigned long long s[4];
   sum_slots(part, 0, nslots, s);
+  if (update && sc.hist_mode == 0 && T.cnt) s[3] += 2 * stale_visit(sc, kin, T);
   if (t == 0) publish_counts(ctl, s, par, update);
 }
 __global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, unsigned* part, int nblk,
-                                                      int par, int update) {
-  pass_reduce_body(ctl, part, min(nblk, DEM_RED_SLOTS), par, update);
+                                                      int par, int update, SysConst sc,
+                                                      const Kin* kin, StaleTab T) {
+  pass_reduce_body(ctl, part, min(nblk, DEM_RED_SLOTS), par, update, sc, kin, T);
 }
 
 // MoveWalls(Time) (src/MDEM.f90:392-408): every axis of every wall, one
@@ -1057,6 +1140,10 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
       else ++ns;
     }
     unsigned long long found = 0;  // bit m: stored entry m of the old list was carried
+    // rows of the stale-pair table (none in the canonical history mode)
+    const StaleTab& T = E.st;
+    const int tn = (s.hist_mode == 0 && T.cnt) ? min(T.cnt[0], DEM_STALE_ROWS) : 0;
+    const int id_i = tn > 0 ? kin_cur[i].id : 0;
     for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int j, double dsq, int cj) {
       if (kf > kb) return;
       const int jo = src ? src[j] : j;
@@ -1072,18 +1159,29 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
             break;
           }
       }
-      const unsigned char fl = mo >= 0 ? 1 : 0;
+      // a pair that comes back into the list resumes its stale history
+      int tr = -1;
+      if (mo < 0 && tn > 0) {
+        const int r = stale_find(T.key, tn, pair_key(id_i, kin_cur[j].id));
+        if (r >= 0 && T.state[r] != 2) tr = r;
+      }
+      const unsigned char fl = (mo >= 0 || tr >= 0) ? 1 : 0;
       if (mo >= 0) found |= 1ull << mo;
       const bool front = fl || s.pair[ci + cj * s.nc].mindijsq - dsq > 0.0;
       const long long e = e0 + 64LL * (front ? kf++ : kb--);
       E.nbr[q][e] = j;
       E.flag[q][e] = fl;
-      if (fl) {
+      double2* g = E.hist[q];
+      if (mo >= 0) {
         const long long om = ob + 64LL * mo;
-        double2* g = E.hist[q];
         g[e] = ohist[om];
         g[ecap + e] = ohist[ecap + om];
         g[2 * ecap + e] = ohist[2 * ecap + om];
+      } else if (tr >= 0) {
+        g[e] = T.hist[tr];
+        g[ecap + e] = T.hist[DEM_STALE_ROWS + tr];
+        g[2 * ecap + e] = T.hist[2 * DEM_STALE_ROWS + tr];
+        T.state[tr] = 1;  // both edges of the pair restore it
       }
     });
     E.cnt[q][i] = ndeg | (kf << 16);
@@ -1093,9 +1191,26 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
       if (!(oflag[om] & 1)) continue;
       if ((found >> m) & 1ull) continue;
       const int jn = o2n ? o2n[onbr[om]] : onbr[om];
-      if (s.hist_mode == 0 &&
-          !ref_adjacent(s, kin_cur[i].rc, kin_cur[i].rxyz, kin_cur[jn].rc, kin_cur[jn].rxyz))
+      // A stored pair lost from the list: the reference visits it at the
+      // next updating pass, and forgets it, when its cells are adjacent;
+      // otherwise it keeps it, and so does the table (one row per pair,
+      // written by the edge of the lower id).
+      if (s.hist_mode != 0 ||
+          ref_adjacent(s, kin_cur[i].rc, kin_cur[i].rxyz, kin_cur[jn].rc, kin_cur[jn].rxyz))
+        continue;
+      const int idi = kin_cur[i].id, idj = kin_cur[jn].id;
+      if (idi > idj) continue;
+      const int r = T.cnt ? T.cnt[0] + atomicAdd(&T.cnt[1], 1) : DEM_STALE_ROWS;
+      if (r >= DEM_STALE_ROWS) {
         ++dropped;
+        continue;
+      }
+      T.key[r] = pair_key(idi, idj);
+      T.slot[r] = make_int2(i, jn);
+      T.hist[r] = ohist[om];
+      T.hist[DEM_STALE_ROWS + r] = ohist[ecap + om];
+      T.hist[2 * DEM_STALE_ROWS + r] = ohist[2 * ecap + om];
+      T.state[r] = 0;
     }
     xref[i] = P;
   }
@@ -1117,16 +1232,102 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
 }
 
 // Make the freshly built edge set current.
+// The stale-pair table after a rebuild (one block of DEM_MERGE_THREADS):
+// the kept rows (slots through o2n when the particles were re-sorted) and
+// the rows the rebuild appended, sorted by key in LDS (bitonic), restored
+// and forgotten rows dropped.  Keys are unique, so the result does not
+// depend on the order in which the rebuild appended.
+#define DEM_MERGE_THREADS 1024
+__device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o2n) {
+  constexpr int R = DEM_STALE_ROWS, PER = DEM_STALE_ROWS / DEM_MERGE_THREADS;
+  __shared__ unsigned long long sk[R];
+  __shared__ unsigned short si[R];
+  __shared__ int s_live;
+  const int t = threadIdx.x;
+  const int n = min(T.cnt[0], R);
+  const int m = min(T.cnt[1], R - n);
+  const int tot = n + m;
+  if (tot == 0) return;  // the same for every thread of the block
+  int P = 2;
+  while (P < tot) P <<= 1;
+  if (t == 0) s_live = 0;
+  for (int k = t; k < P; k += DEM_MERGE_THREADS) {
+    const bool live = k < tot && (k >= n || T.state[k] == 0);
+    sk[k] = live ? T.key[k] : ~0ull;
+    si[k] = (unsigned short)k;
+  }
+  __syncthreads();
+  for (int w = 2; w <= P; w <<= 1)
+    for (int d = w >> 1; d > 0; d >>= 1) {
+      for (int k = t; k < P; k += DEM_MERGE_THREADS) {
+        const int l = k ^ d;
+        if (l > k) {
+          const bool up = (k & w) == 0;
+          const unsigned long long a = sk[k], b = sk[l];
+          if ((a > b) == up) {
+            sk[k] = b;
+            sk[l] = a;
+            const unsigned short x = si[k];
+            si[k] = si[l];
+            si[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = t; k < P; k += DEM_MERGE_THREADS)
+    if (sk[k] != ~0ull && (k + 1 == P || sk[k + 1] == ~0ull)) s_live = k + 1;
+  __syncthreads();
+  const int live = s_live;
+  unsigned long long key[PER];
+  int2 slot[PER];
+  double2 h0[PER], h1[PER], h2[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int k = t + r * DEM_MERGE_THREADS;
+    if (k < live) {
+      const int q = si[k];
+      key[r] = T.key[q];
+      slot[r] = T.slot[q];
+      if (q < n && o2n) slot[r] = make_int2(o2n[slot[r].x], o2n[slot[r].y]);
+      h0[r] = T.hist[q];
+      h1[r] = T.hist[R + q];
+      h2[r] = T.hist[2 * R + q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int k = t + r * DEM_MERGE_THREADS;
+    if (k < live) {
+      T.key[k] = key[r];
+      T.slot[k] = slot[r];
+      T.hist[k] = h0[r];
+      T.hist[R + k] = h1[r];
+      T.hist[2 * R + k] = h2[r];
+      T.state[k] = 0;
+    }
+  }
+  if (t == 0) {
+    T.cnt[0] = live;
+    T.cnt[1] = 0;
+  }
+}
+
+// The flip of a rebuild and the merge of the stale-pair table it changed.
 __device__ __forceinline__ void nbr_flip_body(const EdgeSets& E, const int* gate, int force,
-                                              Ctl* ctl) {
+                                              Ctl* ctl, const int* o2n = nullptr) {
   if (gate_closed(gate, force, ctl)) return;
   if (threadIdx.x == 0) {
     *E.epar = 1 - *E.epar;
     ctl->rebuilds += 1;
   }
+  if (E.st.cnt) stale_merge_body(E.st, o2n);
 }
-__global__ void k_nbr_flip(EdgeSets E, const int* gate, int force, Ctl* ctl) {
-  nbr_flip_body(E, gate, force, ctl);
+__global__ __launch_bounds__(DEM_MERGE_THREADS) void k_nbr_flip(EdgeSets E, const int* gate,
+                                                                int force, Ctl* ctl,
+                                                                const int* o2n) {
+  nbr_flip_body(E, gate, force, ctl, o2n);
 }
 
 // ---------------------------------------------------------------------------

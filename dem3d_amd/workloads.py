@@ -136,14 +136,16 @@ def poly_settling_200k(seed: int = 20240002, steps: int = 5000, n: int = 200_000
 
 def poly_loose_packed(lib, seed: int = 20240002, steps: int = 5000, n: int = 200_000,
                       nbins: int = 16, phi: float = 0.45, lxy: float = 40e-3,
-                      log=None) -> Workload:
+                      log=None, state=None) -> Workload:
     """The settling bed: n polydisperse spheres, radii U[0.4, 0.6] mm (seeded)
     discretised into nbins equal-width components, 2500 kg/m^3, kn = 1e3 N/m,
     e = 0.9, mu = 0.5, in a box periodic in x and y (side lxy) with a floor.
     The spheres are first grown on the GPU (dem3d_amd/packing.py) into a
     random loose packing at volume fraction phi, held between the floor and
     a lid at the height that gives phi and without gravity; the returned
-    workload has no lid and gravity on, so the packing settles."""
+    workload has no lid and gravity on, so the packing settles.  state =
+    (x, v, w) in reduced units, from an earlier call with the same arguments,
+    skips the growth (lib may then be None)."""
     from .packing import Growth, grow, lattice_start
 
     kn = 1e3
@@ -162,12 +164,15 @@ def poly_loose_packed(lib, seed: int = 20240002, steps: int = 5000, n: int = 200
                   total_steps=steps)
     g = Growth(phi=phi, seed=seed, log=log)
     c1 = (comp + 1).astype(np.int32)
-    x0 = lattice_start(gspec, n, floor_z, h, g, c1)
-    x, v, w = grow(lib, gspec, c1, x0, g, e=0.9, mu=0.5)
     spec = _spec((lxy, lxy, lz), list(2.0 * centres), 2500.0, kn, 0.9, 0.5,
                  walls=_floor(lxy, lxy, lz, kn), total_steps=steps)
     red = reduce_spec(spec)
     L, T = red.length_scale, red.time_scale
+    if state is None:
+        x0 = lattice_start(gspec, n, floor_z, h, g, c1)
+        x, v, w = grow(lib, gspec, c1, x0, g, e=0.9, mu=0.5)
+    else:
+        x, v, w = state[0] * L, state[1] * (L / T), state[2] / T
     return Workload(f"poly-loose-{n}-phi{phi:.2f}", spec, red, c1, x / L, v * (T / L), w * T,
                     steps, note=(f"random loose packing grown to phi={phi} (seed {seed}, "
                                  f"dem3d_amd/packing.py) in a {lxy * 1e3:.0f} mm box, bed "

@@ -160,7 +160,7 @@ typedef struct {
   int64_t steps;            /* MoveParticles steps done                        */
   int64_t rebuilds;         /* neighbour-list rebuilds                         */
   int64_t neighbour_pairs;  /* directed entries in the neighbour list          */
-  int64_t stale_dropped;    /* stale history entries lost at a rebuild         */
+  int64_t stale_dropped;    /* stale pairs lost: stale-pair table (4096) full  */
   int64_t ref_ncell[3];     /* reference grid                                  */
   int64_t nb_ncell[3];      /* neighbour-search grid                           */
   double kinetic_energy;    /* KineticEnergy() (src/MDEM.f90:199-213)          */

@@ -23,6 +23,8 @@ REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref.so")
 REF_LIB_BIG = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_big.so")
 # capacity-raised and built for speed (-O3, AVX2/FMA): timing only
 REF_LIB_FAST = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_fast.so")
+# the same with -march=native for this host (built on request): timing only
+REF_LIB_NATIVE = os.path.join(ROOT, "oracle", "_ref", "libdem3d_ref_native.so")
 
 _oracle = None
 
@@ -38,8 +40,14 @@ def oracle_system(red, **kw) -> System:
     return System(oracle_lib(), red, **kw)
 
 
-def ref_available(big=False, fast=False) -> bool:
-    return os.path.exists(REF_LIB_FAST if fast else (REF_LIB_BIG if big else REF_LIB))
+def _ref_path(big=False, fast=False, native=False) -> str:
+    if native:
+        return REF_LIB_NATIVE
+    return REF_LIB_FAST if fast else (REF_LIB_BIG if big else REF_LIB)
+
+
+def ref_available(big=False, fast=False, native=False) -> bool:
+    return os.path.exists(_ref_path(big, fast, native))
 
 
 # ---------------------------------------------------------------------------
@@ -167,7 +175,7 @@ def _d(a):
 
 def _ref_job_body(job):
     red = job["red"]
-    lib = C.CDLL(REF_LIB_FAST if job.get("fast") else (REF_LIB_BIG if job.get("big") else REF_LIB))
+    lib = C.CDLL(_ref_path(job.get("big", False), job.get("fast", False), job.get("native", False)))
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
     P = lambda a: a.ctypes.data_as(dp)

@@ -78,3 +78,57 @@ def shaken_floor(seed=41):
                             velocity_amplitude=-tpf * amp, two_pi_frequency=tpf,
                             phase_angle=phase)]
     return wl
+
+
+def stale_pair_case(kind: str):
+    """Two spheres (d = 1 mm) in a box periodic in x, y, z of 5.01 x 4.08 x
+    4.08 d, no gravity: 5 reference cells of 1.002 d along x, 4 of 1.02 d
+    across.  Sphere 1 rests just below the top of reference cell 1 along x;
+    sphere 2 moves along x at 0.004 d per time step, a displacement larger
+    than the 0.002 d by which a cell exceeds a diameter, so the pair can
+    pass between overlapping and non-adjacent reference cells in one step.
+
+    kind = "separate": sphere 2 overlaps sphere 1 by 0.001 d with a stored
+    (FT, VT), moving away: one step later the pair is neither overlapping nor
+    in adjacent cells, so the reference keeps its history (stale); it then
+    leaves the neighbour list, travels round the periodic box and becomes
+    adjacent again from the other side, where the reference forgets it.
+
+    kind = "return": the pair starts 2.2 d apart, in cells two apart, with a
+    stored (FT, VT) (a stale pair, as a state file of the reference can hold
+    one); sphere 2 approaches, and one step takes it from cell 3 (not
+    adjacent) into an overlap, so the contact starts from the stale history
+    (no new collision counted).
+
+    Returns (workload, contacts, steps)."""
+    from dem3d_amd import workloads
+    from dem3d_amd.spec import reduce_spec
+    d = 1e-3
+    lx, ly = 5.01 * d, 4.08 * d
+    spec = workloads._spec((lx, ly, ly), [d], 2500.0, 1e3, 0.9, 0.5, pbc=(True, True, True),
+                           gravity=(0.0, 0.0, 0.0))
+    red = reduce_spec(spec)
+    L = red.length_scale
+    dr, Lx, Ly = d / L, lx / L, ly / L
+    w = Lx / 5
+    step = 0.004 * dr
+    xa = -0.5 * Lx + 2 * w - 0.0005 * dr
+    yz = -0.5 * Ly + 1.5 * (Ly / 4)
+    if kind == "separate":
+        xb, vb, steps = xa + 0.999 * dr, step, 800
+    elif kind == "return":
+        xb, vb, steps = -0.5 * Lx + 3 * w + 0.001 * dr + 300 * step, -step, 330
+    else:
+        raise ValueError(kind)
+    x = np.array([[xa, xb], [yz, yz], [yz, yz]])
+    v = np.zeros((3, 2))
+    v[0, 1] = vb / red.time_step
+    kn = float(np.asarray(red.kn)[0, 0])
+    f = 1e-4 * kn * dr
+    contacts = dict(keys=np.array([(1 << 32) | 2], np.uint64),
+                    ft=np.array([[0.0, 0.3 * f, -0.2 * f]]),
+                    vt=np.array([[0.0, 2e-3 * dr, 1e-3 * dr]]),
+                    wkeys=np.zeros(0, np.uint64), wft=np.zeros((0, 3)), wvt=np.zeros((0, 3)))
+    wl = workloads.Workload(f"stale-{kind}", spec, red, np.ones(2, np.int32), x, v,
+                            np.zeros((3, 2)), steps)
+    return wl, contacts, steps
