@@ -18,6 +18,14 @@
 #                                 baseline of the benchmark.  Not bit-exact.
 #  oracle/_ref/libdem3d_ref_native.so  the same with -march=native, only when
 #                                 DEM3D_REF_NATIVE=1 (timings on this host).
+#  oracle/_ref/dem3d_ref          the reference's own program DEM (src/DEM.f90
+#                                 over all of its modules), -O2
+#                                 -ffp-contract=off.
+#  oracle/_ref/dem3d_gpu          the same program, built from the same files
+#                                 except that dem3d_amd/fortran/mforces_gpu.f90
+#                                 takes the place of src/MForces.f90: its
+#                                 ComputeAccelerations runs on the GPU through
+#                                 dem3d_amd/libdem3d_hip.so (built first).
 #
 # The reference part is skipped when $REF/src is absent (e.g. on a machine that
 # only runs the prebuilt libraries).
@@ -61,7 +69,41 @@ build_variant() {
   rm -rf "$obj"
 }
 
+# The program DEM, as the reference builds it and with the GPU MForces.
+build_programs() {
+  local obj="$HERE/_ref/obj_prog" flags="-O2 -ffp-contract=off -cpp"
+  local root lib
+  root="$(cd "$HERE/.." && pwd)"
+  lib="$root/dem3d_amd/libdem3d_hip.so"
+  rm -rf "$obj"
+  mkdir -p "$obj/common" "$obj/ref" "$obj/gpu"
+  local common=()
+  for m in MTypes MTime BinFiles MDimension3D MUtil MCells MConfig MContacts mFluid; do
+    "$FC" $flags -module-dir "$obj/common" -c "$REF/src/$m.f90" -o "$obj/common/$m.o"
+    common+=("$obj/common/$m.o")
+  done
+  for m in MForces MDEM DEM; do
+    "$FC" $flags -module-dir "$obj/ref" -I "$obj/common" -c "$REF/src/$m.f90" -o "$obj/ref/$m.o"
+  done
+  "$FC" -o "$HERE/_ref/dem3d_ref" "${common[@]}" "$obj/ref/MForces.o" "$obj/ref/MDEM.o" \
+    "$obj/ref/DEM.o"
+  if [ -f "$lib" ]; then
+    "$FC" $flags -module-dir "$obj/gpu" -c "$root/dem3d_amd/fortran/dem3d_gpu.f90" \
+      -o "$obj/gpu/dem3d_gpu.o"
+    "$FC" $flags -module-dir "$obj/gpu" -I "$obj/common" -c \
+      "$root/dem3d_amd/fortran/mforces_gpu.f90" -o "$obj/gpu/MForces.o"
+    for m in MDEM DEM; do
+      "$FC" $flags -module-dir "$obj/gpu" -I "$obj/common" -c "$REF/src/$m.f90" \
+        -o "$obj/gpu/$m.o"
+    done
+    "$FC" -o "$HERE/_ref/dem3d_gpu" "${common[@]}" "$obj/gpu/dem3d_gpu.o" "$obj/gpu/MForces.o" \
+      "$obj/gpu/MDEM.o" "$obj/gpu/DEM.o" "$lib" -Wl,-rpath,'$ORIGIN/../../dem3d_amd'
+  fi
+  rm -rf "$obj"
+}
+
 mkdir -p "$HERE/_ref"
+build_programs
 build_variant "" ""
 build_variant "_big" "yes"
 # the reference's own optimisation (Makefile: -O3 -march=native, contraction
@@ -71,4 +113,4 @@ build_variant "_fast" "yes" "-O3 -march=x86-64-v3 -fPIC -cpp"
 if [ "${DEM3D_REF_NATIVE:-0}" = "1" ]; then
   build_variant "_native" "yes" "-O3 -march=native -fPIC -cpp"
 fi
-echo "build_ref.sh: built $HERE/_ref/libdem3d_ref.so, libdem3d_ref_big.so and libdem3d_ref_fast.so"
+echo "build_ref.sh: built $HERE/_ref/libdem3d_ref.so, libdem3d_ref_big.so, libdem3d_ref_fast.so, dem3d_ref and dem3d_gpu"

@@ -84,3 +84,34 @@ def test_fortran_bindings_compile_link_and_match_layout(tmp_path):
     # no device here: the error code and message come back through the bindings
     if rc != 0:
         assert rc == -2 and "device" in out.lower(), out
+
+
+def test_integration_level1_excerpt_is_the_compiled_module():
+    """The ComputeAccelerations shown in INTEGRATION.md is, line for line, the
+    one in dem3d_amd/fortran/mforces_gpu.f90 (which oracle/build_ref.sh
+    compiles with the reference's MDEM.f90 and DEM.f90)."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    src = open(os.path.join(ROOT, "dem3d_amd", "fortran", "mforces_gpu.f90")).read()
+    m = re.search(r"```fortran\n(subroutine ComputeAccelerations.*?end subroutine)\n```", doc,
+                  re.S)
+    assert m, "level-1 excerpt not found"
+    norm = lambda t: [re.sub(r"\s+", " ", ln.split("!")[0]).strip() for ln in t.splitlines()]
+    excerpt = [ln for ln in norm(m.group(1)) if ln]
+    module = [ln for ln in norm(src) if ln]
+    # every line of the excerpt appears in the module, in the same order
+    pos = 0
+    for ln in excerpt:
+        key = ln.replace("end subroutine", "end subroutine ComputeAccelerations")
+        while pos < len(module) and module[pos] not in (ln, key):
+            pos += 1
+        assert pos < len(module), ln
+        pos += 1
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src"), reason="reference checkout absent")
+def test_reference_programs_built():
+    """oracle/build_ref.sh built the reference's program DEM and the same
+    program over mforces_gpu.f90 (run by tests/test_gpu_reference_driver.py)."""
+    for name in ("dem3d_ref", "dem3d_gpu"):
+        p = os.path.join(ROOT, "oracle", "_ref", name)
+        assert os.path.exists(p) and os.access(p, os.X_OK), p
