@@ -24,8 +24,10 @@ Prints one JSON line:
   crystal      the ordered bed of the same spheres (rock-salt packing,
                phi = 0.567) as a second, labelled leg
   ensemble     256 independent systems (500..20 000 spheres each) stepped as
-               one batch (one launch per phase) in mixed and in FP64, against
-               the plain FP64 path that steps each system alone
+               one batch (one launch per phase, and persistent: one
+               cooperative launch per 250 steps, a group of resident blocks
+               per system) in mixed and in FP64, against the plain FP64 path
+               that steps each system alone
   serving      the Poisson request trace served with continuous batching:
                p50/p99 step time, waiting and completion latency, aggregate
                particle-steps/s
@@ -243,6 +245,29 @@ def ensemble_run(args):
                                    shape=tune_64["chosen"] if tune_64 else 0)
         print(f"ensemble fp64 batched: {v_64b:.4g} launched one by one, {v_64g:.4g} from graphs",
               file=sys.stderr, flush=True)
+    # the same in persistent launches (one cooperative launch per 250 steps,
+    # a group of resident blocks per system)
+    def persistent(precision, shape):
+        systems = build(precision)
+        b = Batch(lib)
+        b.set_option("persistent", 1)
+        if shape is not None:
+            b.set_option("pass_shape", shape)
+        for s_ in systems:
+            b.add(s_)
+        b.step(20)  # warm-up
+        t0 = time.perf_counter()
+        b.step(args.ens_steps)
+        sec = time.perf_counter() - t0
+        b.close()
+        for s_ in systems:
+            s_.close()
+        return ntot * args.ens_steps / sec
+
+    v_mxp = persistent(DEM_MIXED, tune_mx["chosen"] if tune_mx else None)
+    v_64p = persistent(DEM_FP64, tune_64["chosen"] if tune_64 else None)
+    print(f"ensemble persistent: mixed {v_mxp:.4g}, fp64 {v_64p:.4g} particle-steps/s",
+          file=sys.stderr, flush=True)
     t_alone = 0.0
     ps_alone = 0.0
     for s, w in zip(sys64, members):
@@ -270,9 +295,14 @@ def ensemble_run(args):
         "fp64_batched_graph_step_ms_p50": (float(np.percentile(st_64g, 50))
                                            if st_64g is not None else None),
         "graph_speedup_fp64": v_64g / v_64b if v_64g else None,
+        "mixed_persistent_particle_steps_per_s": v_mxp,
+        "fp64_persistent_particle_steps_per_s": v_64p,
+        "persistent_speedup_mixed": v_mxp / v_mx,
+        "persistent_speedup_fp64": v_64p / v_64b,
         "fp64_alone_particle_steps_per_s": v_alone,
         "fp64_alone_sample_steps": args.alone_steps,
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
+        "speedup_mixed_persistent_vs_fp64_alone": v_mxp / v_alone,
         "speedup_fp64_batched_vs_fp64_alone": v_64b / v_alone,
     })
     return res

@@ -2743,7 +2743,30 @@ struct dem_batch {
   int block_map = 0;  // 0: particle ranges per XCD, 1: dispatch order
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<long long> gkey;
+  // Persistent stepping (k_batch_persistent): steps per cooperative launch,
+  // the group of every block, the group sizes, barrier words, merge scratch
+  // and the abort word.
+  bool persistent = false;
+  int persist_steps = 250;
+  unsigned spin_limit = 1u << 24;
+  int2* d_group = nullptr;
+  size_t group_cap = 0;
+  int* d_gsize = nullptr;
+  GroupSync* d_gsync = nullptr;
+  MergeScratch* d_scratch = nullptr;
+  char* d_scratch_mem = nullptr;
+  size_t pers_cap = 0;  // systems the per-system arrays hold
+  int* d_abort = nullptr;
+  int pers_grid = 0;
+  std::vector<int> pers_key;  // pass blocks per system the groups were cut for
 };
+
+// Whether dem_batch_step runs the persistent launch (per-step timing needs
+// the launch-per-phase path).
+static bool batch_persistent_now(const dem_batch* b) { return b->persistent && !b->timing; }
+// FRONT of the persistent kernel: the batch's launch shape's when it has
+// DEM_BLOCK lanes, else the default shape's.
+static int batch_persistent_front(const dem_batch* b);
 
 static int bfail(dem_batch_t b, int code, const char* fmt, ...) {
   va_list ap;
@@ -2817,7 +2840,9 @@ static void batch_rec_init(dem_handle_t h, BatchRec& r) {
   r.p[1] = b;
   r.r = rebuild_args(h, h->cur, h->rb);
   r.nblk = nblocks(h->np);
-  r.pass_nblk = nblocks(h->np, shape_block(h->batch ? h->batch->shape : 0));
+  r.pass_nblk = nblocks(h->np, h->batch && batch_persistent_now(h->batch)
+                                   ? DEM_BLOCK
+                                   : shape_block(h->batch ? h->batch->shape : 0));
   r.nprev = h->nprev;
   r.first = h->batch_first ? 1 : 0;
   r.step = 0;
@@ -2962,6 +2987,158 @@ static void enqueue_batch_step(dem_batch_t b, int ns, bool mixed, bool moving) {
   hipLaunchKernelGGL(k_nbr_flip_batch, dim3(ns), dim3(DEM_MERGE_THREADS), 0, st, rec);
 }
 
+// ---- persistent stepping -----------------------------------------------------
+
+template <bool MIXED, int FRONT>
+static int persistent_occupancy(dem_batch_t b, int* blocks) {
+  int per_cu = 0, dev = 0, cus = 0;
+  BHIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per_cu, reinterpret_cast<const void*>(&k_batch_persistent<MIXED, FRONT>),
+                 DEM_BLOCK, 0));
+  BHIPCHK(b, hipGetDevice(&dev));
+  BHIPCHK(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  *blocks = per_cu * cus;
+  return DEM_OK;
+}
+
+// Groups of the persistent launch: every system gets blocks in proportion to
+// its pass blocks (at least one, at most one per pass block), within what
+// the GPU holds resident at once for this very kernel.
+static int persistent_groups(dem_batch_t b, int ns, bool mixed, int front) {
+  std::vector<int> nb(ns);
+  long long total = 0;
+  for (int i = 0; i < ns; ++i) {
+    nb[i] = nblocks(b->sys[i]->np, DEM_BLOCK);
+    total += nb[i];
+  }
+  if (nb == b->pers_key && b->pers_grid > 0) return DEM_OK;
+  int resident = 0;
+  int rc = mixed ? (front ? persistent_occupancy<true, 1>(b, &resident)
+                          : persistent_occupancy<true, 0>(b, &resident))
+                 : (front ? persistent_occupancy<false, 1>(b, &resident)
+                          : persistent_occupancy<false, 0>(b, &resident));
+  if (rc) return rc;
+  if (resident < ns)
+    return bfail(b, DEM_ERR_CAPACITY,
+                 "persistent stepping needs a resident block per system: %d systems, %d blocks",
+                 ns, resident);
+  const long long budget = std::min<long long>(resident, total);
+  std::vector<int> g(ns);
+  long long used = 0;
+  for (int i = 0; i < ns; ++i) {
+    g[i] = (int)std::max<long long>(1, std::min<long long>(nb[i], budget * nb[i] / total));
+    used += g[i];
+  }
+  // what is left goes to the systems with the most pass blocks per block
+  while (used < budget) {
+    int best = -1;
+    double worst = 0.0;
+    for (int i = 0; i < ns; ++i) {
+      if (g[i] >= nb[i]) continue;
+      const double load = (double)nb[i] / g[i];
+      if (load > worst) { worst = load; best = i; }
+    }
+    if (best < 0) break;
+    ++g[best];
+    ++used;
+  }
+  std::vector<int2> group;
+  group.reserve((size_t)used);
+  for (int i = 0; i < ns; ++i)
+    for (int r = 0; r < g[i]; ++r) group.push_back(make_int2(i, r));
+  if (group.size() > b->group_cap) {
+    if (b->d_group) (void)hipFree(b->d_group);
+    b->d_group = nullptr;
+    BHIPCHK(b, hipMalloc((void**)&b->d_group, group.size() * sizeof(int2)));
+    b->group_cap = group.size();
+  }
+  if ((size_t)ns > b->pers_cap) {
+    if (b->d_gsize) (void)hipFree(b->d_gsize);
+    if (b->d_gsync) (void)hipFree(b->d_gsync);
+    if (b->d_scratch) (void)hipFree(b->d_scratch);
+    if (b->d_scratch_mem) (void)hipFree(b->d_scratch_mem);
+    b->d_gsize = nullptr;
+    b->d_gsync = nullptr;
+    b->d_scratch = nullptr;
+    b->d_scratch_mem = nullptr;
+    b->pers_cap = 0;
+    const size_t per = (size_t)DEM_STALE_ROWS *
+                       (sizeof(unsigned long long) * 2 + sizeof(unsigned short) + sizeof(int2) +
+                        3 * sizeof(double2));
+    BHIPCHK(b, hipMalloc((void**)&b->d_gsize, (size_t)ns * sizeof(int)));
+    BHIPCHK(b, hipMalloc((void**)&b->d_gsync, (size_t)ns * sizeof(GroupSync)));
+    BHIPCHK(b, hipMalloc((void**)&b->d_scratch, (size_t)ns * sizeof(MergeScratch)));
+    BHIPCHK(b, hipMalloc((void**)&b->d_scratch_mem, (size_t)ns * per));
+    std::vector<MergeScratch> ms((size_t)ns);
+    for (int i = 0; i < ns; ++i) {
+      char* p = b->d_scratch_mem + (size_t)i * per;
+      const size_t R = DEM_STALE_ROWS;
+      ms[i].sk = reinterpret_cast<unsigned long long*>(p);
+      ms[i].key = reinterpret_cast<unsigned long long*>(p + R * 8);
+      ms[i].hist = reinterpret_cast<double2*>(p + R * 16);
+      ms[i].slot = reinterpret_cast<int2*>(p + R * 16 + R * 48);
+      ms[i].si = reinterpret_cast<unsigned short*>(p + R * 16 + R * 48 + R * 8);
+    }
+    BHIPCHK(b, hipMemcpyAsync(b->d_scratch, ms.data(), ms.size() * sizeof(MergeScratch),
+                              hipMemcpyHostToDevice, b->stream));
+    b->pers_cap = (size_t)ns;
+  }
+  if (!b->d_abort) BHIPCHK(b, hipMalloc((void**)&b->d_abort, sizeof(int)));
+  BHIPCHK(b, hipMemcpyAsync(b->d_group, group.data(), group.size() * sizeof(int2),
+                            hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(b->d_gsize, g.data(), (size_t)ns * sizeof(int),
+                            hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->pers_grid = (int)group.size();
+  b->pers_key = nb;
+  return DEM_OK;
+}
+
+static int batch_persistent_front(const dem_batch* b) {
+  int blk[DEM_NSHAPES], wv[DEM_NSHAPES], fr[DEM_NSHAPES];
+  shape_table(blk, wv, fr);
+  return blk[b->shape] == DEM_BLOCK && wv[b->shape] == DEM_PASS_WAVES ? fr[b->shape] : fr[0];
+}
+
+// nsteps steps of every system by persistent launches of persist_steps
+// steps each.
+static int persistent_steps(dem_batch_t b, int ns, bool mixed, int front, int64_t nsteps) {
+  int rc = persistent_groups(b, ns, mixed, front);
+  if (rc) return rc;
+  hipStream_t st = b->stream;
+  for (int64_t s = 0; s < nsteps;) {
+    const int chunk = (int)std::min<int64_t>(b->persist_steps, nsteps - s);
+    BHIPCHK(b, hipMemsetAsync(b->d_gsync, 0, (size_t)ns * sizeof(GroupSync), st));
+    BHIPCHK(b, hipMemsetAsync(b->d_abort, 0, sizeof(int), st));
+    PersistArgs pa;
+    pa.group = b->d_group;
+    pa.gsize = b->d_gsize;
+    pa.gsync = b->d_gsync;
+    pa.scratch = b->d_scratch;
+    pa.abort = b->d_abort;
+    pa.spin_limit = b->spin_limit;
+    pa.nsteps = chunk;
+    const BatchRec* rec = b->d_rec;
+    void* args[] = {(void*)&rec, (void*)&pa};
+    const void* fn = mixed ? (front ? reinterpret_cast<const void*>(&k_batch_persistent<true, 1>)
+                                    : reinterpret_cast<const void*>(&k_batch_persistent<true, 0>))
+                           : (front ? reinterpret_cast<const void*>(&k_batch_persistent<false, 1>)
+                                    : reinterpret_cast<const void*>(&k_batch_persistent<false, 0>));
+    BHIPCHK(b, hipLaunchCooperativeKernel(fn, dim3(b->pers_grid), dim3(DEM_BLOCK), args, 0, st));
+    hipLaunchKernelGGL(k_persist_advance, dim3(nblocks(ns, 64)), dim3(64), 0, st, b->d_rec, ns,
+                       chunk);
+    int abort = 0;
+    BHIPCHK(b, hipMemcpyAsync(&abort, b->d_abort, sizeof(int), hipMemcpyDeviceToHost, st));
+    BHIPCHK(b, hipStreamSynchronize(st));
+    if (abort)
+      return bfail(b, DEM_ERR_DEVICE,
+                   "persistent batch step: a group barrier wait ran out (the systems' states "
+                   "are no longer valid)");
+    s += chunk;
+  }
+  return DEM_OK;
+}
+
 static void drop_graphs(dem_batch_t b) {
   for (int k = 0; k < 2; ++k)
     if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
@@ -3053,6 +3230,12 @@ void dem_batch_destroy(dem_batch_t b) {
   if (b->d_ctl) (void)hipFree(b->d_ctl);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   for (auto e : b->tev) (void)hipEventDestroy(e);
+  if (b->d_group) (void)hipFree(b->d_group);
+  if (b->d_gsize) (void)hipFree(b->d_gsize);
+  if (b->d_gsync) (void)hipFree(b->d_gsync);
+  if (b->d_scratch) (void)hipFree(b->d_scratch);
+  if (b->d_scratch_mem) (void)hipFree(b->d_scratch_mem);
+  if (b->d_abort) (void)hipFree(b->d_abort);
   drop_graphs(b);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -3112,6 +3295,16 @@ int dem_batch_set_option(dem_batch_t b, const char* name, double value) {
   } else if (!strcmp(name, "block_map")) {
     if (value != 0.0 && value != 1.0) return bfail(b, DEM_ERR_ARG, "block_map must be 0 or 1");
     b->block_map = (int)value;
+  } else if (!strcmp(name, "persistent")) {
+    b->persistent = value != 0.0;
+  } else if (!strcmp(name, "persist_steps")) {
+    if (!(value >= 1.0 && value <= 100000.0))
+      return bfail(b, DEM_ERR_ARG, "persist_steps must lie in [1, 100000]");
+    b->persist_steps = (int)value;
+  } else if (!strcmp(name, "spin_limit")) {
+    if (!(value >= 0.0 && value <= 4.0e9))
+      return bfail(b, DEM_ERR_ARG, "spin_limit must lie in [0, 4e9]");
+    b->spin_limit = (unsigned)value;
   } else if (!strcmp(name, "graph_steps")) {
     if (!(value >= 1.0 && value <= 256.0))
       return bfail(b, DEM_ERR_ARG, "graph_steps must lie in [1, 256]");
@@ -3158,12 +3351,18 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   for (size_t i = 0; i < n; ++i) moving = moving || b->sys[i]->nw_move > 0;
   hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
                      ns);
-  hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
-  if (b->use_graph) {
+  const bool pers = batch_persistent_now(b);
+  // the launch-per-phase path prepares each step's arguments in the records;
+  // the persistent launch derives them from the records as uploaded
+  if (!pers) hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);
+  if (pers) {
+    rc = persistent_steps(b, ns, mixed, batch_persistent_front(b), nsteps);
+    if (rc) return rc;
+  } else if (b->use_graph) {
     rc = batch_graphs(b, ns, mixed, moving);
     if (rc) return rc;
   }
-  for (int64_t s = 0; s < nsteps;) {
+  for (int64_t s = pers ? nsteps : 0; s < nsteps;) {
     if (b->use_graph && !b->timing && b->graph_steps > 1 && nsteps - s >= b->graph_steps) {
       BHIPCHK(b, hipGraphLaunch(b->gexec[1], st));
       s += b->graph_steps;

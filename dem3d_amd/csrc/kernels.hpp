@@ -415,8 +415,35 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
 // BS lanes per block.  Neither the block size nor FRONT changes the
 // arithmetic of a contact, the order of a particle's sums or the integer
 // counter sums, so every launch shape gives the same results bit for bit.
+// The arguments of a pass that change from step to step (prepare_step in
+// batch.hpp, step_args on the host); the rest of PassArgs stays fixed.
+struct StepVars {
+  int stamp_next, check_pred, step_idx, par;
+  int* rb_clear;
+  int* rb_set;
+  const double* slot_rd[3];
+  double* slot_wr;
+  const double* psum_rd;
+  double* psum_wr;
+};
+__device__ __forceinline__ StepVars step_vars(const PassArgs& a) {
+  StepVars v;
+  v.stamp_next = a.stamp_next;
+  v.check_pred = a.check_pred;
+  v.step_idx = a.step_idx;
+  v.par = a.par;
+  v.rb_clear = a.rb_clear;
+  v.rb_set = a.rb_set;
+  for (int k = 0; k < 3; ++k) v.slot_rd[k] = a.slot_rd[k];
+  v.slot_wr = a.slot_wr;
+  v.psum_rd = a.psum_rd;
+  v.psum_wr = a.psum_wr;
+  return v;
+}
+
 template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int FRONT = 1>
-__device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
+__device__ __forceinline__ void pass_body_v(const SysConst& s, const PassArgs& a,
+                                            const StepVars& stv, int blk) {
   __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
   __shared__ CompConst sh_comp[DEM_LDS_NC];
   Ctl* ctl = a.ctl;
@@ -424,7 +451,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
   // a pass reads the same value here.
   if (ctl->escaped) return;
   const int t = threadIdx.x;
-  if (blk == 0 && t == 0) *a.rb_clear = 0;
+  if (blk == 0 && t == 0) *stv.rb_clear = 0;
   const bool lds_tab = s.nc <= DEM_LDS_NC;
   if (lds_tab) {
     for (int q = t; q < s.nc * s.nc; q += BS) sh_pair[q] = s.pair[q];
@@ -536,7 +563,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       const d3 wv = add3(wi, scl3(a.h, alp));
       a.pos_out[i] = make_double4(x.x, x.y, x.z, P.w);
       const RefCell rc = ref_cell(s, x.x, x.y, x.z);
-      store_kin(a.kin_out + i, v, wv, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
+      store_kin(a.kin_out + i, v, wv, id_p, rc, (rc.lin != cell_p) ? stv.stamp_next : st_p);
       store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       const double4 R = a.xref[i];
       const d3 dd = min_image3(s, mk3(x.x - R.x, x.y - R.y, x.z - R.z));
@@ -545,15 +572,15 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
     } else if (MODE == MODE_PRED) {
       // AB predictor (src/MDEM.f90:297-331)
       d3 sv = mk3(0, 0, 0), sa = sv, sl = sv;
-      if (a.psum_rd) {
-        ld_ring(a.psum_rd + i, N, sv, sa, sl);
+      if (stv.psum_rd) {
+        ld_ring(stv.psum_rd + i, N, sv, sa, sl);
       } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           if (k >= s.nprev) break;
           const double c = a.coef[k];
           d3 rv, ra, rl;
-          ld_ring(a.slot_rd[k] + i, N, rv, ra, rl);
+          ld_ring(stv.slot_rd[k] + i, N, rv, ra, rl);
           sv = add3(sv, scl3(c, rv));
           sa = add3(sa, scl3(c, ra));
           sl = add3(sl, scl3(c, rl));
@@ -567,13 +594,13 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       if (s.pbc[2]) xp.z = xp.z - s.lbox[2] * fnint(xp.z * s.invl[2]);
       a.pos_out[i] = make_double4(xp.x, xp.y, xp.z, a.pred_w ? a.pred_w[id_p] : P.w);
       const RefCell rc = ref_cell(s, xp.x, xp.y, xp.z);
-      store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
+      store_kin(a.kin_out + i, vp, wp, id_p, rc, (rc.lin != cell_p) ? stv.stamp_next : st_p);
       if (s.nprev > 0) {
-        st_ring(a.slot_wr + i, N, vi, acc, alp);
+        st_ring(stv.slot_wr + i, N, vi, acc, alp);
       } else {
         store_d3x2(a.acc_out + 3 * (size_t)i, acc, alp);
       }
-      if (a.check_pred) {
+      if (stv.check_pred) {
         const double4 R = a.xref[i];
         const d3 dd = min_image3(s, mk3(xp.x - R.x, xp.y - R.y, xp.z - R.z));
         if (dot3(dd, dd) > s.skin_half_sq) err |= DEM_KERR_SKIN;
@@ -593,7 +620,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
         if (k >= s.nprev) break;
         const double c = a.coef[k];
         d3 rv, ra, rl;
-        ld_ring(a.slot_rd[k] + i, N, rv, ra, rl);
+        ld_ring(stv.slot_rd[k] + i, N, rv, ra, rl);
         sv = add3(sv, scl3(c, rv));
         sa = add3(sa, scl3(c, ra));
         sl = add3(sl, scl3(c, rl));
@@ -601,8 +628,8 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
         sba = add3(sba, scl3(a.ab[k], ra));
         sbl = add3(sbl, scl3(a.ab[k], rl));
       }
-      if (a.psum_wr) {
-        st_ring(a.psum_wr + i, N, sb, sba, sbl);
+      if (stv.psum_wr) {
+        st_ring(stv.psum_wr + i, N, sb, sba, sbl);
       }
       d3 x = add3(add3(mk3(X.x, X.y, X.z), scl3(a.coef0, vi)), sv);
       const d3 v = add3(add3(v0, scl3(a.coef0, acc)), sa);
@@ -612,7 +639,7 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
       if (s.pbc[2]) x.z = x.z - s.lbox[2] * fnint(x.z * s.invl[2]);
       a.pos[i] = make_double4(x.x, x.y, x.z, X.w);
       const RefCell rc = ref_cell(s, x.x, x.y, x.z);
-      store_kin(a.kin + i, v, wv, id_p, rc, (rc.lin != cell_p) ? a.stamp_next : st_p);
+      store_kin(a.kin + i, v, wv, id_p, rc, (rc.lin != cell_p) ? stv.stamp_next : st_p);
       // escape scan of UpdateParticleList (src/MDEM.f90:624-625)
       if ((x.x > s.phalf[0]) || (x.x < -s.phalf[0]) || (x.y > s.phalf[1]) ||
           (x.y < -s.phalf[1]) || (x.z > s.phalf[2]) || (x.z < -s.phalf[2]))
@@ -645,10 +672,10 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
     sh_cnt[t >> 6][2] = c2;
     sh_cnt[t >> 6][3] = c3;
     if (err) atomicOr(&ctl->error, err);
-    if (need_rb) atomicOr(a.rb_set, 1);
+    if (need_rb) atomicOr(stv.rb_set, 1);
     if (MODE == MODE_CORR && n_esc) {
       atomicAdd(&ctl->esc_pending, n_esc);
-      atomicMin(&ctl->stop_step, a.step_idx);
+      atomicMin(&ctl->stop_step, stv.step_idx);
     }
   }
   __syncthreads();
@@ -658,6 +685,11 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
     // integer adds: the totals do not depend on the order of the blocks
     if (v) atomicAdd(&a.part[4 * (size_t)(blk % DEM_RED_SLOTS) + t], v);
   }
+}
+
+template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int FRONT = 1>
+__device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, int blk) {
+  pass_body_v<MODE, UPDATE, MIXED, BS, FRONT>(s, a, step_vars(a), blk);
 }
 
 // Launch shapes of the force passes: lanes per block, the occupancy the
@@ -700,12 +732,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) 
 // and publishes the escapes its corrector found.  Runs as one block.
 // Sums of the four counters over the per-block slots [b0, b1) by one block
 // of 1024 threads; the totals are valid in thread 0.
+// Counter slots [b0, b1) summed and cleared by one block of NT threads; the
+// totals on thread 0.
+template <int NT = 1024>
 __device__ __forceinline__ void sum_slots(unsigned* part, int b0, int b1,
                                           unsigned long long s[4]) {
-  __shared__ unsigned long long sh[1024 / 64][4];
+  __shared__ unsigned long long sh[NT / 64][4];
   const int t = threadIdx.x;
   unsigned long long v[4] = {0, 0, 0, 0};
-  for (int b = b0 + t; b < b1; b += 1024)
+  for (int b = b0 + t; b < b1; b += NT)
     for (int c = 0; c < 4; ++c) {
       v[c] += part[4 * (size_t)b + c];
       part[4 * (size_t)b + c] = 0u;
@@ -716,7 +751,7 @@ __device__ __forceinline__ void sum_slots(unsigned* part, int b0, int b1,
   __syncthreads();
   for (int c = 0; c < 4; ++c) s[c] = 0;
   if (t == 0)
-    for (int w = 0; w < 1024 / 64; ++w)
+    for (int w = 0; w < NT / 64; ++w)
       for (int c = 0; c < 4; ++c) s[c] += sh[w][c];
 }
 
@@ -741,14 +776,15 @@ __device__ __forceinline__ void publish_counts(Ctl* ctl, const unsigned long lon
 // After an updating pass: the stale pairs whose reference cells are now
 // adjacent were visited by the reference's cell loop without overlapping
 // (they are no neighbours) and are forgotten; returns, on thread 0 of the
-// block (1024 threads), how many rows are kept.
+// block (NT threads), how many rows are kept.
+template <int NT = 1024>
 __device__ __forceinline__ unsigned long long stale_visit(const SysConst& s, const Kin* kin,
                                                           const StaleTab& T) {
-  __shared__ unsigned long long sh_kept[1024 / 64];
+  __shared__ unsigned long long sh_kept[NT / 64];
   const int t = threadIdx.x;
   const int n = min(T.cnt[0], DEM_STALE_ROWS);
   unsigned long long kept = 0;
-  for (int k = t; k < n; k += 1024) {
+  for (int k = t; k < n; k += NT) {
     if (T.state[k] != 0) continue;
     const int2 q = T.slot[k];
     const int4 a = reinterpret_cast<const int4*>(kin + q.x)[3];
@@ -761,21 +797,22 @@ __device__ __forceinline__ unsigned long long stale_visit(const SysConst& s, con
   __syncthreads();
   kept = 0;
   if (t == 0)
-    for (int w = 0; w < 1024 / 64; ++w) kept += sh_kept[w];
+    for (int w = 0; w < NT / 64; ++w) kept += sh_kept[w];
   return kept;
 }
 
-// nslots <= DEM_RED_SLOTS counter slots of a pass, summed by one block,
-// cleared and published; after an updating pass, the visit of the stale
-// pairs, whose kept rows count as stored pairs (both directions).
+// nslots <= DEM_RED_SLOTS counter slots of a pass, summed by one block of NT
+// threads, cleared and published; after an updating pass, the visit of the
+// stale pairs, whose kept rows count as stored pairs (both directions).
+template <int NT = 1024>
 __device__ __forceinline__ void pass_reduce_body(Ctl* ctl, unsigned* part, int nslots, int par,
                                                  int update, const SysConst& sc, const Kin* kin,
                                                  const StaleTab& T) {
   if (ctl->escaped) return;  // the pass did not run
   const int t = threadIdx.x;
   unsigned long long s[4];
-  sum_slots(part, 0, nslots, s);
-  if (update && sc.hist_mode == 0 && T.cnt) s[3] += 2 * stale_visit(sc, kin, T);
+  sum_slots<NT>(part, 0, nslots, s);
+  if (update && sc.hist_mode == 0 && T.cnt) s[3] += 2 * stale_visit<NT>(sc, kin, T);
   if (t == 0) publish_counts(ctl, s, par, update);
 }
 __global__ __launch_bounds__(1024) void k_pass_reduce(Ctl* ctl, unsigned* part, int nblk,
@@ -1232,16 +1269,17 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
 }
 
 // Make the freshly built edge set current.
-// The stale-pair table after a rebuild (one block of DEM_MERGE_THREADS):
-// the kept rows (slots through o2n when the particles were re-sorted) and
-// the rows the rebuild appended, sorted by key in LDS (bitonic), restored
-// and forgotten rows dropped.  Keys are unique, so the result does not
-// depend on the order in which the rebuild appended.
+// The stale-pair table after a rebuild (one block of NT threads): the kept
+// rows (slots through o2n when the particles were re-sorted) and the rows
+// the rebuild appended, sorted by key (bitonic, in sk/si: LDS, or scratch
+// memory of the system), restored and forgotten rows dropped.  Keys are
+// unique, so the result does not depend on the order in which the rebuild
+// appended.
 #define DEM_MERGE_THREADS 1024
-__device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o2n) {
-  constexpr int R = DEM_STALE_ROWS, PER = DEM_STALE_ROWS / DEM_MERGE_THREADS;
-  __shared__ unsigned long long sk[R];
-  __shared__ unsigned short si[R];
+template <int NT>
+__device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o2n,
+                                                 unsigned long long* sk, unsigned short* si) {
+  constexpr int R = DEM_STALE_ROWS, PER = DEM_STALE_ROWS / NT;
   __shared__ int s_live;
   const int t = threadIdx.x;
   const int n = min(T.cnt[0], R);
@@ -1251,7 +1289,7 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
   int P = 2;
   while (P < tot) P <<= 1;
   if (t == 0) s_live = 0;
-  for (int k = t; k < P; k += DEM_MERGE_THREADS) {
+  for (int k = t; k < P; k += NT) {
     const bool live = k < tot && (k >= n || T.state[k] == 0);
     sk[k] = live ? T.key[k] : ~0ull;
     si[k] = (unsigned short)k;
@@ -1259,7 +1297,7 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
   __syncthreads();
   for (int w = 2; w <= P; w <<= 1)
     for (int d = w >> 1; d > 0; d >>= 1) {
-      for (int k = t; k < P; k += DEM_MERGE_THREADS) {
+      for (int k = t; k < P; k += NT) {
         const int l = k ^ d;
         if (l > k) {
           const bool up = (k & w) == 0;
@@ -1275,7 +1313,7 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
       }
       __syncthreads();
     }
-  for (int k = t; k < P; k += DEM_MERGE_THREADS)
+  for (int k = t; k < P; k += NT)
     if (sk[k] != ~0ull && (k + 1 == P || sk[k + 1] == ~0ull)) s_live = k + 1;
   __syncthreads();
   const int live = s_live;
@@ -1284,7 +1322,7 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
   double2 h0[PER], h1[PER], h2[PER];
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
-    const int k = t + r * DEM_MERGE_THREADS;
+    const int k = t + r * NT;
     if (k < live) {
       const int q = si[k];
       key[r] = T.key[q];
@@ -1298,7 +1336,7 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
-    const int k = t + r * DEM_MERGE_THREADS;
+    const int k = t + r * NT;
     if (k < live) {
       T.key[k] = key[r];
       T.slot[k] = slot[r];
@@ -1314,15 +1352,18 @@ __device__ __forceinline__ void stale_merge_body(const StaleTab& T, const int* o
   }
 }
 
-// The flip of a rebuild and the merge of the stale-pair table it changed.
+// The flip of a rebuild and the merge of the stale-pair table it changed
+// (one block of DEM_MERGE_THREADS, sorting in LDS).
 __device__ __forceinline__ void nbr_flip_body(const EdgeSets& E, const int* gate, int force,
                                               Ctl* ctl, const int* o2n = nullptr) {
+  __shared__ unsigned long long sk[DEM_STALE_ROWS];
+  __shared__ unsigned short si[DEM_STALE_ROWS];
   if (gate_closed(gate, force, ctl)) return;
   if (threadIdx.x == 0) {
     *E.epar = 1 - *E.epar;
     ctl->rebuilds += 1;
   }
-  if (E.st.cnt) stale_merge_body(E.st, o2n);
+  if (E.st.cnt) stale_merge_body<DEM_MERGE_THREADS>(E.st, o2n, sk, si);
 }
 __global__ __launch_bounds__(DEM_MERGE_THREADS) void k_nbr_flip(EdgeSets E, const int* gate,
                                                                 int force, Ctl* ctl,

@@ -627,7 +627,7 @@ class Batch:
 
     def set_option(self, name: str, value: float):
         """dem_batch_set_option: "timing", "graph", "graph_steps", "pass_shape",
-        "block_map"."""
+        "block_map", "persistent", "persist_steps", "spin_limit"."""
         self._check(self.lib.batch_set_option(self.b, name.encode(), float(value)))
 
     def autotune(self, steps_per_shape=20):

@@ -498,7 +498,17 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done,
  * membership changes).  "graph_steps": steps per replayed graph (1..256,
  * default 8); with "timing" every step is its own replay.
  * "pass_shape", "block_map": launch shape and block order of the force
- * passes (see dem_tune_t). */
+ * passes (see dem_tune_t).
+ * "persistent": 1 = step the batch with persistent cooperative launches:
+ * every system is stepped by its own group of resident blocks (sized by its
+ * particles), the groups synchronise only their own blocks between the
+ * phases of a step, and one launch runs "persist_steps" steps (1..100000,
+ * default 250) of every system; results are bitwise those of the
+ * launch-per-phase path.  Needs one resident block per system (else
+ * DEM_ERR_CAPACITY); "timing" uses the launch-per-phase path.
+ * "spin_limit": bound of a group barrier's wait (polls, default 2^24); a
+ * wait that runs out ends the launch and the call returns DEM_ERR_DEVICE
+ * (the systems' states are then not valid). */
 int dem_batch_set_option(dem_batch_t b, const char* name, double value);
 int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
 
