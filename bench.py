@@ -346,6 +346,9 @@ def main(argv=None):
                          "after packing (profiler runs pack once, outside the profiler)")
     ap.add_argument("--pack-only", action="store_true",
                     help="pack the bed (and write --bed-cache), then exit")
+    ap.add_argument("--crystal-only", action="store_true",
+                    help="time only the ordered (rock-salt) bed of --cells for --steps steps "
+                         "and print its pass timings (profiling runs)")
     ap.add_argument("--crystal-steps", type=int, default=500,
                     help="timed steps of the crystal leg (0 = skip it)")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -407,6 +410,21 @@ def main(argv=None):
     from dem3d_amd.native import DEM_FP64, DEM_HIST_CANONICAL, DEM_HIST_REFERENCE, DEM_MIXED, load_hip
     lib = load_hip()
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.crystal_only:
+        # the ordered bed alone, --steps timed steps (profiling runs)
+        hmode = DEM_HIST_REFERENCE if args.history == "reference" else DEM_HIST_CANONICAL
+        cwl = workloads.bidisperse_sheared_4m(cells=tuple(int(x) for x in args.cells.split(",")))
+        rc4 = timed_run(cwl, args, DEM_FP64, hmode)
+        cst = rc4["stats"]
+        print(json.dumps({"workload": cwl.name, "spheres": cwl.np, "value": rc4["value"],
+                          "unit": "particle-steps/s", "ms_per_step": rc4["ms"],
+                          "steps": args.steps, "ms_pass1": cst["ms_pass1"],
+                          "ms_pass2": cst["ms_pass2"], "ms_rebuild_per_step": cst["ms_rebuild"],
+                          "pair_contacts_mean": cst["mean_pair_contacts"],
+                          "wall_contacts": cst["wall_contacts"],
+                          "neighbour_entries_per_sphere": cst["neighbour_pairs"] / cwl.np,
+                          "autotune": rc4["tune"]}), flush=True)
+        return 0
     t0 = time.perf_counter()
     state = None
     if args.bed_cache and os.path.exists(args.bed_cache):
