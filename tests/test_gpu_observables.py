@@ -80,7 +80,8 @@ def test_observables_match_oracle_and_checker(hip, kw):
 
 def test_observables_of_a_packed_bed(hip):
     """A bed grown to phi 0.60 (the headline protocol at 12 000 spheres):
-    contact counts and bulk packing fraction against the checkers."""
+    contact counts and bulk packing fraction against the checkers, and the
+    global bond order Q6 of a random packing."""
     wl = workloads.packed_bed(hip, n=12000, settle_steps=100)
     g = System(hip, wl.red)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
@@ -88,4 +89,12 @@ def test_observables_of_a_packed_bed(hip):
     ob = _check(g, wl.red, st, nl=64)
     assert 0.57 < ob["packing_fraction"] < 0.66
     assert ob["coordination"] > 1.0
+    # disordered: the global bond order of a random packing (FCC 0.575, the
+    # rock-salt crystal of the second leg 0.354)
+    from bond_order import global_q6
+    dmean = 2.0 * float(np.mean(np.asarray(wl.red.radius)[wl.comp - 1]))
+    q6 = global_q6(st["x"], 1.2 * dmean, box=None)
+    print(f"packed bed: phi {ob['packing_fraction']:.4f}, Z {ob['coordination']:.3f}, Q6 {q6:.4f}",
+          flush=True)
+    assert q6 < 0.1, q6
     g.close()

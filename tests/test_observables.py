@@ -82,3 +82,17 @@ def test_pair_count_equals_brute_force():
             count += int(np.count_nonzero(mdsq[c[i], c[i + 1:]] - dsq > 0.0))
         assert count > 0
         assert overlapping_pairs(red, x, wl.comp) == count
+
+
+def test_global_q6_of_lattices_and_a_gas():
+    """tests/bond_order.py global_q6 against its published values: FCC
+    0.57452, simple cubic 1/(2 sqrt 2), an ideal gas nearly 0."""
+    from bond_order import global_q6
+    n = 5
+    g = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)], float)
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    fcc = (g[:, None, :] + base[None, :, :]).reshape(-1, 3).T
+    assert abs(global_q6(fcc, 0.75, box=[n] * 3) - 0.57452) < 1e-4
+    assert abs(global_q6(g.T, 1.1, box=[n] * 3) - 1.0 / (2.0 * np.sqrt(2.0))) < 1e-12
+    rng = np.random.default_rng(1)
+    assert global_q6(rng.uniform(0, 10, (3, 4000)), 1.0, box=[10] * 3) < 0.02
