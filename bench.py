@@ -527,6 +527,8 @@ def main(argv=None):
             "ms_pass1": st["ms_pass1"],
             "ms_pass2": st["ms_pass2"],
             "ms_rebuild_per_step": st["ms_rebuild"],
+            "max_halo": st["max_halo"],
+            "halo_overflow_tiles": st["halo_overflow"],
             "steps_before_timing": r64["steps_before"],
             "autotune": r64["tune"],
             "parallelism": "single GPU",

@@ -63,6 +63,8 @@ struct Ctl {
   unsigned long long stale_dropped; // stored pairs lost from the list at a rebuild
   unsigned long long pairs_upd_sum; // pair contacts summed over updating passes
   unsigned long long upd_evals;     // updating passes summed
+  int max_halo;                     // high-water mark of a tile's halo (staged passes)
+  int halo_over;                    // tiles whose halo exceeded DEM_HALO_CAP
 };
 
 enum {

@@ -201,6 +201,7 @@ static void free_state(dem_handle_t h) {
     dfree(h->pos[k]); dfree(h->kin[k]);
     dfree(h->es.cnt[k]); dfree(h->es.soff[k]);
     dfree(h->es.nbr[k]); dfree(h->es.flag[k]); dfree(h->es.hist[k]);
+    dfree(h->es.slot[k]); dfree(h->es.halo[k]); dfree(h->es.hcnt[k]);
     dfree(h->wflag[k]); dfree(h->whist[k]);
   }
   dfree(h->es.epar);
@@ -594,6 +595,8 @@ static int ensure_tmp(dem_handle_t h, size_t bytes) {
   return DEM_OK;
 }
 
+static int clear_contacts(dem_handle_t h);
+
 static int alloc_state(dem_handle_t h, int np) {
   free_state(h);
   const size_t N = (size_t)std::max(np, 1);
@@ -611,6 +614,9 @@ static int alloc_state(dem_handle_t h, int np) {
     if (e == hipSuccess) e = dalloc(&h->es.nbr[k], h->ecap);
     if (e == hipSuccess) e = dalloc(&h->es.flag[k], h->ecap);
     if (e == hipSuccess) e = dalloc(&h->es.hist[k], 3 * h->ecap);
+    if (e == hipSuccess) e = dalloc(&h->es.slot[k], h->ecap);
+    if (e == hipSuccess) e = dalloc(&h->es.halo[k], (size_t)nblocks(N, DEM_TILE) * DEM_HALO_CAP);
+    if (e == hipSuccess) e = dalloc(&h->es.hcnt[k], (size_t)nblocks(N, DEM_TILE));
     if (e == hipSuccess) e = dalloc(&h->wflag[k], N);
     if (e == hipSuccess) e = dalloc(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
   }
@@ -659,7 +665,9 @@ static int alloc_state(dem_handle_t h, int np) {
   HIPCHK(h, hipMemsetAsync(h->cell_count, 0, ((size_t)h->gncell + 1) * sizeof(int), h->stream));
   h->np = np;
   h->sc.np = np;
-  return DEM_OK;
+  // empty edge sets, current set 0 (also for a state of no particles, which
+  // dem_upload_state leaves before it clears the contacts)
+  return clear_contacts(h);
 }
 
 // Empty edge sets, no wall history, current set 0.
@@ -2119,6 +2127,8 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->ms_rebuild = h->ms_rebuild;
   out->ms_step = h->ms_step;
   out->mean_pair_contacts = c.upd_evals ? (double)c.pairs_upd_sum / (double)c.upd_evals : 0.0;
+  out->max_halo = c.max_halo;
+  out->halo_overflow = c.halo_over;
   return DEM_OK;
 }
 

@@ -146,12 +146,25 @@ struct StaleTab {
   int* cnt;
 };
 
+// Tiles of the staged force passes: particles [DEM_TILE*b, DEM_TILE*b +
+// DEM_TILE) form tile b (consecutive in cell-key order, so a compact region).
+// Its halo is the set of neighbours outside it, ascending, at most
+// DEM_HALO_CAP of them; the slot of an edge is the index of its neighbour in
+// the tile's staged records: j - DEM_TILE*b inside the tile, DEM_TILE + the
+// halo row outside it, DEM_SLOT_NONE when the halo overflowed.
+#define DEM_TILE DEM_BLOCK
+#define DEM_HALO_CAP 512
+#define DEM_SLOT_NONE 0xffff
+
 struct EdgeSets {
   int* cnt[2];
   int* soff[2];
   int* nbr[2];
   unsigned char* flag[2];
   double2* hist[2];  // three planes of ecap entries each
+  unsigned short* slot[2];  // staged slot of each edge's neighbour
+  int* halo[2];             // DEM_HALO_CAP rows per tile
+  int* hcnt[2];             // halo rows of each tile
   long long ecap;
   int* epar;  // device word: index of the current set
   StaleTab st;  // null pointers in the canonical history mode
@@ -312,6 +325,113 @@ __device__ __forceinline__ void load_hist(const double2* __restrict__ hist, long
   h2 = ld_nt(hist + 2 * ecap + e);
 }
 
+// The contribution of one contact to its particle's sums: fn and ft (FP64;
+// the FP32 sums of the mixed mode round them there) and the torque (the
+// mixed mode's FP32 torque, widened exactly).
+struct EdgeOut {
+  d3 fn, ft, tq;
+};
+
+// The contact of an edge whose pair overlaps (delta = MinDijSq - DijSq > 0):
+// the role of the pair ("i" of the reference), CommonCalculations, the
+// torque and, in an updating pass, the new history (src/MForces.f90:223-263).
+// KJ is the partner's Kin line.  Counters and errors go to acc; the force
+// and torque contributions are returned (acc_add sums them).
+template <bool UPDATE, bool MIXED>
+__device__ __forceinline__ EdgeOut edge_eval(const SysConst& s, const PassArgs& a,
+                                             const PairConst& pc, double rj, const d3& rij,
+                                             double dsq, double delta, const Kin& KJ,
+                                             unsigned char fl, double2 h0, double2 h1, double2 h2,
+                                             unsigned char* __restrict__ flag,
+                                             double2* __restrict__ hist, long long ecap,
+                                             long long e, d3 vi, d3 wi, double ri, int cell_p,
+                                             int xyz_p, int st_p, int id_p, EdgeAcc& acc) {
+  int role;
+  if (s.hist_mode == 0) {
+    role = ref_role(s, cell_p, xyz_p, KJ.rc, KJ.rxyz, st_p, KJ.st, id_p, KJ.id);
+    if (role < 0) { acc.err |= DEM_KERR_ROLE; role = id_p < KJ.id; }
+  } else {
+    role = id_p < KJ.id;
+  }
+  const double sg = role ? 1.0 : -1.0;
+  ContactIn in;
+  in.contacting = fl & 1;
+  in.ft0 = scl3(sg, mk3(h0.x, h0.y, h1.x));
+  in.vt0 = scl3(sg, mk3(h1.y, h2.x, h2.y));
+  if (!in.contacting && role) ++acc.n_col;
+  if (role) ++acc.n_pair;
+  const double dij = sqrt(dsq);
+  in.nij = div3(rij, dij);
+  in.disp_n = scl3(dij - pc.mindij, in.nij);
+  in.vij = sub3(kin_v(KJ), vi);
+  in.vijc = sub3(in.vij, cross3(add3(scl3(ri, wi), scl3(rj, kin_w(KJ))), in.nij));
+  in.delta_dsq = delta;
+  in.half_dt = a.half_dt;
+  const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
+  EdgeOut r;
+  r.fn = o.fn;
+  r.ft = o.ft;
+  if (MIXED) {
+    // the contact in FP64, its torque formed in FP32 as the sums (mixed.hpp)
+    const f3 t = sclf((float)ri, crossf(tof(in.nij), tof(o.ft)));
+    r.tq = mk3((double)t.x, (double)t.y, (double)t.z);
+  } else {
+    // torque of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
+    r.tq = role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft));
+  }
+  if (UPDATE) {
+    const d3 f = scl3(sg, o.ft), g = scl3(sg, o.vt);
+    st_nt(hist + e, make_double2(f.x, f.y));
+    st_nt(hist + ecap + e, make_double2(f.z, g.x));
+    st_nt(hist + 2 * ecap + e, make_double2(g.y, g.z));
+    if (!(fl & 1)) flag[e] = 1;
+    ++acc.n_stored;
+  }
+  return r;
+}
+
+// iForce = iForce + ijForceN + ijForceT, iTorque = iTorque + torque
+// (src/MForces.f90:381-385), in FP64, or in the compensated FP32 sums of the
+// mixed mode (mixed.hpp).
+template <bool MIXED>
+__device__ __forceinline__ void acc_add(EdgeAcc& acc, const d3& fn, const d3& ft, const d3& tq) {
+  if (MIXED) {
+    kahan_add3(acc.Ff, tof(fn));
+    kahan_add3(acc.Ff, tof(ft));
+    kahan_add3(acc.Tf, tof(tq));
+  } else {
+    acc.F = add3(add3(acc.F, fn), ft);
+    acc.T = add3(acc.T, tq);
+  }
+}
+
+template <bool UPDATE, bool MIXED>
+__device__ __forceinline__ void edge_contact(const SysConst& s, const PassArgs& a,
+                                             const PairConst& pc, double rj, const d3& rij,
+                                             double dsq, double delta, const Kin& KJ,
+                                             unsigned char fl, double2 h0, double2 h1, double2 h2,
+                                             unsigned char* __restrict__ flag,
+                                             double2* __restrict__ hist, long long ecap,
+                                             long long e, d3 vi, d3 wi, double ri, int cell_p,
+                                             int xyz_p, int st_p, int id_p, EdgeAcc& acc) {
+  const EdgeOut o = edge_eval<UPDATE, MIXED>(s, a, pc, rj, rij, dsq, delta, KJ, fl, h0, h1, h2,
+                                             flag, hist, ecap, e, vi, wi, ri, cell_p, xyz_p,
+                                             st_p, id_p, acc);
+  acc_add<MIXED>(acc, o.fn, o.ft, o.tq);
+}
+
+// A stored pair that no longer overlaps, in an updating pass.  The reference
+// removes it only when its cell loop visits it (RemovePairContact,
+// src/MForces.f90:268): reference cells the same or neighbours (rc, rxyz: the
+// partner's cell state); otherwise it persists.
+__device__ __forceinline__ void edge_separated(const SysConst& s, int cell_p, int xyz_p, int rc,
+                                               int rxyz, unsigned char* __restrict__ flag,
+                                               long long e, EdgeAcc& acc) {
+  const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, xyz_p, rc, rxyz);
+  if (keep) ++acc.n_stored;
+  else flag[e] = 0;
+}
+
 template <bool UPDATE, bool MIXED, int FRONT>
 __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
                                           const PairConst* ptab, const CompConst* ctab,
@@ -348,53 +468,9 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
     const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
     if (delta > 0.0) {
       if (!front) KJ = load_kin(a.kin_in + j);
-      int role;
-      if (s.hist_mode == 0) {
-        role = ref_role(s, cell_p, xyz_p, KJ.rc, KJ.rxyz, st_p, KJ.st, id_p, KJ.id);
-        if (role < 0) { acc.err |= DEM_KERR_ROLE; role = id_p < KJ.id; }
-      } else {
-        role = id_p < KJ.id;
-      }
-      const double sg = role ? 1.0 : -1.0;
-      ContactIn in;
-      in.contacting = fl & 1;
-      in.ft0 = scl3(sg, mk3(h0.x, h0.y, h1.x));
-      in.vt0 = scl3(sg, mk3(h1.y, h2.x, h2.y));
-      if (!in.contacting && role) ++acc.n_col;
-      if (role) ++acc.n_pair;
-      const double rj = ctab[cj].radius;
-      const double dij = sqrt(dsq);
-      in.nij = div3(rij, dij);
-      in.disp_n = scl3(dij - pc.mindij, in.nij);
-      in.vij = sub3(kin_v(KJ), vi);
-      in.vijc = sub3(in.vij, cross3(add3(scl3(ri, wi), scl3(rj, kin_w(KJ))), in.nij));
-      in.delta_dsq = delta;
-      in.half_dt = a.half_dt;
-      const ContactOut o = common_calc(in, rij, pc.kn, pc.yn, pc.kt, pc.yt, pc.musq);
-      if (MIXED) {
-        // the contact in FP64, its contributions summed in FP32 (mixed.hpp)
-        const f3 ft = tof(o.ft);
-        kahan_add3(acc.Ff, tof(o.fn));
-        kahan_add3(acc.Ff, ft);
-        kahan_add3(acc.Tf, sclf((float)ri, crossf(tof(in.nij), ft)));
-      } else {
-        // iForce = iForce + ijForceN + ijForceT (src/MForces.f90:381); torque
-        // of "i" is (Ri*Nij) x FT, of "j" Rj*(Nij x FT) (src/MForces.f90:259,385)
-        acc.F = add3(add3(acc.F, o.fn), o.ft);
-        acc.T = add3(acc.T,
-                     role ? cross3(scl3(ri, in.nij), o.ft) : scl3(ri, cross3(in.nij, o.ft)));
-      }
-      if (UPDATE) {
-        const d3 f = scl3(sg, o.ft), g = scl3(sg, o.vt);
-        st_nt(hist + e, make_double2(f.x, f.y));
-        st_nt(hist + ecap + e, make_double2(f.z, g.x));
-        st_nt(hist + 2 * ecap + e, make_double2(g.y, g.z));
-        if (!(fl & 1)) flag[e] = 1;
-        ++acc.n_stored;
-      }
+      edge_contact<UPDATE, MIXED>(s, a, pc, ctab[cj].radius, rij, dsq, delta, KJ, fl, h0, h1, h2,
+                                  flag, hist, ecap, e, vi, wi, ri, cell_p, xyz_p, st_p, id_p, acc);
     } else if (UPDATE && (fl & 1)) {
-      // The reference removes a stored pair only when its cell loop visits
-      // it (RemovePairContact, src/MForces.f90:268); otherwise it persists.
       int rc, rxyz;
       if (front) {
         rc = KJ.rc;
@@ -404,13 +480,84 @@ __device__ __forceinline__ void pair_loop(const SysConst& s, const PassArgs& a,
         rc = c.y;
         rxyz = c.w;
       }
-      const bool keep = (s.hist_mode == 0) && !ref_adjacent(s, cell_p, xyz_p, rc, rxyz);
-      if (keep) ++acc.n_stored;
-      else flag[e] = 0;
+      edge_separated(s, cell_p, xyz_p, rc, rxyz, flag, e, acc);
     }
   }
 }
 
+// The same walk with the partner records staged in LDS by the pass (tile
+// records and the tile's halo, pass_body_v): the position of every partner,
+// and the Kin line of partners inside the tile, come from LDS; the Kin line of
+// a halo partner is gathered from memory (its index from the staged halo
+// rows), and an edge whose partner is not staged (DEM_SLOT_NONE) gathers both.
+// The arithmetic and its order are those of pair_loop.
+template <bool UPDATE, bool MIXED, int FRONT>
+__device__ __forceinline__ void pair_loop_staged(
+    const SysConst& s, const PassArgs& a, const PairConst* ptab, const CompConst* ctab,
+    const unsigned short* __restrict__ slot, const int* __restrict__ nbr,
+    unsigned char* __restrict__ flag, double2* __restrict__ hist, long long ecap, long long e,
+    int deg, int nfront, const double4& P, d3 vi, d3 wi, int ci, double ri, int cell_p, int xyz_p,
+    int st_p, int id_p, const double4* sp, const Kin* sk, const int* shalo, EdgeAcc& acc) {
+  if (!FRONT) nfront = deg;
+  unsigned sl_n = 0;
+  unsigned char fl_n = 0;
+  if (deg > 0) {
+    sl_n = slot[e];
+    fl_n = flag[e];
+  }
+  for (int k = 0; k < deg; ++k, e += 64) {
+    const unsigned sl = sl_n;
+    const unsigned char fl = fl_n;
+    if (k + 1 < deg) {
+      sl_n = slot[e + 64];
+      fl_n = flag[e + 64];
+    }
+    const bool front = k < nfront;
+    const bool inside = sl < DEM_TILE;
+    int j = 0;
+    double4 Q;
+    Kin KJ;
+    if (inside) {
+      Q = sp[sl];
+      if (front) KJ = sk[sl];
+    } else {
+      if (sl != DEM_SLOT_NONE) {
+        Q = sp[sl];
+        j = shalo[sl - DEM_TILE];
+      } else {
+        j = nbr[e];
+        Q = a.pos_in[j];
+      }
+      if (front) KJ = load_kin(a.kin_in + j);
+    }
+    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
+    if (fl & 1) load_hist(hist, ecap, e, h0, h1, h2);
+    const int cj = (int)Q.w;
+    const d3 rij = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
+    const double dsq = dot3(rij, rij);
+    const PairConst& pc = ptab[ci + cj * s.nc];
+    const double delta = pc.mindijsq - dsq;   // overlap test (src/MForces.f90:223-224)
+    if (delta > 0.0) {
+      if (!front) KJ = inside ? sk[sl] : load_kin(a.kin_in + j);
+      edge_contact<UPDATE, MIXED>(s, a, pc, ctab[cj].radius, rij, dsq, delta, KJ, fl, h0, h1, h2,
+                                  flag, hist, ecap, e, vi, wi, ri, cell_p, xyz_p, st_p, id_p, acc);
+    } else if (UPDATE && (fl & 1)) {
+      int rc, rxyz;
+      if (front) {
+        rc = KJ.rc;
+        rxyz = KJ.rxyz;
+      } else if (inside) {
+        rc = sk[sl].rc;
+        rxyz = sk[sl].rxyz;
+      } else {
+        const int4 c = reinterpret_cast<const int4*>(a.kin_in + j)[3];
+        rc = c.y;
+        rxyz = c.w;
+      }
+      edge_separated(s, cell_p, xyz_p, rc, rxyz, flag, e, acc);
+    }
+  }
+}
 // One force pass over the particles [blk*BS, blk*BS+BS) of one system,
 // BS lanes per block.  Neither the block size nor FRONT changes the
 // arithmetic of a contact, the order of a particle's sums or the integer
@@ -441,18 +588,34 @@ __device__ __forceinline__ StepVars step_vars(const PassArgs& a) {
   return v;
 }
 
+// Pass variants (the FRONT template argument of the passes): bit 0 is the
+// front rule of the pair walk (1: the partner's Kin line is requested with
+// its position only for front edges); bit 1 stages the partner records of
+// the block's tile in LDS (pair_loop_staged; BS == DEM_TILE), which leaves
+// room in LDS for the pair tables of at most DEM_LDS_NC_STAGED components.
+#define DEM_PV_STAGED 2
+#define DEM_LDS_NC_STAGED 8
+
 template <int MODE, bool UPDATE, bool MIXED, int BS = DEM_BLOCK, int FRONT = 1>
 __device__ __forceinline__ void pass_body_v(const SysConst& s, const PassArgs& a,
                                             const StepVars& stv, int blk) {
-  __shared__ PairConst sh_pair[DEM_LDS_NC * DEM_LDS_NC];
-  __shared__ CompConst sh_comp[DEM_LDS_NC];
+  constexpr bool STAGED = (FRONT & DEM_PV_STAGED) != 0;
+  constexpr int NCL = STAGED ? DEM_LDS_NC_STAGED : DEM_LDS_NC;
+  static_assert(!STAGED || BS == DEM_TILE, "a staged pass runs one block per tile");
+
+  __shared__ PairConst sh_pair[NCL * NCL];
+  __shared__ CompConst sh_comp[NCL];
+  // staged partner records: the tile's own particles, then its halo
+  __shared__ double4 st_pos[STAGED ? DEM_TILE + DEM_HALO_CAP : 1];
+  __shared__ Kin st_kin[STAGED ? DEM_TILE : 1];
+  __shared__ int st_halo[STAGED ? DEM_HALO_CAP : 1];
   Ctl* ctl = a.ctl;
   // Escapes are published between passes (k_pass_reduce_*), so every block of
   // a pass reads the same value here.
   if (ctl->escaped) return;
   const int t = threadIdx.x;
   if (blk == 0 && t == 0) *stv.rb_clear = 0;
-  const bool lds_tab = s.nc <= DEM_LDS_NC;
+  const bool lds_tab = s.nc <= NCL;
   if (lds_tab) {
     for (int q = t; q < s.nc * s.nc; q += BS) sh_pair[q] = s.pair[q];
     if (t < s.nc) sh_comp[t] = s.comp[t];
@@ -482,6 +645,20 @@ __device__ __forceinline__ void pass_body_v(const SysConst& s, const PassArgs& a
     deg = cnt_deg(cn);
     nfront = cnt_front(cn);
     e = sell_edge(a.E.soff[ep], i, 0);
+    if (STAGED) {
+      st_pos[t] = P;
+      st_kin[t] = K;
+    }
+  }
+  if (STAGED) {
+    // the halo's positions (independent gathers, all in flight together)
+    const int hn = a.E.hcnt[ep][blk];
+    const int* hrow = a.E.halo[ep] + (size_t)blk * DEM_HALO_CAP;
+    for (int k = t; k < hn; k += BS) {
+      const int j = hrow[k];
+      st_halo[k] = j;
+      st_pos[DEM_TILE + k] = a.pos_in[j];
+    }
   }
   const int ci = (int)P.w;
   const CompConst cc = s.comp[active ? ci : 0];
@@ -493,10 +670,17 @@ __device__ __forceinline__ void pass_body_v(const SysConst& s, const PassArgs& a
   acc.Ff.s = acc.Ff.c = acc.Tf.s = acc.Tf.c = mkf(0.0f, 0.0f, 0.0f);
   acc.n_col = acc.n_pair = acc.n_stored = 0;
   acc.err = 0;
-  if (lds_tab) __syncthreads();
-  pair_loop<UPDATE, MIXED, FRONT>(s, a, lds_tab ? sh_pair : s.pair, lds_tab ? sh_comp : s.comp,
-                                  a.E.nbr[ep], a.E.flag[ep], a.E.hist[ep], a.E.ecap, e, deg,
-                                  nfront, P, vi, wi, ci, ri, cell_p, xyz_p, st_p, id_p, acc);
+  if (STAGED || lds_tab) __syncthreads();
+  if (STAGED)
+    pair_loop_staged<UPDATE, MIXED, FRONT & 1>(
+        s, a, lds_tab ? sh_pair : s.pair, lds_tab ? sh_comp : s.comp, a.E.slot[ep], a.E.nbr[ep],
+        a.E.flag[ep], a.E.hist[ep], a.E.ecap, e, deg, nfront, P, vi, wi, ci, ri, cell_p, xyz_p,
+        st_p, id_p, st_pos, st_kin, st_halo, acc);
+  else
+    pair_loop<UPDATE, MIXED, FRONT & 1>(s, a, lds_tab ? sh_pair : s.pair,
+                                        lds_tab ? sh_comp : s.comp, a.E.nbr[ep], a.E.flag[ep],
+                                        a.E.hist[ep], a.E.ecap, e, deg, nfront, P, vi, wi, ci, ri,
+                                        cell_p, xyz_p, st_p, id_p, acc);
   d3 F = acc.F, T = acc.T;
   if (MIXED) {
     F = add3(F, kahan_value(acc.Ff));
@@ -694,10 +878,12 @@ __device__ __forceinline__ void pass_body(const SysConst& s, const PassArgs& a, 
 
 // Launch shapes of the force passes: lanes per block, the occupancy the
 // register allocation targets (wavefronts per SIMD, 512/WAVES VGPRs each;
-// above 3 the pass spills to scratch) and FRONT (pair_loop: 1 = the partner's
-// Kin line is gathered with its position only for the front edges, 0 = for
-// every edge).  Shape 0 is the default; the others are candidates of the
-// autotuner (dem_autotune).  Every shape gives the same results bit for bit.
+// above 3 the pass spills to scratch) and the pass variant FRONT (bit 0, the
+// front rule of pair_loop: 1 = the partner's Kin line is gathered with its
+// position only for the front edges, 0 = for every edge; bit 1: partner
+// records staged in LDS per tile, pair_loop_staged).  Shape 0 is the
+// default; the others are candidates of the autotuner (dem_autotune).  Every
+// shape gives the same results bit for bit.
 #ifndef DEM_NSHAPES
 #define DEM_NSHAPES 8
 #endif
@@ -713,9 +899,9 @@ struct PassShape;
 DEM_PASS_SHAPE(0, 256, DEM_PASS_WAVES, 1)
 DEM_PASS_SHAPE(1, 128, DEM_PASS_WAVES, 1)
 DEM_PASS_SHAPE(2, 64, DEM_PASS_WAVES, 1)
-DEM_PASS_SHAPE(3, 512, DEM_PASS_WAVES, 1)
-DEM_PASS_SHAPE(4, 256, 4, 1)
-DEM_PASS_SHAPE(5, 256, 2, 1)
+DEM_PASS_SHAPE(3, 256, 2, 1)
+DEM_PASS_SHAPE(4, 256, DEM_PASS_WAVES, 1 | DEM_PV_STAGED)
+DEM_PASS_SHAPE(5, 256, DEM_PASS_WAVES, 0 | DEM_PV_STAGED)
 DEM_PASS_SHAPE(6, 128, DEM_PASS_WAVES, 0)
 DEM_PASS_SHAPE(7, 256, DEM_PASS_WAVES, 0)
 #undef DEM_PASS_SHAPE
@@ -1123,6 +1309,104 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
 
 #define DEM_FILL_STORED 24
 
+// The halo of tile blk of the edge set q just filled, and the staged slot of
+// every edge of the tile (one block of DEM_TILE threads, one per particle).
+// The neighbours outside the tile are collected in an LDS hash set, compacted
+// and sorted ascending, so the halo does not depend on the order in which the
+// lanes inserted them; the first DEM_HALO_CAP rows are kept.  The slot of an
+// edge is found by binary search in the sorted rows.
+#define DEM_HALO_HASH 4096
+#define DEM_HALO_SORT 2048
+__device__ __forceinline__ void halo_build_body(const SysConst& s, const EdgeSets& E, int q,
+                                                Ctl* ctl, int blk) {
+  static_assert(DEM_TILE == DEM_BLOCK, "one thread per particle of a tile");
+  __shared__ int h_tab[DEM_HALO_HASH];
+  __shared__ int h_row[DEM_HALO_SORT];
+  const int t = threadIdx.x;
+  const int t0 = blk * DEM_TILE, i = t0 + t;
+  for (int k = t; k < DEM_HALO_HASH; k += DEM_BLOCK) h_tab[k] = -1;
+  __syncthreads();
+  int deg = 0;
+  long long e0 = 0;
+  if (i < s.np) {
+    deg = cnt_deg(E.cnt[q][i]);
+    e0 = sell_edge(E.soff[q], i, 0);
+  }
+  const int* nbr = E.nbr[q];
+  for (int m = 0; m < deg; ++m) {
+    const int j = nbr[e0 + 64LL * m];
+    if (j >= t0 && j < t0 + DEM_TILE) continue;
+    unsigned hs = ((unsigned)j * 2654435761u) >> 20;  // 12 bits
+    // at most DEM_HALO_HASH probes: a full table leaves j unstaged
+    for (int probe = 0; probe < DEM_HALO_HASH; ++probe) {
+      const int old = atomicCAS(&h_tab[hs], -1, j);
+      if (old == -1 || old == j) break;
+      hs = (hs + 1) & (DEM_HALO_HASH - 1);
+    }
+  }
+  __syncthreads();
+  constexpr int PER = DEM_HALO_HASH / DEM_BLOCK;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) c += h_tab[t * PER + k] >= 0;
+  int tot;
+  int off = block_scan_excl(c, &tot);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int v = h_tab[t * PER + k];
+    if (v >= 0) {
+      if (off < DEM_HALO_SORT) h_row[off] = v;
+      ++off;
+    }
+  }
+  const int n = min(tot, DEM_HALO_SORT);
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int k = n + t; k < P; k += DEM_BLOCK) h_row[k] = 0x7fffffff;
+  __syncthreads();
+  for (int w = 2; w <= P; w <<= 1)
+    for (int d = w >> 1; d > 0; d >>= 1) {
+      for (int k = t; k < P; k += DEM_BLOCK) {
+        const int l = k ^ d;
+        if (l > k) {
+          const bool up = (k & w) == 0;
+          const int a = h_row[k], b = h_row[l];
+          if ((a > b) == up) {
+            h_row[k] = b;
+            h_row[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  const int hn = min(n, DEM_HALO_CAP);
+  int* halo = E.halo[q] + (size_t)blk * DEM_HALO_CAP;
+  for (int k = t; k < hn; k += DEM_BLOCK) halo[k] = h_row[k];
+  if (t == 0) {
+    E.hcnt[q][blk] = hn;
+    atomicMax(&ctl->max_halo, tot);
+    if (tot > DEM_HALO_CAP) atomicAdd(&ctl->halo_over, 1);
+  }
+  unsigned short* slot = E.slot[q];
+  for (int m = 0; m < deg; ++m) {
+    const long long e = e0 + 64LL * m;
+    const int j = nbr[e];
+    int sl;
+    if (j >= t0 && j < t0 + DEM_TILE) {
+      sl = j - t0;
+    } else {
+      int lo = 0, hi = hn;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (h_row[mid] < j) lo = mid + 1;
+        else hi = mid;
+      }
+      sl = (lo < hn && h_row[lo] == j) ? DEM_TILE + lo : DEM_SLOT_NONE;
+    }
+    slot[e] = (unsigned short)sl;
+  }
+}
+
 // Pass 2 of the list build: fill the new edge set and carry the history of
 // every pair that is still listed.  src maps new particle indices to the ones
 // the current edge set uses (null = identity; a resort passes its
@@ -1257,6 +1541,7 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
     if (dropped) atomicAdd(&ctl->stale_dropped, dropped);
     if (nedges) atomicAdd(&ctl->nbpairs, nedges);
   }
+  halo_build_body(s, E, q, ctl, blk);
 }
 __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
                                                         const int* start,

@@ -96,6 +96,7 @@ class Stats(C.Structure):
         ("kinetic_energy", C.c_double), ("max_overlap", C.c_double), ("ms_pass1", C.c_double),
         ("ms_pass2", C.c_double), ("ms_rebuild", C.c_double), ("ms_step", C.c_double),
         ("mean_pair_contacts", C.c_double), ("time", C.c_double), ("next_step", C.c_int64),
+        ("max_halo", C.c_int64), ("halo_overflow", C.c_int64),
     ]
 
     def asdict(self):

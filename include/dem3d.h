@@ -178,6 +178,10 @@ typedef struct {
   int64_t next_step;        /* step number a batch gives the system's next
                                MoveParticles (first_step + steps of the last
                                dem_step; MethodOrder initially)                */
+  int64_t max_halo;         /* largest halo of a 256-particle tile of the
+                               neighbour lists built so far (staged passes)    */
+  int64_t halo_overflow;    /* tiles whose halo exceeded its 512 staged rows
+                               (their outer partners are read from memory)     */
 } dem_stats_t;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -515,10 +519,14 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
 /* ---- launch shapes and the autotuner ------------------------------------ */
 /* No counterpart in the reference.  The force passes come in DEM_NSHAPES
  * launch shapes: lanes per block, the occupancy (wavefronts per SIMD) their
- * register allocation targets, and whether the partner's 64-byte velocity
- * record is gathered with its position for every list entry or only for the
- * entries the list rebuild put first as contacts (the others fetch it when
- * they overlap).  Every shape gives bitwise the same results, so a shape
+ * register allocation targets, and a variant: bit 0, whether the partner's
+ * 64-byte velocity record is gathered with its position for every list entry
+ * or only for the entries the list rebuild put first as contacts (the others
+ * fetch it when they overlap); bit 1, whether each block first stages in LDS
+ * the records of its tile of 256 particles and the positions of the tile's
+ * halo (its neighbours outside the tile, listed by every neighbour-list
+ * rebuild), so that most partner records come from LDS.  Every shape gives
+ * bitwise the same results, so a shape
  * changes speed only; shape 0 (256 lanes, 3 waves, contacts only) is the
  * default.  The autotuner times every shape on the live system while
  * stepping it: each shape runs 1 untimed step (its first launch) and then
@@ -539,9 +547,11 @@ typedef struct {
   int64_t steps;               /* MoveParticles steps the tuning advanced      */
   int32_t block[DEM_NSHAPES];  /* lanes per block of each shape                */
   int32_t waves[DEM_NSHAPES];  /* wavefronts per SIMD it targets               */
-  int32_t front[DEM_NSHAPES];  /* 1: partner velocities gathered with the
-                                  position only for the contact-first entries
-                                  of a list; 0: for every entry               */
+  int32_t front[DEM_NSHAPES];  /* variant: bit 0 set, partner velocities
+                                  gathered with the position only for the
+                                  contact-first entries of a list (clear: for
+                                  every entry); bit 1, tile and halo staged
+                                  in LDS                                       */
   double ms[DEM_NSHAPES];      /* mean device ms per step: the two passes
                                   (dem_autotune), the whole step
                                   (dem_batch_autotune)                         */
