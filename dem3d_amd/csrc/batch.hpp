@@ -196,7 +196,7 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count_batch(const BatchRec* _
                  b.r.ctl, m.y);
 }
 
-__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill_batch(const BatchRec* __restrict__ rec,
+__global__ __launch_bounds__(DEM_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nbr_fill_batch(const BatchRec* __restrict__ rec,
                                                               const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const BatchRec& b = rec[m.x];

@@ -1666,6 +1666,25 @@ static int shape_block(int shape) {
   shape_table(b, w, d);
   return b[shape];
 }
+static bool shape_staged(int shape) {
+  int b[DEM_NSHAPES], w[DEM_NSHAPES], d[DEM_NSHAPES];
+  shape_table(b, w, d);
+  return (d[shape] & DEM_PV_STAGED) != 0;
+}
+
+// Tile halos are listed by the neighbour-list rebuilds only while a staged
+// launch shape is in use: switching one on lists the halos of the current
+// lists (k_halo_build, on stream st), switching it off stops the listing.
+static int sync_halo(dem_handle_t h, bool want, hipStream_t st) {
+  if (want == (h->es.halo_on != 0)) return DEM_OK;
+  h->es.halo_on = want ? 1 : 0;
+  HIPCHK(h, hipMemcpyAsync(h->d_es, &h->es, sizeof(EdgeSets), hipMemcpyHostToDevice, st));
+  if (want && h->np > 0)
+    hipLaunchKernelGGL(k_halo_build, dim3(nblocks(h->np, DEM_TILE)), dim3(DEM_TILE), 0, st,
+                       h->sc, h->es, h->ctl);
+  HIPCHK(h, hipGetLastError());
+  return DEM_OK;
+}
 
 // The pass kernel in launch shape `shape` (nb blocks).  Only the steps'
 // passes (predictor, corrector) come in every shape; the others run in the
@@ -1927,6 +1946,10 @@ int dem_step(dem_handle_t h, int64_t nsteps, int64_t first_step, int64_t* steps_
     return DEM_OK;
   }
   hipStream_t st = h->stream;
+  {
+    const int rc = sync_halo(h, shape_staged(h->shape), st);
+    if (rc) return rc;
+  }
   {
     int big = 0x7fffffff;
     HIPCHK(h, hipMemcpyAsync(&h->ctl->stop_step, &big, sizeof(int), hipMemcpyHostToDevice, st));
@@ -3107,7 +3130,8 @@ static int persistent_groups(dem_batch_t b, int ns, bool mixed, int front) {
 static int batch_persistent_front(const dem_batch* b) {
   int blk[DEM_NSHAPES], wv[DEM_NSHAPES], fr[DEM_NSHAPES];
   shape_table(blk, wv, fr);
-  return blk[b->shape] == DEM_BLOCK && wv[b->shape] == DEM_PASS_WAVES ? fr[b->shape] : fr[0];
+  // the persistent kernel walks the lists without LDS staging (bit 1)
+  return (blk[b->shape] == DEM_BLOCK && wv[b->shape] == DEM_PASS_WAVES ? fr[b->shape] : fr[0]) & 1;
 }
 
 // nsteps steps of every system by persistent launches of persist_steps
@@ -3350,6 +3374,11 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   }
   const int ns = (int)n;
   // the table of this call, once; every step then advances it on the device
+  const bool pers = batch_persistent_now(b);
+  for (size_t i = 0; i < n; ++i) {
+    rc = sync_halo(b->sys[i], !pers && shape_staged(b->shape), st);
+    if (rc) return bfail(b, rc, "system %zu: %s", i, dem_last_error(b->sys[i]));
+  }
   BatchRec* R = b->h_rec[0];
   if (b->ev_used[0]) BHIPCHK(b, hipEventSynchronize(b->ev[0]));
   for (size_t i = 0; i < n; ++i) batch_rec_init(b->sys[i], R[i]);
@@ -3361,7 +3390,6 @@ int dem_batch_step(dem_batch_t b, int64_t nsteps, int64_t* steps_done, int32_t* 
   for (size_t i = 0; i < n; ++i) moving = moving || b->sys[i]->nw_move > 0;
   hipLaunchKernelGGL(k_reset_escape, dim3(nblocks(ns, 64)), dim3(64), 0, st, (const BatchRec*)rec,
                      ns);
-  const bool pers = batch_persistent_now(b);
   // the launch-per-phase path prepares each step's arguments in the records;
   // the persistent launch derives them from the records as uploaded
   if (!pers) hipLaunchKernelGGL(k_batch_prepare, dim3(nblocks(ns, 64)), dim3(64), 0, st, rec, ns);

@@ -168,6 +168,7 @@ struct EdgeSets {
   long long ecap;
   int* epar;  // device word: index of the current set
   StaleTab st;  // null pointers in the canonical history mode
+  int halo_on;  // the rebuilds list tile halos and slots (a staged shape is in use)
 };
 
 __host__ __device__ __forceinline__ unsigned long long pair_key(int ida, int idb) {
@@ -1222,13 +1223,17 @@ __global__ void k_cell_sort(int ncell, const int* start, int* cell_idx, const in
 }
 
 // Visits every j != i with |r_ij|^2 < (r_i + r_j + skin)^2 (minimum image),
-// scanning the 27 cells around i's cell in a fixed order: fn(j, |r_ij|^2,
-// component of j).
-template <typename F>
+// scanning the 27 cells around i's cell in a fixed order (dz, dy, dx, then
+// the cell's slice in order): fn(j, |r_ij|^2, component of j).  rc2 is the
+// table of (r_i + r_j + skin)^2 by component pair (LDS, staged by the
+// caller).  The slices of a row of three cells are requested together, and
+// their particles B at a time (indices, then positions), so a lane waits for
+// a few round trips per row instead of two per candidate.
+template <int B = 4, typename F>
 __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const double4* pos,
                                                    int i, const double4& P,
                                                    const int* start, const int* cell_idx,
-                                                   F&& fn) {
+                                                   const double* rc2, F&& fn) {
   const int ci = (int)P.w;
   const Int3 g = grid_coords(s, P.x, P.y, P.z);
   const int gx = g.x, gy = g.y, gz = g.z;
@@ -1246,27 +1251,70 @@ __device__ __forceinline__ void for_each_neighbour(const SysConst& s, const doub
         if (s.gn[1] == 2 && dy == 1) continue;
         cy = (cy + s.gn[1]) % s.gn[1];
       } else if (cy < 0 || cy >= s.gn[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
+      int cb[3], ce[3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const int dx = u - 1;
         int cx = gx + dx;
+        bool skip = false;
         if (s.pbc[0]) {
-          if (s.gn[0] == 1 && dx != 0) continue;
-          if (s.gn[0] == 2 && dx == 1) continue;
+          if (s.gn[0] == 1 && dx != 0) skip = true;
+          if (s.gn[0] == 2 && dx == 1) skip = true;
           cx = (cx + s.gn[0]) % s.gn[0];
-        } else if (cx < 0 || cx >= s.gn[0]) continue;
-        const int c = grid_key(s, cx, cy, cz);
-        const int e = start[c + 1];
-        for (int p = start[c]; p < e; ++p) {
-          const int j = cell_idx[p];
-          if (j == i) continue;
-          const double4 Q = pos[j];
-          const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
-          const double dsq = dot3(r, r);
-          const int cj = (int)Q.w;
-          if (dsq < (s.cut_all ? s.rc2_all : s.pair[ci + cj * s.nc].rc2)) fn(j, dsq, cj);
+        } else if (cx < 0 || cx >= s.gn[0]) {
+          skip = true;
+        }
+        cb[u] = ce[u] = 0;
+        if (!skip) {
+          const int c = grid_key(s, cx, cy, cz);
+          cb[u] = start[c];
+          ce[u] = start[c + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        for (int p0 = cb[u]; p0 < ce[u]; p0 += B) {
+          int jj[B];
+          double4 QQ[B];
+#pragma unroll
+          for (int v = 0; v < B; ++v) jj[v] = (p0 + v < ce[u]) ? cell_idx[p0 + v] : -1;
+#pragma unroll
+          for (int v = 0; v < B; ++v)
+            if (jj[v] >= 0) QQ[v] = pos[jj[v]];
+#pragma unroll
+          for (int v = 0; v < B; ++v) {
+            const int j = jj[v];
+            if (j < 0 || j == i) continue;
+            const double4 Q = QQ[v];
+            const d3 r = min_image3(s, mk3(Q.x - P.x, Q.y - P.y, Q.z - P.z));
+            const double dsq = dot3(r, r);
+            const int cj = (int)Q.w;
+            const int q = ci + cj * s.nc;
+            if (dsq < (s.cut_all ? s.rc2_all : rc2 ? rc2[q] : s.pair[q].rc2)) fn(j, dsq, cj);
+          }
         }
       }
     }
   }
+}
+
+// The pair tables the rebuild kernels read per candidate, (r_i + r_j +
+// skin)^2 and MinDijSq by component pair, staged in LDS by one block of
+// DEM_BLOCK threads (ok = 0: more than DEM_LDS_NC components, read from
+// SysConst instead).
+struct NbrTab {
+  double rc2[DEM_LDS_NC * DEM_LDS_NC];
+  double mindsq[DEM_LDS_NC * DEM_LDS_NC];
+  bool ok;
+};
+__device__ __forceinline__ void stage_nbr_tab(const SysConst& s, NbrTab& t) {
+  t.ok = s.nc <= DEM_LDS_NC;
+  if (t.ok)
+    for (int q = threadIdx.x; q < s.nc * s.nc; q += DEM_BLOCK) {
+      t.rc2[q] = s.pair[q].rc2;
+      t.mindsq[q] = s.pair[q].mindijsq;
+    }
+  __syncthreads();
 }
 
 // Pass 1 of the list build: neighbours per particle (capped at max_nb) into
@@ -1277,12 +1325,15 @@ __device__ __forceinline__ void nbr_count_body(const SysConst& s, const double4*
                                                const EdgeSets& E, int* slice_w,
                                                const int* gate, int force, Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
+  __shared__ NbrTab tab;
+  stage_nbr_tab(s, tab);
   const int q = 1 - *E.epar;
   const int i = blk * DEM_BLOCK + threadIdx.x;
   int n = 0, err = 0;
   if (i < s.np) {
     const double4 P = pos[i];
-    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int, double, int) { ++n; });
+    for_each_neighbour<4>(s, pos, i, P, start, cell_idx, tab.ok ? tab.rc2 : nullptr,
+                          [&](int, double, int) { ++n; });
     if (n > s.max_nb) {
       err = DEM_KERR_NBR_OVERFLOW;
       n = s.max_nb;
@@ -1418,6 +1469,8 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
                                               const Kin* kin_cur, double4* xref,
                                               const int* gate, int force, Ctl* ctl, int blk) {
   if (gate_closed(gate, force, ctl)) return;
+  __shared__ NbrTab tab;
+  stage_nbr_tab(s, tab);
   const int p = *E.epar, q = 1 - p;
   const int i = blk * DEM_BLOCK + threadIdx.x;
   const long long ecap = E.ecap;
@@ -1443,42 +1496,81 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
     // new neighbour is looked up there instead of in the old list in memory.
     // More than DEM_FILL_STORED of them (a dense cluster around a large
     // sphere) falls back to the search in memory.
-    int sj[DEM_FILL_STORED], sm[DEM_FILL_STORED];
+    // (row t of the table is the t-th set bit of oldmask)
+    int sj[DEM_FILL_STORED];
     int ns = 0;
     bool many = false;
 #pragma unroll
-    for (int t = 0; t < DEM_FILL_STORED; ++t) sj[t] = sm[t] = -1;
-    for (int m = 0; m < odeg; ++m) {
-      if (!(oflag[ob + 64LL * m] & 1)) continue;
-      const int jo = onbr[ob + 64LL * m];
+    for (int t = 0; t < DEM_FILL_STORED; ++t) sj[t] = -1;
+    // bit m: entry m of the old list stores history (flags requested eight at
+    // a time, so the lane waits for one round trip per eight entries)
+    unsigned long long oldmask = 0;
+    for (int m0 = 0; m0 < odeg; m0 += 8) {
+      unsigned char f[8];
 #pragma unroll
-      for (int t = 0; t < DEM_FILL_STORED; ++t)
-        if (t == ns) {
-          sj[t] = jo;
-          sm[t] = m;
-        }
-      if (ns == DEM_FILL_STORED) many = true;
-      else ++ns;
+      for (int u = 0; u < 8; ++u) f[u] = (m0 + u < odeg) ? oflag[ob + 64LL * (m0 + u)] : 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) oldmask |= (unsigned long long)(f[u] & 1) << (m0 + u);
+    }
+    // their partners, four requests at a time
+    for (unsigned long long mm = oldmask; mm;) {
+      int mq[4], jq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        mq[u] = mm ? __builtin_ctzll(mm) : -1;
+        if (mm) mm &= mm - 1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jq[u] = mq[u] >= 0 ? onbr[ob + 64LL * mq[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (mq[u] < 0) continue;
+#pragma unroll
+        for (int t = 0; t < DEM_FILL_STORED; ++t)
+          if (t == ns) sj[t] = jq[u];
+        if (ns == DEM_FILL_STORED) many = true;
+        else ++ns;
+      }
     }
     unsigned long long found = 0;  // bit m: stored entry m of the old list was carried
     // rows of the stale-pair table (none in the canonical history mode)
     const StaleTab& T = E.st;
     const int tn = (s.hist_mode == 0 && T.cnt) ? min(T.cnt[0], DEM_STALE_ROWS) : 0;
     const int id_i = tn > 0 ? kin_cur[i].id : 0;
-    for_each_neighbour(s, pos, i, P, start, cell_idx, [&](int j, double dsq, int cj) {
+    // one carried history in flight: edge pe of the new set, values ph*
+    long long pe = -1;
+    double2 ph0, ph1, ph2;
+    double2* g = E.hist[q];
+    auto put_pending = [&]() {
+      if (pe >= 0) {
+        g[pe] = ph0;
+        g[ecap + pe] = ph1;
+        g[2 * ecap + pe] = ph2;
+      }
+    };
+    for_each_neighbour<2>(s, pos, i, P, start, cell_idx, tab.ok ? tab.rc2 : nullptr,
+                          [&](int j, double dsq, int cj) {
       if (kf > kb) return;
       const int jo = src ? src[j] : j;
       int mo = -1;
       if (!many) {
+        int tf = -1;
 #pragma unroll
         for (int t = 0; t < DEM_FILL_STORED; ++t)
-          if (sj[t] == jo) mo = sm[t];
+          if (sj[t] == jo) tf = t;
+        if (tf >= 0) {
+          unsigned long long mm = oldmask;
+          for (int t = 0; t < tf; ++t) mm &= mm - 1;
+          mo = __builtin_ctzll(mm);
+        }
       } else {
-        for (int m = 0; m < odeg; ++m)
-          if ((oflag[ob + 64LL * m] & 1) && onbr[ob + 64LL * m] == jo) {
+        for (unsigned long long mm = oldmask; mm; mm &= mm - 1) {
+          const int m = __builtin_ctzll(mm);
+          if (onbr[ob + 64LL * m] == jo) {
             mo = m;
             break;
           }
+        }
       }
       // a pair that comes back into the list resumes its stale history
       int tr = -1;
@@ -1488,29 +1580,36 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
       }
       const unsigned char fl = (mo >= 0 || tr >= 0) ? 1 : 0;
       if (mo >= 0) found |= 1ull << mo;
-      const bool front = fl || s.pair[ci + cj * s.nc].mindijsq - dsq > 0.0;
+      const int pq = ci + cj * s.nc;
+      const bool front = fl || (tab.ok ? tab.mindsq[pq] : s.pair[pq].mindijsq) - dsq > 0.0;
       const long long e = e0 + 64LL * (front ? kf++ : kb--);
       E.nbr[q][e] = j;
       E.flag[q][e] = fl;
-      double2* g = E.hist[q];
-      if (mo >= 0) {
-        const long long om = ob + 64LL * mo;
-        g[e] = ohist[om];
-        g[ecap + e] = ohist[ecap + om];
-        g[2 * ecap + e] = ohist[2 * ecap + om];
-      } else if (tr >= 0) {
-        g[e] = T.hist[tr];
-        g[ecap + e] = T.hist[DEM_STALE_ROWS + tr];
-        g[2 * ecap + e] = T.hist[2 * DEM_STALE_ROWS + tr];
-        T.state[tr] = 1;  // both edges of the pair restore it
+      if (fl) {
+        // the carried history is requested now and written when the next
+        // carried pair is found (or after the scan), so the lane does not
+        // wait for it here
+        put_pending();
+        if (mo >= 0) {
+          const long long om = ob + 64LL * mo;
+          ph0 = ohist[om];
+          ph1 = ohist[ecap + om];
+          ph2 = ohist[2 * ecap + om];
+        } else {
+          ph0 = T.hist[tr];
+          ph1 = T.hist[DEM_STALE_ROWS + tr];
+          ph2 = T.hist[2 * DEM_STALE_ROWS + tr];
+          T.state[tr] = 1;  // both edges of the pair restore it
+        }
+        pe = e;
       }
     });
+    put_pending();
     E.cnt[q][i] = ndeg | (kf << 16);
     nedges = (unsigned long long)ndeg;
-    for (int m = 0; m < odeg; ++m) {
+    for (unsigned long long lost = oldmask & ~found; lost; lost &= lost - 1) {
+      const int m = __builtin_ctzll(lost);
       const long long om = ob + 64LL * m;
-      if (!(oflag[om] & 1)) continue;
-      if ((found >> m) & 1ull) continue;
       const int jn = o2n ? o2n[onbr[om]] : onbr[om];
       // A stored pair lost from the list: the reference visits it at the
       // next updating pass, and forgets it, when its cells are adjacent;
@@ -1541,9 +1640,9 @@ __device__ __forceinline__ void nbr_fill_body(const SysConst& s, const double4* 
     if (dropped) atomicAdd(&ctl->stale_dropped, dropped);
     if (nedges) atomicAdd(&ctl->nbpairs, nedges);
   }
-  halo_build_body(s, E, q, ctl, blk);
+  if (E.halo_on) halo_build_body(s, E, q, ctl, blk);
 }
-__global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double4* pos,
+__global__ __launch_bounds__(DEM_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nbr_fill(SysConst s, const double4* pos,
                                                         const int* start,
                                                         const int* cell_idx, EdgeSets E,
                                                         const int* src, const int* o2n,
@@ -1551,6 +1650,12 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_fill(SysConst s, const double
                                                         const int* gate, int force, Ctl* ctl) {
   nbr_fill_body(s, pos, start, cell_idx, E, src, o2n, kin_cur, xref, gate, force, ctl,
                 blockIdx.x);
+}
+
+// The halos and slots of the current edge set (when a staged launch shape
+// is switched on between rebuilds).
+__global__ __launch_bounds__(DEM_TILE) void k_halo_build(SysConst s, EdgeSets E, Ctl* ctl) {
+  halo_build_body(s, E, *E.epar, ctl, blockIdx.x);
 }
 
 // Make the freshly built edge set current.
