@@ -161,6 +161,7 @@ struct dem_system {
   int device = 0;             // the GPU of the handle's stream
   double* d_obs = nullptr;    // dem_observables partial sums
   size_t obs_cap = 0;         // doubles in d_obs
+  size_t state_bytes = 0;     // device bytes of the state arrays (state_arrays)
 };
 
 static int fail(dem_handle_t h, int code, const char* fmt, ...) {
@@ -213,6 +214,7 @@ static void free_state(dem_handle_t h) {
   dfree(h->cell_start); dfree(h->cursor); dfree(h->cell_idx); dfree(h->tile_sum);
   dfree(h->perm_src); dfree(h->old_to_new); dfree(h->slice_w);
   dfree(h->d_aos); dfree(h->d_tmp); dfree(h->d_epart); dfree(h->part);
+  h->state_bytes = 0;
   h->np = 0;
   h->has_state = false;
   h->tmp_bytes = 0;
@@ -221,6 +223,36 @@ static void free_state(dem_handle_t h) {
 extern "C" {
 
 const char* dem_last_error(dem_handle_t h) { return h ? h->err : g_err; }
+
+// Device bytes of the per-handle constants and control words of dem_create.
+static size_t const_bytes(int nc, int nw) {
+  return (size_t)nc * nc * sizeof(PairConst) + (size_t)nc * sizeof(CompConst) +
+         (size_t)std::max(1, nc * nw) * sizeof(WallConst) +
+         (size_t)std::max(1, nw) * (sizeof(WallGeom) + sizeof(WallMotion)) + sizeof(Ctl) +
+         2 * sizeof(int);
+}
+
+// The neighbour-search grid of a box: gn[d] cells of at least rcmax along
+// each axis; returns the number of cell keys (8x8x8-cell bricks).
+static long long nb_grid(const double lbox[3], double rcmax, int gn[3]) {
+  long long keys = 512;
+  for (int d = 0; d < 3; ++d) {
+    gn[d] = std::max(1, (int)(lbox[d] / rcmax));
+    keys *= (gn[d] + 7) / 8;
+  }
+  return keys;
+}
+// The neighbour-list skin of a parameter set: 0.1 of the smallest diameter
+// unless given.
+static double params_skin(const dem_params_t* p, double* rmax_out) {
+  double rmax = 0.0, rmin = 1e300;
+  for (int c = 0; c < p->nc; ++c) {
+    rmax = std::max(rmax, p->radius[c]);
+    rmin = std::min(rmin, p->radius[c]);
+  }
+  if (rmax_out) *rmax_out = rmax;
+  return p->skin > 0.0 ? p->skin : 0.2 * rmin;
+}
 
 int dem_create(const dem_params_t* p, dem_handle_t* out) {
   if (!p || !out) return fail(nullptr, DEM_ERR_ARG, "null argument");
@@ -252,15 +284,13 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
   h->next_step = p->method_order;  // InitialStep after the start-up (src/DEM.f90:87-92)
   s.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 48;
   s.max_c = 0;
-  double rmax = 0.0, rmin = 1e300;
   for (int c = 0; c < nc; ++c) {
-    rmax = std::max(rmax, p->radius[c]);
-    rmin = std::min(rmin, p->radius[c]);
     h->radius_h.push_back(p->radius[c]);
     h->mass_h.push_back(p->mass ? p->mass[c] : 0.0);
     h->inertia_h.push_back(p->inertia ? p->inertia[c] : 0.0);
   }
-  h->skin = p->skin > 0.0 ? p->skin : 0.2 * rmin;  // 0.1 of the smallest diameter
+  double rmax = 0.0;
+  h->skin = params_skin(p, &rmax);  // 0.1 of the smallest diameter unless given
   s.skin_half_sq = 0.25 * h->skin * h->skin;
   for (int d = 0; d < 3; ++d) {
     s.pbc[d] = p->pbc[d] != 0;
@@ -299,14 +329,11 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     s.rn_by2[d] = (double)rn[d] * 0.5;
   }
   // neighbour-search grid: cells of at least 2*rmax + skin
-  const double rcmax = 2.0 * rmax + h->skin;
-  long long gn = 512;  // keys: 8x8x8 cells per brick
+  const long long gn = nb_grid(p->lbox, 2.0 * rmax + h->skin, s.gn);
   for (int d = 0; d < 3; ++d) {
-    s.gn[d] = std::max(1, (int)(p->lbox[d] / rcmax));
     s.gb[d] = (s.gn[d] + 7) / 8;
     s.g_by_l[d] = (double)s.gn[d] / p->lbox[d];
     s.g_by2[d] = (double)s.gn[d] * 0.5;
-    gn *= s.gb[d];
   }
   if (gn > (1LL << 30)) {
     delete h;
@@ -597,57 +624,75 @@ static int ensure_tmp(dem_handle_t h, size_t bytes) {
 
 static int clear_contacts(dem_handle_t h);
 
+// Sizes of a system's state for N (>= 1) particles.
+static void state_dims(dem_handle_t h, size_t N) {
+  h->nslices = (int)((N + 63) / 64);
+  h->ecap = (size_t)h->nslices * 64 * (size_t)h->sc.max_nb;
+  h->es.ecap = (long long)h->ecap;
+  h->ntiles = (h->gncell + SCAN_TILE - 1) / SCAN_TILE;
+  h->ntiles_np = (h->nslices + SCAN_TILE - 1) / SCAN_TILE;
+}
+
+// The device arrays of a system's state for N particles (state_dims set):
+// f(&pointer, elements) for each.  alloc_state allocates them, and
+// dem_footprint counts them without a device.
+template <class F>
+static void state_arrays(dem_handle_t h, size_t N, F&& f) {
+  const SysConst& s = h->sc;
+  const size_t nring = (size_t)std::max(h->nprev, 1) * 9;
+  for (int k = 0; k < 2; ++k) {
+    f(&h->pos[k], N);
+    f(&h->kin[k], N);
+    f(&h->es.cnt[k], N);
+    f(&h->es.soff[k], (size_t)h->nslices + 1);
+    f(&h->es.nbr[k], h->ecap);
+    f(&h->es.flag[k], h->ecap);
+    f(&h->es.hist[k], 3 * h->ecap);
+    f(&h->es.slot[k], h->ecap);
+    f(&h->es.halo[k], (size_t)nblocks(N, DEM_TILE) * DEM_HALO_CAP);
+    f(&h->es.hcnt[k], (size_t)nblocks(N, DEM_TILE));
+    f(&h->wflag[k], N);
+    f(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
+  }
+  f(&h->es.epar, 1);
+  f(&h->es.st.key, DEM_STALE_ROWS);
+  f(&h->es.st.slot, DEM_STALE_ROWS);
+  f(&h->es.st.hist, 3 * DEM_STALE_ROWS);
+  f(&h->es.st.state, DEM_STALE_ROWS);
+  f(&h->es.st.cnt, 3);
+  f(&h->d_es, 1);
+  f(&h->acc, 3 * N);
+  f(&h->acc_p, 3 * N);
+  f(&h->ring, nring * N);
+  f(&h->ring2, nring * N);
+  f(&h->psum, (size_t)9 * N);
+  f(&h->id, N);
+  f(&h->id2, N);
+  f(&h->xref, N);
+  f(&h->key, N);
+  f(&h->cell_count, (size_t)h->gncell + 1);
+  f(&h->cell_start, (size_t)h->gncell + 1);
+  f(&h->cursor, (size_t)h->gncell + 1);
+  f(&h->cell_idx, N);
+  f(&h->tile_sum, (size_t)std::max(h->ntiles, h->ntiles_np) + 1);
+  f(&h->perm_src, N);
+  f(&h->old_to_new, N);
+  f(&h->slice_w, (size_t)h->nslices);
+  f(&h->d_epart, (size_t)nblocks(N) + 1);
+  f(&h->part, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK));
+}
+
 static int alloc_state(dem_handle_t h, int np) {
   free_state(h);
   const size_t N = (size_t)std::max(np, 1);
-  const SysConst& s = h->sc;
-  const size_t nring = (size_t)std::max(h->nprev, 1) * 9;
-  h->nslices = (int)((N + 63) / 64);
-  h->ecap = (size_t)h->nslices * 64 * (size_t)s.max_nb;
-  h->es.ecap = (long long)h->ecap;
+  state_dims(h, N);
   hipError_t e = hipSuccess;
-  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-    e = dalloc(&h->pos[k], N);
-    if (e == hipSuccess) e = dalloc(&h->kin[k], N);
-    if (e == hipSuccess) e = dalloc(&h->es.cnt[k], N);
-    if (e == hipSuccess) e = dalloc(&h->es.soff[k], (size_t)h->nslices + 1);
-    if (e == hipSuccess) e = dalloc(&h->es.nbr[k], h->ecap);
-    if (e == hipSuccess) e = dalloc(&h->es.flag[k], h->ecap);
-    if (e == hipSuccess) e = dalloc(&h->es.hist[k], 3 * h->ecap);
-    if (e == hipSuccess) e = dalloc(&h->es.slot[k], h->ecap);
-    if (e == hipSuccess) e = dalloc(&h->es.halo[k], (size_t)nblocks(N, DEM_TILE) * DEM_HALO_CAP);
-    if (e == hipSuccess) e = dalloc(&h->es.hcnt[k], (size_t)nblocks(N, DEM_TILE));
-    if (e == hipSuccess) e = dalloc(&h->wflag[k], N);
-    if (e == hipSuccess) e = dalloc(&h->whist[k], (size_t)std::max(s.nw, 1) * 6 * N);
-  }
-  if (e == hipSuccess) e = dalloc(&h->es.epar, 1);
-  if (e == hipSuccess) e = dalloc(&h->es.st.key, DEM_STALE_ROWS);
-  if (e == hipSuccess) e = dalloc(&h->es.st.slot, DEM_STALE_ROWS);
-  if (e == hipSuccess) e = dalloc(&h->es.st.hist, 3 * DEM_STALE_ROWS);
-  if (e == hipSuccess) e = dalloc(&h->es.st.state, DEM_STALE_ROWS);
-  if (e == hipSuccess) e = dalloc(&h->es.st.cnt, 3);
-  if (e == hipSuccess) e = dalloc(&h->d_es, 1);
-  if (e == hipSuccess) e = dalloc(&h->acc, 3 * N);
-  if (e == hipSuccess) e = dalloc(&h->acc_p, 3 * N);
-  if (e == hipSuccess) e = dalloc(&h->ring, nring * N);
-  if (e == hipSuccess) e = dalloc(&h->ring2, nring * N);
-  if (e == hipSuccess) e = dalloc(&h->psum, (size_t)9 * N);
-  if (e == hipSuccess) e = dalloc(&h->id, N);
-  if (e == hipSuccess) e = dalloc(&h->id2, N);
-  if (e == hipSuccess) e = dalloc(&h->xref, N);
-  if (e == hipSuccess) e = dalloc(&h->key, N);
-  if (e == hipSuccess) e = dalloc(&h->cell_count, (size_t)h->gncell + 1);
-  if (e == hipSuccess) e = dalloc(&h->cell_start, (size_t)h->gncell + 1);
-  if (e == hipSuccess) e = dalloc(&h->cursor, (size_t)h->gncell + 1);
-  if (e == hipSuccess) e = dalloc(&h->cell_idx, N);
-  h->ntiles = (h->gncell + SCAN_TILE - 1) / SCAN_TILE;
-  h->ntiles_np = (h->nslices + SCAN_TILE - 1) / SCAN_TILE;
-  if (e == hipSuccess) e = dalloc(&h->tile_sum, (size_t)std::max(h->ntiles, h->ntiles_np) + 1);
-  if (e == hipSuccess) e = dalloc(&h->perm_src, N);
-  if (e == hipSuccess) e = dalloc(&h->old_to_new, N);
-  if (e == hipSuccess) e = dalloc(&h->slice_w, (size_t)h->nslices);
-  if (e == hipSuccess) e = dalloc(&h->d_epart, (size_t)nblocks(N) + 1);
-  if (e == hipSuccess) e = dalloc(&h->part, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK));
+  size_t bytes = 0;
+  state_arrays(h, N, [&](auto** ptr, size_t n) {
+    bytes += std::max<size_t>(n, 1) * sizeof(**ptr);
+    if (e == hipSuccess) e = dalloc(ptr, n);
+  });
+  h->state_bytes = bytes;
   // the counter slots start at zero; every reduction clears what it read
   if (e == hipSuccess)
     e = hipMemsetAsync(h->part, 0, 4 * (size_t)nblocks(N, DEM_MIN_PASS_BLOCK) * sizeof(unsigned),
@@ -668,6 +713,30 @@ static int alloc_state(dem_handle_t h, int np) {
   // empty edge sets, current set 0 (also for a state of no particles, which
   // dem_upload_state leaves before it clears the contacts)
   return clear_contacts(h);
+}
+
+extern "C" int dem_footprint(const dem_params_t* p, int32_t np, int64_t* bytes) {
+  if (!p || !bytes || np < 0) return fail(nullptr, DEM_ERR_ARG, "bad footprint arguments");
+  *bytes = 0;
+  if (p->nc < 1 || p->nc > DEM_MAX_COMPONENTS || p->nw < 0 || p->nw > DEM_MAX_WALLS ||
+      p->method_order < 1 || p->method_order > 4 || p->max_neighbours < 0 ||
+      p->max_neighbours > 64)
+    return fail(nullptr, DEM_ERR_ARG, "bad parameters");
+  // the state arrays alloc_state allocates, counted on a handle that holds
+  // only their dimensions
+  dem_system t;
+  double rmax = 0.0;
+  const double skin = params_skin(p, &rmax);
+  t.gncell = (int)nb_grid(p->lbox, 2.0 * rmax + skin, t.sc.gn);
+  t.sc.max_nb = p->max_neighbours > 0 ? p->max_neighbours : 48;
+  t.sc.nw = p->nw;
+  t.nprev = p->method_order - 1;
+  const size_t N = (size_t)std::max(np, 1);
+  state_dims(&t, N);
+  size_t sum = 0;
+  state_arrays(&t, N, [&](auto** ptr, size_t n) { sum += std::max<size_t>(n, 1) * sizeof(**ptr); });
+  *bytes = (int64_t)(sum + const_bytes(p->nc, p->nw));
+  return DEM_OK;
 }
 
 // Empty edge sets, no wall history, current set 0.
@@ -2152,6 +2221,7 @@ int dem_stats(dem_handle_t h, dem_stats_t* out) {
   out->mean_pair_contacts = c.upd_evals ? (double)c.pairs_upd_sum / (double)c.upd_evals : 0.0;
   out->max_halo = c.max_halo;
   out->halo_overflow = c.halo_over;
+  out->device_bytes = (int64_t)(h->state_bytes + const_bytes(h->sc.nc, h->sc.nw));
   return DEM_OK;
 }
 

@@ -73,7 +73,7 @@ module dem3d_gpu
     real(c_double) :: ms_pass1, ms_pass2, ms_rebuild, ms_step
     real(c_double) :: mean_pair_contacts, time
     integer(c_int64_t) :: next_step
-    integer(c_int64_t) :: max_halo, halo_overflow
+    integer(c_int64_t) :: max_halo, halo_overflow, device_bytes
   end type dem_stats_t
 
   ! dem_tune_t: the autotuner's report (launch shapes of the force passes).

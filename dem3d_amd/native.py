@@ -96,7 +96,7 @@ class Stats(C.Structure):
         ("kinetic_energy", C.c_double), ("max_overlap", C.c_double), ("ms_pass1", C.c_double),
         ("ms_pass2", C.c_double), ("ms_rebuild", C.c_double), ("ms_step", C.c_double),
         ("mean_pair_contacts", C.c_double), ("time", C.c_double), ("next_step", C.c_int64),
-        ("max_halo", C.c_int64), ("halo_overflow", C.c_int64),
+        ("max_halo", C.c_int64), ("halo_overflow", C.c_int64), ("device_bytes", C.c_int64),
     ]
 
     def asdict(self):
@@ -250,6 +250,7 @@ class Library:
         "batch_step_times": (C.c_int64, [C.c_void_p, C.c_int64, _dp]),
         "autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(TuneT)]),
         "batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TuneT)]),
+        "footprint": (C.c_int, [C.POINTER(Params), C.c_int32, _lp]),
     }
 
     def __init__(self, path: str, prefix: str):
@@ -285,6 +286,16 @@ def load_hip() -> Library:
                 f"{HIP_LIB} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
         _hip = Library(HIP_LIB, "dem_")
     return _hip
+
+
+def footprint(lib: "Library", red: Reduced, n: int, **kw) -> int:
+    """dem_footprint: the device bytes a system of n particles with the
+    reduced constants red (and System's keyword options) allocates."""
+    out = C.c_int64(0)
+    rc = lib.footprint(C.byref(make_params(red, **kw)), int(n), C.byref(out))
+    if rc != DEM_OK:
+        raise DemError(rc, lib.last_error(None).decode())
+    return int(out.value)
 
 
 def make_params(red: Reduced, precision=DEM_FP64, history_mode=DEM_HIST_REFERENCE,

@@ -2,7 +2,8 @@
 
 Requests (workloads.Request: arrival time, size, steps, contact parameters)
 arrive on their own clock.  The server admits a waiting request as soon as
-its estimated device footprint fits the budget next to the systems already
+its device footprint -- the library's own count of what the system will
+allocate (dem_footprint) -- fits the budget next to the systems already
 resident.  A worker thread builds and starts each admitted request -- the
 packing on the host, then upload and Euler start-up on the system's own
 stream -- while the main thread keeps stepping the batch, so the device
@@ -29,17 +30,14 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import workloads
-from .native import DEM_MIXED, Batch, Library, System
+from .native import DEM_MIXED, Batch, Library, System, footprint
 
-# Device bytes per particle of one system (two state buffers, two edge sets
-# of max_nb slots with 4 + 1 + 48 bytes each, ring and staging) and a fixed
-# part per system; an upper estimate used for admission.
-BYTES_PER_PARTICLE = 2 * (32 + 64 + 4 + 4 + 48) + 2 * 48 * (4 + 1 + 48) + 2 * 216 + 200
-BYTES_PER_SYSTEM = 32 << 20
-
-
-def estimate_device_bytes(n: int) -> int:
-    return BYTES_PER_SYSTEM + BYTES_PER_PARTICLE * int(n)
+def request_bytes(lib: Library, req: workloads.Request, precision: int = DEM_MIXED) -> int:
+    """The device bytes the request's system will hold: the library's own
+    count of what it allocates for those constants and that many particles
+    (dem_footprint), computed without building the particles."""
+    red = workloads.ensemble_member_reduced(req.n, req.mu, req.e, req.seed, steps=req.steps)
+    return footprint(lib, red, req.n, precision=precision)
 
 
 @dataclass
@@ -51,6 +49,7 @@ class Record:
     finished: float = float("nan")
     remaining: int = 0
     steps_done: int = 0
+    bytes: int = -1  # device footprint (request_bytes), once computed
     escaped: int = 0
     system: Optional[System] = None
     result: Optional[dict] = None
@@ -85,7 +84,7 @@ class Report:
             "batch_steps": int(st.size),
             "chunks": self.chunks,
             "max_resident": self.max_resident,
-            "peak_device_bytes_est": self.peak_bytes,
+            "peak_device_bytes": self.peak_bytes,
         }
 
 
@@ -128,14 +127,16 @@ def serve(lib: Library, trace: List[workloads.Request], precision: int = DEM_MIX
         batch.remove(r.system)
         r.system.close()
         r.system = None
-        used -= estimate_device_bytes(r.request.n)
+        used -= r.bytes
 
     try:
         while pending or building or active:
             # admit everything that has arrived and fits: its build starts on
             # the worker thread
             while pending and (not realtime or pending[0].request.arrival <= now()):
-                need = estimate_device_bytes(pending[0].request.n)
+                if pending[0].bytes < 0:
+                    pending[0].bytes = request_bytes(lib, pending[0].request, precision)
+                need = pending[0].bytes
                 if (active or building) and used + need > budget_bytes:
                     break
                 r = pending.pop(0)

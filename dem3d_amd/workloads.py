@@ -451,13 +451,9 @@ def fluidized_bed(seed: int = 20240008, n_side=(6, 6, 6), velocity: float = 0.02
                     note="upward water flow, drag of src/mFluid.f90")
 
 
-def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,
-                    steps: int = 10000, phi: float = 0.5) -> Workload:
-    """One system of an ensemble: n spheres with radii U[0.9, 1.1] r0
-    discretised into nbins equal-width components, on a jittered FCC lattice
-    at volume fraction ~phi in a box periodic in x,y with a floor, settling
-    under gravity; friction mu and restitution e for every pair and the
-    floor, kn = 1e3 N/m."""
+def _member_layout(n, mu, e, seed, nbins, r0, steps, phi):
+    """The radii, box and constants of an ensemble member (the first draws of
+    its generator, which the member's positions and velocities continue)."""
     kn = 1e3
     rng = np.random.default_rng(seed)
     edges = np.linspace(0.9 * r0, 1.1 * r0, nbins + 1)
@@ -471,15 +467,33 @@ def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: 
     nz = int(math.ceil(n / (4 * nxy * nxy)))
     lx = ly = nxy * a
     lz = nz * a + 6.0 * r0 + 0.5 * a
+    spec = _spec((lx, ly, lz), list(2.0 * centres), 2500.0, kn, e, mu,
+                 walls=_floor(lx, ly, lz, kn), total_steps=steps)
+    return rng, comp, a, (nxy, nz), spec, reduce_spec(spec)
+
+
+def ensemble_member_reduced(n: int, mu: float, e: float, seed: int, nbins: int = 8,
+                            r0: float = 0.5e-3, steps: int = 10000, phi: float = 0.5):
+    """The reduced constants of ensemble_member(n, mu, e, seed, ...) without
+    building its particles (what admission by device footprint needs)."""
+    return _member_layout(n, mu, e, seed, nbins, r0, steps, phi)[5]
+
+
+def ensemble_member(n: int, mu: float, e: float, seed: int, nbins: int = 8, r0: float = 0.5e-3,
+                    steps: int = 10000, phi: float = 0.5) -> Workload:
+    """One system of an ensemble: n spheres with radii U[0.9, 1.1] r0
+    discretised into nbins equal-width components, on a jittered FCC lattice
+    at volume fraction ~phi in a box periodic in x,y with a floor, settling
+    under gravity; friction mu and restitution e for every pair and the
+    floor, kn = 1e3 N/m."""
+    rng, comp, a, (nxy, nz), spec, red = _member_layout(n, mu, e, seed, nbins, r0, steps, phi)
+    lx, ly, lz = spec.lbox
     pos = _fcc(nxy, nxy, nz, a)[:, :n]
     pos[0] -= 0.5 * lx
     pos[1] -= 0.5 * ly
     pos[2] += -0.5 * lz + 1.1 * r0
     gap = a / math.sqrt(2.0) - 2.2 * r0
     pos = pos + rng.uniform(-0.45 * gap, 0.45 * gap, pos.shape)
-    spec = _spec((lx, ly, lz), list(2.0 * centres), 2500.0, kn, e, mu,
-                 walls=_floor(lx, ly, lz, kn), total_steps=steps)
-    red = reduce_spec(spec)
     L, T = red.length_scale, red.time_scale
     vel = rng.uniform(-0.05, 0.05, pos.shape)
     return Workload(f"member-{n}", spec, red, (comp + 1).astype(np.int32), pos / L,

@@ -182,10 +182,20 @@ typedef struct {
                                neighbour lists built so far (staged passes)    */
   int64_t halo_overflow;    /* tiles whose halo exceeded its 512 staged rows
                                (their outer partners are read from memory)     */
+  int64_t device_bytes;     /* device memory the handle holds for its state and
+                               constants (dem_footprint of its parameters and
+                               particle count); buffers made on demand (the
+                               observables' partial sums, the level-1 drop-in's
+                               staging, the fluid layers) are not counted      */
 } dem_stats_t;
 
 /* ---- lifetime ------------------------------------------------------------ */
 int dem_create(const dem_params_t* params, dem_handle_t* out);
+/* No counterpart in the reference.  The device bytes dem_create and a state
+ * of np particles allocate for params (the figure dem_stats reports as
+ * device_bytes once such a state is uploaded), computed on the host without
+ * a device: what a request loop admits by. */
+int dem_footprint(const dem_params_t* params, int32_t np, int64_t* bytes);
 void dem_destroy(dem_handle_t h);
 /* Message of the last error on h (h may be NULL for dem_create failures). */
 const char* dem_last_error(dem_handle_t h);

@@ -115,3 +115,21 @@ def test_reference_programs_built():
     for name in ("dem3d_ref", "dem3d_gpu"):
         p = os.path.join(ROOT, "oracle", "_ref", name)
         assert os.path.exists(p) and os.access(p, os.X_OK), p
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libdem3d_hip.so not built")
+def test_footprint_needs_no_device():
+    """dem_footprint answers on the host: positive, growing with the particle
+    count by at least the two edge sets' history planes per particle, larger
+    with more neighbour slots, and refusing bad arguments."""
+    from dem3d_amd import native, workloads
+    lib = native.Library(LIB, "dem_")
+    red = workloads.ensemble_member_reduced(5000, 0.3, 0.8, 7, steps=100)
+    b1 = native.footprint(lib, red, 5000)
+    b2 = native.footprint(lib, red, 10000)
+    assert b1 > 0 and b2 > b1
+    # two edge sets of 48 slots per particle, 48 bytes of history each
+    assert b2 - b1 >= 5000 * 2 * 48 * 48
+    assert native.footprint(lib, red, 5000, max_neighbours=32) < b1
+    with pytest.raises(native.DemError):
+        native.footprint(lib, red, -1)

@@ -142,8 +142,8 @@ def test_serving_trace_prefix_same_alone_and_in_the_mix(hip):
         st = r.system.download()
         return {k: st[k].copy() for k in ("x", "v", "w")}
 
-    # room for about a third of the trace's spheres at once
-    budget = serve.estimate_device_bytes(sum(q.n for q in trace) // 3)
+    # room for about a third of the trace's footprint at once
+    budget = sum(serve.request_bytes(hip, q, DEM_MIXED) for q in trace) // 3
     rep = serve.serve(hip, trace, precision=DEM_MIXED, budget_bytes=budget, chunk=25,
                       realtime=True, keep=keep)
     assert rep.max_resident >= 2

@@ -33,13 +33,18 @@ def _keep(r):
 @pytest.mark.parametrize("precision", [DEM_FP64, DEM_MIXED])
 def test_each_request_same_alone_and_in_the_mix(hip, precision):
     trace = _trace()
-    # a small footprint budget forces some requests to wait for others
-    budget = 2 * serve.estimate_device_bytes(2500)
+    # a small footprint budget (the two largest requests) forces some
+    # requests to wait for others
+    need = [serve.request_bytes(hip, q, precision) for q in trace]
+    budget = sum(sorted(need)[-2:])
     rep = serve.serve(hip, trace, precision=precision, budget_bytes=budget, chunk=25,
                       realtime=False, keep=_keep)
     assert rep.max_resident >= 2
+    assert rep.peak_bytes <= budget
     for r in rep.records:
         s = serve.build_system(hip, r.request, precision)
+        # admission counted what the library allocates for the system
+        assert s.stats()["device_bytes"] == r.bytes == serve.request_bytes(hip, r.request, precision)
         s.step(r.request.steps)
         st = s.download()
         s.close()
