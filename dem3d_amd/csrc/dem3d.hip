@@ -271,6 +271,7 @@ int dem_create(const dem_params_t* p, dem_handle_t* out) {
     return fail(nullptr, DEM_ERR_DEVICE, "no HIP device available");
   dem_handle_t h = new dem_system();
   h->err[0] = 0;
+  h->es.halo_rows = DEM_HALO_CAP;
   const int nc = p->nc, nw = p->nw;
   SysConst& s = h->sc;
   s.nc = nc;
@@ -452,6 +453,8 @@ void dem_destroy(dem_handle_t h) {
   delete h;
 }
 
+static int sync_halo(dem_handle_t h, bool want, hipStream_t st);
+
 int dem_set_option(dem_handle_t h, const char* name, double value) {
   if (!h || !name) return DEM_ERR_ARG;
   if (!strcmp(name, "timing")) h->timing = value != 0.0;
@@ -466,6 +469,17 @@ int dem_set_option(dem_handle_t h, const char* name, double value) {
   } else if (!strcmp(name, "block_map")) {
     if (value != 0.0 && value != 1.0) return fail(h, DEM_ERR_ARG, "block_map must be 0 or 1");
     h->block_map = (int)value;
+  } else if (!strcmp(name, "halo_rows")) {
+    if (!(value >= 1.0 && value <= (double)DEM_HALO_CAP) || value != (double)(int)value)
+      return fail(h, DEM_ERR_ARG, "halo_rows must be an integer in [1, %d]", DEM_HALO_CAP);
+    if (h->in_batch) return fail(h, DEM_ERR_STATE, "the system is stepped by a batch");
+    h->es.halo_rows = (int)value;
+    // halos already listed are listed again with the new bound
+    if (h->es.halo_on) {
+      h->es.halo_on = 0;
+      const int rc = sync_halo(h, true, h->stream);
+      if (rc) return rc;
+    }
   } else return fail(h, DEM_ERR_ARG, "unknown option '%s'", name);
   return DEM_OK;
 }

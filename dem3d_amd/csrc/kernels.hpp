@@ -169,6 +169,7 @@ struct EdgeSets {
   int* epar;  // device word: index of the current set
   StaleTab st;  // null pointers in the canonical history mode
   int halo_on;  // the rebuilds list tile halos and slots (a staged shape is in use)
+  int halo_rows;  // rows listed per tile, <= DEM_HALO_CAP (option "halo_rows")
 };
 
 __host__ __device__ __forceinline__ unsigned long long pair_key(int ida, int idb) {
@@ -1364,7 +1365,8 @@ __global__ __launch_bounds__(DEM_BLOCK) void k_nbr_count(SysConst s, const doubl
 // every edge of the tile (one block of DEM_TILE threads, one per particle).
 // The neighbours outside the tile are collected in an LDS hash set, compacted
 // and sorted ascending, so the halo does not depend on the order in which the
-// lanes inserted them; the first DEM_HALO_CAP rows are kept.  The slot of an
+// lanes inserted them; the first E.halo_rows (at most DEM_HALO_CAP) rows are
+// kept, the other neighbours get DEM_SLOT_NONE.  The slot of an
 // edge is found by binary search in the sorted rows.
 #define DEM_HALO_HASH 4096
 #define DEM_HALO_SORT 2048
@@ -1430,13 +1432,13 @@ __device__ __forceinline__ void halo_build_body(const SysConst& s, const EdgeSet
       }
       __syncthreads();
     }
-  const int hn = min(n, DEM_HALO_CAP);
+  const int hn = min(n, E.halo_rows);
   int* halo = E.halo[q] + (size_t)blk * DEM_HALO_CAP;
   for (int k = t; k < hn; k += DEM_BLOCK) halo[k] = h_row[k];
   if (t == 0) {
     E.hcnt[q][blk] = hn;
     atomicMax(&ctl->max_halo, tot);
-    if (tot > DEM_HALO_CAP) atomicAdd(&ctl->halo_over, 1);
+    if (tot > E.halo_rows) atomicAdd(&ctl->halo_over, 1);
   }
   unsigned short* slot = E.slot[q];
   for (int m = 0; m < deg; ++m) {

@@ -274,7 +274,10 @@ int dem_stats(dem_handle_t h, dem_stats_t* out);
  *                       configuration before an Euler start-up
  *   "next_step"         sets the step number of the next batched step
  *   "pass_shape"        launch shape of the force passes (see dem_tune_t)
- *   "block_map"         block order of the force passes (see dem_tune_t) */
+ *   "block_map"         block order of the force passes (see dem_tune_t)
+ *   "halo_rows"         rows of a tile's halo the LDS-staged launch shapes
+ *                       stage (1..512, default 512); neighbours beyond them are
+ *                       read from memory (the results do not change) */
 int dem_set_option(dem_handle_t h, const char* name, double value);
 
 /* ---- observables ---------------------------------------------------------- */

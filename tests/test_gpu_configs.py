@@ -259,6 +259,28 @@ def test_config2_disordered_960k_against_oracle(hip):
     assert g.stats()["ncol"] - ncol0[0] == o.stats()["ncol"] - ncol0[1]
     g.close()
     o.close()
+    # the launch shapes that stage each tile and its halo in LDS (with all
+    # halo rows, and with 8, so that most partners overflow to memory) give
+    # the global-gather shape's steps bit for bit on this bed
+    runs = []
+    for shape, rows in ((7, 512), (5, 512), (4, 512), (4, 8)):
+        gs = System(hip, wl.red)
+        upload_same([gs], wl, state=st, contacts=con)
+        gs.set_option("pass_shape", shape)
+        gs.set_option("halo_rows", rows)
+        gs.step(3)
+        d, c, stt = gs.download(), gs.download_contacts(), gs.stats()
+        runs.append((d, c, stt))
+        gs.close()
+        print(f"shape {shape}, {rows} halo rows: largest halo {stt['max_halo']}, "
+              f"{stt['halo_overflow']} tile listings over the rows", flush=True)
+    for d, c, stt in runs[1:]:
+        for k in ("x", "v", "w", "a", "alpha"):
+            assert np.array_equal(d[k], runs[0][0][k]), k
+        for k in ("keys", "ft", "vt", "wkeys", "wft", "wvt"):
+            assert np.array_equal(c[k], runs[0][1][k]), k
+    assert runs[1][2]["max_halo"] > 0 and runs[1][2]["halo_overflow"] == 0
+    assert runs[3][2]["halo_overflow"] > 0
 
 
 def test_config2_packed_4m_pair_count_against_kdtree(hip):

@@ -118,3 +118,23 @@ def test_bad_shape_is_refused(hip):
     with pytest.raises(Exception, match="block_map"):
         s.set_option("block_map", 2)
     s.close()
+
+
+def test_staged_shapes_with_few_halo_rows(hip):
+    """The LDS-staged shapes read the partners beyond a tile's staged halo
+    rows from memory: with 4 rows most of them do, and the trajectory is
+    still bitwise the default shape's."""
+    kw = CASES[0]
+    ref = _make(hip, kw)
+    ref.step(12)
+    for rows in (4, 512):
+        s = _make(hip, kw)
+        s.set_option("pass_shape", 4)
+        s.set_option("halo_rows", rows)
+        s.step(12)
+        _same(_state(s), _state(ref), rows)
+        st = s.stats()
+        assert st["max_halo"] > 4
+        assert (st["halo_overflow"] > 0) == (rows == 4)
+    with pytest.raises(Exception):
+        ref.set_option("halo_rows", 0)
