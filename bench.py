@@ -27,7 +27,9 @@ Prints one JSON line:
                one batch (one launch per phase, and persistent: one
                cooperative launch per 250 steps, a group of resident blocks
                per system) in mixed and in FP64, against the plain FP64 path
-               that steps each system alone
+               that steps each system alone and against the reference
+               stepping every system on the host's cores, one process per
+               system, --cpu-workers at a time (--cpu-ens-steps steps each)
   serving      the Poisson request trace served with continuous batching:
                p50/p99 step time, waiting and completion latency, aggregate
                particle-steps/s
@@ -71,6 +73,43 @@ def cpu_baseline(wl, steps):
         "sample": (f"reference Fortran MoveParticles (MaxNeighbours/MaxNCell raised; amdflang "
                    f"-O3 -march=x86-64-v3), {steps} steps of the benchmark's own packed bed "
                    f"({wl.np} spheres) from a zero ABM ring, {sec:.2f} s, 1 core, {cpu_model()}"),
+    }
+
+
+def cpu_ensemble_baseline(members, steps, workers):
+    """The reference on every member of the ensemble, `steps` MoveParticles
+    steps each from a zero ABM ring, `workers` members at a time, each in a
+    spawned process of its own (one core each): the host's ensemble
+    throughput mode.  The value is workers x particle-steps / the members'
+    summed stepping time (set-up and process start-up excluded; the wall time
+    including them is reported beside it)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+    from checkers import ref_available, run_reference
+    if not ref_available(fast=True) or steps <= 0:
+        return None
+
+    def one(w):
+        res = run_reference(dict(red=w.red, comp=w.comp, x=w.x, v=w.v, w=w.w, fast=True,
+                                 actions=[("time", 1, steps)]), timeout=900, method="spawn")
+        return res["seconds"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        secs = list(ex.map(one, members))
+    wall = time.perf_counter() - t0
+    ps = sum(w.np for w in members) * steps
+    return {
+        "value": workers * ps / sum(secs),
+        "unit": "particle-steps/s",
+        "cores": workers,
+        "kind": "reference",
+        "wall_particle_steps_per_s": ps / wall,
+        "sample": (f"reference Fortran MoveParticles (amdflang -O3 -march=x86-64-v3), all "
+                   f"{len(members)} systems x {steps} steps from a zero ABM ring, {workers} "
+                   f"systems at a time, one process and core each; stepping time summed over "
+                   f"the systems {sum(secs):.1f} s, wall time with process start-up "
+                   f"{wall:.1f} s, {cpu_model()}"),
     }
 
 
@@ -227,6 +266,11 @@ def ensemble_run(args):
 
     res = {"systems": len(members), "spheres": ntot,
            "sizes": [int(min(w.np for w in members)), int(max(w.np for w in members))]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_ensemble_baseline(members, args.cpu_ens_steps,
+                                    min(args.cpu_workers, os.cpu_count() or 1))
+        print(f"ensemble cpu baseline: {cpu}", file=sys.stderr, flush=True)
     sysm = build(DEM_MIXED)
     v_mx, st_mx, tune_mx = batched(sysm, args.ens_steps, tune_steps=args.tune_steps)
     for s in sysm:
@@ -304,6 +348,8 @@ def ensemble_run(args):
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
         "speedup_mixed_persistent_vs_fp64_alone": v_mxp / v_alone,
         "speedup_fp64_batched_vs_fp64_alone": v_64b / v_alone,
+        "cpu_reference_throughput": cpu,
+        "speedup_mixed_batched_vs_cpu_reference": v_mx / cpu["value"] if cpu else None,
     })
     return res
 
@@ -379,6 +425,11 @@ def main(argv=None):
     ap.add_argument("--pass-shape", type=int, default=-1,
                     help="launch shape of the 4M runs' force passes instead of autotuning "
                          "(profiling runs)")
+    ap.add_argument("--cpu-ens-steps", type=int, default=20,
+                    help="steps per system of the reference's ensemble throughput sample "
+                         "(0 = skip it)")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="host cores (one process each) of the ensemble throughput sample")
     ap.add_argument("--alone-steps", type=int, default=100,
                     help="steps per system of the one-at-a-time FP64 sample")
     ap.add_argument("--serve-requests", type=int, default=200,
