@@ -183,7 +183,7 @@ def timed_run(wl, args, precision, hmode):
     bracketed by device synchronisations.  Returns the rate, the stats of the
     library (pass timings from HIP events on its stream) and the final
     positions."""
-    from dem3d_amd.native import System, load_hip
+    from dem3d_amd.native import NSHAPES, System, load_hip
     g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode, precision=precision)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
@@ -192,6 +192,11 @@ def timed_run(wl, args, precision, hmode):
         g.set_option("pass_shape", args.pass_shape)
         if args.block_map >= 0:
             g.set_option("block_map", args.block_map)
+        if args.tune_steps > 0:
+            # the steps the autotuner would take first: the timed steps are
+            # then those of an autotuned run (every shape gives the same
+            # trajectory), as profiling runs of one shape need
+            g.step((NSHAPES + 1) * (args.tune_steps + 1))
     elif args.tune_steps > 0:
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
@@ -424,7 +429,8 @@ def main(argv=None):
                     help="skip the graph-replay run of the ensemble (profiler runs)")
     ap.add_argument("--pass-shape", type=int, default=-1,
                     help="launch shape of the 4M runs' force passes instead of autotuning "
-                         "(profiling runs)")
+                         "(profiling runs; with --tune-steps > 0 the run first takes the "
+                         "steps the autotuner would, so its timed steps are the same)")
     ap.add_argument("--cpu-ens-steps", type=int, default=20,
                     help="steps per system of the reference's ensemble throughput sample "
                          "(0 = skip it)")
