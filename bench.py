@@ -70,6 +70,8 @@ def cpu_baseline(wl, steps):
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "reference",
+        "host_cpus": os.cpu_count(),
+        "cpu_model": cpu_model(),
         "sample": (f"reference Fortran MoveParticles (MaxNeighbours/MaxNCell raised; amdflang "
                    f"-O3 -march=x86-64-v3), {steps} steps of the benchmark's own packed bed "
                    f"({wl.np} spheres) from a zero ABM ring, {sec:.2f} s, 1 core, {cpu_model()}"),
@@ -104,6 +106,8 @@ def cpu_ensemble_baseline(members, steps, workers):
         "unit": "particle-steps/s",
         "cores": workers,
         "kind": "reference",
+        "host_cpus": os.cpu_count(),
+        "cpu_model": cpu_model(),
         "wall_particle_steps_per_s": ps / wall,
         "sample": (f"reference Fortran MoveParticles (amdflang -O3 -march=x86-64-v3), all "
                    f"{len(members)} systems x {steps} steps from a zero ABM ring, {workers} "
@@ -313,10 +317,12 @@ def ensemble_run(args):
             s_.close()
         return ntot * args.ens_steps / sec
 
-    v_mxp = persistent(DEM_MIXED, tune_mx["chosen"] if tune_mx else None)
-    v_64p = persistent(DEM_FP64, tune_64["chosen"] if tune_64 else None)
-    print(f"ensemble persistent: mixed {v_mxp:.4g}, fp64 {v_64p:.4g} particle-steps/s",
-          file=sys.stderr, flush=True)
+    v_mxp = v_64p = None
+    if not args.no_ens_persistent:
+        v_mxp = persistent(DEM_MIXED, tune_mx["chosen"] if tune_mx else None)
+        v_64p = persistent(DEM_FP64, tune_64["chosen"] if tune_64 else None)
+        print(f"ensemble persistent: mixed {v_mxp:.4g}, fp64 {v_64p:.4g} particle-steps/s",
+              file=sys.stderr, flush=True)
     t_alone = 0.0
     ps_alone = 0.0
     for s, w in zip(sys64, members):
@@ -346,12 +352,12 @@ def ensemble_run(args):
         "graph_speedup_fp64": v_64g / v_64b if v_64g else None,
         "mixed_persistent_particle_steps_per_s": v_mxp,
         "fp64_persistent_particle_steps_per_s": v_64p,
-        "persistent_speedup_mixed": v_mxp / v_mx,
-        "persistent_speedup_fp64": v_64p / v_64b,
+        "persistent_speedup_mixed": v_mxp / v_mx if v_mxp else None,
+        "persistent_speedup_fp64": v_64p / v_64b if v_64p else None,
         "fp64_alone_particle_steps_per_s": v_alone,
         "fp64_alone_sample_steps": args.alone_steps,
         "speedup_mixed_batched_vs_fp64_alone": v_mx / v_alone,
-        "speedup_mixed_persistent_vs_fp64_alone": v_mxp / v_alone,
+        "speedup_mixed_persistent_vs_fp64_alone": v_mxp / v_alone if v_mxp else None,
         "speedup_fp64_batched_vs_fp64_alone": v_64b / v_alone,
         "cpu_reference_throughput": cpu,
         "speedup_mixed_batched_vs_cpu_reference": v_mx / cpu["value"] if cpu else None,
@@ -427,6 +433,9 @@ def main(argv=None):
                          "ranges per XCD, 1 = dispatch order)")
     ap.add_argument("--no-ens-graph", action="store_true",
                     help="skip the graph-replay run of the ensemble (profiler runs)")
+    ap.add_argument("--no-ens-persistent", action="store_true",
+                    help="skip the persistent (cooperative) runs of the ensemble (profiler runs: "
+                         "rocprofv3 can fail at exit after cooperative launches)")
     ap.add_argument("--pass-shape", type=int, default=-1,
                     help="launch shape of the 4M runs' force passes instead of autotuning "
                          "(profiling runs; with --tune-steps > 0 the run first takes the "
