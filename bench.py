@@ -187,7 +187,7 @@ def timed_run(wl, args, precision, hmode):
     bracketed by device synchronisations.  Returns the rate, the stats of the
     library (pass timings from HIP events on its stream) and the final
     positions."""
-    from dem3d_amd.native import NSHAPES, System, load_hip
+    from dem3d_amd.native import System, autotune_steps, load_hip
     g = System(load_hip(), wl.red, skin=args.skin, history_mode=hmode, precision=precision)
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
@@ -200,7 +200,7 @@ def timed_run(wl, args, precision, hmode):
             # the steps the autotuner would take first: the timed steps are
             # then those of an autotuned run (every shape gives the same
             # trajectory), as profiling runs of one shape need
-            g.step((NSHAPES + 1) * (args.tune_steps + 1))
+            g.step(autotune_steps(args.tune_steps))
     elif args.tune_steps > 0:
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
@@ -230,7 +230,7 @@ def ensemble_run(args):
     mixed precision for --ens-steps steps, batched FP64, and the plain FP64
     path (each system stepped alone, on a --alone-steps sample)."""
     from dem3d_amd import workloads
-    from dem3d_amd.native import DEM_FP64, DEM_MIXED, NSHAPES, Batch, System, load_hip
+    from dem3d_amd.native import DEM_FP64, DEM_MIXED, Batch, System, autotune_steps, load_hip
     lib = load_hip()
     members = workloads.ensemble_256()
     ntot = sum(w.np for w in members)
@@ -262,7 +262,7 @@ def ensemble_run(args):
         if tune_steps:
             tune = b.autotune(steps_per_shape=tune_steps)
         else:
-            b.step(NSHAPES * (args.tune_steps + 1))  # the same trajectory as a tuned run
+            b.step(autotune_steps(args.tune_steps))  # the same trajectory as a tuned run
         b.step(8)
         t0 = time.perf_counter()
         b.step(steps)

@@ -2129,56 +2129,69 @@ int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, de
   if (!h->has_state || h->np == 0) return fail(h, DEM_ERR_STATE, "upload a non-empty state first");
   const bool timing = h->timing;
   const int keep = h->shape;
+  const int map0 = h->block_map;
   int64_t next = first_step;
+  // one segment: an untimed step if warm (the first launch of a shape loads
+  // its code), then nsteps steps timed with HIP events in shape c and block
+  // order m; *ms = their summed time (both passes)
+  auto seg = [&](int c, int m, int64_t nsteps, bool warm, double* ms) -> int {
+    h->shape = c;
+    h->block_map = m;
+    *ms = 0.0;
+    int64_t done = 0;
+    if (warm) {
+      h->timing = false;
+      const int rc = dem_step(h, 1, next, &done, nullptr);
+      out->steps += done;
+      next += done;
+      if (rc != DEM_OK) return rc;
+    }
+    if (nsteps <= 0) return DEM_OK;
+    h->timing = true;
+    const int rc = dem_step(h, nsteps, next, &done, nullptr);
+    out->steps += done;
+    next += done;
+    if (rc == DEM_OK) *ms = (h->ms_pass1 + h->ms_pass2) * (double)nsteps;
+    return rc;
+  };
+  auto undo = [&](int rc) {
+    h->timing = timing;
+    h->shape = keep;
+    h->block_map = map0;
+    return rc;
+  };
+  // every shape in a forward sweep, then in a backward sweep, half the steps
+  // each: a trend in the system (contacts forming or relaxing) weighs on
+  // every shape alike
+  const int64_t h1 = (steps_per_shape + 1) / 2, h2 = steps_per_shape - h1;
+  double t[DEM_NSHAPES], x;
+  for (int c = 0; c < DEM_NSHAPES; ++c) {
+    const int rc = seg(c, map0, h1, true, &t[c]);
+    if (rc != DEM_OK) return undo(rc);
+  }
+  for (int c = DEM_NSHAPES - 1; c >= 0; --c) {
+    const int rc = seg(c, map0, h2, false, &x);
+    if (rc != DEM_OK) return undo(rc);
+    t[c] += x;
+  }
   int best = 0;
   for (int c = 0; c < DEM_NSHAPES; ++c) {
-    h->shape = c;
-    int64_t done = 0;
-    h->timing = false;  // the first launch of a shape loads its code
-    int rc = dem_step(h, 1, next, &done, nullptr);
-    out->steps += done;
-    next += done;
-    if (rc == DEM_OK) {
-      h->timing = true;
-      rc = dem_step(h, steps_per_shape, next, &done, nullptr);
-      out->steps += done;
-      next += done;
-    }
-    if (rc != DEM_OK) {
-      h->timing = timing;
-      h->shape = keep;
-      return rc;
-    }
-    shape_table(out->block, out->waves, out->front);
-    out->ms[c] = h->ms_pass1 + h->ms_pass2;
+    out->ms[c] = t[c] / (double)steps_per_shape;
     if (out->ms[c] < out->ms[best]) best = c;
   }
-  // the other block order, in the best shape
-  const int map0 = h->block_map;
-  out->ms_map[map0] = out->ms[best];
+  shape_table(out->block, out->waves, out->front);
+  // the two block orders in the best shape, timed in the order A B B A
+  double ta = 0.0, tb = 0.0;
+  int rc = seg(best, map0, h1, false, &x);
+  ta += x;
+  if (rc == DEM_OK) rc = seg(best, 1 - map0, h1, true, &x), tb += x;
+  if (rc == DEM_OK) rc = seg(best, 1 - map0, h2, false, &x), tb += x;
+  if (rc == DEM_OK) rc = seg(best, map0, h2, false, &x), ta += x;
+  if (rc != DEM_OK) return undo(rc);
+  out->ms_map[map0] = ta / (double)steps_per_shape;
+  out->ms_map[1 - map0] = tb / (double)steps_per_shape;
   h->shape = best;
-  h->block_map = 1 - map0;
-  {
-    int64_t done = 0;
-    h->timing = false;
-    int rc = dem_step(h, 1, next, &done, nullptr);
-    out->steps += done;
-    next += done;
-    if (rc == DEM_OK) {
-      h->timing = true;
-      rc = dem_step(h, steps_per_shape, next, &done, nullptr);
-      out->steps += done;
-      next += done;
-    }
-    if (rc != DEM_OK) {
-      h->timing = timing;
-      h->shape = keep;
-      h->block_map = map0;
-      return rc;
-    }
-    out->ms_map[1 - map0] = h->ms_pass1 + h->ms_pass2;
-  }
-  if (!(out->ms_map[1 - map0] < out->ms_map[map0])) h->block_map = map0;
+  h->block_map = (out->ms_map[1 - map0] < out->ms_map[map0]) ? 1 - map0 : map0;
   h->timing = timing;
   out->nshapes = DEM_NSHAPES;
   out->chosen = best;
@@ -3546,58 +3559,65 @@ int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) 
   if (n == 0) return bfail(b, DEM_ERR_STATE, "the batch has no systems");
   const bool timing = b->timing;
   const int keep = b->shape;
+  const int map0 = b->block_map;
   std::vector<int64_t> done(n);
-  int best = 0;
-  for (int c = 0; c < DEM_NSHAPES; ++c) {
+  // one segment, as in dem_autotune; *ms = the summed device time of the
+  // timed steps (the whole batched step)
+  auto seg = [&](int c, int m, int64_t nsteps, bool warm, double* ms) -> int {
     if (c != b->shape) b->dirty = true;
     b->shape = c;
-    b->timing = false;  // the first launch of a shape loads its code
-    int rc = dem_batch_step(b, 1, done.data(), nullptr);
-    out->steps += done[0];
-    if (rc == DEM_OK) {
-      b->timing = true;
-      rc = dem_batch_step(b, steps_per_shape, done.data(), nullptr);
+    b->block_map = m;
+    *ms = 0.0;
+    if (warm) {
+      b->timing = false;
+      const int rc = dem_batch_step(b, 1, done.data(), nullptr);
       out->steps += done[0];
+      if (rc != DEM_OK) return rc;
     }
-    if (rc != DEM_OK) {
-      b->timing = timing;
-      if (b->shape != keep) b->dirty = true;
-      b->shape = keep;
-      return rc;
-    }
-    double sum = 0.0;
-    for (double t : b->step_ms) sum += t;
-    shape_table(out->block, out->waves, out->front);
-    out->ms[c] = sum / (double)steps_per_shape;
+    if (nsteps <= 0) return DEM_OK;
+    b->timing = true;
+    const int rc = dem_batch_step(b, nsteps, done.data(), nullptr);
+    out->steps += done[0];
+    if (rc == DEM_OK)
+      for (double t : b->step_ms) *ms += t;
+    return rc;
+  };
+  auto undo = [&](int rc) {
+    b->timing = timing;
+    if (b->shape != keep) b->dirty = true;
+    b->shape = keep;
+    b->block_map = map0;
+    return rc;
+  };
+  const int64_t h1 = (steps_per_shape + 1) / 2, h2 = steps_per_shape - h1;
+  double t[DEM_NSHAPES], x;
+  for (int c = 0; c < DEM_NSHAPES; ++c) {
+    const int rc = seg(c, map0, h1, true, &t[c]);
+    if (rc != DEM_OK) return undo(rc);
+  }
+  for (int c = DEM_NSHAPES - 1; c >= 0; --c) {
+    const int rc = seg(c, map0, h2, false, &x);
+    if (rc != DEM_OK) return undo(rc);
+    t[c] += x;
+  }
+  int best = 0;
+  for (int c = 0; c < DEM_NSHAPES; ++c) {
+    out->ms[c] = t[c] / (double)steps_per_shape;
     if (out->ms[c] < out->ms[best]) best = c;
   }
-  // the other block order, in the best shape
-  if (b->shape != best) b->dirty = true;
+  shape_table(out->block, out->waves, out->front);
+  double ta = 0.0, tb = 0.0;
+  int rc = seg(best, map0, h1, false, &x);
+  ta += x;
+  if (rc == DEM_OK) rc = seg(best, 1 - map0, h1, true, &x), tb += x;
+  if (rc == DEM_OK) rc = seg(best, 1 - map0, h2, false, &x), tb += x;
+  if (rc == DEM_OK) rc = seg(best, map0, h2, false, &x), ta += x;
+  if (rc != DEM_OK) return undo(rc);
+  out->ms_map[map0] = ta / (double)steps_per_shape;
+  out->ms_map[1 - map0] = tb / (double)steps_per_shape;
+  if (best != b->shape) b->dirty = true;
   b->shape = best;
-  const int map0 = b->block_map;
-  out->ms_map[map0] = out->ms[best];
-  b->block_map = 1 - map0;
-  {
-    b->timing = false;
-    int rc = dem_batch_step(b, 1, done.data(), nullptr);
-    out->steps += done[0];
-    if (rc == DEM_OK) {
-      b->timing = true;
-      rc = dem_batch_step(b, steps_per_shape, done.data(), nullptr);
-      out->steps += done[0];
-    }
-    if (rc != DEM_OK) {
-      b->timing = timing;
-      if (b->shape != keep) b->dirty = true;
-      b->shape = keep;
-      b->block_map = map0;
-      return rc;
-    }
-    double sum = 0.0;
-    for (double t : b->step_ms) sum += t;
-    out->ms_map[1 - map0] = sum / (double)steps_per_shape;
-  }
-  if (!(out->ms_map[1 - map0] < out->ms_map[map0])) b->block_map = map0;
+  b->block_map = (out->ms_map[1 - map0] < out->ms_map[map0]) ? 1 - map0 : map0;
   b->timing = timing;
   out->nshapes = DEM_NSHAPES;
   out->chosen = best;

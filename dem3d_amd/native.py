@@ -72,6 +72,11 @@ assert C.sizeof(ParticleT) == 128
 NSHAPES = 8
 
 
+def autotune_steps(steps_per_shape):
+    """Steps that dem_autotune / dem_batch_autotune advance a system by."""
+    return NSHAPES * (steps_per_shape + 1) + 2 * steps_per_shape + 1
+
+
 class TuneT(C.Structure):
     """dem_tune_t: the autotuner's report."""
     _fields_ = [("nshapes", C.c_int32), ("chosen", C.c_int32), ("steps", C.c_int64),

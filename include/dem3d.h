@@ -542,15 +542,21 @@ int64_t dem_batch_step_times(dem_batch_t b, int64_t cap, double* ms);
  * bitwise the same results, so a shape
  * changes speed only; shape 0 (256 lanes, 3 waves, contacts only) is the
  * default.  The autotuner times every shape on the live system while
- * stepping it: each shape runs 1 untimed step (its first launch) and then
- * steps_per_shape steps timed with HIP events, and the fastest becomes the
- * system's (batch's) shape.  The steps count as ordinary MoveParticles steps
- * (numbered from first_step, resp. each system's next_step), and the
- * trajectory is bitwise the one dem_step (dem_batch_step) gives for the same
- * number of steps.  Then the best shape runs steps_per_shape + 1 more steps
- * in the other block order and the faster order is kept: "block_map" 0 gives
- * each XCD consecutive particle ranges, 1 keeps the dispatch order (all XCDs
- * in one region of the particles at a time).  Escapes and errors stop the
+ * stepping it, over steps_per_shape steps timed with HIP events: half of
+ * them in a sweep over the shapes in order and the rest in a sweep back
+ * (the first launch of a shape, which loads its code, is an untimed step
+ * before its first timed steps), so that a trend in the system (contacts
+ * forming or relaxing) weighs on every shape alike; the fastest becomes the
+ * system's (batch's) shape.  Then the best shape is timed in both block
+ * orders, current and other, in the order A B B A with the same halves (one
+ * untimed step before the other order's first steps), and the faster order
+ * is kept: "block_map" 0 gives each XCD consecutive particle ranges, 1
+ * keeps the dispatch order (all XCDs in one region of the particles at a
+ * time).  For s = steps_per_shape the tuning takes
+ * DEM_NSHAPES * (s + 1) + 2 * s + 1 steps.  They count as ordinary
+ * MoveParticles steps (numbered from first_step, resp. each system's
+ * next_step), and the trajectory is bitwise the one dem_step
+ * (dem_batch_step) gives for the same number of steps.  Escapes and errors stop the
  * tuning with dem_step's return code, the shape and order unchanged.  Both
  * can also be set directly with the options "pass_shape" and "block_map". */
 #define DEM_NSHAPES 8

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from dem3d_amd import workloads
-from dem3d_amd.native import DEM_FP64, DEM_MIXED, NSHAPES, Batch, System
+from dem3d_amd.native import DEM_FP64, DEM_MIXED, NSHAPES, Batch, System, autotune_steps
 
 pytestmark = pytest.mark.gpu
 
@@ -67,7 +67,10 @@ def test_autotune_keeps_the_trajectory(hip):
     kw = CASES[0]
     a = _make(hip, kw)
     rep = a.autotune(steps_per_shape=3)
-    assert rep["steps"] == NSHAPES * 4 + 4   # every shape, then the other block order
+    # every shape in a sweep forward (an untimed first step, 2 timed) and
+    # back (1 timed), then the best shape's block orders A B B A (2, 1 + 2,
+    # 1, 1 steps)
+    assert rep["steps"] == NSHAPES * 4 + 7 == autotune_steps(3)
     assert rep["block_map"] in (0, 1) and all(m > 0 for m in rep["ms_map"])
     assert rep["ms_map"][rep["block_map"]] <= rep["ms_map"][1 - rep["block_map"]]
     assert len(rep["ms"]) == NSHAPES and all(m > 0 for m in rep["ms"])
@@ -90,7 +93,7 @@ def test_batch_shapes_and_autotune_equal_alone(hip):
     alone = []
     for kw in systems_kw:
         s = _make(hip, kw)
-        s.step(NSHAPES * 3 + 3 + 9 + 9)
+        s.step(autotune_steps(2) + 9 + 9)
         alone.append(_state(s))
         s.close()
     systems = [_make(hip, kw) for kw in systems_kw]
@@ -101,7 +104,7 @@ def test_batch_shapes_and_autotune_equal_alone(hip):
     b.set_option("block_map", 1)
     b.step(9)
     rep = b.autotune(steps_per_shape=2)
-    assert rep["steps"] == NSHAPES * 3 + 3
+    assert rep["steps"] == NSHAPES * 3 + 5 == autotune_steps(2)
     assert rep["chosen"] == int(np.argmin(rep["ms"]))
     b.step(9)
     for s, ref in zip(systems, alone):
