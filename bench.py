@@ -171,15 +171,20 @@ def cpu_model():
     return "unknown CPU"
 
 
-def traffic_from_profiles(workload, kernel):
-    """HBM bytes per launch of kernel ("pass1" or "pass2") measured with
-    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) for this workload, committed in
-    profiles/; None when absent."""
+def traffic_from_profiles(workload, kernel, shape, block_map):
+    """HBM bytes per launch of kernel ("pass1" or "pass2") in the given launch
+    shape and block order, measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE) for this workload over the benchmark's timed steps and
+    committed in profiles/; None when not measured."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get(f"{kernel}_bytes_per_launch")
+        d = json.load(f)
+    for e in d.get("launches", []):
+        if (e["shape"], e["block_map"]) == (shape, block_map):
+            return e.get(f"{kernel}_bytes_per_launch")
+    return None
 
 
 def timed_run(wl, args, precision, hmode):
@@ -192,7 +197,9 @@ def timed_run(wl, args, precision, hmode):
     g.upload(wl.comp, wl.x, wl.v, wl.w)
     g.startup()
     tune = None
+    shape = (0, 0)  # the library's default launch shape and block order
     if args.pass_shape >= 0:
+        shape = (args.pass_shape, max(args.block_map, 0))
         g.set_option("pass_shape", args.pass_shape)
         if args.block_map >= 0:
             g.set_option("block_map", args.block_map)
@@ -205,6 +212,7 @@ def timed_run(wl, args, precision, hmode):
         # dem_autotune: every launch shape of the passes timed on the live bed
         # (these are ordinary steps: the trajectory does not change)
         tune = g.autotune(steps_per_shape=args.tune_steps)
+        shape = (tune["chosen"], tune["block_map"])
         print(f"autotune: shape {tune['chosen']} of {tune['block']}/{tune['waves']}/"
               f"{tune['front']}, block order {tune['block_map']} {tune['ms_map']}, "
               f"ms {[round(m, 4) for m in tune['ms']]}", file=sys.stderr, flush=True)
@@ -219,7 +227,7 @@ def timed_run(wl, args, precision, hmode):
     st = g.stats()
     sec = t1 - t0
     out = {"ms": 1e3 * sec / args.steps, "value": wl.np * args.steps / sec, "stats": st,
-           "obs": g.observables(), "tune": tune,
+           "obs": g.observables(), "tune": tune, "shape": shape,
            "steps_before": g.next_step - args.steps - wl.red.method_order}
     g.close()
     return out
@@ -562,7 +570,7 @@ def main(argv=None):
     if st["ms_pass2"] > st["ms_pass1"]:
         dom_ms, dom_b, dom = (st["ms_pass2"], b2, "pass2")
     achieved = dom_b / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = traffic_from_profiles(wl.name, dom)
+    traffic = traffic_from_profiles(wl.name, dom, *r64["shape"])
     out = {
         "metric": "particle-steps/sec at 4M spheres (FP64 vs mixed) + batched p50/p99 step latency",
         "value": value,
@@ -607,6 +615,7 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_launch_shape": list(r64["shape"]),
             "bytes_per_launch": dom_b,
             "step_frac": (b1 + b2) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
