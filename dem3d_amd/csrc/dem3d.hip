@@ -2181,15 +2181,19 @@ int dem_autotune(dem_handle_t h, int64_t steps_per_shape, int64_t first_step, de
   }
   shape_table(out->block, out->waves, out->front);
   // the two block orders in the best shape, timed in the order A B B A
-  double ta = 0.0, tb = 0.0;
-  int rc = seg(best, map0, h1, false, &x);
-  ta += x;
-  if (rc == DEM_OK) rc = seg(best, 1 - map0, h1, true, &x), tb += x;
-  if (rc == DEM_OK) rc = seg(best, 1 - map0, h2, false, &x), tb += x;
-  if (rc == DEM_OK) rc = seg(best, map0, h2, false, &x), ta += x;
-  if (rc != DEM_OK) return undo(rc);
-  out->ms_map[map0] = ta / (double)steps_per_shape;
-  out->ms_map[1 - map0] = tb / (double)steps_per_shape;
+  double tm[2] = {0.0, 0.0};
+  const struct {
+    int map;
+    int64_t nsteps;
+    bool warm;
+  } abba[4] = {{map0, h1, false}, {1 - map0, h1, true}, {1 - map0, h2, false}, {map0, h2, false}};
+  for (const auto& q : abba) {
+    const int rc = seg(best, q.map, q.nsteps, q.warm, &x);
+    if (rc != DEM_OK) return undo(rc);
+    tm[q.map] += x;
+  }
+  out->ms_map[0] = tm[0] / (double)steps_per_shape;
+  out->ms_map[1] = tm[1] / (double)steps_per_shape;
   h->shape = best;
   h->block_map = (out->ms_map[1 - map0] < out->ms_map[map0]) ? 1 - map0 : map0;
   h->timing = timing;
@@ -3606,15 +3610,20 @@ int dem_batch_autotune(dem_batch_t b, int64_t steps_per_shape, dem_tune_t* out) 
     if (out->ms[c] < out->ms[best]) best = c;
   }
   shape_table(out->block, out->waves, out->front);
-  double ta = 0.0, tb = 0.0;
-  int rc = seg(best, map0, h1, false, &x);
-  ta += x;
-  if (rc == DEM_OK) rc = seg(best, 1 - map0, h1, true, &x), tb += x;
-  if (rc == DEM_OK) rc = seg(best, 1 - map0, h2, false, &x), tb += x;
-  if (rc == DEM_OK) rc = seg(best, map0, h2, false, &x), ta += x;
-  if (rc != DEM_OK) return undo(rc);
-  out->ms_map[map0] = ta / (double)steps_per_shape;
-  out->ms_map[1 - map0] = tb / (double)steps_per_shape;
+  // the two block orders in the best shape, timed in the order A B B A
+  double tm[2] = {0.0, 0.0};
+  const struct {
+    int map;
+    int64_t nsteps;
+    bool warm;
+  } abba[4] = {{map0, h1, false}, {1 - map0, h1, true}, {1 - map0, h2, false}, {map0, h2, false}};
+  for (const auto& q : abba) {
+    const int rc = seg(best, q.map, q.nsteps, q.warm, &x);
+    if (rc != DEM_OK) return undo(rc);
+    tm[q.map] += x;
+  }
+  out->ms_map[0] = tm[0] / (double)steps_per_shape;
+  out->ms_map[1] = tm[1] / (double)steps_per_shape;
   if (best != b->shape) b->dirty = true;
   b->shape = best;
   b->block_map = (out->ms_map[1 - map0] < out->ms_map[map0]) ? 1 - map0 : map0;
