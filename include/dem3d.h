@@ -216,7 +216,13 @@ int dem_download_walls(dem_handle_t h, int32_t nw, dem_wall_t* walls);
 /* Particle state.  a/alpha/ring/ipc/cell_rank may be NULL (zeros; fresh ring;
  * fresh cell lists as DistributeParticles3D builds them).  cell_rank gives the
  * order of the reference's cell lists: within one cell a particle with a larger
- * rank precedes one with a smaller rank (ties: larger id first). */
+ * rank precedes one with a smaller rank (ties: larger id first).  Right after
+ * dem_euler_startup, until the next evaluation, the lists still hold the
+ * particles that the last Euler sub-step moved into another cell in their old
+ * cell; a state downloaded then (or written by dem_save_state_file then) does
+ * not carry that, and a pair of such a particle in its new cell can resume
+ * with the other role.  States taken after a MoveParticles step resume within
+ * rounding (the rebuilt lists sum in another order). */
 int dem_upload_state(dem_handle_t h, int32_t np, const int32_t* comp,
                      const double* x, const double* v, const double* w,
                      const double* a, const double* alpha,

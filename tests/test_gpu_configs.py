@@ -9,7 +9,8 @@
   reached, one evaluation and three ABM steps against the oracle (FP64 bar);
   300 steps compared with the oracle after every step (accelerations and
   kinetic energy), the oracle running six 50-step windows from the GPU's
-  checkpoints in parallel threads; and the mixed mode by shadow evaluation --
+  checkpoints in parallel threads; the configuration's 5 000 steps sampled
+  by eleven such windows, one every 500 steps and the last 50; and the mixed mode by shadow evaluation --
   at each of three successive FP64 states, the mixed forces of the same state
   against the oracle's (mixed bar).
 * the dense sheared bed, packed disordered at volume fraction 0.60
@@ -119,63 +120,70 @@ def test_config1_poly_200k_fp64_against_oracle(hip, poly200k):
     g.close()
 
 
-def test_config1_every_step_for_300_steps(hip, poly200k):
-    """Six windows of 50 steps.  The GPU steps one step at a time, keeping the
-    accelerations of every step's first evaluation and the kinetic energy;
-    from each window's starting state the oracle takes the same 50 steps in
-    a thread of its own.  After every step: on a window's first step
-    (identical input) accelerations within 1e-12 of each particle's force
-    scale; on the later ones, where the two trajectories separate by
-    rounding, within 1e-10 of the largest force scale of the system; the
-    kinetic energy within 1e-12 and 1e-10 relative; at a window's end the
+def _oracle_window(wl, nstep, st, con, a, al, ke, keys):
+    """The oracle's nstep steps from a GPU checkpoint (state st, contacts
+    con) against the GPU's accelerations a, al and kinetic energies ke of the
+    same steps and its contact keys at the end.  Returns the worst relative
+    errors (first-step a, later a, first-step ke, later ke) and whether the
     contact sets are identical."""
-    import threading
-    wl, g0 = poly200k
     im, ii = _inv(wl.comp, wl.red)
-    nwin, nstep = 6, 50
-    out = [None] * nwin
+    o = oracle_system(wl.red)
+    upload_same([o], wl, state=st, contacts=con)
+    worst = [0.0, 0.0, 0.0, 0.0]
+    for k in range(nstep):
+        o.step(1)
+        so = o.download()
+        fs, ts = force_scale(o)
+        if k == 0:
+            _, ra = accel_tolerance_ok(a[k], so["a"], fs, im, 1.0)
+            _, rb = accel_tolerance_ok(al[k], so["alpha"], ts + 1e-300, ii, 1.0)
+        else:
+            ra = np.abs(a[k] - so["a"]).max() / (fs * im).max()
+            rb = np.abs(al[k] - so["alpha"]).max() / (ts * ii).max()
+        ko = o.stats()["kinetic_energy"]
+        q = 0 if k == 0 else 1
+        worst[q] = max(worst[q], ra, rb)
+        worst[2 + q] = max(worst[2 + q], abs(ke[k] - ko) / ko)
+    same = bool(np.array_equal(o.download_contacts()["keys"], keys))
+    o.close()
+    return worst, same
 
-    def oracle_window(w, st, con, a, al, ke, keys):
+
+def _windows_against_oracle(wl, g, starts, nstep):
+    """The GPU system g steps to each window's first step, then one step at a
+    time for nstep steps, keeping the accelerations of every step's first
+    evaluation and the kinetic energy; from each window's starting state the
+    oracle takes the same steps in a thread of its own, while the GPU goes
+    on.  Asserts the bars of the window test on every window."""
+    import threading
+    out = [None] * len(starts)
+
+    def run(w, *args):
         try:
-            o = oracle_system(wl.red)
-            upload_same([o], wl, state=st, contacts=con)
-            worst = [0.0, 0.0, 0.0, 0.0]  # first-step a, later a, first-step ke, later ke
-            for k in range(nstep):
-                o.step(1)
-                so = o.download()
-                fs, ts = force_scale(o)
-                if k == 0:
-                    _, ra = accel_tolerance_ok(a[k], so["a"], fs, im, 1.0)
-                    _, rb = accel_tolerance_ok(al[k], so["alpha"], ts + 1e-300, ii, 1.0)
-                else:
-                    ra = np.abs(a[k] - so["a"]).max() / (fs * im).max()
-                    rb = np.abs(al[k] - so["alpha"]).max() / (ts * ii).max()
-                ko = o.stats()["kinetic_energy"]
-                q = 0 if k == 0 else 1
-                worst[q] = max(worst[q], ra, rb)
-                worst[2 + q] = max(worst[2 + q], abs(ke[k] - ko) / ko)
-            same = bool(np.array_equal(o.download_contacts()["keys"], keys))
-            o.close()
-            out[w] = (worst, same)
+            out[w] = _oracle_window(wl, nstep, *args)
         except BaseException as exc:  # reported by the main thread
             out[w] = exc
 
     threads = []
-    for w in range(nwin):
-        st = full_state(g0, with_rank=True)
-        con = g0.download_contacts()
+    done = 0
+    for w, s0 in enumerate(starts):
+        if s0 > done:
+            g.step(s0 - done)
+        st = full_state(g, with_rank=True)
+        con = g.download_contacts()
         a, al, ke = [], [], []
         for _ in range(nstep):
-            g0.step(1)
-            s = g0.download()
+            g.step(1)
+            s = g.download()
             a.append(s["a"])
             al.append(s["alpha"])
-            ke.append(g0.stats()["kinetic_energy"])
-        keys = g0.download_contacts()["keys"]
-        t = threading.Thread(target=oracle_window, args=(w, st, con, a, al, ke, keys))
+            ke.append(g.stats()["kinetic_energy"])
+        done = s0 + nstep
+        keys = g.download_contacts()["keys"]
+        t = threading.Thread(target=run, args=(w, st, con, a, al, ke, keys))
         t.start()
         threads.append(t)
-        print(f"window {w}: GPU done, oracle started", flush=True)
+        print(f"window {w} (steps {s0}-{s0 + nstep}): GPU done, oracle started", flush=True)
     for t in threads:
         t.join()
     for w, r in enumerate(out):
@@ -186,6 +194,43 @@ def test_config1_every_step_for_300_steps(hip, poly200k):
         assert worst[0] <= 1e-12 and worst[1] <= 1e-10, (w, worst)
         assert worst[2] <= 1e-12 and worst[3] <= 1e-10, (w, worst)
         assert same, w
+
+
+def test_config1_every_step_for_300_steps(hip, poly200k):
+    """Six consecutive windows of 50 steps.  The GPU steps one step at a
+    time, keeping the accelerations of every step's first evaluation and the
+    kinetic energy; from each window's starting state the oracle takes the
+    same 50 steps in a thread of its own.  After every step: on a window's
+    first step (identical input) accelerations within 1e-12 of each
+    particle's force scale; on the later ones, where the two trajectories
+    separate by rounding, within 1e-10 of the largest force scale of the
+    system; the kinetic energy within 1e-12 and 1e-10 relative; at a window's
+    end the contact sets are identical."""
+    wl, g0 = poly200k
+    _windows_against_oracle(wl, g0, [50 * w for w in range(6)], 50)
+
+
+def test_config1_windows_across_5000_steps(hip, poly200k):
+    """The configuration's 5 000 steps from the Euler start-up of the loose
+    packing, sampled: eleven windows of 50 steps, one every 500 steps and the
+    last 50 (the settling from the first steps to the end), each checked against the
+    oracle from the GPU's state at its start with the bars of the 300-step
+    test.  The first window starts after the first MoveParticles step: the
+    state of the handle right after the start-up holds particles that the
+    last Euler sub-step moved into another cell in their old cell list until
+    the next evaluation, which a state upload (or a state file the handle
+    writes at that point) does not represent -- from such a checkpoint the
+    GPU and the oracle both step alike, and both differ from the run that
+    goes on in one pair of this packing."""
+    wl, _ = poly200k
+    g = System(hip, wl.red)
+    g.upload(wl.comp, wl.x, wl.v, wl.w)
+    g.startup()
+    _windows_against_oracle(wl, g, [1] + [500 * w for w in range(1, 10)] + [4950], 50)
+    ob = g.observables()
+    print(f"after {g.stats()['steps']} steps: phi {ob['packing_fraction']:.4f}, "
+          f"{ob['pair_contacts']} pairs", flush=True)
+    g.close()
 
 
 def test_config1_poly_200k_mixed_shadow_evaluation(hip, poly200k):
